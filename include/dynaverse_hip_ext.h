@@ -1,0 +1,45 @@
+/*
+ * dynaverse_hip_ext.h -- device-resident entry points of the MI355X library.
+ *
+ * These have no counterpart in the reference ABI.  They run the same kernels
+ * as dynaverse_gpu.h on buffers that already live in device memory (HBM), so
+ * that callers which keep data resident (the batching and serving layers,
+ * and throughput measurement) skip the host<->device copies.  Pointers are
+ * device addresses on HIP device 0; every call is synchronous on the owner's
+ * stream and returns the dynaverse_gpu.h status codes.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The owner's HIP stream (hipStream_t), for callers that time with events. */
+int dvgpu_ext_stream(void* owner, void** stream);
+
+/* Canonical little-endian affine points (64 B G1 / 128 B G2, zero = infinity)
+ * -> Montgomery words, device to device. g2 selects the group. */
+int dvgpu_ext_points_to_mont(void* owner, const void* canonical, void* mont, size_t n, int g2,
+                             char* error, size_t error_capacity);
+
+/* MSM over device-resident Montgomery points and canonical scalars.
+ * result: 64 (G1) or 128 (G2) canonical bytes on the host.  window_bits 0
+ * picks the default.  phase_ms, when not null, receives MSM_PHASE_EVENTS-1 = 7
+ * event-timed phase durations: digits, scan, scatter, tasks, accumulate,
+ * bucket reduce, window tree. */
+int dvgpu_ext_msm_device(void* owner, const void* points_mont, const void* scalars, size_t n,
+                         int g2, int window_bits, uint8_t* result, float* phase_ms,
+                         char* error, size_t error_capacity);
+
+/* rows x 2^log_n natural-order NTTs on canonical device words; in may equal
+ * out.  root as in dvgpu_bn254_ntt.  kernel_ms (optional) receives the
+ * event-timed duration of the transform. */
+int dvgpu_ext_ntt_device(void* owner, const void* in, void* out, size_t length, size_t rows,
+                         int inverse, const uint8_t* root, float* kernel_ms,
+                         char* error, size_t error_capacity);
+
+#ifdef __cplusplus
+}
+#endif

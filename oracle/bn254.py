@@ -1,0 +1,561 @@
+"""CPU oracle for the BN254 proving hot path (Python big-int restatement).
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports this
+module; only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
+leg of ``bench.py`` use it, and only as the checker.
+
+It restates, from the textbook definitions, what the reference computes with
+arkworks 0.4 (ark-bn254 0.4.0, ark-ec/ff/poly 0.4.2, a crates.io dependency
+that is absent from the reference checkout):
+
+* Fq, Fr, Fq2 arithmetic and the BN254 curves E: y^2 = x^3 + 3 over Fq and
+  E': y^2 = x^3 + 3/(9+u) over Fq2 (``crates/gpu/src/codec.rs:1-18``).
+* ``straight_msm``: left-to-right double-and-add MSM, the same algorithm as
+  ``crates/gpu/src/lib.rs:120-139`` (``CpuBackend::msm``).
+* ``ntt``: natural-order radix-2 DFT over <omega_n>, omega_n derived from the
+  2^20-th root the wrapper passes (``crates/gpu/src/icicle.rs:429-456``,
+  ``Radix2EvaluationDomain`` in ``crates/gpu/src/lib.rs:99-117``); inverse
+  carries the 1/n factor.
+* ``fixed_base``: [s]G followed by normalisation
+  (``crates/protocol/src/setup.rs:13-31``).
+* G2 wire parsing and subgroup predicates with masks 1/2/4
+  (``crates/gpu/src/g2_validation.rs:214-299``,
+  ``native/include/icicle_g2_wire.cuh:9-35``,
+  ``native/src/icicle_g2_validation.cu:20-116``).
+
+Every function here is pinned by a check in ``tests/test_oracle.py``
+(closed forms from the reference's own tests, brute-force DFT, group-order
+identities, cross-check against the C oracle).  Functions without such a
+check would say "parity unpinned" here; none are left in that state.
+"""
+
+from __future__ import annotations
+
+# --------------------------------------------------------------------------
+# Moduli and curve constants
+# --------------------------------------------------------------------------
+Q = 0x30644E72E131A029B85045B68181585D97816A916871CA8D3C208C16D87CFD47  # Fq
+R = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001  # Fr
+B1 = 3
+G1_GEN = (1, 2)
+# EIP-197 / ark-bn254 G2 generator, (c0, c1) pairs.
+G2_GEN = (
+    (10857046999023057135944570762232829481370756359578518086990519993285655852781,
+     11559732032986387107991004021392285783925812861821192530917403151452391805634),
+    (8495653923123431417604973247489272438418190587263600148770280649306958101930,
+     4082367875863433681332203403145435568316851327593401208105741076214120093531),
+)
+BN_SEED = 4965661367192848881  # BN parameter x; 63 bits, popcount 28
+LAMBDA = 6 * BN_SEED * BN_SEED  # psi acts as [6x^2] on the r-torsion of E'
+FR_GENERATOR = 5  # multiplicative generator arkworks derives roots from
+TWO_ADICITY = 28
+
+
+def fr_root_of_unity(n: int) -> int:
+    """omega_n = 5^((r-1)/n) for a power of two n <= 2^28."""
+    assert n >= 1 and n & (n - 1) == 0 and n <= (1 << TWO_ADICITY)
+    return pow(FR_GENERATOR, (R - 1) // n, R)
+
+
+ROOT_2_20 = fr_root_of_unity(1 << 20)
+
+# --------------------------------------------------------------------------
+# Fq2 = Fq[u]/(u^2+1)
+# --------------------------------------------------------------------------
+
+
+def f2_add(a, b):
+    return ((a[0] + b[0]) % Q, (a[1] + b[1]) % Q)
+
+
+def f2_sub(a, b):
+    return ((a[0] - b[0]) % Q, (a[1] - b[1]) % Q)
+
+
+def f2_neg(a):
+    return ((-a[0]) % Q, (-a[1]) % Q)
+
+
+def f2_mul(a, b):
+    return ((a[0] * b[0] - a[1] * b[1]) % Q, (a[0] * b[1] + a[1] * b[0]) % Q)
+
+
+def f2_inv(a):
+    n = (a[0] * a[0] + a[1] * a[1]) % Q
+    ni = pow(n, Q - 2, Q)
+    return ((a[0] * ni) % Q, (-a[1] * ni) % Q)
+
+
+def f2_pow(a, e):
+    out = (1, 0)
+    while e:
+        if e & 1:
+            out = f2_mul(out, a)
+        a = f2_mul(a, a)
+        e >>= 1
+    return out
+
+
+def f2_conj(a):
+    return (a[0], (-a[1]) % Q)
+
+
+XI = (9, 1)
+B2 = f2_mul((3, 0), f2_inv(XI))  # twist coefficient 3/(9+u)
+
+# --------------------------------------------------------------------------
+# Field abstraction so the curve code serves both G1 (Fq) and G2 (Fq2)
+# --------------------------------------------------------------------------
+
+
+class _Fq:
+    zero = 0
+    one = 1
+    b = B1
+
+    @staticmethod
+    def add(a, b):
+        return (a + b) % Q
+
+    @staticmethod
+    def sub(a, b):
+        return (a - b) % Q
+
+    @staticmethod
+    def mul(a, b):
+        return (a * b) % Q
+
+    @staticmethod
+    def neg(a):
+        return (-a) % Q
+
+    @staticmethod
+    def inv(a):
+        return pow(a, Q - 2, Q)
+
+    @staticmethod
+    def is_zero(a):
+        return a == 0
+
+
+class _Fq2:
+    zero = (0, 0)
+    one = (1, 0)
+    b = B2
+    add = staticmethod(f2_add)
+    sub = staticmethod(f2_sub)
+    mul = staticmethod(f2_mul)
+    neg = staticmethod(f2_neg)
+    inv = staticmethod(f2_inv)
+
+    @staticmethod
+    def is_zero(a):
+        return a == (0, 0)
+
+
+FQ = _Fq
+FQ2 = _Fq2
+
+# --------------------------------------------------------------------------
+# Affine group law.  None is the point at infinity.  Affine arithmetic with an
+# inversion per step is slow but obviously correct; it is the oracle.
+# --------------------------------------------------------------------------
+
+
+def on_curve(p, F=FQ):
+    if p is None:
+        return True
+    x, y = p
+    return F.sub(F.mul(y, y), F.add(F.mul(F.mul(x, x), x), F.b)) == F.zero
+
+
+def neg(p, F=FQ):
+    if p is None:
+        return None
+    return (p[0], F.neg(p[1]))
+
+
+def add(p, q, F=FQ):
+    if p is None:
+        return q
+    if q is None:
+        return p
+    x1, y1 = p
+    x2, y2 = q
+    if x1 == x2:
+        if F.add(y1, y2) == F.zero:
+            return None
+        return double(p, F)
+    lam = F.mul(F.sub(y2, y1), F.inv(F.sub(x2, x1)))
+    x3 = F.sub(F.sub(F.mul(lam, lam), x1), x2)
+    y3 = F.sub(F.mul(lam, F.sub(x1, x3)), y1)
+    return (x3, y3)
+
+
+def double(p, F=FQ):
+    if p is None:
+        return None
+    x, y = p
+    if F.is_zero(y):
+        return None
+    three_x2 = F.mul(F.add(F.add(x, x), x), x)
+    lam = F.mul(three_x2, F.inv(F.add(y, y)))
+    x3 = F.sub(F.mul(lam, lam), F.add(x, x))
+    y3 = F.sub(F.mul(lam, F.sub(x, x3)), y)
+    return (x3, y3)
+
+
+# Jacobian arithmetic for the scalar-multiplication inner loops (same group
+# law, fewer inversions); converted back to affine at the end.
+
+def _jac_double(P, F):
+    X, Y, Z = P
+    if F.is_zero(Z) or F.is_zero(Y):
+        return (F.one, F.one, F.zero)
+    A = F.mul(X, X)
+    Bv = F.mul(Y, Y)
+    C = F.mul(Bv, Bv)
+    t = F.add(X, Bv)
+    D = F.sub(F.sub(F.mul(t, t), A), C)
+    D = F.add(D, D)
+    E = F.add(F.add(A, A), A)
+    Fv = F.mul(E, E)
+    X3 = F.sub(Fv, F.add(D, D))
+    C8 = F.add(C, C)
+    C8 = F.add(C8, C8)
+    C8 = F.add(C8, C8)
+    Y3 = F.sub(F.mul(E, F.sub(D, X3)), C8)
+    Z3 = F.mul(F.add(Y, Y), Z)
+    return (X3, Y3, Z3)
+
+
+def _jac_add(P, Qp, F):
+    X1, Y1, Z1 = P
+    X2, Y2, Z2 = Qp
+    if F.is_zero(Z1):
+        return Qp
+    if F.is_zero(Z2):
+        return P
+    Z1Z1 = F.mul(Z1, Z1)
+    Z2Z2 = F.mul(Z2, Z2)
+    U1 = F.mul(X1, Z2Z2)
+    U2 = F.mul(X2, Z1Z1)
+    S1 = F.mul(F.mul(Y1, Z2), Z2Z2)
+    S2 = F.mul(F.mul(Y2, Z1), Z1Z1)
+    if U1 == U2:
+        if S1 == S2:
+            return _jac_double(P, F)
+        return (F.one, F.one, F.zero)
+    H = F.sub(U2, U1)
+    Rr = F.sub(S2, S1)
+    H2 = F.mul(H, H)
+    H3 = F.mul(H2, H)
+    U1H2 = F.mul(U1, H2)
+    X3 = F.sub(F.sub(F.mul(Rr, Rr), H3), F.add(U1H2, U1H2))
+    Y3 = F.sub(F.mul(Rr, F.sub(U1H2, X3)), F.mul(S1, H3))
+    Z3 = F.mul(F.mul(Z1, Z2), H)
+    return (X3, Y3, Z3)
+
+
+def _to_jac(p, F):
+    if p is None:
+        return (F.one, F.one, F.zero)
+    return (p[0], p[1], F.one)
+
+
+def _from_jac(P, F):
+    X, Y, Z = P
+    if F.is_zero(Z):
+        return None
+    zi = F.inv(Z)
+    zi2 = F.mul(zi, zi)
+    return (F.mul(X, zi2), F.mul(Y, F.mul(zi2, zi)))
+
+
+def mul(p, k: int, F=FQ):
+    """[k]p for an unreduced non-negative integer k (double-and-add)."""
+    assert k >= 0
+    acc = _to_jac(None, F)
+    base = _to_jac(p, F)
+    while k:
+        if k & 1:
+            acc = _jac_add(acc, base, F)
+        base = _jac_double(base, F)
+        k >>= 1
+    return _from_jac(acc, F)
+
+
+def straight_msm(points, scalars, F=FQ):
+    """sum_i s_i * P_i by per-term double-and-add (``lib.rs:122-139``)."""
+    assert len(points) == len(scalars)
+    acc = _to_jac(None, F)
+    for p, s in zip(points, scalars):
+        assert 0 <= s < R
+        cur = _to_jac(p, F)
+        k = s
+        while k:
+            if k & 1:
+                acc = _jac_add(acc, cur, F)
+            cur = _jac_double(cur, F)
+            k >>= 1
+    return _from_jac(acc, F)
+
+
+def g1_mul(k):
+    return mul(G1_GEN, k % R, FQ)
+
+
+def g2_mul(k):
+    return mul(G2_GEN, k % R, FQ2)
+
+
+# --------------------------------------------------------------------------
+# Wire codec (crates/gpu/src/codec.rs:32-77; resident.rs encode_g2)
+# --------------------------------------------------------------------------
+
+
+def fr_to_bytes(v: int) -> bytes:
+    return int(v).to_bytes(32, "little")
+
+
+def bytes_to_int(b: bytes) -> int:
+    return int.from_bytes(bytes(b), "little")
+
+
+def g1_to_bytes(p) -> bytes:
+    if p is None:
+        return bytes(64)
+    return p[0].to_bytes(32, "little") + p[1].to_bytes(32, "little")
+
+
+def g1_from_bytes(b: bytes):
+    b = bytes(b)
+    assert len(b) == 64
+    if b == bytes(64):
+        return None
+    x = int.from_bytes(b[:32], "little")
+    y = int.from_bytes(b[32:], "little")
+    assert x < Q and y < Q and on_curve((x, y)), "invalid G1 bytes"
+    return (x, y)
+
+
+def g2_to_bytes(p) -> bytes:
+    if p is None:
+        return bytes(128)
+    (x0, x1), (y0, y1) = p
+    return b"".join(v.to_bytes(32, "little") for v in (x0, x1, y0, y1))
+
+
+def g2_from_bytes(b: bytes):
+    b = bytes(b)
+    assert len(b) == 128
+    if b == bytes(128):
+        return None
+    v = [int.from_bytes(b[32 * i:32 * i + 32], "little") for i in range(4)]
+    return ((v[0], v[1]), (v[2], v[3]))
+
+
+# --------------------------------------------------------------------------
+# NTT (natural order in and out; inverse scaled by 1/n)
+# --------------------------------------------------------------------------
+
+
+def dft_bruteforce(values, inverse=False):
+    """O(n^2) evaluation sum_j v_j w^(jk): the definition itself."""
+    n = len(values)
+    w = fr_root_of_unity(n)
+    if inverse:
+        w = pow(w, R - 2, R)
+    out = []
+    for k in range(n):
+        wk = pow(w, k, R)
+        acc = 0
+        t = 1
+        for v in values:
+            acc = (acc + v * t) % R
+            t = (t * wk) % R
+        out.append(acc)
+    if inverse:
+        ninv = pow(n, R - 2, R)
+        out = [(o * ninv) % R for o in out]
+    return out
+
+
+def ntt(values, inverse=False):
+    """Iterative radix-2 DIT transform, equal to ``dft_bruteforce``."""
+    n = len(values)
+    assert n >= 1 and n & (n - 1) == 0
+    a = [v % R for v in values]
+    if n == 1:
+        return a
+    logn = n.bit_length() - 1
+    rev = [int(bin(i)[2:].zfill(logn)[::-1], 2) for i in range(n)]
+    a = [a[rev[i]] for i in range(n)]
+    w_n = fr_root_of_unity(n)
+    if inverse:
+        w_n = pow(w_n, R - 2, R)
+    h = 1
+    while h < n:
+        w = pow(w_n, n // (2 * h), R)
+        tw = [1] * h
+        for j in range(1, h):
+            tw[j] = (tw[j - 1] * w) % R
+        for start in range(0, n, 2 * h):
+            for j in range(h):
+                u = a[start + j]
+                v = (a[start + j + h] * tw[j]) % R
+                a[start + j] = (u + v) % R
+                a[start + j + h] = (u - v) % R
+        h *= 2
+    if inverse:
+        ninv = pow(n, R - 2, R)
+        a = [(x * ninv) % R for x in a]
+    return a
+
+
+# --------------------------------------------------------------------------
+# Fixed-base setup (crates/protocol/src/setup.rs:13-31)
+# --------------------------------------------------------------------------
+
+
+def fixed_base_g1(scalars):
+    return [g1_mul(s) for s in scalars]
+
+
+def fixed_base_g2(scalars):
+    return [g2_mul(s) for s in scalars]
+
+
+# --------------------------------------------------------------------------
+# G2 wire validation (g2_validation.rs:214-299, icicle_g2_wire.cuh:9-35)
+# --------------------------------------------------------------------------
+MASK_ENCODING, MASK_CURVE, MASK_SUBGROUP = 1, 2, 4
+
+
+def g2_parse_wire(b: bytes):
+    """Return (status, point).  status 1 = encoding error, else 0.
+
+    All-zero is infinity.  Byte 127 carries arkworks' two flag bits: 0xC0 is
+    invalid, 0x40 marks infinity (coordinates must still be canonical), and
+    the flag bits are masked off the top limb before the canonical check.
+    """
+    b = bytes(b)
+    assert len(b) == 128
+    if b == bytes(128):
+        return 0, None
+    flags = b[127] & 0xC0
+    if flags == 0xC0:
+        return 1, None
+    raw = bytearray(b)
+    raw[127] &= 0x3F
+    v = [int.from_bytes(raw[32 * i:32 * i + 32], "little") for i in range(4)]
+    if any(x >= Q for x in v):
+        return 1, None
+    if flags == 0x40:
+        return 0, None
+    return 0, ((v[0], v[1]), (v[2], v[3]))
+
+
+_PSI_CX = f2_pow(XI, (Q - 1) // 3)
+_PSI_CY = f2_pow(XI, (Q - 1) // 2)
+
+
+def psi(p):
+    """Untwist-Frobenius-twist endomorphism on E'(Fq2)."""
+    if p is None:
+        return None
+    x, y = p
+    return (f2_mul(f2_conj(x), _PSI_CX), f2_mul(f2_conj(y), _PSI_CY))
+
+
+def subgroup_full_order(p):
+    return mul(p, R, FQ2) is None
+
+
+def subgroup_frobenius_127(p):
+    return psi(p) == mul(p, LAMBDA, FQ2)
+
+
+def subgroup_frobenius_63(p):
+    a = mul(p, BN_SEED, FQ2)
+    b = psi(a)
+    c = psi(b)
+    lhs = add(add(add(a, p, FQ2), b, FQ2), c, FQ2)
+    return lhs == double(psi(c), FQ2)
+
+
+SUBGROUP_ALGORITHMS = {1: subgroup_full_order, 2: subgroup_frobenius_127, 3: subgroup_frobenius_63}
+
+
+def g2_masks(wire: bytes, algorithm: int):
+    """Per-point masks and the 32-byte report fields for a wire array."""
+    assert len(wire) % 128 == 0
+    pred = SUBGROUP_ALGORITHMS[algorithm]
+    masks = []
+    for i in range(len(wire) // 128):
+        status, p = g2_parse_wire(wire[128 * i:128 * i + 128])
+        if status:
+            masks.append(MASK_ENCODING)
+        elif not on_curve(p, FQ2):
+            masks.append(MASK_CURVE)
+        elif p is not None and not pred(p):
+            masks.append(MASK_SUBGROUP)
+        else:
+            masks.append(0)
+    invalid = [i for i, m in enumerate(masks) if m]
+    agg = 0
+    for m in masks:
+        agg |= m
+    report = dict(checked_points=len(masks), invalid_points=len(invalid),
+                  first_invalid=invalid[0] if invalid else (1 << 64) - 1,
+                  failure_mask=agg, algorithm=algorithm)
+    return masks, report
+
+
+def twist_point_from_x(x0: int):
+    """Smallest twist point (x0 + k, 0 i + ...) found by incrementing x.c0.
+
+    Twist points of E'(Fq2) outside the r-torsion exist because the twist
+    has cofactor > 1; these feed the non-subgroup corpus.
+    """
+    k = x0
+    while True:
+        x = (k % Q, 1)
+        rhs = f2_add(f2_mul(f2_mul(x, x), x), B2)
+        y = f2_sqrt(rhs)
+        if y is not None:
+            return (x, y)
+        k += 1
+
+
+def _fq_sqrt(a):
+    a %= Q
+    if a == 0:
+        return 0
+    if pow(a, (Q - 1) // 2, Q) != 1:
+        return None
+    return pow(a, (Q + 1) // 4, Q)  # Q = 3 mod 4
+
+
+def f2_sqrt(a):
+    """Square root in Fq2 (complex method, Q = 3 mod 4)."""
+    a0, a1 = a
+    if a1 == 0:
+        s = _fq_sqrt(a0)
+        if s is not None:
+            return (s, 0)
+        s = _fq_sqrt((-a0) % Q)
+        return (0, s) if s is not None else None
+    norm = (a0 * a0 + a1 * a1) % Q
+    alpha = _fq_sqrt(norm)
+    if alpha is None:
+        return None
+    inv2 = pow(2, Q - 2, Q)
+    for cand in ((a0 + alpha) * inv2 % Q, (a0 - alpha) * inv2 % Q):
+        x0 = _fq_sqrt(cand)
+        if x0 is not None and x0 != 0:
+            x1 = a1 * pow(2 * x0, Q - 2, Q) % Q
+            r = (x0, x1)
+            if f2_mul(r, r) == (a0 % Q, a1 % Q):
+                return r
+    return None

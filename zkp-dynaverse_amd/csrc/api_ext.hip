@@ -1,0 +1,121 @@
+// Device-resident extension entry points (dynaverse_hip_ext.h).
+#include <cstring>
+
+#include "../../include/dynaverse_hip_ext.h"
+#include "owner.hpp"
+
+using namespace dvgpu;
+
+namespace {
+Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
+
+struct Events {
+    hipEvent_t e[MSM_PHASE_EVENTS] = {};
+    explicit Events(bool on) {
+        if (on)
+            for (auto& x : e) HIP_CHECK(hipEventCreate(&x));
+    }
+    ~Events() {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+template <class F>
+int msm_device(Owner* o, const void* points, const void* scalars, size_t n, int window_bits,
+               uint8_t* result, float* phase_ms) {
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
+    MsmConfig cfg = o->msm_config;
+    if (window_bits) cfg.window_bits = window_bits;
+    Events ev(phase_ms != nullptr);
+    const MsmPlan p0 = msm_plan(n, cfg);
+    uint32_t* window_words = o->msm.windows.as<uint32_t>(size_t(p0.windows) * XW);
+    o->msm.phase_events = phase_ms ? ev.e : nullptr;
+    MsmPlan plan;
+    try {
+        plan = msm_enqueue<F>(static_cast<const uint32_t*>(points), static_cast<const uint32_t*>(scalars),
+                              n, o->msm, o->stream, cfg, window_words);
+    } catch (...) {
+        o->msm.phase_events = nullptr;
+        throw;
+    }
+    o->msm.phase_events = nullptr;
+    uint32_t* host = o->msm.host_windows.as<uint32_t>(size_t(plan.windows) * XW);
+    HIP_CHECK(hipMemcpyAsync(host, window_words, size_t(plan.windows) * XW * 4, hipMemcpyDeviceToHost,
+                             o->stream));
+    HIP_CHECK(hipStreamSynchronize(o->stream));
+    if (phase_ms)
+        for (int k = 0; k + 1 < MSM_PHASE_EVENTS; ++k)
+            HIP_CHECK(hipEventElapsedTime(&phase_ms[k], ev.e[k], ev.e[k + 1]));
+    affine_to_bytes<F>(msm_combine_windows<F>(host, plan), result);
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int dvgpu_ext_stream(void* owner, void** stream) {
+    Owner* o = as_owner(owner);
+    if (!o || o->retired) return -4;
+    if (!stream) return -1;
+    *stream = o->stream;
+    return 0;
+}
+
+int dvgpu_ext_points_to_mont(void* owner, const void* canonical, void* mont, size_t n, int g2,
+                             char* error, size_t capacity) {
+    Owner* o = as_owner(owner);
+    return guarded(o, error, capacity, [&]() -> int {
+        require((canonical && mont) || !n, "null device buffers");
+        require(g2 == 0 || g2 == 1, "g2 must be 0 or 1");
+        if (g2)
+            points_to_mont<Fq2Ops>(static_cast<const uint8_t*>(canonical), static_cast<uint32_t*>(mont), n, o->stream);
+        else
+            points_to_mont<FqOps>(static_cast<const uint8_t*>(canonical), static_cast<uint32_t*>(mont), n, o->stream);
+        HIP_CHECK(hipStreamSynchronize(o->stream));
+        return 0;
+    });
+}
+
+int dvgpu_ext_msm_device(void* owner, const void* points, const void* scalars, size_t n, int g2,
+                         int window_bits, uint8_t* result, float* phase_ms, char* error,
+                         size_t capacity) {
+    Owner* o = as_owner(owner);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(result && n <= (size_t(1) << 24) && (!n || (points && scalars)), "invalid MSM arguments");
+        require(g2 == 0 || g2 == 1, "g2 must be 0 or 1");
+        if (!n) {
+            std::memset(result, 0, g2 ? 128 : 64);
+            return 0;
+        }
+        return g2 ? msm_device<Fq2Ops>(o, points, scalars, n, window_bits, result, phase_ms)
+                  : msm_device<FqOps>(o, points, scalars, n, window_bits, result, phase_ms);
+    });
+}
+
+int dvgpu_ext_ntt_device(void* owner, const void* in, void* out, size_t length, size_t rows,
+                         int inverse, const uint8_t* root, float* kernel_ms, char* error,
+                         size_t capacity) {
+    Owner* o = as_owner(owner);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(in && out && root && length && length <= DVGPU_MAX_NTT_SIZE && !(length & (length - 1)) &&
+                    rows && (inverse == 0 || inverse == 1),
+                "invalid NTT arguments");
+        uint32_t rw[8];
+        std::memcpy(rw, root, 32);
+        require(ntt_root_has_order_2_20(rw), "NTT root must have order 2^20");
+        ntt_domain_prepare(o->domain, rw, o->stream);
+        int log_n = 0;
+        while ((size_t(1) << log_n) < length) ++log_n;
+        Events ev(kernel_ms != nullptr);
+        if (kernel_ms) HIP_CHECK(hipEventRecord(ev.e[0], o->stream));
+        ntt_rows(o->domain, static_cast<const uint32_t*>(in), static_cast<uint32_t*>(out), log_n, rows,
+                 inverse != 0, o->ntt, o->stream);
+        if (kernel_ms) HIP_CHECK(hipEventRecord(ev.e[1], o->stream));
+        HIP_CHECK(hipStreamSynchronize(o->stream));
+        if (kernel_ms) HIP_CHECK(hipEventElapsedTime(kernel_ms, ev.e[0], ev.e[1]));
+        return 0;
+    });
+}
+
+}  // extern "C"

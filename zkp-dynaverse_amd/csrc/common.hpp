@@ -1,0 +1,115 @@
+// Shared host-side plumbing: HIP error checks, a grow-only device arena and
+// the exception types that map onto the C ABI's status codes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace dvgpu {
+
+// Invalid caller input: maps to status -1 and leaves the owner healthy.
+struct InvalidArgument : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+
+// Device/runtime failure: maps to status -2 and retires the owner.
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t status, const char* what) {
+    if (status != hipSuccess)
+        throw DeviceError(std::string(what) + ": " + hipGetErrorString(status));
+}
+#define HIP_CHECK(expr) ::dvgpu::hip_check((expr), #expr)
+#define HIP_CHECK_LAUNCH() ::dvgpu::hip_check(hipGetLastError(), "kernel launch")
+
+inline void require(bool ok, const char* message) {
+    if (!ok) throw InvalidArgument(message);
+}
+
+// Memory accounting shared by every device allocation an owner makes.  The
+// limit models the reference's per-owner quota; 0 means unlimited.
+struct Quota {
+    uint64_t limit = 0;
+    uint64_t used = 0;
+    uint64_t peak = 0;
+    void charge(uint64_t bytes) {
+        if (limit && (used > limit || bytes > limit - used))
+            throw InvalidArgument("device memory quota exceeded");
+        used += bytes;
+        if (used > peak) peak = used;
+    }
+    void release(uint64_t bytes) { used = bytes > used ? 0 : used - bytes; }
+};
+
+// A named, grow-only device buffer.  Contents are not preserved on growth.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    Quota* quota = nullptr;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() noexcept {
+        if (ptr) {
+            (void)hipFree(ptr);
+            if (quota) quota->release(bytes);
+        }
+        ptr = nullptr;
+        bytes = 0;
+    }
+    void* ensure(size_t need) {
+        if (need <= bytes) return ptr;
+        size_t grow = need + need / 4;
+        grow = (grow + 255) & ~size_t(255);
+        if (quota) quota->charge(grow);
+        void* fresh = nullptr;
+        const hipError_t st = hipMalloc(&fresh, grow);
+        if (st != hipSuccess) {
+            if (quota) quota->release(grow);
+            throw DeviceError(std::string("hipMalloc: ") + hipGetErrorString(st));
+        }
+        release();
+        ptr = fresh;
+        bytes = grow;
+        return ptr;
+    }
+    template <class T>
+    T* as(size_t count) { return static_cast<T*>(ensure(count * sizeof(T))); }
+};
+
+// Pinned host staging buffer (hipHostMalloc), grow-only.
+struct HostBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    HostBuffer() = default;
+    HostBuffer(const HostBuffer&) = delete;
+    HostBuffer& operator=(const HostBuffer&) = delete;
+    ~HostBuffer() { release(); }
+    void release() noexcept {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    void* ensure(size_t need) {
+        if (need <= bytes) return ptr;
+        size_t grow = (need + need / 4 + 4095) & ~size_t(4095);
+        void* fresh = nullptr;
+        HIP_CHECK(hipHostMalloc(&fresh, grow, hipHostMallocDefault));
+        release();
+        ptr = fresh;
+        bytes = grow;
+        return ptr;
+    }
+    template <class T>
+    T* as(size_t count) { return static_cast<T*>(ensure(count * sizeof(T))); }
+};
+
+inline unsigned grid_for(size_t n, unsigned block) { return unsigned((n + block - 1) / block); }
+
+}  // namespace dvgpu

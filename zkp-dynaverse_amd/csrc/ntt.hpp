@@ -1,0 +1,46 @@
+// Natural-order BN254 Fr NTT / INTT on gfx950.
+#pragma once
+#include "common.hpp"
+#include "field.hpp"
+
+namespace dvgpu {
+
+constexpr int NTT_MAX_LOG = 20;  // domain built from a root of order 2^20
+
+// Twiddles omega^k (k < 2^19) for one direction, in Montgomery memory form,
+// built from the 2^20-th root the caller supplies.
+struct NttDomain {
+    uint32_t root_words[8] = {0};  // canonical root this domain was built from
+    bool ready = false;
+    DeviceBuffer fwd, inv;         // 2^19 x 8 words each
+    void release() {
+        fwd.release();
+        inv.release();
+        ready = false;
+    }
+};
+
+// Validate that `root` (canonical LE words) has multiplicative order exactly
+// 2^20 in Fr.  Returns false otherwise.
+bool ntt_root_has_order_2_20(const uint32_t (&root)[8]);
+
+// Build (or reuse) the twiddle tables for `root` on `stream`.
+void ntt_domain_prepare(NttDomain& d, const uint32_t (&root)[8], hipStream_t stream);
+
+struct NttWorkspace {
+    DeviceBuffer a, b;
+    void attach(Quota* q) { a.quota = b.quota = q; }
+    void release() { a.release(); b.release(); }
+};
+
+// rows independent transforms of length n = 2^log_n, each row contiguous.
+// in: canonical LE words (8 per element); out: canonical LE words.
+// in and out may alias.  `scale_inverse` applies 1/n on inverse.
+void ntt_rows(const NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows,
+              bool inverse, NttWorkspace& ws, hipStream_t stream);
+
+// Same, but input and output in Montgomery memory form (resident data).
+void ntt_rows_mont(const NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows,
+                   bool inverse, NttWorkspace& ws, hipStream_t stream);
+
+}  // namespace dvgpu

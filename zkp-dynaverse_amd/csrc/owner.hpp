@@ -1,0 +1,89 @@
+// The opaque owner behind every `void* backend` of the C ABI.
+//
+// The reference binds one isolated CUDA context per owner
+// (native/src/icicle_msm.cu:92-144).  HIP has no user-visible contexts, so an
+// owner here is: device 0, one non-blocking stream, its own grow-only device
+// scratch and domains, and a retired flag.  Any device error retires the
+// owner (all its device memory is released and later calls return -4);
+// invalid arguments never do.
+#pragma once
+#include <chrono>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "../../include/dynaverse_gpu.h"
+#include "common.hpp"
+#include "msm.hpp"
+#include "ntt.hpp"
+
+namespace dvgpu {
+
+struct ResidentState;  // resident.hip
+struct SetupState;     // setup.hip
+
+struct Owner {
+    hipStream_t stream = nullptr;
+    bool retired = false;
+    uint32_t large_bucket_factor = 0;
+    std::string identity;
+    Quota quota;                // limit 0 until the resident engine is configured
+    MsmWorkspace msm;
+    NttWorkspace ntt;
+    NttDomain domain;
+    DeviceBuffer stage_a, stage_b, stage_c;  // per-call device inputs/outputs
+    HostBuffer pinned;                       // per-call host staging
+    DvgpuBatchStats batch_stats{};
+    MsmConfig msm_config;
+    // Fixed-base setup and G2 validation state.
+    SetupState* setup = nullptr;
+    DvgpuG2ValidationStats g2_stats{};
+    DvgpuG2ValidationTraffic g2_traffic{};
+    DvgpuG2KernelConfig g2_kernel{128, 0, 0, 0};
+    DvgpuG2ExperimentStats g2_experiment{};
+    int64_t g2_corrupt_next = -1;
+    // Resident operator engine.
+    ResidentState* resident = nullptr;
+    DvgpuResidentStats retired_resident_stats{};
+    DvgpuResidentDiagnostics retired_resident_diagnostics{};
+
+    Owner();
+    ~Owner();
+    void retire() noexcept;
+    void release_device() noexcept;
+};
+
+// Helpers shared by the ABI translation units.
+int fail(int status, const char* message, char* error, size_t capacity);
+
+template <class Body>
+int guarded(Owner* owner, char* error, size_t capacity, Body&& body) {
+    if (!owner || owner->retired)
+        return fail(-4, "backend retired after a device error; create a new backend", error, capacity);
+    try {
+        return body();
+    } catch (const InvalidArgument& e) {
+        return fail(-1, e.what(), error, capacity);
+    } catch (const std::exception& e) {
+        owner->retire();
+        return fail(-2, e.what(), error, capacity);
+    } catch (...) {
+        owner->retire();
+        return fail(-3, "unknown native exception", error, capacity);
+    }
+}
+
+inline uint64_t now_ns() {
+    return uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(
+                        std::chrono::steady_clock::now().time_since_epoch())
+                        .count());
+}
+
+// Resident engine hooks (resident.hip).
+void resident_destroy(ResidentState* state) noexcept;
+DvgpuResidentStats resident_stats(const ResidentState* state);
+DvgpuResidentDiagnostics resident_diagnostics(const ResidentState* state);
+// Setup hooks (setup.hip).
+void setup_destroy(SetupState* state) noexcept;
+
+}  // namespace dvgpu

@@ -1,0 +1,191 @@
+"""ctypes binding of libdynaverse_hip.so (the C ABI in include/*.h).
+
+The library is loaded from this directory only.  A missing library is an
+error: there is no CPU fallback anywhere in this package.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdynaverse_hip.so")
+
+c_u8p = ctypes.POINTER(ctypes.c_uint8)
+c_u32p = ctypes.POINTER(ctypes.c_uint32)
+c_u64p = ctypes.POINTER(ctypes.c_uint64)
+c_void_p = ctypes.c_void_p
+c_size_t = ctypes.c_size_t
+c_char_p = ctypes.c_char_p
+
+
+class DvgpuBatchStats(ctypes.Structure):
+    _fields_ = [("icicle_calls", ctypes.c_uint64), ("buffer_grows", ctypes.c_uint64),
+                ("max_batch_size", ctypes.c_uint64), ("input_capacity", ctypes.c_uint64)]
+
+
+class DvgpuSetupStats(ctypes.Structure):
+    _fields_ = [("table_builds", ctypes.c_uint64 * 2), ("calls", ctypes.c_uint64 * 2),
+                ("points", ctypes.c_uint64 * 2), ("table_bytes", ctypes.c_uint64),
+                ("device_ns", ctypes.c_uint64 * 5), ("native_wall_ns", ctypes.c_uint64)]
+
+
+class DvgpuG2ValidationReport(ctypes.Structure):
+    _fields_ = [("checked_points", ctypes.c_uint64), ("invalid_points", ctypes.c_uint64),
+                ("first_invalid", ctypes.c_uint64), ("failure_mask", ctypes.c_uint32),
+                ("algorithm", ctypes.c_uint32)]
+
+
+class DvgpuG2ValidationStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "parse_curve_device_ns", "subgroup_device_ns", "reduce_device_ns", "h2d_device_ns",
+        "d2h_device_ns", "native_wall_ns", "checked_points", "accepted_points")]
+
+
+class DvgpuG2ValidationTraffic(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "scalar_h2d_bytes", "point_h2d_bytes", "point_d2h_bytes", "status_d2h_bytes")]
+
+
+class DvgpuMemoryInfo(ctypes.Structure):
+    _fields_ = [("free_bytes", ctypes.c_uint64), ("total_bytes", ctypes.c_uint64)]
+
+
+class DvgpuResidentOp(ctypes.Structure):
+    _fields_ = [("opcode", ctypes.c_uint32), ("kind", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("a", ctypes.c_uint64), ("b", ctypes.c_uint64),
+                ("c", ctypes.c_uint64), ("a_offset", ctypes.c_uint64), ("b_offset", ctypes.c_uint64),
+                ("c_offset", ctypes.c_uint64), ("length", ctypes.c_uint64), ("rows", ctypes.c_uint64),
+                ("cols", ctypes.c_uint64), ("input", c_void_p), ("output", c_void_p)]
+
+
+class DvgpuResidentStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "h2d_bytes", "d2h_bytes", "bytes_allocated", "fr_transform_elements",
+        "g1_transform_elements", "msm_g1_points", "msm_g2_points", "calls")]
+
+
+class DvgpuResidentDiagnostics(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "resident_allocations", "temporary_allocations", "temporary_peak_bytes",
+        "journal_saved_bytes", "journal_saved_ranges", "journal_peak_bytes",
+        "device_operation_ns", "device_operations", "physical_h2d_bytes", "physical_d2h_bytes",
+        "explicit_d2d_bytes", "journal_metadata_capacity_bytes")]
+
+
+class DvgpuResidentProfile(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "native_wall_ns", "journal_host_ns", "conversion_host_ns", "operation_host_ns",
+        "wait_host_ns", "event_total_ns", "event_journal_ns", "event_operation_ns", "event_mask",
+        "h2d_bytes", "d2h_bytes", "d2d_bytes")]
+
+
+class DvgpuMsmTimings(ctypes.Structure):
+    _fields_ = [("conversion_ns", ctypes.c_uint64), ("icicle_ns", ctypes.c_uint64),
+                ("output_ns", ctypes.c_uint64)]
+
+
+class DvgpuMsmTimeline(ctypes.Structure):
+    _fields_ = [("wall_ns", ctypes.c_uint64 * 9), ("event_ns", ctypes.c_uint64 * 9),
+                ("calls", ctypes.c_uint64 * 9), ("event_calls", ctypes.c_uint64 * 9),
+                ("bytes", ctypes.c_uint64 * 9), ("kernel_ns", ctypes.c_uint64 * 10),
+                ("kernel_calls", ctypes.c_uint64 * 10),
+                ("instrumentation_inclusive_ns", ctypes.c_uint64),
+                ("instrumentation_total_ns", ctypes.c_uint64), ("c", ctypes.c_uint64),
+                ("bitsize", ctypes.c_uint64), ("windows", ctypes.c_uint64),
+                ("buckets", ctypes.c_uint64)]
+
+
+class DvgpuG2KernelConfig(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_uint32), ("schedule", ctypes.c_uint32),
+                ("layout", ctypes.c_uint32), ("fused", ctypes.c_uint32)]
+
+
+class DvgpuG2ExperimentStats(ctypes.Structure):
+    _fields_ = [("repack_device_ns", ctypes.c_uint64), ("subgroup_kernel_ns", ctypes.c_uint64),
+                ("logical_repack_bytes", ctypes.c_uint64)]
+
+
+class DvgpuG2KernelResources(ctypes.Structure):
+    _fields_ = [("registers", ctypes.c_uint32), ("block_threads", ctypes.c_uint32),
+                ("active_blocks", ctypes.c_uint32), ("max_threads_per_sm", ctypes.c_uint32),
+                ("local_bytes", ctypes.c_uint64), ("shared_bytes", ctypes.c_uint64),
+                ("normalize_registers", ctypes.c_uint32), ("normalize_active_blocks", ctypes.c_uint32),
+                ("normalize_local_bytes", ctypes.c_uint64), ("normalize_shared_bytes", ctypes.c_uint64)]
+
+
+_I = ctypes.c_int
+_E = (ctypes.c_char_p, c_size_t)  # error, capacity
+
+# name -> (restype, argtypes); every symbol the C headers declare.
+SIGNATURES = {
+    "dvgpu_backend_create": (_I, [ctypes.POINTER(c_void_p), *_E]),
+    "dvgpu_backend_create_with_large_bucket_factor": (_I, [ctypes.c_uint32, ctypes.POINTER(c_void_p), *_E]),
+    "dvgpu_backend_destroy": (None, [c_void_p]),
+    "dvgpu_backend_device_identity": (_I, [c_void_p, ctypes.c_char_p, c_size_t, *_E]),
+    "dvgpu_bn254_msm": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, *_E]),
+    "dvgpu_bn254_g2_msm": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, *_E]),
+    "dvgpu_bn254_ntt": (_I, [c_void_p, c_void_p, c_size_t, _I, c_void_p, c_void_p, *_E]),
+    "dvgpu_bn254_msm_batch": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_void_p, *_E]),
+    "dvgpu_setup_mode": (_I, [c_void_p, _I]),
+    "dvgpu_setup_stats": (_I, [c_void_p, ctypes.POINTER(DvgpuSetupStats)]),
+    "dvgpu_bn254_g2_validation_inspect": (_I, [c_void_p, c_void_p, c_size_t, ctypes.c_uint32, c_void_p,
+                                               ctypes.POINTER(DvgpuG2ValidationReport), *_E]),
+    "dvgpu_g2_validation_stats": (_I, [c_void_p, ctypes.POINTER(DvgpuG2ValidationStats)]),
+    "dvgpu_g2_validation_traffic": (_I, [c_void_p, ctypes.POINTER(DvgpuG2ValidationTraffic)]),
+    "dvgpu_bn254_fixed_base_g2_checked": (_I, [c_void_p, c_void_p, c_size_t, ctypes.c_uint32, c_void_p,
+                                               ctypes.POINTER(DvgpuG2ValidationReport), *_E]),
+    "dvgpu_backend_batch_stats": (_I, [c_void_p, ctypes.POINTER(DvgpuBatchStats)]),
+    "dvgpu_backend_memory_info": (_I, [c_void_p, ctypes.POINTER(DvgpuMemoryInfo), *_E]),
+    "dvgpu_resident_configure": (_I, [c_void_p, ctypes.c_uint64, ctypes.c_uint64, c_void_p, *_E]),
+    "dvgpu_resident_call": (_I, [c_void_p, ctypes.POINTER(DvgpuResidentOp), c_u64p, *_E]),
+    "dvgpu_resident_seal": (None, [c_void_p]),
+    "dvgpu_resident_commit": (None, [c_void_p]),
+    "dvgpu_resident_stats": (_I, [c_void_p, ctypes.POINTER(DvgpuResidentStats)]),
+    "dvgpu_resident_diagnostics": (_I, [c_void_p, ctypes.POINTER(DvgpuResidentDiagnostics)]),
+    "dvgpu_resident_call_profiled": (_I, [c_void_p, ctypes.POINTER(DvgpuResidentOp), c_u64p,
+                                          ctypes.POINTER(DvgpuResidentProfile), *_E]),
+    "dvgpu_bn254_msm_profiled": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p,
+                                      ctypes.POINTER(DvgpuMsmTimings), *_E]),
+    "dvgpu_bn254_msm_timeline": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, ctypes.c_char_p,
+                                      ctypes.POINTER(DvgpuMsmTimings), ctypes.POINTER(DvgpuMsmTimeline), *_E]),
+    "dvgpu_reserve_batch": (_I, [c_void_p, c_size_t, *_E]),
+    "dvgpu_bn254_fixed_base": (_I, [c_void_p, c_void_p, c_size_t, _I, c_void_p, *_E]),
+    "dvgpu_g2_psi_probe": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, *_E]),
+    "dvgpu_g2_test_corrupt_next": (_I, [c_void_p, ctypes.c_uint64]),
+    "dvgpu_g2_kernel_config_set": (_I, [c_void_p, ctypes.POINTER(DvgpuG2KernelConfig)]),
+    "dvgpu_g2_kernel_config_get": (_I, [c_void_p, ctypes.POINTER(DvgpuG2KernelConfig)]),
+    "dvgpu_g2_kernel_resources": (_I, [c_void_p, ctypes.c_uint32, ctypes.POINTER(DvgpuG2KernelResources)]),
+    "dvgpu_g2_experiment_stats": (_I, [c_void_p, ctypes.POINTER(DvgpuG2ExperimentStats)]),
+    "dvgpu_bn254_tower_inspect": (_I, [c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                       c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t,
+                                       c_void_p, c_size_t, *_E]),
+    "dvgpu_bn254_lines_inspect": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p,
+                                       c_size_t, c_void_p, c_size_t, *_E]),
+    "dvgpu_bn254_miller_inspect": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p,
+                                        c_size_t, c_void_p, c_size_t, *_E]),
+    # dynaverse_hip_ext.h
+    "dvgpu_ext_stream": (_I, [c_void_p, ctypes.POINTER(c_void_p)]),
+    "dvgpu_ext_points_to_mont": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, _I, *_E]),
+    "dvgpu_ext_msm_device": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, _I, _I, c_void_p,
+                                  ctypes.POINTER(ctypes.c_float), *_E]),
+    "dvgpu_ext_ntt_device": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, _I, c_void_p,
+                                  ctypes.POINTER(ctypes.c_float), *_E]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libdynaverse_hip.so is not built (run __graft_entry__.build()); "
+                "this package has no CPU fallback")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
