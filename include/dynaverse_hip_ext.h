@@ -16,6 +16,15 @@
 extern "C" {
 #endif
 
+/* Device memory owned by this owner (freed by dvgpu_ext_free or when the
+ * owner is destroyed or retired).  Copies are synchronous. */
+int dvgpu_ext_alloc(void* owner, size_t bytes, void** ptr, char* error, size_t error_capacity);
+int dvgpu_ext_free(void* owner, void* ptr);
+int dvgpu_ext_copy_to_device(void* owner, void* dst, const void* src, size_t bytes,
+                             char* error, size_t error_capacity);
+int dvgpu_ext_copy_to_host(void* owner, void* dst, const void* src, size_t bytes,
+                           char* error, size_t error_capacity);
+
 /* The owner's HIP stream (hipStream_t), for callers that time with events. */
 int dvgpu_ext_stream(void* owner, void** stream);
 
@@ -39,6 +48,11 @@ int dvgpu_ext_msm_device(void* owner, const void* points_mont, const void* scala
 int dvgpu_ext_ntt_device(void* owner, const void* in, void* out, size_t length, size_t rows,
                          int inverse, const uint8_t* root, float* kernel_ms,
                          char* error, size_t error_capacity);
+
+/* [s_i] G for n device-resident canonical scalars into device Montgomery
+ * affine points (64 / 128 bytes each): device-side SRS generation. */
+int dvgpu_ext_fixed_base_device(void* owner, const void* scalars, size_t n, int g2, void* points_mont,
+                                char* error, size_t error_capacity);
 
 #ifdef __cplusplus
 }

@@ -75,17 +75,13 @@ def test_window_sizes_agree(backend):
     enc_p = b"".join(o.g1_to_bytes(p) for p in pts)
     enc_s = b"".join(o.fr_to_bytes(s) for s in sc)
     exp = o.g1_to_bytes(closed_form_msm(logs, sc))
-    import ctypes
-    import numpy as np
-    import torch
-    dev_p = torch.tensor(np.frombuffer(enc_p, dtype=np.uint8).copy(), device="cuda")
-    dev_s = torch.tensor(np.frombuffer(enc_s, dtype=np.uint8).copy(), device="cuda")
-    mont = torch.empty(len(enc_p), dtype=torch.uint8, device="cuda")
-    torch.cuda.synchronize()
-    backend.points_to_mont_device(dev_p.data_ptr(), mont.data_ptr(), 3000)
+    dev_p = backend.to_device(enc_p)
+    dev_s = backend.to_device(enc_s)
+    mont = backend.alloc(len(enc_p))
+    backend.points_to_mont_device(dev_p.ptr, mont.ptr, 3000)
     for c in (4, 5, 7, 8, 10, 11, 12, 13, 14, 15, 16):
         print(f"case: window c={c}", flush=True)
-        assert backend.msm_device(mont.data_ptr(), dev_s.data_ptr(), 3000, window_bits=c) == exp, c
+        assert backend.msm_device(mont.ptr, dev_s.ptr, 3000, window_bits=c) == exp, c
 
 
 def test_batch_offsets_and_rejections(backend):

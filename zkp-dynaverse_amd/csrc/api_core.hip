@@ -41,6 +41,11 @@ void Owner::release_device() noexcept {
     stage_b.release();
     stage_c.release();
     pinned.release();
+    for (auto& kv : ext_allocations) {
+        (void)hipFree(kv.first);
+        quota.release(kv.second);
+    }
+    ext_allocations.clear();
 }
 
 void Owner::retire() noexcept {

@@ -54,6 +54,62 @@ int msm_device(Owner* o, const void* points, const void* scalars, size_t n, int 
 
 extern "C" {
 
+int dvgpu_ext_alloc(void* owner, size_t bytes, void** ptr, char* error, size_t capacity) {
+    Owner* o = as_owner(owner);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(ptr && bytes, "invalid allocation request");
+        o->quota.charge(bytes);
+        void* p = nullptr;
+        const hipError_t st = hipMalloc(&p, bytes);
+        if (st != hipSuccess) {
+            o->quota.release(bytes);
+            throw DeviceError(std::string("hipMalloc: ") + hipGetErrorString(st));
+        }
+        o->ext_allocations[p] = bytes;
+        *ptr = p;
+        return 0;
+    });
+}
+
+int dvgpu_ext_free(void* owner, void* ptr) {
+    Owner* o = as_owner(owner);
+    if (!o) return -1;
+    if (o->retired) return -4;
+    auto it = o->ext_allocations.find(ptr);
+    if (it == o->ext_allocations.end()) return -1;
+    (void)hipStreamSynchronize(o->stream);
+    (void)hipFree(it->first);
+    o->quota.release(it->second);
+    o->ext_allocations.erase(it);
+    return 0;
+}
+
+int dvgpu_ext_copy_to_device(void* owner, void* dst, const void* src, size_t bytes, char* error,
+                             size_t capacity) {
+    Owner* o = as_owner(owner);
+    return guarded(o, error, capacity, [&]() -> int {
+        require((dst && src) || !bytes, "null copy buffers");
+        if (bytes) {
+            HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, o->stream));
+            HIP_CHECK(hipStreamSynchronize(o->stream));
+        }
+        return 0;
+    });
+}
+
+int dvgpu_ext_copy_to_host(void* owner, void* dst, const void* src, size_t bytes, char* error,
+                           size_t capacity) {
+    Owner* o = as_owner(owner);
+    return guarded(o, error, capacity, [&]() -> int {
+        require((dst && src) || !bytes, "null copy buffers");
+        if (bytes) {
+            HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, o->stream));
+            HIP_CHECK(hipStreamSynchronize(o->stream));
+        }
+        return 0;
+    });
+}
+
 int dvgpu_ext_stream(void* owner, void** stream) {
     Owner* o = as_owner(owner);
     if (!o || o->retired) return -4;

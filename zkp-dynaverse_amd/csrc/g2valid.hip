@@ -1,0 +1,537 @@
+// G2 subgroup validation kernels and the setup / G2 entry points of the ABI.
+//
+// Wire rules (native/include/icicle_g2_wire.cuh:9-35): 128 bytes
+// x.c0 || x.c1 || y.c0 || y.c1 little endian; all zero = infinity; byte 127
+// carries two flag bits: 0xC0 is an encoding error, 0x40 marks infinity (the
+// coordinates must still be canonical); the flag bits are masked off before
+// the canonical (< q) check.  Masks: 1 encoding, 2 curve, 4 subgroup.
+// Subgroup predicates (native/src/icicle_g2_validation.cu:20-71,
+// crates/gpu/src/g2_validation.rs:250-299):
+//   1: [r] P = O               (r as a 254-bit integer)
+//   2: psi(P) = [6 x^2] P      (127-bit multiplier)
+//   3: [x+1]P + psi([x]P) + psi^2([x]P) = 2 psi^3([x]P)   (63-bit seed x)
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "owner.hpp"
+#include "setup_impl.hpp"
+
+namespace dvgpu {
+
+namespace {
+
+using G2 = Xyzz<Fq2Ops>;
+
+__device__ uint32_t parse_wire(const uint8_t* wire, size_t index, Affine<Fq2Ops>& p) {
+    const uint4* q = reinterpret_cast<const uint4*>(wire + index * 128);
+    uint32_t w[32];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint4 v = q[k];
+        w[4 * k] = v.x;
+        w[4 * k + 1] = v.y;
+        w[4 * k + 2] = v.z;
+        w[4 * k + 3] = v.w;
+    }
+    uint32_t any = 0;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) any |= w[k];
+    p.inf = true;
+    if (!any) return 0;
+    const uint32_t flags = (w[31] >> 24) & 0xC0u;
+    if (flags == 0xC0u) return 1;
+    w[31] &= 0x3FFFFFFFu;
+    uint32_t f[4][8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) f[j][k] = w[8 * j + k];
+        if (!is_canonical<FqParams>(f[j])) return 1;
+    }
+    if (flags == 0x40u) return 0;
+    p.x = {to_mont<FqParams>(f[0]), to_mont<FqParams>(f[1])};
+    p.y = {to_mont<FqParams>(f[2]), to_mont<FqParams>(f[3])};
+    p.inf = false;
+    return 0;
+}
+
+__device__ bool in_subgroup(const Affine<Fq2Ops>& a, uint32_t algorithm) {
+    const G2 P = xyzz_from_affine<Fq2Ops>(a);
+    if (algorithm == 1) {
+        uint32_t r[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r[k] = FR_MOD32[k];
+        return xyzz_is_inf<Fq2Ops>(xyzz_mul_words<Fq2Ops, 8>(P, r, 254));
+    }
+    if (algorithm == 2) {
+        uint32_t lam[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lam[k] = G2_LAMBDA[k];
+        return xyzz_eq<Fq2Ops>(psi(P), xyzz_mul_words<Fq2Ops, 4>(P, lam, 127));
+    }
+    const G2 A = xyzz_mul_u64<Fq2Ops>(P, BN_SEED);
+    const G2 Bp = psi(A);
+    const G2 C = psi(Bp);
+    const G2 lhs = xyzz_add<Fq2Ops>(xyzz_add<Fq2Ops>(xyzz_add<Fq2Ops>(A, P), Bp), C);
+    return xyzz_eq<Fq2Ops>(lhs, xyzz_dbl<Fq2Ops>(psi(C)));
+}
+
+__global__ void k_parse_curve(const uint8_t* __restrict__ wire, size_t n, uint32_t* __restrict__ masks) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Affine<Fq2Ops> p;
+    const uint32_t status = parse_wire(wire, i, p);
+    masks[i] = status ? status : (affine_on_curve<Fq2Ops>(p) ? 0u : 2u);
+}
+
+__global__ void k_subgroup(const uint8_t* __restrict__ wire, size_t n, uint32_t algorithm,
+                           uint32_t* __restrict__ masks) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || masks[i]) return;
+    Affine<Fq2Ops> p;
+    parse_wire(wire, i, p);
+    if (p.inf) return;
+    masks[i] = in_subgroup(p, algorithm) ? 0u : 4u;
+}
+
+__global__ void k_reduce_status(const uint32_t* __restrict__ masks, size_t n, uint32_t algorithm,
+                                DvgpuG2ValidationReport* __restrict__ report) {
+    __shared__ uint64_t counts[256], first[256];
+    __shared__ uint32_t bits[256];
+    const unsigned t = threadIdx.x;
+    uint64_t count = 0, idx = UINT64_MAX;
+    uint32_t mask = 0;
+    for (size_t i = t; i < n; i += 256) {
+        const uint32_t m = masks[i];
+        if (m) {
+            ++count;
+            mask |= m;
+            if (i < idx) idx = i;
+        }
+    }
+    counts[t] = count;
+    first[t] = idx;
+    bits[t] = mask;
+    __syncthreads();
+    for (unsigned s = 128; s; s >>= 1) {
+        if (t < s) {
+            counts[t] += counts[t + s];
+            bits[t] |= bits[t + s];
+            first[t] = min(first[t], first[t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        DvgpuG2ValidationReport r;
+        r.checked_points = n;
+        r.invalid_points = counts[0];
+        r.first_invalid = first[0];
+        r.failure_mask = bits[0];
+        r.algorithm = algorithm;
+        *report = r;
+    }
+}
+
+__global__ void k_corrupt(uint8_t* wire, size_t index) { wire[index * 128] ^= 1u; }
+
+// psi images of homogeneous (X : Y : Z) points, canonical 192-byte records.
+__global__ void k_psi_probe(const uint8_t* __restrict__ in, size_t n, uint8_t* __restrict__ out,
+                            uint32_t* __restrict__ masks) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(in + i * 192);
+    uint32_t f[6][8];
+    bool valid = true;
+    for (int j = 0; j < 6; ++j) {
+        for (int k = 0; k < 8; ++k) f[j][k] = w[8 * j + k];
+        valid &= is_canonical<FqParams>(f[j]);
+    }
+    uint32_t* o = reinterpret_cast<uint32_t*>(out + i * 576);
+    Fq2 X{}, Y{}, Z{};
+    if (valid) {
+        X = {to_mont<FqParams>(f[0]), to_mont<FqParams>(f[1])};
+        Y = {to_mont<FqParams>(f[2]), to_mont<FqParams>(f[3])};
+        Z = {to_mont<FqParams>(f[4]), to_mont<FqParams>(f[5])};
+        // Y^2 Z = X^3 + b Z^3
+        const Fq2 lhs = f2_mul(f2_sqr(Y), Z);
+        const Fq2 rhs = f2_add(f2_mul(f2_sqr(X), X), f2_mul(Fq2Ops::b(), f2_mul(f2_sqr(Z), Z)));
+        valid = f2_eq(lhs, rhs);
+    }
+    masks[i] = valid ? 0u : 1u;
+    for (int j = 0; j < 3; ++j) {
+        uint32_t* r = o + j * 48;
+        if (!valid) {
+            for (int k = 0; k < 48; ++k) r[k] = 0;
+            continue;
+        }
+        X = f2_mul(f2_conj(X), psi_cx());
+        Y = f2_mul(f2_conj(Y), psi_cy());
+        Z = f2_conj(Z);
+        put_canon(r, X);
+        put_canon(r + 16, Y);
+        put_canon(r + 32, Z);
+    }
+}
+
+}  // namespace
+
+void g2_validate_enqueue(const uint8_t* wire, size_t n, uint32_t algorithm, uint32_t* masks,
+                         DvgpuG2ValidationReport* report, unsigned threads, hipStream_t stream,
+                         hipEvent_t parsed, hipEvent_t subgroup_done) {
+    k_parse_curve<<<grid_for(n, threads), threads, 0, stream>>>(wire, n, masks);
+    HIP_CHECK_LAUNCH();
+    if (parsed) HIP_CHECK(hipEventRecord(parsed, stream));
+    k_subgroup<<<grid_for(n, threads), threads, 0, stream>>>(wire, n, algorithm, masks);
+    HIP_CHECK_LAUNCH();
+    if (subgroup_done) HIP_CHECK(hipEventRecord(subgroup_done, stream));
+    k_reduce_status<<<1, 256, 0, stream>>>(masks, n, algorithm, report);
+    HIP_CHECK_LAUNCH();
+}
+
+void g2_corrupt_point(uint8_t* wire, size_t index, hipStream_t stream) {
+    k_corrupt<<<1, 1, 0, stream>>>(wire, index);
+    HIP_CHECK_LAUNCH();
+}
+
+void g2_psi_probe_enqueue(const uint8_t* points, size_t n, uint8_t* out, uint32_t* masks, hipStream_t stream) {
+    k_psi_probe<<<grid_for(n, 64), 64, 0, stream>>>(points, n, out, masks);
+    HIP_CHECK_LAUNCH();
+}
+
+void g2_kernel_attributes(uint32_t, unsigned threads, DvgpuG2KernelResources& out) {
+    hipFuncAttributes a{};
+    HIP_CHECK(hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_subgroup)));
+    int blocks = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(k_subgroup),
+                                                           int(threads), 0));
+    int device = 0, maximum = 0;
+    HIP_CHECK(hipGetDevice(&device));
+    HIP_CHECK(hipDeviceGetAttribute(&maximum, hipDeviceAttributeMaxThreadsPerMultiProcessor, device));
+    out.registers = uint32_t(a.numRegs);
+    out.block_threads = threads;
+    out.active_blocks = uint32_t(blocks);
+    out.max_threads_per_sm = uint32_t(maximum);
+    out.local_bytes = a.localSizeBytes;
+    out.shared_bytes = a.sharedSizeBytes;
+    hipFuncAttributes b{};
+    HIP_CHECK(hipFuncGetAttributes(&b, reinterpret_cast<const void*>(k_fixed_base<Fq2Ops, true>)));
+    int nblocks = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nblocks, reinterpret_cast<const void*>(k_fixed_base<Fq2Ops, true>), 256, 0));
+    out.normalize_registers = uint32_t(b.numRegs);
+    out.normalize_active_blocks = uint32_t(nblocks);
+    out.normalize_local_bytes = b.localSizeBytes;
+    out.normalize_shared_bytes = b.sharedSizeBytes;
+}
+
+void setup_destroy(SetupState* s) noexcept { delete s; }
+
+}  // namespace dvgpu
+
+using namespace dvgpu;
+
+namespace {
+
+Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
+
+SetupState& setup_of(Owner* o) {
+    if (!o->setup) {
+        o->setup = new SetupState();
+        o->setup->attach(&o->quota);
+    }
+    return *o->setup;
+}
+
+struct EventSet {
+    hipEvent_t e[8] = {};
+    EventSet() {
+        for (auto& x : e) HIP_CHECK(hipEventCreate(&x));
+    }
+    ~EventSet() {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    uint64_t ns(int a, int b) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e[a], e[b]));
+        return uint64_t(double(ms) * 1e6);
+    }
+};
+
+// [s_i] G into host `output` (canonical affine); events 0..5 are the
+// table/H2D/generate/normalize/D2H boundaries of DvgpuSetupStats.device_ns.
+template <class F>
+void run_fixed_base(Owner* o, const uint8_t* scalars, size_t n, uint8_t* output, EventSet& ev,
+                    int64_t corrupt, uint32_t algorithm, DvgpuG2ValidationReport* report_out,
+                    uint32_t* dev_masks_out) {
+    constexpr int PB = PointLayout<F>::CANON_BYTES;
+    SetupState& st = setup_of(o);
+    HIP_CHECK(hipEventRecord(ev.e[0], o->stream));
+    if (st.mode) {
+        // Building the table is part of the first call's "table" span.
+        (void)setup_table<F>(st, o->stream);
+    }
+    HIP_CHECK(hipEventRecord(ev.e[1], o->stream));
+    uint32_t* dsc = st.scalars.as<uint32_t>(n * 8);
+    uint8_t* dpts = st.points.as<uint8_t>(n * PB);
+    HIP_CHECK(hipMemcpyAsync(dsc, scalars, n * 32, hipMemcpyHostToDevice, o->stream));
+    HIP_CHECK(hipEventRecord(ev.e[2], o->stream));
+    fixed_base_enqueue<F>(st, dsc, n, dpts, o->stream);
+    HIP_CHECK(hipEventRecord(ev.e[3], o->stream));
+    // Generation and normalization run fused in one kernel; the normalize span
+    // is recorded as empty.
+    HIP_CHECK(hipEventRecord(ev.e[4], o->stream));
+    if (report_out) {
+        if (corrupt >= 0) g2_corrupt_point(dpts, size_t(corrupt), o->stream);
+        uint32_t* masks = st.masks.as<uint32_t>(n);
+        auto* rep = st.report.as<DvgpuG2ValidationReport>(1);
+        g2_validate_enqueue(dpts, n, algorithm, masks, rep, o->g2_kernel.threads, o->stream, ev.e[6], ev.e[7]);
+        HIP_CHECK(hipMemcpyAsync(report_out, rep, sizeof(*rep), hipMemcpyDeviceToHost, o->stream));
+        (void)dev_masks_out;
+    }
+    HIP_CHECK(hipMemcpyAsync(output, dpts, n * PB, hipMemcpyDeviceToHost, o->stream));
+    HIP_CHECK(hipEventRecord(ev.e[5], o->stream));
+    HIP_CHECK(hipStreamSynchronize(o->stream));
+    const int g2 = PB == 128 ? 1 : 0;
+    for (int k = 0; k < 5; ++k) st.stats.device_ns[k] += ev.ns(k, k + 1);
+    ++st.stats.calls[g2];
+    st.stats.points[g2] += n;
+}
+
+}  // namespace
+
+extern "C" {
+
+// icicle_msm.cu:523-532
+int dvgpu_setup_mode(void* backend, int mode) {
+    Owner* o = as_owner(backend);
+    if (!o || o->retired || mode < 0 || mode > 2) return -1;
+    setup_of(o).mode = mode;
+    return 0;
+}
+
+int dvgpu_setup_stats(void* backend, DvgpuSetupStats* stats) {
+    Owner* o = as_owner(backend);
+    if (!o || !stats) return -1;
+    *stats = o->setup ? o->setup->stats : DvgpuSetupStats{};
+    return 0;
+}
+
+// icicle_msm.cu:494-521
+int dvgpu_bn254_fixed_base(void* backend, const uint8_t* scalars, size_t length, int g2, uint8_t* output,
+                           char* error, size_t capacity) {
+    Owner* o = as_owner(backend);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(length <= 4096 && (g2 == 0 || g2 == 1) && (!length || (scalars && output)),
+                "invalid fixed-base buffers or chunk length");
+        if (!length) return 0;
+        const uint64_t t0 = now_ns();
+        EventSet ev;
+        uint8_t* staged = o->pinned.as<uint8_t>(length * (g2 ? 128 : 64));
+        if (g2)
+            run_fixed_base<Fq2Ops>(o, scalars, length, staged, ev, -1, 0, nullptr, nullptr);
+        else
+            run_fixed_base<FqOps>(o, scalars, length, staged, ev, -1, 0, nullptr, nullptr);
+        std::memcpy(output, staged, length * (g2 ? 128 : 64));
+        o->setup->stats.native_wall_ns += now_ns() - t0;
+        if (g2) {
+            o->g2_traffic.scalar_h2d_bytes += length * 32;
+            o->g2_traffic.point_d2h_bytes += length * 128;
+        }
+        return 0;
+    });
+}
+
+// icicle_msm.cu:583-618: generate, validate on the device, publish only if
+// every point is accepted; -5 publishes the report alone.
+int dvgpu_bn254_fixed_base_g2_checked(void* backend, const uint8_t* scalars, size_t n, uint32_t algorithm,
+                                      uint8_t* output, DvgpuG2ValidationReport* report, char* error,
+                                      size_t capacity) {
+    Owner* o = as_owner(backend);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(n <= 4096 && algorithm >= 1 && algorithm <= 3 && report && (!n || (scalars && output)),
+                "invalid checked G2 buffers or algorithm");
+        if (!n) {
+            *report = {0, 0, UINT64_MAX, 0, algorithm};
+            return 0;
+        }
+        const int64_t corrupt = o->g2_corrupt_next;
+        o->g2_corrupt_next = -1;
+        require(corrupt < int64_t(n), "test corruption index outside chunk");
+        const uint64_t t0 = now_ns();
+        EventSet ev;
+        uint8_t* staged = o->pinned.as<uint8_t>(n * 128 + sizeof(DvgpuG2ValidationReport));
+        auto* rep = reinterpret_cast<DvgpuG2ValidationReport*>(staged + n * 128);
+        run_fixed_base<Fq2Ops>(o, scalars, n, staged, ev, corrupt, algorithm, rep, nullptr);
+        const uint64_t wall = now_ns() - t0;
+        o->setup->stats.native_wall_ns += wall;
+        auto& s = o->g2_stats;
+        s.parse_curve_device_ns += ev.ns(4, 6);
+        s.subgroup_device_ns += ev.ns(6, 7);
+        s.d2h_device_ns += ev.ns(7, 5);
+        s.native_wall_ns += wall;
+        s.checked_points += n;
+        o->g2_traffic.scalar_h2d_bytes += n * 32;
+        o->g2_traffic.point_d2h_bytes += n * 128;
+        o->g2_traffic.status_d2h_bytes += sizeof(DvgpuG2ValidationReport);
+        *report = *rep;
+        if (rep->invalid_points) return fail(-5, "G2 content validation rejected", error, capacity);
+        s.accepted_points += n;
+        std::memcpy(output, staged, n * 128);
+        return 0;
+    });
+}
+
+// icicle_msm.cu:619-654
+int dvgpu_bn254_g2_validation_inspect(void* backend, const uint8_t* wire, size_t n, uint32_t algorithm,
+                                      uint32_t* masks, DvgpuG2ValidationReport* report, char* error,
+                                      size_t capacity) {
+    Owner* o = as_owner(backend);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(n <= 4096 && algorithm >= 1 && algorithm <= 3 && report && (!n || (wire && masks)),
+                "invalid G2 validation buffers, algorithm or chunk length");
+        if (!n) {
+            *report = {0, 0, UINT64_MAX, 0, algorithm};
+            return 0;
+        }
+        const uint64_t t0 = now_ns();
+        SetupState& st = setup_of(o);
+        EventSet ev;
+        uint8_t* dwire = st.points.as<uint8_t>(n * 128);
+        uint32_t* dmasks = st.masks.as<uint32_t>(n);
+        auto* drep = st.report.as<DvgpuG2ValidationReport>(1);
+        uint8_t* staged = o->pinned.as<uint8_t>(n * 4 + sizeof(DvgpuG2ValidationReport));
+        HIP_CHECK(hipEventRecord(ev.e[0], o->stream));
+        HIP_CHECK(hipMemcpyAsync(dwire, wire, n * 128, hipMemcpyHostToDevice, o->stream));
+        HIP_CHECK(hipEventRecord(ev.e[1], o->stream));
+        g2_validate_enqueue(dwire, n, algorithm, dmasks, drep, o->g2_kernel.threads, o->stream, ev.e[2], ev.e[3]);
+        HIP_CHECK(hipEventRecord(ev.e[4], o->stream));
+        HIP_CHECK(hipMemcpyAsync(staged, dmasks, n * 4, hipMemcpyDeviceToHost, o->stream));
+        HIP_CHECK(hipMemcpyAsync(staged + n * 4, drep, sizeof(DvgpuG2ValidationReport), hipMemcpyDeviceToHost,
+                                 o->stream));
+        HIP_CHECK(hipEventRecord(ev.e[5], o->stream));
+        HIP_CHECK(hipStreamSynchronize(o->stream));
+        auto& s = o->g2_stats;
+        s.h2d_device_ns += ev.ns(0, 1);
+        s.parse_curve_device_ns += ev.ns(1, 2);
+        s.subgroup_device_ns += ev.ns(2, 3);
+        s.reduce_device_ns += ev.ns(3, 4);
+        s.d2h_device_ns += ev.ns(4, 5);
+        s.native_wall_ns += now_ns() - t0;
+        s.checked_points += n;
+        o->g2_experiment.subgroup_kernel_ns += ev.ns(2, 3);
+        o->g2_traffic.point_h2d_bytes += n * 128;
+        o->g2_traffic.status_d2h_bytes += n * 4 + sizeof(DvgpuG2ValidationReport);
+        std::memcpy(masks, staged, n * 4);
+        std::memcpy(report, staged + n * 4, sizeof(DvgpuG2ValidationReport));
+        return 0;
+    });
+}
+
+int dvgpu_g2_validation_stats(void* backend, DvgpuG2ValidationStats* stats) {
+    Owner* o = as_owner(backend);
+    if (!o || !stats) return -1;
+    *stats = o->g2_stats;
+    return 0;
+}
+
+int dvgpu_g2_validation_traffic(void* backend, DvgpuG2ValidationTraffic* stats) {
+    Owner* o = as_owner(backend);
+    if (!o || !stats) return -1;
+    *stats = o->g2_traffic;
+    return 0;
+}
+
+// icicle_msm.cu:543-556
+int dvgpu_g2_kernel_config_set(void* backend, const DvgpuG2KernelConfig* config) {
+    Owner* o = as_owner(backend);
+    if (!o || o->retired) return -4;
+    if (!config || (config->threads != 64 && config->threads != 128 && config->threads != 256) ||
+        config->schedule > 1 || config->layout > 1 || config->fused > 1)
+        return -1;
+    o->g2_kernel = *config;
+    return 0;
+}
+
+int dvgpu_g2_kernel_config_get(void* backend, DvgpuG2KernelConfig* config) {
+    Owner* o = as_owner(backend);
+    if (!o || !config) return -1;
+    *config = o->g2_kernel;
+    return 0;
+}
+
+int dvgpu_g2_experiment_stats(void* backend, DvgpuG2ExperimentStats* stats) {
+    Owner* o = as_owner(backend);
+    if (!o || !stats) return -1;
+    *stats = o->g2_experiment;
+    return 0;
+}
+
+// icicle_msm.cu:557-576
+int dvgpu_g2_kernel_resources(void* backend, uint32_t algorithm, DvgpuG2KernelResources* output) {
+    Owner* o = as_owner(backend);
+    if (!o || o->retired) return -4;
+    if (!output || algorithm < 1 || algorithm > 3) return -1;
+    try {
+        DvgpuG2KernelResources r{};
+        g2_kernel_attributes(algorithm, o->g2_kernel.threads, r);
+        *output = r;
+        return 0;
+    } catch (...) {
+        o->retire();
+        return -2;
+    }
+}
+
+// icicle_msm.cu:577-582
+int dvgpu_g2_test_corrupt_next(void* backend, uint64_t index) {
+    Owner* o = as_owner(backend);
+    if (!o || o->retired || index >= 4096) return -1;
+    o->g2_corrupt_next = int64_t(index);
+    return 0;
+}
+
+// dynaverse_hip_ext.h: device-resident SRS generation.
+int dvgpu_ext_fixed_base_device(void* backend, const void* scalars, size_t n, int g2, void* points,
+                                char* error, size_t capacity) {
+    Owner* o = as_owner(backend);
+    return guarded(o, error, capacity, [&]() -> int {
+        require((g2 == 0 || g2 == 1) && (!n || (scalars && points)), "invalid fixed-base arguments");
+        SetupState& st = setup_of(o);
+        if (g2)
+            fixed_base_mont_enqueue<Fq2Ops>(st, static_cast<const uint32_t*>(scalars), n,
+                                            static_cast<uint32_t*>(points), o->stream);
+        else
+            fixed_base_mont_enqueue<FqOps>(st, static_cast<const uint32_t*>(scalars), n,
+                                           static_cast<uint32_t*>(points), o->stream);
+        HIP_CHECK(hipStreamSynchronize(o->stream));
+        return 0;
+    });
+}
+
+// icicle_msm.cu:750-765
+int dvgpu_g2_psi_probe(void* backend, const uint8_t* points, size_t n, uint8_t* output, char* error,
+                       size_t capacity) {
+    Owner* o = as_owner(backend);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(n <= 129 && (!n || (points && output)), "invalid psi probe buffers");
+        if (!n) return 0;
+        SetupState& st = setup_of(o);
+        uint8_t* din = st.points.as<uint8_t>(n * 192 + n * 576);
+        uint32_t* dmasks = st.masks.as<uint32_t>(n);
+        uint8_t* staged = o->pinned.as<uint8_t>(n * 576 + n * 4);
+        HIP_CHECK(hipMemcpyAsync(din, points, n * 192, hipMemcpyHostToDevice, o->stream));
+        g2_psi_probe_enqueue(din, n, din + n * 192, dmasks, o->stream);
+        HIP_CHECK(hipMemcpyAsync(staged, din + n * 192, n * 576, hipMemcpyDeviceToHost, o->stream));
+        HIP_CHECK(hipMemcpyAsync(staged + n * 576, dmasks, n * 4, hipMemcpyDeviceToHost, o->stream));
+        HIP_CHECK(hipStreamSynchronize(o->stream));
+        const uint32_t* m = reinterpret_cast<const uint32_t*>(staged + n * 576);
+        for (size_t i = 0; i < n; ++i)
+            if (m[i]) throw InvalidArgument("invalid homogeneous psi input");
+        std::memcpy(output, staged, n * 576);
+        return 0;
+    });
+}
+
+}  // extern "C"

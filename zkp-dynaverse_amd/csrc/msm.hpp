@@ -36,19 +36,19 @@ struct MsmConfig {
 // Scratch owned by one backend; grows to the largest MSM seen.
 struct MsmWorkspace {
     DeviceBuffer digits, ranks, counts, offsets, sorted, tasks, partials, chunks, tree_a, tree_b,
-        block_sums, total, points, scalars, windows;
+        block_sums, total, points, scalars, windows, heavy;
     HostBuffer host_windows;
     // Optional phase events (MSM_PHASE_EVENTS of them) recorded by msm_enqueue:
     // start, digits, scan, scatter, tasks, accumulate, reduce, tree.
     hipEvent_t* phase_events = nullptr;
     void attach(Quota* q) {
         for (DeviceBuffer* b : {&digits, &ranks, &counts, &offsets, &sorted, &tasks, &partials,
-                                &chunks, &tree_a, &tree_b, &block_sums, &total, &points, &scalars, &windows})
+                                &chunks, &tree_a, &tree_b, &block_sums, &total, &points, &scalars, &windows, &heavy})
             b->quota = q;
     }
     void release() {
         for (DeviceBuffer* b : {&digits, &ranks, &counts, &offsets, &sorted, &tasks, &partials,
-                                &chunks, &tree_a, &tree_b, &block_sums, &total, &points, &scalars, &windows})
+                                &chunks, &tree_a, &tree_b, &block_sums, &total, &points, &scalars, &windows, &heavy})
             b->release();
         host_windows.release();
     }

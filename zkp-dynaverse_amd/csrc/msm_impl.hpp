@@ -240,14 +240,21 @@ static __global__ void k_scatter(const uint32_t* __restrict__ digits, const uint
 }
 
 // One thread per bucket: emit (start, length) of each of its tasks.
+// Buckets cut into more than HEAVY_TASKS tasks have their partial sums
+// combined by a whole block (k_heavy) instead of serially in k_reduce_chunks.
+constexpr uint32_t HEAVY_TASKS = 8;
+
 static __global__ void k_tasks(const uint32_t* __restrict__ counts, const uint2* __restrict__ offsets,
-                        uint32_t m, uint32_t task_points, uint2* __restrict__ tasks) {
+                               uint32_t m, uint32_t task_points, uint2* __restrict__ tasks,
+                               uint32_t* __restrict__ heavy, uint32_t* __restrict__ heavy_count) {
     const uint32_t wb = blockIdx.x * blockDim.x + threadIdx.x;
     if (wb >= m) return;
     const uint32_t cnt = counts[wb];
     const uint2 off = offsets[wb];
-    for (uint32_t k = 0, s = 0; s < cnt; ++k, s += task_points)
+    uint32_t k = 0;
+    for (uint32_t s = 0; s < cnt; ++k, s += task_points)
         tasks[off.y + k] = make_uint2(off.x + s, min(task_points, cnt - s));
+    if (k > HEAVY_TASKS) heavy[atomicAdd(heavy_count, 1u)] = wb;
 }
 
 template <class F>
@@ -271,6 +278,39 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t* __restrict__
     Mem<F>::store_xyzz(partials + size_t(t) * PointLayout<F>::XYZZ_WORDS, acc);
 }
 
+// One block per heavy bucket (grid-strided): strided serial sums, then an
+// LDS tree; the bucket total overwrites the bucket's first task partial.
+template <class F>
+__global__ void __launch_bounds__(256) k_heavy(const uint32_t* __restrict__ heavy,
+                                               const uint32_t* __restrict__ heavy_count,
+                                               const uint32_t* __restrict__ counts,
+                                               const uint2* __restrict__ offsets, uint32_t task_points,
+                                               uint32_t* __restrict__ partials) {
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
+    __shared__ uint4 lds[256 * XW / 4];
+    uint32_t* slot = reinterpret_cast<uint32_t*>(lds) + threadIdx.x * XW;
+    const uint32_t nheavy = *heavy_count;
+    for (uint32_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
+        const uint32_t wb = heavy[h];
+        const uint32_t t0 = offsets[wb].y;
+        const uint32_t nt = (counts[wb] + task_points - 1) / task_points;
+        Xyzz<F> acc = xyzz_inf<F>();
+        for (uint32_t k = threadIdx.x; k < nt; k += 256)
+            acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(partials + size_t(t0 + k) * XW));
+        Mem<F>::store_xyzz(slot, acc);
+        __syncthreads();
+        for (uint32_t s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(slot + s * XW));
+                Mem<F>::store_xyzz(slot, acc);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) Mem<F>::store_xyzz(partials + size_t(t0) * XW, acc);
+        __syncthreads();
+    }
+}
+
 // One thread per chunk of L buckets of one window:
 //   out = sum_i (i + 1) S_{jL+i} + (jL) * sum_i S_{jL+i}
 // where S_b is the sum of bucket b's task partials.
@@ -291,7 +331,8 @@ __global__ void __launch_bounds__(256) k_reduce_chunks(const uint32_t* __restric
         const uint32_t cnt = counts[wb];
         if (cnt) {
             const uint32_t t0 = offsets[wb].y;
-            const uint32_t nt = (cnt + task_points - 1) / task_points;
+            uint32_t nt = (cnt + task_points - 1) / task_points;
+            if (nt > HEAVY_TASKS) nt = 1;  // combined by k_heavy
             Xyzz<F> s = Mem<F>::load_xyzz(partials + size_t(t0) * PointLayout<F>::XYZZ_WORDS);
             for (uint32_t k = 1; k < nt; ++k)
                 s = xyzz_add<F>(s, Mem<F>::load_xyzz(partials + size_t(t0 + k) * PointLayout<F>::XYZZ_WORDS));
@@ -299,8 +340,11 @@ __global__ void __launch_bounds__(256) k_reduce_chunks(const uint32_t* __restric
         }
         acc = xyzz_add<F>(acc, run);
     }
-    const uint64_t offset = uint64_t(j) * uint64_t(chunk);
-    if (offset) acc = xyzz_add<F>(acc, xyzz_mul_u64<F>(run, offset));
+    const uint32_t offset = j * uint32_t(chunk);
+    if (offset) {
+        const uint32_t k[1] = {offset};
+        acc = xyzz_add<F>(acc, xyzz_mul_words<F, 1>(run, k, 32 - __clz(offset)));
+    }
     Mem<F>::store_xyzz(out + size_t(g) * PointLayout<F>::XYZZ_WORDS, acc);
 }
 
@@ -383,7 +427,7 @@ MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, M
     uint32_t* ranks = ws.ranks.as<uint32_t>(entries);
     uint32_t* sorted = ws.sorted.as<uint32_t>(entries);
     uint2* tile_sums = ws.block_sums.as<uint2>(tiles);
-    uint32_t* total = ws.total.as<uint32_t>(4);
+    uint32_t* total = ws.total.as<uint32_t>(4);  // [0] entries, [1] tasks, [2] heavy buckets
     uint2* tasks = ws.tasks.as<uint2>(plan.max_tasks);
     uint32_t* partials = ws.partials.as<uint32_t>(plan.max_tasks * XW);
     const uint32_t chunks_per_window = B / uint32_t(cfg.chunk);
@@ -394,7 +438,10 @@ MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, M
         if (ev) HIP_CHECK(hipEventRecord(ev[k], stream));
     };
     mark(0);
+    const size_t max_heavy = std::min<size_t>(m, entries / (size_t(cfg.task_points) * HEAVY_TASKS) + 1);
+    uint32_t* heavy = ws.heavy.as<uint32_t>(max_heavy);
     HIP_CHECK(hipMemsetAsync(counts, 0, m * sizeof(uint32_t), stream));
+    HIP_CHECK(hipMemsetAsync(total, 0, 4 * sizeof(uint32_t), stream));
     k_digits<<<grid_for(n, 256), 256, 0, stream>>>(scalars, uint32_t(n), plan.c, W, B, counts, digits, ranks);
     HIP_CHECK_LAUNCH();
     mark(1);
@@ -409,10 +456,13 @@ MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, M
     k_scatter<<<grid_for(entries, 256), 256, 0, stream>>>(digits, ranks, offsets, uint32_t(n), W, B, sorted);
     HIP_CHECK_LAUNCH();
     mark(3);
-    k_tasks<<<grid_for(m, 256), 256, 0, stream>>>(counts, offsets, uint32_t(m), tp, tasks);
+    k_tasks<<<grid_for(m, 256), 256, 0, stream>>>(counts, offsets, uint32_t(m), tp, tasks, heavy, total + 2);
     HIP_CHECK_LAUNCH();
     mark(4);
     k_accumulate<F><<<grid_for(plan.max_tasks, 256), 256, 0, stream>>>(points, sorted, tasks, total, partials);
+    HIP_CHECK_LAUNCH();
+    k_heavy<F><<<unsigned(std::min<size_t>(max_heavy, 1024)), 256, 0, stream>>>(heavy, total + 2, counts, offsets,
+                                                                               tp, partials);
     HIP_CHECK_LAUNCH();
     mark(5);
     const size_t nchunks = size_t(chunks_per_window) * W;

@@ -11,6 +11,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 
 #include "../../include/dynaverse_gpu.h"
 #include "common.hpp"
@@ -46,6 +47,8 @@ struct Owner {
     ResidentState* resident = nullptr;
     DvgpuResidentStats retired_resident_stats{};
     DvgpuResidentDiagnostics retired_resident_diagnostics{};
+    // Caller-visible device allocations (dynaverse_hip_ext.h).
+    std::unordered_map<void*, size_t> ext_allocations;
 
     Owner();
     ~Owner();

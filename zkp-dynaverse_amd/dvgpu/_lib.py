@@ -163,7 +163,12 @@ SIGNATURES = {
     "dvgpu_bn254_miller_inspect": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p,
                                         c_size_t, c_void_p, c_size_t, *_E]),
     # dynaverse_hip_ext.h
+    "dvgpu_ext_alloc": (_I, [c_void_p, c_size_t, ctypes.POINTER(c_void_p), *_E]),
+    "dvgpu_ext_free": (_I, [c_void_p, c_void_p]),
+    "dvgpu_ext_copy_to_device": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, *_E]),
+    "dvgpu_ext_copy_to_host": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, *_E]),
     "dvgpu_ext_stream": (_I, [c_void_p, ctypes.POINTER(c_void_p)]),
+    "dvgpu_ext_fixed_base_device": (_I, [c_void_p, c_void_p, c_size_t, _I, c_void_p, *_E]),
     "dvgpu_ext_points_to_mont": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, _I, *_E]),
     "dvgpu_ext_msm_device": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, _I, _I, c_void_p,
                                   ctypes.POINTER(ctypes.c_float), *_E]),

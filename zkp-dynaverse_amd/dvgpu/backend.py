@@ -64,6 +64,40 @@ def _ptr(buf) -> ctypes.c_void_p:
     return ctypes.cast(ctypes.c_char_p(buf), ctypes.c_void_p)
 
 
+class DeviceArray:
+    """Device memory owned by a HipBackend owner (freed with it)."""
+
+    def __init__(self, backend: "HipBackend", nbytes: int):
+        self.backend = backend
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        err = _err()
+        _check(backend.lib.dvgpu_ext_alloc(backend.handle, self.nbytes, ctypes.byref(p), err, 256), err)
+        self.ptr = p.value
+
+    def upload(self, arr) -> None:
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("upload larger than allocation")
+        err = _err()
+        _check(self.backend.lib.dvgpu_ext_copy_to_device(self.backend.handle, self.ptr,
+                                                         ctypes.c_void_p(arr.ctypes.data), arr.nbytes,
+                                                         err, 256), err)
+
+    def download(self, nbytes: Optional[int] = None) -> np.ndarray:
+        n = self.nbytes if nbytes is None else int(nbytes)
+        out = np.empty(n, dtype=np.uint8)
+        err = _err()
+        _check(self.backend.lib.dvgpu_ext_copy_to_host(self.backend.handle, ctypes.c_void_p(out.ctypes.data),
+                                                       self.ptr, n, err, 256), err)
+        return out
+
+    def free(self) -> None:
+        if self.ptr and self.backend.handle is not None and self.backend.handle.value:
+            self.backend.lib.dvgpu_ext_free(self.backend.handle, self.ptr)
+        self.ptr = None
+
+
 class HipBackend:
     """One native owner on HIP device 0 (not shareable across threads)."""
 
@@ -294,6 +328,16 @@ class HipBackend:
         return {"g1_msm": True, "fr_ntt": True, "g2_msm": True}
 
     # ---- device-resident extension (dynaverse_hip_ext.h) -------------------
+    def alloc(self, nbytes: int) -> "DeviceArray":
+        return DeviceArray(self, nbytes)
+
+    def to_device(self, data) -> "DeviceArray":
+        arr = np.ascontiguousarray(np.frombuffer(bytes(data), dtype=np.uint8)
+                                   if not isinstance(data, np.ndarray) else data)
+        d = DeviceArray(self, arr.nbytes)
+        d.upload(arr)
+        return d
+
     def stream_handle(self) -> int:
         s = ctypes.c_void_p()
         _check(self.lib.dvgpu_ext_stream(self.handle, ctypes.byref(s)), _err())
@@ -321,3 +365,144 @@ class HipBackend:
         _check(self.lib.dvgpu_ext_ntt_device(self.handle, in_ptr, out_ptr, length, rows, int(inverse),
                                              _ptr(root), ctypes.byref(ms) if timed else None, err, 256), err)
         return ms.value if timed else None
+
+
+# ---- fixed-base setup and G2 validation (setup.rs, g2_validation.rs) ------
+
+SETUP_CHUNK = 4096
+G2_ALGORITHMS = {"full_order": 1, "frobenius127": 2, "frobenius63": 3}
+
+
+def _setup_methods():
+    def set_setup_mode(self, mode: int) -> None:
+        """setup.rs:82-86: 0 ordinary, 1 table, 2 table + batch normalize."""
+        if self.lib.dvgpu_setup_mode(self.handle, int(mode)) != 0:
+            raise BackendError(-1, "invalid or retired setup owner")
+
+    def setup_stats(self):
+        st = _lib.DvgpuSetupStats()
+        if self.lib.dvgpu_setup_stats(self.handle, ctypes.byref(st)) != 0:
+            raise BackendError(-1, "invalid setup stats owner")
+        return {"table_builds": list(st.table_builds), "calls": list(st.calls),
+                "points": list(st.points), "table_bytes": st.table_bytes,
+                "device_ns": list(st.device_ns), "native_wall_ns": st.native_wall_ns}
+
+    def fixed_base_encoded(self, scalars: bytes, n: int, g2: bool) -> bytes:
+        out = ctypes.create_string_buffer(n * (128 if g2 else 64))
+        err = _err()
+        _check(self.lib.dvgpu_bn254_fixed_base(self.handle, _ptr(scalars), n, int(g2), out, err, 256), err)
+        return out.raw
+
+    def fixed_base_g1(self, scalars: Sequence[int]) -> list:
+        """setup.rs:113-126: 4096-scalar chunks, decoded with the on-curve check."""
+        out = []
+        for c in range(0, len(scalars), SETUP_CHUNK):
+            chunk = scalars[c:c + SETUP_CHUNK]
+            raw = self.fixed_base_encoded(b"".join(encode_fr(s) for s in chunk), len(chunk), False)
+            out.extend(decode_g1(raw[64 * i:64 * i + 64]) for i in range(len(chunk)))
+        return out
+
+    def fixed_base_g2(self, scalars: Sequence[int], algorithm: int = 3) -> list:
+        """setup.rs:128-143 with device validation (g2_validation.rs:149-169):
+        every chunk is validated on the GPU before it is published."""
+        out = []
+        for c in range(0, len(scalars), SETUP_CHUNK):
+            chunk = scalars[c:c + SETUP_CHUNK]
+            raw, report = self.fixed_base_g2_checked_encoded(
+                b"".join(encode_fr(s) for s in chunk), len(chunk), algorithm)
+            if report["invalid_points"]:
+                raise BackendError(-5, "G2 validation rejected index %d (mask %d)"
+                                   % (c + report["first_invalid"], report["failure_mask"]))
+            out.extend(decode_g2(raw[128 * i:128 * i + 128]) for i in range(len(chunk)))
+        return out
+
+    def fixed_base_g2_checked_encoded(self, scalars: bytes, n: int, algorithm: int):
+        out = ctypes.create_string_buffer(max(1, n * 128))
+        rep = _lib.DvgpuG2ValidationReport()
+        err = _err()
+        status = self.lib.dvgpu_bn254_fixed_base_g2_checked(self.handle, _ptr(scalars), n, algorithm, out,
+                                                            ctypes.byref(rep), err, 256)
+        report = {k: getattr(rep, k) for k, _ in rep._fields_}
+        if status == -5:
+            return None, report
+        _check(status, err)
+        return out.raw[:n * 128], report
+
+    def inspect_g2_validation(self, wire: bytes, algorithm: int):
+        """g2_validation.rs:191-206: masks (1 encoding, 2 curve, 4 subgroup)."""
+        if len(wire) % 128 or len(wire) // 128 > SETUP_CHUNK or algorithm not in (1, 2, 3):
+            raise ValueError("invalid G2 validation input")
+        n = len(wire) // 128
+        masks = (ctypes.c_uint32 * max(1, n))()
+        rep = _lib.DvgpuG2ValidationReport()
+        err = _err()
+        _check(self.lib.dvgpu_bn254_g2_validation_inspect(self.handle, _ptr(wire or b"\0"), n, algorithm,
+                                                          masks, ctypes.byref(rep), err, 256), err)
+        return list(masks)[:n], {k: getattr(rep, k) for k, _ in rep._fields_}
+
+    def inspect_g2_validation_batched(self, wire: bytes, algorithm: int):
+        """g2_validation.rs:114-125: 4096-point blocks, global first_invalid."""
+        masks = []
+        comb = {"checked_points": 0, "invalid_points": 0, "first_invalid": (1 << 64) - 1,
+                "failure_mask": 0, "algorithm": algorithm}
+        for block, c in enumerate(range(0, len(wire), SETUP_CHUNK * 128)):
+            m, r = self.inspect_g2_validation(wire[c:c + SETUP_CHUNK * 128], algorithm)
+            comb["checked_points"] += r["checked_points"]
+            comb["invalid_points"] += r["invalid_points"]
+            comb["failure_mask"] |= r["failure_mask"]
+            if r["invalid_points"]:
+                comb["first_invalid"] = min(comb["first_invalid"], block * SETUP_CHUNK + r["first_invalid"])
+            masks.extend(m)
+        return masks, comb
+
+    def _struct_getter(fn_name, cls):
+        def get(self):
+            st = cls()
+            if getattr(self.lib, fn_name)(self.handle, ctypes.byref(st)) != 0:
+                raise BackendError(-1, fn_name + " failed")
+            return {k: getattr(st, k) for k, _ in st._fields_}
+        return get
+
+    def set_g2_kernel_config(self, threads=128, schedule=0, layout=0, fused=0):
+        cfg = _lib.DvgpuG2KernelConfig(threads, schedule, layout, fused)
+        status = self.lib.dvgpu_g2_kernel_config_set(self.handle, ctypes.byref(cfg))
+        if status != 0:
+            raise BackendError(status, "invalid or retired G2 kernel config")
+
+    def g2_kernel_resources(self, algorithm: int):
+        r = _lib.DvgpuG2KernelResources()
+        status = self.lib.dvgpu_g2_kernel_resources(self.handle, algorithm, ctypes.byref(r))
+        if status != 0:
+            raise BackendError(status, "G2 kernel resource query failed")
+        return {k: getattr(r, k) for k, _ in r._fields_}
+
+    def g2_test_corrupt_next(self, index: int) -> int:
+        return self.lib.dvgpu_g2_test_corrupt_next(self.handle, index)
+
+    def g2_psi_probe(self, points: bytes, n: int) -> bytes:
+        out = ctypes.create_string_buffer(max(1, n * 576))
+        err = _err()
+        _check(self.lib.dvgpu_g2_psi_probe(self.handle, _ptr(points or b"\0"), n, out, err, 256), err)
+        return out.raw[:n * 576]
+
+    def fixed_base_device(self, scalars_ptr: int, n: int, out_ptr: int, g2: bool = False):
+        """[s_i] G from device scalars straight into device Montgomery points."""
+        err = _err()
+        _check(self.lib.dvgpu_ext_fixed_base_device(self.handle, scalars_ptr, n, int(g2), out_ptr, err, 256), err)
+
+    ns = dict(set_setup_mode=set_setup_mode, setup_stats=setup_stats,
+              fixed_base_encoded=fixed_base_encoded, fixed_base_g1=fixed_base_g1,
+              fixed_base_g2=fixed_base_g2, fixed_base_g2_checked_encoded=fixed_base_g2_checked_encoded,
+              inspect_g2_validation=inspect_g2_validation,
+              inspect_g2_validation_batched=inspect_g2_validation_batched,
+              g2_validation_stats=_struct_getter("dvgpu_g2_validation_stats", _lib.DvgpuG2ValidationStats),
+              g2_validation_traffic=_struct_getter("dvgpu_g2_validation_traffic", _lib.DvgpuG2ValidationTraffic),
+              g2_experiment_stats=_struct_getter("dvgpu_g2_experiment_stats", _lib.DvgpuG2ExperimentStats),
+              set_g2_kernel_config=set_g2_kernel_config, g2_kernel_resources=g2_kernel_resources,
+              g2_test_corrupt_next=g2_test_corrupt_next, g2_psi_probe=g2_psi_probe,
+              fixed_base_device=fixed_base_device)
+    for k, v in ns.items():
+        setattr(HipBackend, k, v)
+
+
+_setup_methods()
