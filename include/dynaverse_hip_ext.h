@@ -25,6 +25,11 @@ int dvgpu_ext_copy_to_device(void* owner, void* dst, const void* src, size_t byt
 int dvgpu_ext_copy_to_host(void* owner, void* dst, const void* src, size_t bytes,
                            char* error, size_t error_capacity);
 
+/* Device-to-device copy by a plain load/store kernel (16-byte vectors):
+ * the reference copy for HBM traffic-counter calibration.  bytes % 16 == 0. */
+int dvgpu_ext_device_copy(void* owner, void* dst, const void* src, size_t bytes,
+                          char* error, size_t error_capacity);
+
 /* The owner's HIP stream (hipStream_t), for callers that time with events. */
 int dvgpu_ext_stream(void* owner, void** stream);
 

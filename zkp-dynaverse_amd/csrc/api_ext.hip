@@ -1,4 +1,5 @@
 // Device-resident extension entry points (dynaverse_hip_ext.h).
+#include <algorithm>
 #include <cstring>
 
 #include "../../include/dynaverse_hip_ext.h"
@@ -8,6 +9,11 @@ using namespace dvgpu;
 
 namespace {
 Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
+
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
 
 struct Events {
     hipEvent_t e[MSM_PHASE_EVENTS] = {};
@@ -104,6 +110,22 @@ int dvgpu_ext_copy_to_host(void* owner, void* dst, const void* src, size_t bytes
         require((dst && src) || !bytes, "null copy buffers");
         if (bytes) {
             HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, o->stream));
+            HIP_CHECK(hipStreamSynchronize(o->stream));
+        }
+        return 0;
+    });
+}
+
+int dvgpu_ext_device_copy(void* owner, void* dst, const void* src, size_t bytes, char* error,
+                          size_t capacity) {
+    Owner* o = as_owner(owner);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(dst && src && bytes % 16 == 0, "copy needs device buffers and a multiple of 16 bytes");
+        const size_t n = bytes / 16;
+        if (n) {
+            k_copy16<<<unsigned(std::min<size_t>(grid_for(n, 256), 8192)), 256, 0, o->stream>>>(
+                static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+            HIP_CHECK_LAUNCH();
             HIP_CHECK(hipStreamSynchronize(o->stream));
         }
         return 0;

@@ -168,6 +168,7 @@ SIGNATURES = {
     "dvgpu_ext_copy_to_device": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, *_E]),
     "dvgpu_ext_copy_to_host": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, *_E]),
     "dvgpu_ext_stream": (_I, [c_void_p, ctypes.POINTER(c_void_p)]),
+    "dvgpu_ext_device_copy": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, *_E]),
     "dvgpu_ext_fixed_base_device": (_I, [c_void_p, c_void_p, c_size_t, _I, c_void_p, *_E]),
     "dvgpu_ext_points_to_mont": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, _I, *_E]),
     "dvgpu_ext_msm_device": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, _I, _I, c_void_p,

@@ -338,6 +338,10 @@ class HipBackend:
         d.upload(arr)
         return d
 
+    def device_copy(self, dst_ptr: int, src_ptr: int, nbytes: int) -> None:
+        err = _err()
+        _check(self.lib.dvgpu_ext_device_copy(self.handle, dst_ptr, src_ptr, nbytes, err, 256), err)
+
     def stream_handle(self) -> int:
         s = ctypes.c_void_p()
         _check(self.lib.dvgpu_ext_stream(self.handle, ctypes.byref(s)), _err())
