@@ -559,3 +559,25 @@ def f2_sqrt(a):
             if f2_mul(r, r) == (a0 % Q, a1 % Q):
                 return r
     return None
+
+
+# --------------------------------------------------------------------------
+# Group NTT over G1 (resident ECNTT rows; operators.rs:413-440)
+# --------------------------------------------------------------------------
+
+
+def ecntt(points, inverse=False):
+    """out_k = sum_j [w^(jk)] P_j over <w_n>, inverse scaled by n^-1: the DFT
+    definition evaluated in the group, O(n^2) scalar multiplications."""
+    n = len(points)
+    w = fr_root_of_unity(n)
+    if inverse:
+        w = pow(w, R - 2, R)
+    ninv = pow(n, R - 2, R) if inverse else 1
+    out = []
+    for k in range(n):
+        acc = None
+        for j, p in enumerate(points):
+            acc = add(acc, mul(p, pow(w, j * k, R) * ninv % R))
+        out.append(acc)
+    return out
