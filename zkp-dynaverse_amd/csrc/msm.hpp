@@ -1,5 +1,9 @@
-// Variable-base multi-scalar multiplication on BN254 G1 / G2 (Pippenger).
+// Variable-base multi-scalar multiplication on BN254 G1 / G2 (Pippenger),
+// batched: one fixed sequence of kernels computes any number of MSMs of
+// uneven sizes, each with its own window width.
 #pragma once
+#include <vector>
+
 #include "common.hpp"
 #include "curve.hpp"
 
@@ -27,62 +31,104 @@ struct PointLayout<Fq2Ops> {
 };
 
 struct MsmConfig {
-    int window_bits = 0;  // c; 0 selects by size
+    int window_bits = 0;  // c for every MSM of a call; 0 selects by size
     int task_points = 32; // maximum points one accumulation thread adds
     int chunk = 8;        // buckets per first-level reduction thread
-    int tree_fanin = 4;   // values per thread in later reduction levels
 };
 
-// Scratch owned by one backend; grows to the largest MSM seen.
-struct MsmWorkspace {
-    DeviceBuffer digits, ranks, counts, offsets, sorted, tasks, partials, chunks, tree_a, tree_b,
-        block_sums, total, points, scalars, windows, heavy;
-    HostBuffer host_windows;
-    // Optional phase events (MSM_PHASE_EVENTS of them) recorded by msm_enqueue:
-    // start, digits, scan, scatter, tasks, accumulate, reduce, tree.
-    hipEvent_t* phase_events = nullptr;
-    void attach(Quota* q) {
-        for (DeviceBuffer* b : {&digits, &ranks, &counts, &offsets, &sorted, &tasks, &partials,
-                                &chunks, &tree_a, &tree_b, &block_sums, &total, &points, &scalars, &windows, &heavy})
-            b->quota = q;
-    }
-    void release() {
-        for (DeviceBuffer* b : {&digits, &ranks, &counts, &offsets, &sorted, &tasks, &partials,
-                                &chunks, &tree_a, &tree_b, &block_sums, &total, &points, &scalars, &windows, &heavy})
-            b->release();
-        host_windows.release();
-    }
+// Window width by log2 size (index 0..24), filled from measurement; an
+// entry of 0 falls back to the built-in rule.
+struct WindowTable {
+    int c[25] = {0};
 };
+int msm_window_for(size_t n, const WindowTable* table);
+
+// One MSM of a batch, as the caller describes it.
+struct MsmItem {
+    size_t n;
+    uint32_t point_base;   // first point (or first entry of the index array)
+    uint32_t scalar_base;  // first scalar
+    int c;                 // 0 = by size
+};
+
+// Per-MSM descriptor as the kernels read it (all bases are exclusive
+// prefix sums over the batch).
+struct MsmDesc {
+    uint32_t n, c, windows, buckets;
+    uint32_t point_base, scalar_base;
+    uint32_t thread_base, entry_base, bucket_base, chunk_base, window_base, pad;
+};
+
+struct MsmBatchPlan {
+    std::vector<MsmDesc> descs;
+    uint32_t total_threads = 0, total_entries = 0, total_buckets = 0, total_chunks = 0, total_windows = 0;
+    size_t max_tasks = 0, max_heavy = 0;
+    int chunk = 8;
+    uint32_t task_points = 32;
+};
+
+MsmBatchPlan msm_batch_plan(const std::vector<MsmItem>& items, const MsmConfig& cfg,
+                            const WindowTable* table = nullptr);
 
 constexpr int MSM_PHASE_EVENTS = 8;
 
-int msm_default_window(size_t n);
+// Scratch owned by one backend; grows to the largest batch seen.
+struct MsmWorkspace {
+    DeviceBuffer digits, ranks, counts, offsets, sorted, tasks, partials, chunks, block_sums, total,
+        points, scalars, windows, heavy, descs;
+    HostBuffer host_windows, host_descs;
+    // Optional phase events (MSM_PHASE_EVENTS of them) recorded by the
+    // enqueue: start, digits, scan, scatter, tasks, accumulate, reduce, tree.
+    hipEvent_t* phase_events = nullptr;
+    DeviceBuffer* all[15] = {&digits, &ranks, &counts, &offsets, &sorted, &tasks, &partials, &chunks,
+                             &block_sums, &total, &points, &scalars, &windows, &heavy, &descs};
+    void attach(Quota* q) {
+        for (DeviceBuffer* b : all) b->quota = q;
+    }
+    void release() {
+        for (DeviceBuffer* b : all) b->release();
+        host_windows.release();
+        host_descs.release();
+    }
+};
 
-// Plan of one MSM: window layout and buffer sizes.
+// Size every buffer of `plan` and copy its descriptors to the device
+// (enqueued on `stream`).  Must precede msm_batch_enqueue; a captured graph
+// calls it once before capture.
+void msm_batch_prepare(const MsmBatchPlan& plan, int xyzz_words, MsmWorkspace& ws, hipStream_t stream);
+
+// points: device Montgomery affine words; point_index: optional indices
+// into `points` (sparse MSMs over resident bases); scalars: canonical words.
+// Leaves plan.total_windows XYZZ values in window_out.  Launches only
+// kernels (no memset, no allocation), so it can be captured in a hipGraph.
+template <class F>
+void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, const uint32_t* scalars,
+                       const MsmBatchPlan& plan, MsmWorkspace& ws, hipStream_t stream, uint32_t* window_out);
+
+// Host epilogue: for every MSM, sum_w 2^(w c) V_w then canonical affine.
+template <class F>
+void msm_batch_combine(const uint32_t* window_words, const MsmBatchPlan& plan, Affine<F>* out);
+
+// ---- single MSM conveniences ------------------------------------------------
+
 struct MsmPlan {
     int c, windows;
-    uint32_t buckets;      // 2^(c-1) per window
+    uint32_t buckets;
     size_t n;
-    size_t max_tasks;
 };
+int msm_default_window(size_t n);
 MsmPlan msm_plan(size_t n, const MsmConfig& cfg);
 
-// points: device Montgomery affine words; scalars: device canonical words
-// (8 per scalar).  Enqueues the device phases on `stream` and leaves one
-// XYZZ value per window in ws.host_windows after an implicit synchronize,
-// then combines the windows on the host.  Returns the canonical affine sum.
-template <class F>
-Affine<F> msm_run(const uint32_t* points, const uint32_t* scalars, size_t n, MsmWorkspace& ws,
-                  hipStream_t stream, const MsmConfig& cfg);
-
-// Device-only part: leaves W window values (XYZZ words) in `window_out`.
 template <class F>
 MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, MsmWorkspace& ws,
                     hipStream_t stream, const MsmConfig& cfg, uint32_t* window_out);
 
-// Host epilogue: sum_w 2^(w c) V_w, then canonical affine.
 template <class F>
 Affine<F> msm_combine_windows(const uint32_t* window_words, const MsmPlan& plan);
+
+template <class F>
+Affine<F> msm_run(const uint32_t* points, const uint32_t* scalars, size_t n, MsmWorkspace& ws,
+                  hipStream_t stream, const MsmConfig& cfg);
 
 // Canonical little-endian points (64 B G1 / 128 B G2, all-zero = infinity)
 // on the device -> Montgomery words (all-zero stays all-zero).

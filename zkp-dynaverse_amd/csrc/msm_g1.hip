@@ -4,12 +4,27 @@
 namespace dvgpu {
 
 int msm_default_window(size_t n) { return msm_default_window_impl(n); }
+int msm_window_for(size_t n, const WindowTable* table) { return msm_window_for_impl(n, table); }
 MsmPlan msm_plan(size_t n, const MsmConfig& cfg) { return msm_plan_impl(n, cfg); }
+MsmBatchPlan msm_batch_plan(const std::vector<MsmItem>& items, const MsmConfig& cfg, const WindowTable* table) {
+    return msm_batch_plan_impl(items, cfg, table);
+}
+void msm_batch_prepare(const MsmBatchPlan& plan, int xyzz_words, MsmWorkspace& ws, hipStream_t stream) {
+    msm_batch_prepare_impl(plan, xyzz_words, ws, stream);
+}
 
-template MsmPlan msm_enqueue<FqOps>(const uint32_t*, const uint32_t*, size_t, MsmWorkspace&, hipStream_t, const MsmConfig&, uint32_t*);
-template Affine<FqOps> msm_run<FqOps>(const uint32_t*, const uint32_t*, size_t, MsmWorkspace&, hipStream_t, const MsmConfig&);
-template Affine<FqOps> msm_combine_windows<FqOps>(const uint32_t*, const MsmPlan&);
-template void points_to_mont<FqOps>(const uint8_t*, uint32_t*, size_t, hipStream_t);
-template void affine_to_bytes<FqOps>(const Affine<FqOps>&, uint8_t*);
+#define DVGPU_MSM_INSTANTIATE(F)                                                                              \
+    template void msm_batch_enqueue<F>(const uint32_t*, const uint32_t*, const uint32_t*, const MsmBatchPlan&, \
+                                       MsmWorkspace&, hipStream_t, uint32_t*);                                \
+    template void msm_batch_combine<F>(const uint32_t*, const MsmBatchPlan&, Affine<F>*);                     \
+    template MsmPlan msm_enqueue<F>(const uint32_t*, const uint32_t*, size_t, MsmWorkspace&, hipStream_t,    \
+                                    const MsmConfig&, uint32_t*);                                             \
+    template Affine<F> msm_run<F>(const uint32_t*, const uint32_t*, size_t, MsmWorkspace&, hipStream_t,      \
+                                  const MsmConfig&);                                                          \
+    template Affine<F> msm_combine_windows<F>(const uint32_t*, const MsmPlan&);                               \
+    template void points_to_mont<F>(const uint8_t*, uint32_t*, size_t, hipStream_t);                          \
+    template void affine_to_bytes<F>(const Affine<F>&, uint8_t*);
+
+DVGPU_MSM_INSTANTIATE(FqOps)
 
 }  // namespace dvgpu

@@ -3,10 +3,18 @@
 
 namespace dvgpu {
 
-template MsmPlan msm_enqueue<Fq2Ops>(const uint32_t*, const uint32_t*, size_t, MsmWorkspace&, hipStream_t, const MsmConfig&, uint32_t*);
-template Affine<Fq2Ops> msm_run<Fq2Ops>(const uint32_t*, const uint32_t*, size_t, MsmWorkspace&, hipStream_t, const MsmConfig&);
-template Affine<Fq2Ops> msm_combine_windows<Fq2Ops>(const uint32_t*, const MsmPlan&);
-template void points_to_mont<Fq2Ops>(const uint8_t*, uint32_t*, size_t, hipStream_t);
-template void affine_to_bytes<Fq2Ops>(const Affine<Fq2Ops>&, uint8_t*);
+#define DVGPU_MSM_INSTANTIATE(F)                                                                              \
+    template void msm_batch_enqueue<F>(const uint32_t*, const uint32_t*, const uint32_t*, const MsmBatchPlan&, \
+                                       MsmWorkspace&, hipStream_t, uint32_t*);                                \
+    template void msm_batch_combine<F>(const uint32_t*, const MsmBatchPlan&, Affine<F>*);                     \
+    template MsmPlan msm_enqueue<F>(const uint32_t*, const uint32_t*, size_t, MsmWorkspace&, hipStream_t,    \
+                                    const MsmConfig&, uint32_t*);                                             \
+    template Affine<F> msm_run<F>(const uint32_t*, const uint32_t*, size_t, MsmWorkspace&, hipStream_t,      \
+                                  const MsmConfig&);                                                          \
+    template Affine<F> msm_combine_windows<F>(const uint32_t*, const MsmPlan&);                               \
+    template void points_to_mont<F>(const uint8_t*, uint32_t*, size_t, hipStream_t);                          \
+    template void affine_to_bytes<F>(const Affine<F>&, uint8_t*);
+
+DVGPU_MSM_INSTANTIATE(Fq2Ops)
 
 }  // namespace dvgpu

@@ -3,6 +3,9 @@
 // Phases (one stream, no host synchronisation until the window values):
 //   1. digits   : scalar -> ceil(255/c) signed c-bit digits; each nonzero digit
 //                 takes a rank in its bucket with one global atomic.
+//   (Every kernel serves a whole batch of MSMs: a thread finds its MSM by a
+//   binary search over the descriptor table, whose prefix bases place each
+//   MSM's entries, buckets, chunks and windows in global arrays.)
 //   2. scan     : exclusive scan of (bucket count, task count) pairs.
 //   3. scatter  : point index (plus sign bit) to its bucket-sorted slot.
 //   4. tasks    : every bucket is cut into tasks of at most T points, so one
@@ -10,7 +13,7 @@
 //   5. accumulate: one thread per task, XYZZ mixed additions -> partial sums.
 //   6. reduce   : one thread per chunk of L buckets computes
 //                 sum_i (i+1) S_i by running sums and adds (jL) * sum_i S_i.
-//   7. tree     : per-window tree sum of the chunk values.
+//   7. tree     : one block per window sums its chunk values (LDS tree).
 // The W window values go to the host, which forms sum_w 2^(wc) V_w: ~250
 // serial doublings that one CPU core finishes far sooner than one GPU lane.
 #pragma once
@@ -18,6 +21,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 
 namespace dvgpu {
 
@@ -109,35 +113,72 @@ __device__ __forceinline__ uint32_t scalar_bits(const uint32_t (&s)[8], int bit,
     return uint32_t(v >> sh) & ((1u << c) - 1u);
 }
 
-static __global__ void k_digits(const uint32_t* __restrict__ scalars, uint32_t n, int c, int windows,
-                         uint32_t buckets, uint32_t* __restrict__ counts,
-                         uint32_t* __restrict__ digits, uint32_t* __restrict__ ranks) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(i) * 8);
+
+// Index of the descriptor owning `key` under the prefix field `field`: the
+// last descriptor whose base is <= key.
+template <uint32_t MsmDesc::*field>
+__device__ __forceinline__ uint32_t find_desc(const MsmDesc* __restrict__ d, uint32_t m, uint32_t key) {
+    uint32_t lo = 0, hi = m - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (d[mid].*field <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+static __global__ void k_fill(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+static __global__ void k_digits(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_threads,
+                                const uint32_t* __restrict__ scalars, uint32_t* __restrict__ counts,
+                                uint32_t* __restrict__ digits, uint32_t* __restrict__ ranks) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total_threads) return;
+    const MsmDesc d = descs[find_desc<&MsmDesc::thread_base>(descs, m, t)];
+    const uint32_t i = t - d.thread_base;
+    const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(d.scalar_base + i) * 8);
     const uint4 a = sp[0], b = sp[1];
     const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    uint32_t carry = 0;
+    const int c = int(d.c);
     const uint32_t full = 1u << c;
-    for (int w = 0; w < windows; ++w) {
-        uint32_t d = scalar_bits(s, w * c, c) + carry;
+    uint32_t carry = 0;
+    for (uint32_t w = 0; w < d.windows; ++w) {
+        uint32_t v = scalar_bits(s, int(w) * c, c) + carry;
         uint32_t sign = 0;
-        if (d > buckets) {
-            d = full - d;
+        if (v > d.buckets) {
+            v = full - v;
             sign = 1;
             carry = 1;
         } else {
             carry = 0;
         }
-        const size_t slot = size_t(w) * n + i;
-        if (d == 0) {
+        const size_t slot = size_t(d.entry_base) + size_t(w) * d.n + i;
+        if (v == 0) {
             digits[slot] = NO_DIGIT;
         } else {
-            const uint32_t bucket = d - 1;
-            ranks[slot] = atomicAdd(&counts[size_t(w) * buckets + bucket], 1u);
+            const uint32_t bucket = v - 1;
+            ranks[slot] = atomicAdd(&counts[d.bucket_base + w * d.buckets + bucket], 1u);
             digits[slot] = (bucket << 1) | sign;
         }
     }
+}
+
+static __global__ void k_scatter(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_entries,
+                                 const uint32_t* __restrict__ digits, const uint32_t* __restrict__ ranks,
+                                 const uint2* __restrict__ offsets, const uint32_t* __restrict__ point_index,
+                                 uint32_t* __restrict__ sorted) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total_entries) return;
+    const uint32_t dig = digits[e];
+    if (dig == NO_DIGIT) return;
+    const MsmDesc d = descs[find_desc<&MsmDesc::entry_base>(descs, m, e)];
+    const uint32_t local = e - d.entry_base, w = local / d.n, i = local % d.n;
+    const uint32_t pos = offsets[d.bucket_base + w * d.buckets + (dig >> 1)].x + ranks[e];
+    const uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
+    sorted[pos] = pidx | ((dig & 1u) << 31);
 }
 
 // Exclusive scan of (count, tasks(count)) pairs in three phases.
@@ -227,18 +268,6 @@ static __global__ void k_scan_apply(const uint32_t* __restrict__ counts, uint32_
     }
 }
 
-static __global__ void k_scatter(const uint32_t* __restrict__ digits, const uint32_t* __restrict__ ranks,
-                          const uint2* __restrict__ offsets, uint32_t n, int windows, uint32_t buckets,
-                          uint32_t* __restrict__ sorted) {
-    const size_t slot = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (slot >= size_t(windows) * n) return;
-    const uint32_t d = digits[slot];
-    if (d == NO_DIGIT) return;
-    const uint32_t w = uint32_t(slot / n), i = uint32_t(slot % n);
-    const uint32_t pos = offsets[size_t(w) * buckets + (d >> 1)].x + ranks[slot];
-    sorted[pos] = i | ((d & 1u) << 31);
-}
-
 // One thread per bucket: emit (start, length) of each of its tasks.
 // Buckets cut into more than HEAVY_TASKS tasks have their partial sums
 // combined by a whole block (k_heavy) instead of serially in k_reduce_chunks.
@@ -311,23 +340,24 @@ __global__ void __launch_bounds__(256) k_heavy(const uint32_t* __restrict__ heav
     }
 }
 
+
 // One thread per chunk of L buckets of one window:
 //   out = sum_i (i + 1) S_{jL+i} + (jL) * sum_i S_{jL+i}
 // where S_b is the sum of bucket b's task partials.
 template <class F>
-__global__ void __launch_bounds__(256) k_reduce_chunks(const uint32_t* __restrict__ counts,
+__global__ void __launch_bounds__(256) k_reduce_chunks(const MsmDesc* __restrict__ descs, uint32_t m,
+                                                       uint32_t total_chunks, const uint32_t* __restrict__ counts,
                                                        const uint2* __restrict__ offsets,
-                                                       const uint32_t* __restrict__ partials,
-                                                       uint32_t buckets, int windows, int chunk,
-                                                       uint32_t task_points,
-                                                       uint32_t* __restrict__ out) {
-    const uint32_t per_window = buckets / chunk;
+                                                       const uint32_t* __restrict__ partials, int chunk,
+                                                       uint32_t task_points, uint32_t* __restrict__ out) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= per_window * uint32_t(windows)) return;
-    const uint32_t w = g / per_window, j = g % per_window;
+    if (g >= total_chunks) return;
+    const MsmDesc d = descs[find_desc<&MsmDesc::chunk_base>(descs, m, g)];
+    const uint32_t per_window = d.buckets / uint32_t(chunk);
+    const uint32_t local = g - d.chunk_base, w = local / per_window, j = local % per_window;
     Xyzz<F> run = xyzz_inf<F>(), acc = xyzz_inf<F>();
     for (int i = chunk - 1; i >= 0; --i) {
-        const size_t wb = size_t(w) * buckets + size_t(j) * chunk + i;
+        const size_t wb = size_t(d.bucket_base) + size_t(w) * d.buckets + size_t(j) * chunk + i;
         const uint32_t cnt = counts[wb];
         if (cnt) {
             const uint32_t t0 = offsets[wb].y;
@@ -348,20 +378,32 @@ __global__ void __launch_bounds__(256) k_reduce_chunks(const uint32_t* __restric
     Mem<F>::store_xyzz(out + size_t(g) * PointLayout<F>::XYZZ_WORDS, acc);
 }
 
-// Per-window tree level: out[w][k] = sum of in[w][k*fan .. k*fan + fan).
+// One block per window of the batch: strided sums of the window's chunk
+// values, then an LDS tree; thread 0 writes the window value.
 template <class F>
-__global__ void __launch_bounds__(256) k_tree(const uint32_t* __restrict__ in, uint32_t in_per_window,
-                                              int windows, int fan, uint32_t* __restrict__ out) {
-    const uint32_t out_per_window = (in_per_window + fan - 1) / fan;
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= out_per_window * uint32_t(windows)) return;
-    const uint32_t w = g / out_per_window, k = g % out_per_window;
-    const uint32_t first = k * fan, last = min(first + fan, in_per_window);
-    const size_t base = size_t(w) * in_per_window;
-    Xyzz<F> acc = Mem<F>::load_xyzz(in + (base + first) * PointLayout<F>::XYZZ_WORDS);
-    for (uint32_t e = first + 1; e < last; ++e)
-        acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(in + (base + e) * PointLayout<F>::XYZZ_WORDS));
-    Mem<F>::store_xyzz(out + size_t(g) * PointLayout<F>::XYZZ_WORDS, acc);
+__global__ void __launch_bounds__(256) k_window_tree(const MsmDesc* __restrict__ descs, uint32_t m, int chunk,
+                                                     const uint32_t* __restrict__ chunk_vals,
+                                                     uint32_t* __restrict__ out) {
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
+    __shared__ uint4 lds[256 * XW / 4];
+    uint32_t* slot = reinterpret_cast<uint32_t*>(lds) + threadIdx.x * XW;
+    const uint32_t gw = blockIdx.x;
+    const MsmDesc d = descs[find_desc<&MsmDesc::window_base>(descs, m, gw)];
+    const uint32_t w = gw - d.window_base, per_window = d.buckets / uint32_t(chunk);
+    const size_t base = size_t(d.chunk_base) + size_t(w) * per_window;
+    Xyzz<F> acc = xyzz_inf<F>();
+    for (uint32_t k = threadIdx.x; k < per_window; k += 256)
+        acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(chunk_vals + (base + k) * XW));
+    Mem<F>::store_xyzz(slot, acc);
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(slot + s * XW));
+            Mem<F>::store_xyzz(slot, acc);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Mem<F>::store_xyzz(out + size_t(gw) * XW, acc);
 }
 
 // Canonical little-endian bytes -> Montgomery words; all-zero stays zero.
@@ -390,106 +432,145 @@ __global__ void k_to_mont(const uint8_t* __restrict__ canonical, uint32_t* __res
 inline int msm_default_window_impl(size_t n) {
     int lg = 0;
     while ((size_t(1) << (lg + 1)) <= n) ++lg;
-    int c = lg - 3;
-    return std::max(4, std::min(16, c));
+    return std::max(4, std::min(16, lg - 3));
 }
 
-inline MsmPlan msm_plan_impl(size_t n, const MsmConfig& cfg) {
-    MsmPlan p;
-    p.c = cfg.window_bits ? cfg.window_bits : msm_default_window_impl(n);
-    require(p.c >= 4 && p.c <= 20, "MSM window bits must be within 4..20");
-    p.windows = (255 + p.c - 1) / p.c;
-    p.buckets = 1u << (p.c - 1);
-    p.n = n;
-    const size_t entries = size_t(p.windows) * n;
-    const size_t nonempty = std::min<size_t>(entries, size_t(p.windows) * p.buckets);
+inline int msm_window_for_impl(size_t n, const WindowTable* table) {
+    int lg = 0;
+    while ((size_t(1) << (lg + 1)) <= n) ++lg;
+    if (table && lg < 25 && table->c[lg] >= 4 && table->c[lg] <= 20) return table->c[lg];
+    return msm_default_window_impl(n);
+}
+
+inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const MsmConfig& cfg,
+                                        const WindowTable* table) {
+    require(cfg.chunk >= 1 && cfg.chunk <= 8 && (cfg.chunk & (cfg.chunk - 1)) == 0 && cfg.task_points >= 1,
+            "invalid MSM config");
+    MsmBatchPlan p;
+    p.chunk = cfg.chunk;
+    p.task_points = uint32_t(cfg.task_points);
+    uint64_t threads = 0, entries = 0, buckets = 0, chunks = 0, windows = 0;
+    for (const MsmItem& it : items) {
+        require(it.n > 0 && it.n <= (size_t(1) << 24), "MSM length must be within 1..2^24");
+        const int c = it.c ? it.c : (cfg.window_bits ? cfg.window_bits : msm_window_for_impl(it.n, table));
+        require(c >= 4 && c <= 20, "MSM window bits must be within 4..20");
+        MsmDesc d{};
+        d.n = uint32_t(it.n);
+        d.c = uint32_t(c);
+        d.windows = uint32_t((255 + c - 1) / c);
+        d.buckets = 1u << (c - 1);
+        d.point_base = it.point_base;
+        d.scalar_base = it.scalar_base;
+        d.thread_base = uint32_t(threads);
+        d.entry_base = uint32_t(entries);
+        d.bucket_base = uint32_t(buckets);
+        d.chunk_base = uint32_t(chunks);
+        d.window_base = uint32_t(windows);
+        threads += it.n;
+        entries += uint64_t(d.windows) * it.n;
+        buckets += uint64_t(d.windows) * d.buckets;
+        chunks += uint64_t(d.windows) * (d.buckets / uint32_t(cfg.chunk));
+        windows += d.windows;
+        p.descs.push_back(d);
+    }
+    require(entries < (uint64_t(1) << 31) && buckets < (uint64_t(1) << 31), "MSM batch too large");
+    p.total_threads = uint32_t(threads);
+    p.total_entries = uint32_t(entries);
+    p.total_buckets = uint32_t(buckets);
+    p.total_chunks = uint32_t(chunks);
+    p.total_windows = uint32_t(windows);
+    const size_t nonempty = std::min<size_t>(entries, buckets);
     p.max_tasks = entries / size_t(cfg.task_points) + nonempty + 1;
+    p.max_heavy = std::min<size_t>(buckets, entries / (size_t(cfg.task_points) * HEAVY_TASKS) + 1);
     return p;
 }
 
-template <class F>
-MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, MsmWorkspace& ws,
-                    hipStream_t stream, const MsmConfig& cfg, uint32_t* window_out) {
-    require(n > 0 && n <= (size_t(1) << 24), "MSM length must be within 1..2^24");
-    require(cfg.chunk >= 1 && cfg.task_points >= 1 && cfg.tree_fanin >= 2, "invalid MSM config");
-    const MsmPlan plan = msm_plan_impl(n, cfg);
-    require(plan.buckets % uint32_t(cfg.chunk) == 0, "chunk must divide bucket count");
-    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
-    const int W = plan.windows;
-    const uint32_t B = plan.buckets;
-    const size_t m = size_t(W) * B;
-    const size_t entries = size_t(W) * n;
-    const uint32_t tiles = uint32_t((m + SCAN_TILE - 1) / SCAN_TILE);
+inline void msm_batch_prepare_impl(const MsmBatchPlan& plan, int xw, MsmWorkspace& ws, hipStream_t stream) {
+    const size_t m = plan.descs.size();
+    require(m > 0, "empty MSM batch");
+    const uint32_t tiles = uint32_t((plan.total_buckets + SCAN_TILE - 1) / SCAN_TILE);
+    ws.counts.as<uint32_t>(plan.total_buckets);
+    ws.offsets.as<uint2>(plan.total_buckets);
+    ws.digits.as<uint32_t>(plan.total_entries);
+    ws.ranks.as<uint32_t>(plan.total_entries);
+    ws.sorted.as<uint32_t>(plan.total_entries);
+    ws.block_sums.as<uint2>(tiles);
+    ws.total.as<uint32_t>(4);
+    ws.tasks.as<uint2>(plan.max_tasks);
+    ws.partials.as<uint32_t>(plan.max_tasks * xw);
+    ws.heavy.as<uint32_t>(plan.max_heavy);
+    ws.chunks.as<uint32_t>(size_t(plan.total_chunks) * xw);
+    // Earlier work on this stream may still read the pinned staging area.
+    HIP_CHECK(hipStreamSynchronize(stream));
+    MsmDesc* host = ws.host_descs.as<MsmDesc>(m);
+    std::memcpy(host, plan.descs.data(), m * sizeof(MsmDesc));
+    MsmDesc* dev = ws.descs.as<MsmDesc>(m);
+    HIP_CHECK(hipMemcpyAsync(dev, host, m * sizeof(MsmDesc), hipMemcpyHostToDevice, stream));
+}
 
-    uint32_t* counts = ws.counts.as<uint32_t>(m);
-    uint2* offsets = ws.offsets.as<uint2>(m);
-    uint32_t* digits = ws.digits.as<uint32_t>(entries);
-    uint32_t* ranks = ws.ranks.as<uint32_t>(entries);
-    uint32_t* sorted = ws.sorted.as<uint32_t>(entries);
-    uint2* tile_sums = ws.block_sums.as<uint2>(tiles);
-    uint32_t* total = ws.total.as<uint32_t>(4);  // [0] entries, [1] tasks, [2] heavy buckets
-    uint2* tasks = ws.tasks.as<uint2>(plan.max_tasks);
-    uint32_t* partials = ws.partials.as<uint32_t>(plan.max_tasks * XW);
-    const uint32_t chunks_per_window = B / uint32_t(cfg.chunk);
-    uint32_t* chunk_vals = ws.chunks.as<uint32_t>(size_t(chunks_per_window) * W * XW);
+template <class F>
+void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, const uint32_t* scalars,
+                       const MsmBatchPlan& plan, MsmWorkspace& ws, hipStream_t stream, uint32_t* window_out) {
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
+    const uint32_t m = uint32_t(plan.descs.size());
+    const uint32_t tiles = uint32_t((plan.total_buckets + SCAN_TILE - 1) / SCAN_TILE);
+    // The buffers were sized by msm_batch_prepare; check instead of growing.
+    require(ws.counts.bytes >= size_t(plan.total_buckets) * 4 && ws.sorted.bytes >= size_t(plan.total_entries) * 4 &&
+                ws.partials.bytes >= plan.max_tasks * XW * 4 && ws.chunks.bytes >= size_t(plan.total_chunks) * XW * 4 &&
+                ws.descs.bytes >= m * sizeof(MsmDesc) && ws.heavy.bytes >= plan.max_heavy * 4 &&
+                ws.block_sums.bytes >= tiles * sizeof(uint2),
+            "MSM workspace not prepared for this batch");
+    const MsmDesc* descs = static_cast<const MsmDesc*>(ws.descs.ptr);
+    uint32_t* counts = static_cast<uint32_t*>(ws.counts.ptr);
+    uint2* offsets = static_cast<uint2*>(ws.offsets.ptr);
+    uint32_t* digits = static_cast<uint32_t*>(ws.digits.ptr);
+    uint32_t* ranks = static_cast<uint32_t*>(ws.ranks.ptr);
+    uint32_t* sorted = static_cast<uint32_t*>(ws.sorted.ptr);
+    uint2* tile_sums = static_cast<uint2*>(ws.block_sums.ptr);
+    uint32_t* total = static_cast<uint32_t*>(ws.total.ptr);  // [0] entries, [1] tasks, [2] heavy buckets
+    uint2* tasks = static_cast<uint2*>(ws.tasks.ptr);
+    uint32_t* partials = static_cast<uint32_t*>(ws.partials.ptr);
+    uint32_t* heavy = static_cast<uint32_t*>(ws.heavy.ptr);
+    uint32_t* chunk_vals = static_cast<uint32_t*>(ws.chunks.ptr);
+    const uint32_t tp = plan.task_points;
 
     hipEvent_t* ev = ws.phase_events;
     auto mark = [&](int k) {
         if (ev) HIP_CHECK(hipEventRecord(ev[k], stream));
     };
     mark(0);
-    const size_t max_heavy = std::min<size_t>(m, entries / (size_t(cfg.task_points) * HEAVY_TASKS) + 1);
-    uint32_t* heavy = ws.heavy.as<uint32_t>(max_heavy);
-    HIP_CHECK(hipMemsetAsync(counts, 0, m * sizeof(uint32_t), stream));
-    HIP_CHECK(hipMemsetAsync(total, 0, 4 * sizeof(uint32_t), stream));
-    k_digits<<<grid_for(n, 256), 256, 0, stream>>>(scalars, uint32_t(n), plan.c, W, B, counts, digits, ranks);
+    k_fill<<<unsigned(std::min<size_t>(grid_for(plan.total_buckets, 256), 4096)), 256, 0, stream>>>(
+        counts, plan.total_buckets, 0u);
+    k_fill<<<1, 64, 0, stream>>>(total, 4, 0u);
+    k_digits<<<grid_for(plan.total_threads, 256), 256, 0, stream>>>(descs, m, plan.total_threads, scalars, counts,
+                                                                   digits, ranks);
     HIP_CHECK_LAUNCH();
     mark(1);
-    const uint32_t tp = uint32_t(cfg.task_points);
-    k_scan_tiles<<<tiles, SCAN_THREADS, 0, stream>>>(counts, uint32_t(m), tp, tile_sums);
-    HIP_CHECK_LAUNCH();
+    k_scan_tiles<<<tiles, SCAN_THREADS, 0, stream>>>(counts, plan.total_buckets, tp, tile_sums);
     k_scan_sums<<<1, SCAN_THREADS, 0, stream>>>(tile_sums, tiles, total);
-    HIP_CHECK_LAUNCH();
-    k_scan_apply<<<tiles, SCAN_THREADS, 0, stream>>>(counts, uint32_t(m), tp, tile_sums, offsets);
+    k_scan_apply<<<tiles, SCAN_THREADS, 0, stream>>>(counts, plan.total_buckets, tp, tile_sums, offsets);
     HIP_CHECK_LAUNCH();
     mark(2);
-    k_scatter<<<grid_for(entries, 256), 256, 0, stream>>>(digits, ranks, offsets, uint32_t(n), W, B, sorted);
+    k_scatter<<<grid_for(plan.total_entries, 256), 256, 0, stream>>>(descs, m, plan.total_entries, digits, ranks,
+                                                                    offsets, point_index, sorted);
     HIP_CHECK_LAUNCH();
     mark(3);
-    k_tasks<<<grid_for(m, 256), 256, 0, stream>>>(counts, offsets, uint32_t(m), tp, tasks, heavy, total + 2);
+    k_tasks<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(counts, offsets, plan.total_buckets, tp, tasks,
+                                                                  heavy, total + 2);
     HIP_CHECK_LAUNCH();
     mark(4);
     k_accumulate<F><<<grid_for(plan.max_tasks, 256), 256, 0, stream>>>(points, sorted, tasks, total, partials);
     HIP_CHECK_LAUNCH();
-    k_heavy<F><<<unsigned(std::min<size_t>(max_heavy, 1024)), 256, 0, stream>>>(heavy, total + 2, counts, offsets,
-                                                                               tp, partials);
-    HIP_CHECK_LAUNCH();
     mark(5);
-    const size_t nchunks = size_t(chunks_per_window) * W;
-    k_reduce_chunks<F><<<grid_for(nchunks, 256), 256, 0, stream>>>(counts, offsets, partials, B, W,
-                                                                   cfg.chunk, tp, chunk_vals);
+    k_heavy<F><<<unsigned(std::min<size_t>(plan.max_heavy, 1024)), 256, 0, stream>>>(heavy, total + 2, counts,
+                                                                                     offsets, tp, partials);
+    k_reduce_chunks<F><<<grid_for(plan.total_chunks, 256), 256, 0, stream>>>(
+        descs, m, plan.total_chunks, counts, offsets, partials, plan.chunk, tp, chunk_vals);
     HIP_CHECK_LAUNCH();
     mark(6);
-    // Tree levels down to one value per window.
-    uint32_t per_window = chunks_per_window;
-    const uint32_t* src = chunk_vals;
-    int flip = 0;
-    while (per_window > 1) {
-        const uint32_t next = (per_window + cfg.tree_fanin - 1) / cfg.tree_fanin;
-        uint32_t* dst = next == 1 ? window_out
-                                  : (flip ? ws.tree_b.as<uint32_t>(size_t(next) * W * XW)
-                                          : ws.tree_a.as<uint32_t>(size_t(next) * W * XW));
-        k_tree<F><<<grid_for(size_t(next) * W, 256), 256, 0, stream>>>(src, per_window, W, cfg.tree_fanin, dst);
-        HIP_CHECK_LAUNCH();
-        src = dst;
-        per_window = next;
-        flip ^= 1;
-    }
-    if (src != window_out)
-        HIP_CHECK(hipMemcpyAsync(window_out, src, size_t(W) * XW * sizeof(uint32_t),
-                                 hipMemcpyDeviceToDevice, stream));
+    k_window_tree<F><<<plan.total_windows, 256, 0, stream>>>(descs, m, plan.chunk, chunk_vals, window_out);
+    HIP_CHECK_LAUNCH();
     mark(7);
-    return plan;
 }
 
 template <class F>
@@ -509,15 +590,68 @@ struct HostMem<Fq2Ops> {
     static Xyzz<Fq2Ops> load_xyzz(const uint32_t* p) { return {ld(p), ld(p + 16), ld(p + 32), ld(p + 48)}; }
 };
 
+
 template <class F>
-Affine<F> msm_combine_windows(const uint32_t* words, const MsmPlan& plan) {
+Affine<F> combine_one(const uint32_t* words, const MsmDesc& d) {
     constexpr int XW = PointLayout<F>::XYZZ_WORDS;
-    Xyzz<F> acc = HostMem<F>::load_xyzz(words + size_t(plan.windows - 1) * XW);
-    for (int w = plan.windows - 2; w >= 0; --w) {
-        for (int k = 0; k < plan.c; ++k) acc = xyzz_dbl<F>(acc);
-        acc = xyzz_add<F>(acc, HostMem<F>::load_xyzz(words + size_t(w) * XW));
+    const uint32_t* base = words + size_t(d.window_base) * XW;
+    Xyzz<F> acc = HostMem<F>::load_xyzz(base + size_t(d.windows - 1) * XW);
+    for (int w = int(d.windows) - 2; w >= 0; --w) {
+        for (uint32_t k = 0; k < d.c; ++k) acc = xyzz_dbl<F>(acc);
+        acc = xyzz_add<F>(acc, HostMem<F>::load_xyzz(base + size_t(w) * XW));
     }
     return xyzz_to_affine<F>(acc);
+}
+
+// Window combination on the host; large batches spread over a few threads.
+template <class F>
+void msm_batch_combine(const uint32_t* words, const MsmBatchPlan& plan, Affine<F>* out) {
+    const size_t m = plan.descs.size();
+    const size_t workers = std::min<size_t>(m / 8, 8);
+    if (workers < 2) {
+        for (size_t i = 0; i < m; ++i) out[i] = combine_one<F>(words, plan.descs[i]);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < workers; ++t)
+        pool.emplace_back([&, t] {
+            for (size_t i = t; i < m; i += workers) out[i] = combine_one<F>(words, plan.descs[i]);
+        });
+    for (auto& th : pool) th.join();
+}
+
+// ---- single-MSM conveniences -------------------------------------------------
+
+inline MsmPlan msm_plan_impl(size_t n, const MsmConfig& cfg) {
+    MsmPlan p;
+    p.c = cfg.window_bits ? cfg.window_bits : msm_default_window_impl(n);
+    p.windows = (255 + p.c - 1) / p.c;
+    p.buckets = 1u << (p.c - 1);
+    p.n = n;
+    return p;
+}
+
+template <class F>
+MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, MsmWorkspace& ws,
+                    hipStream_t stream, const MsmConfig& cfg, uint32_t* window_out) {
+    const MsmBatchPlan plan = msm_batch_plan_impl({MsmItem{n, 0, 0, cfg.window_bits}}, cfg, nullptr);
+    msm_batch_prepare_impl(plan, PointLayout<F>::XYZZ_WORDS, ws, stream);
+    msm_batch_enqueue<F>(points, nullptr, scalars, plan, ws, stream, window_out);
+    MsmPlan p;
+    p.c = int(plan.descs[0].c);
+    p.windows = int(plan.descs[0].windows);
+    p.buckets = plan.descs[0].buckets;
+    p.n = n;
+    return p;
+}
+
+template <class F>
+Affine<F> msm_combine_windows(const uint32_t* words, const MsmPlan& plan) {
+    MsmDesc d{};
+    d.c = uint32_t(plan.c);
+    d.windows = uint32_t(plan.windows);
+    d.window_base = 0;
+    return combine_one<F>(words, d);
 }
 
 template <class F>
