@@ -41,6 +41,8 @@ void Owner::release_device() noexcept {
     stage_b.release();
     stage_c.release();
     pinned.release();
+    for (BatchObj* b : batches) batch_destroy(b);
+    batches.clear();
     for (auto& kv : ext_allocations) {
         (void)hipFree(kv.first);
         quota.release(kv.second);
@@ -100,10 +102,12 @@ int msm_entry(Owner* o, const uint8_t* points, const uint8_t* scalars, size_t le
         }
         Affine<F> r;
         MsmPlan plan;
+        MsmConfig cfg = o->msm_config;
+        if (!cfg.window_bits) cfg.window_bits = msm_window_for(length, &o->window_table);
         try {
             uint32_t* window_words = o->msm.windows.as<uint32_t>(
-                size_t(msm_plan(length, o->msm_config).windows) * PointLayout<F>::XYZZ_WORDS);
-            plan = msm_enqueue<F>(mont, dsc, length, o->msm, o->stream, o->msm_config, window_words);
+                size_t(msm_plan(length, cfg).windows) * PointLayout<F>::XYZZ_WORDS);
+            plan = msm_enqueue<F>(mont, dsc, length, o->msm, o->stream, cfg, window_words);
             o->msm.phase_events = nullptr;
             uint32_t* host = o->msm.host_windows.as<uint32_t>(size_t(plan.windows) * PointLayout<F>::XYZZ_WORDS);
             HIP_CHECK(hipMemcpyAsync(host, window_words,
@@ -308,8 +312,10 @@ int dvgpu_bn254_msm_batch(void* backend, const uint8_t* points, const uint8_t* s
         points_to_mont<FqOps>(dpts, mont, total, o->stream);
         uint8_t encoded[DVGPU_MAX_MSM_BATCH_SIZE * 64];
         for (size_t i = 0; i < count; ++i) {
+            MsmConfig cfg = o->msm_config;
+            if (!cfg.window_bits) cfg.window_bits = msm_window_for(length, &o->window_table);
             const Affine<FqOps> r = msm_run<FqOps>(mont + i * length * 16, dsc + i * length * 8, length,
-                                                   o->msm, o->stream, o->msm_config);
+                                                   o->msm, o->stream, cfg);
             affine_to_bytes<FqOps>(r, encoded + i * 64);
         }
         std::memcpy(results, encoded, count * 64);

@@ -32,7 +32,7 @@ int msm_device(Owner* o, const void* points, const void* scalars, size_t n, int 
                uint8_t* result, float* phase_ms) {
     constexpr int XW = PointLayout<F>::XYZZ_WORDS;
     MsmConfig cfg = o->msm_config;
-    if (window_bits) cfg.window_bits = window_bits;
+    cfg.window_bits = window_bits ? window_bits : msm_window_for(n, &o->window_table);
     Events ev(phase_ms != nullptr);
     const MsmPlan p0 = msm_plan(n, cfg);
     uint32_t* window_words = o->msm.windows.as<uint32_t>(size_t(p0.windows) * XW);

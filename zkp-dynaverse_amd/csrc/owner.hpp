@@ -21,6 +21,7 @@
 namespace dvgpu {
 
 struct ResidentState;  // resident.hip
+struct BatchObj;       // batch.hip
 struct SetupState;     // setup.hip
 
 struct Owner {
@@ -49,6 +50,10 @@ struct Owner {
     DvgpuResidentDiagnostics retired_resident_diagnostics{};
     // Caller-visible device allocations (dynaverse_hip_ext.h).
     std::unordered_map<void*, size_t> ext_allocations;
+    // Batches created through dynaverse_hip_ext.h (freed with the owner).
+    std::vector<BatchObj*> batches;
+    // Measured window width per log2 MSM size (dvgpu_ext_set_window_table).
+    WindowTable window_table;
 
     Owner();
     ~Owner();
@@ -86,6 +91,8 @@ inline uint64_t now_ns() {
 void resident_destroy(ResidentState* state) noexcept;
 DvgpuResidentStats resident_stats(const ResidentState* state);
 DvgpuResidentDiagnostics resident_diagnostics(const ResidentState* state);
+// Batch hooks (batch.hip).
+void batch_destroy(BatchObj* b) noexcept;
 // Setup hooks (setup.hip).
 void setup_destroy(SetupState* state) noexcept;
 

@@ -112,6 +112,27 @@ class DvgpuG2KernelResources(ctypes.Structure):
                 ("normalize_local_bytes", ctypes.c_uint64), ("normalize_shared_bytes", ctypes.c_uint64)]
 
 
+class DvgpuMsmItem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("point_base", ctypes.c_uint32), ("scalar_base", ctypes.c_uint32),
+                ("window_bits", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class DvgpuNttRows(ctypes.Structure):
+    _fields_ = [("in_", c_void_p), ("out", c_void_p), ("length", ctypes.c_uint64), ("rows", ctypes.c_uint64),
+                ("inverse", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class DvgpuBatchSpec(ctypes.Structure):
+    _fields_ = [("g1_points", c_void_p), ("g1_index", c_void_p), ("g1_scalars", c_void_p),
+                ("g1_items", ctypes.POINTER(DvgpuMsmItem)), ("g1_count", ctypes.c_uint64),
+                ("g2_points", c_void_p), ("g2_index", c_void_p), ("g2_scalars", c_void_p),
+                ("g2_items", ctypes.POINTER(DvgpuMsmItem)), ("g2_count", ctypes.c_uint64),
+                ("ntt", ctypes.POINTER(DvgpuNttRows)), ("ntt_count", ctypes.c_uint64),
+                ("g2_wire", c_void_p), ("g2_wire_n", ctypes.c_uint64),
+                ("g2_algorithm", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("g2_masks", c_void_p), ("root", c_void_p)]
+
+
 _I = ctypes.c_int
 _E = (ctypes.c_char_p, c_size_t)  # error, capacity
 
@@ -169,6 +190,13 @@ SIGNATURES = {
     "dvgpu_ext_copy_to_host": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, *_E]),
     "dvgpu_ext_stream": (_I, [c_void_p, ctypes.POINTER(c_void_p)]),
     "dvgpu_ext_device_copy": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, *_E]),
+    "dvgpu_ext_set_window_table": (_I, [c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "dvgpu_ext_msm_batch_device": (_I, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(DvgpuMsmItem),
+                                        c_size_t, _I, c_void_p, ctypes.POINTER(ctypes.c_float), *_E]),
+    "dvgpu_ext_batch_create": (_I, [c_void_p, ctypes.POINTER(DvgpuBatchSpec), _I, ctypes.POINTER(c_void_p), *_E]),
+    "dvgpu_ext_batch_run": (_I, [c_void_p, c_void_p, c_void_p, c_void_p,
+                                 ctypes.POINTER(DvgpuG2ValidationReport), ctypes.POINTER(ctypes.c_float), *_E]),
+    "dvgpu_ext_batch_destroy": (_I, [c_void_p, c_void_p]),
     "dvgpu_ext_fixed_base_device": (_I, [c_void_p, c_void_p, c_size_t, _I, c_void_p, *_E]),
     "dvgpu_ext_points_to_mont": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, _I, *_E]),
     "dvgpu_ext_msm_device": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, _I, _I, c_void_p,
