@@ -63,15 +63,26 @@ int dvgpu_ext_fixed_base_device(void* owner, const void* scalars, size_t n, int 
 
 /* ---- batching: many uneven MSMs and NTT rows in one launch sequence ---- */
 
-/* Window width per log2 MSM size (25 entries, 0 = built-in rule), e.g. the
- * table an autotuning sweep measured.  Used whenever window_bits is 0. */
-int dvgpu_ext_set_window_table(void* owner, const int32_t* window_bits_by_log2);
+/* Launch shapes chosen per size, e.g. by an autotuning sweep.  Every entry
+ * of 0 keeps the built-in choice.  Applies to every later call of the owner
+ * (plain, extension and batch entry points) and to batches created later. */
+typedef struct DvgpuTuning {
+    int32_t msm_window_bits[25];  /* by log2 MSM size: window width c, 4..20 */
+    int32_t ntt_pass_stages[21];  /* by log2 NTT length: butterfly stages per
+                                     LDS pass (radix 2^s), 1..11 */
+    int32_t msm_task_points;      /* max points one accumulation task adds, 1..1024 */
+    int32_t msm_chunk;            /* buckets per first-level reduction thread: 1, 2, 4, 8 */
+    int32_t msm_device_finalize_min; /* batches of at least this many MSMs sum their
+                                        windows on the device (one lane per MSM) */
+} DvgpuTuning;
+int dvgpu_ext_set_tuning(void* owner, const DvgpuTuning* tuning);
+int dvgpu_ext_get_tuning(void* owner, DvgpuTuning* tuning);
 
 typedef struct DvgpuMsmItem {
     uint64_t n;              /* points in this MSM */
     uint32_t point_base;     /* first point, or first entry of point_index */
     uint32_t scalar_base;    /* first scalar */
-    int32_t window_bits;     /* 0 = from the window table */
+    int32_t window_bits;     /* 0 = from the tuning table */
     int32_t reserved;
 } DvgpuMsmItem;
 
@@ -82,12 +93,38 @@ int dvgpu_ext_msm_batch_device(void* owner, const void* points, const void* poin
                                const void* scalars, const DvgpuMsmItem* items, size_t count, int g2,
                                uint8_t* results, float* phase_ms, char* error, size_t error_capacity);
 
+/* Same MSMs with the window sums finished on the device (one lane per MSM):
+ * results_device (device, count x 64 / 128 bytes; null = internal) receives
+ * the canonical affine results, results_host (optional) a copy.  With
+ * slot_points non-null, MSM i also stores its result as a Montgomery affine
+ * point at index result_slots[i] (host array, UINT32_MAX = none) of
+ * slot_points, after every MSM of the call has read its inputs: a result may
+ * replace one of its own input points (a cached commitment updated in place).
+ * Bounds: item ranges are checked against point_count / index_count /
+ * scalar_count and slots against slot_count on the host; indexed points at or
+ * beyond point_count are caught on the device, and then the call fails with
+ * -1 and stores no slot.  Empty MSMs give the identity and may not name a
+ * slot. */
+int dvgpu_ext_msm_batch_finalize(void* owner, const void* points, size_t point_count,
+                                 const void* point_index, size_t index_count,
+                                 const void* scalars, size_t scalar_count,
+                                 const DvgpuMsmItem* items, size_t count, int g2,
+                                 void* results_device, uint8_t* results_host,
+                                 const uint32_t* result_slots, void* slot_points, size_t slot_count,
+                                 char* error, size_t error_capacity);
+
 typedef struct DvgpuNttRows {
     const void* in;          /* device canonical words */
     void* out;
     uint64_t length, rows;   /* rows x length, length a power of two <= 2^20 */
     int32_t inverse, reserved;
 } DvgpuNttRows;
+
+/* All row batches in one launch sequence (one launch per pass for every
+ * entry, then one bit-reversal launch), whatever their lengths.  Canonical in
+ * and out; entries may not overlap each other (each may run in place). */
+int dvgpu_ext_ntt_batch_device(void* owner, const DvgpuNttRows* rows, size_t count, const uint8_t* root,
+                               float* kernel_ms, char* error, size_t error_capacity);
 
 typedef struct DvgpuBatchSpec {
     const void* g1_points; const void* g1_index; const void* g1_scalars;
@@ -103,7 +140,9 @@ typedef struct DvgpuBatchSpec {
 
 /* Plan, size and warm a batch; with use_graph the whole sequence (G1 MSMs,
  * G2 MSMs, NTT rows, G2 validation, result copies) is captured into one
- * hipGraph that dvgpu_ext_batch_run replays with a single launch. */
+ * hipGraph that dvgpu_ext_batch_run replays with a single launch.  NTT
+ * entries run in order; consecutive entries that do not overlap run fused
+ * (an entry may read an earlier entry's output). */
 int dvgpu_ext_batch_create(void* owner, const DvgpuBatchSpec* spec, int use_graph, void** batch,
                            char* error, size_t error_capacity);
 /* Runs the batch once; outputs are host buffers (any may be null). */

@@ -13,7 +13,8 @@ import numpy as np
 import pytest
 
 from dvgpu import BackendError, HipBackend
-from dvgpu.batching import DeviceBatch, MsmItem, NttRows, msm_batch_device, set_window_table
+from dvgpu import autotune
+from dvgpu.batching import DeviceBatch, MsmItem, NttRows, msm_batch_device, msm_batch_finalize, ntt_batch_device
 from dvgpu.codec import FR_MODULUS, encode_fr, root_of_unity_2_20
 from dvgpu.rng import ChaCha20Stream
 from dvgpu.workload import tau_powers
@@ -199,18 +200,27 @@ def test_full_batch_eager_and_graph_replays(backend, wl, use_graph):
         batch.close()
 
 
-def test_window_table_changes_plan_not_results(backend, wl):
-    table = [0] * 25
-    for lg in range(10, 19):
-        table[lg] = max(4, lg - 6)
-    set_window_table(backend, table)
+def test_tuning_changes_plan_not_results(backend, wl):
+    base = autotune.get(backend)
+    table = {"msm_window_bits": {lg: max(4, lg - 6) for lg in range(10, 19)}, "msm_task_points": 7,
+             "msm_chunk": 2, "ntt_pass_stages": {12: 4, 16: 6}}
+    autotune.apply(backend, table)
     try:
-        print("case: window table override", flush=True)
+        got = autotune.get(backend)
+        assert got["msm_task_points"] == 7 and got["msm_chunk"] == 2 and got["ntt_pass_stages"] == {12: 4, 16: 6}
+        print("case: tuning override msm", flush=True)
         assert msm_batch_device(backend, wl.srs, wl.d_sc, wl.items[:16]) == wl.exp1[:16]
+        print("case: tuning override ntt", flush=True)
+        for length, d_in, d_out, exp in ((1 << 12, wl.in12, wl.out12, wl.exp12), (1 << 16, wl.in16, wl.out16, wl.exp16)):
+            backend.ntt_device(d_in.ptr, d_out.ptr, length, 16, False)
+            assert d_out.download().tobytes() == exp
     finally:
-        set_window_table(backend, [0] * 25)
-    with pytest.raises(BackendError):
-        set_window_table(backend, [3] * 25)
+        autotune.apply(backend, base)
+    for bad in ({"msm_window_bits": {12: 3}}, {"ntt_pass_stages": {12: 12}}, {"msm_chunk": 3},
+                {"msm_task_points": 5000}):
+        with pytest.raises(BackendError):
+            autotune.apply(backend, bad)
+    assert autotune.get(backend) == base
 
 
 def test_invalid_batches_leave_owner_healthy(backend, wl):
@@ -220,3 +230,86 @@ def test_invalid_batches_leave_owner_healthy(backend, wl):
         DeviceBatch(backend, ntt=[NttRows(wl.in12, wl.out12, 3000, 1)])
     assert msm_batch_device(backend, wl.srs, wl.d_sc, [MsmItem(0, 0, 0)]) == [bytes(64)]
     assert msm_batch_device(backend, wl.srs, wl.d_sc, wl.items[:2]) == wl.exp1[:2]
+
+
+def test_fused_ntt_batch_mixed_lengths(backend):
+    # Lengths 1 .. 2^16 with several row counts, forward and inverse, in one
+    # fused launch sequence; each row against the C oracle.
+    root = encode_fr(root_of_unity_2_20())
+    shapes = [(1, 3, False), (2, 1, True), (256, 5, False), (1024, 2, True), (2048, 1, False), (4096, 3, False),
+              (1 << 16, 2, True), (8, 7, True)]
+    rng = ChaCha20Stream(3020)
+    rows, host = [], []
+    for length, r, inv in shapes:
+        vals = rng.uniform_fr_array(length * r).reshape(-1)
+        d_in = backend.to_device(vals)
+        d_out = backend.alloc(vals.nbytes)
+        rows.append(NttRows(d_in, d_out, length, r, inv))
+        host.append((vals, length, r, inv))
+    in_place = rng.uniform_fr_array(512 * 2).reshape(-1)
+    d_ip = backend.to_device(in_place)
+    rows.append(NttRows(d_ip, d_ip, 512, 2, False))
+    host.append((in_place, 512, 2, False))
+    print("case: fused ntt batch", flush=True)
+    ms = ntt_batch_device(backend, rows, timed=True)
+    assert ms > 0
+    for (vals, length, r, inv), row in zip(host, rows):
+        exp = b"".join(cref.ntt(vals[32 * length * k:32 * length * (k + 1)].tobytes(), length, inv, root, 8)
+                       for k in range(r))
+        assert row.out.download().tobytes() == exp, (length, r, inv)
+    # Overlapping entries are rejected before any work.
+    with pytest.raises(BackendError):
+        ntt_batch_device(backend, [NttRows(rows[2].inp, rows[2].out, 256, 1), NttRows(rows[2].out, rows[3].out, 256, 1)])
+
+
+def test_finalize_on_device_and_result_slots(backend, wl):
+    # Device-side window sums equal the host epilogue; results stored into
+    # slots replace inputs only after every MSM of the call has read them.
+    items = wl.items[:24]
+    print("case: finalize vs host epilogue", flush=True)
+    assert msm_batch_finalize(backend, wl.srs, wl.d_sc, items) == wl.exp1[:24]
+    # Commitment update in place: C' = C + sum d_j P_j, where C sits in a slot
+    # array and enters the MSM through the index with scalar 1.
+    u = ChaCha20Stream(3021)
+    base = backend.alloc(64 * 8)
+    # Points 0..3: P_0..P_3 (Montgomery, resident); 4..7: cached points, identity at first.
+    base.upload(np.concatenate([wl.srs.download(64 * 4), np.zeros(64 * 4, dtype=np.uint8)]))
+    ks = [3, 1, 5, 2]
+    state = [0] * len(ks)                               # discrete log of each cached point
+    for rnd in range(3):
+        idx, sc, its, slots, exps = [], b"", [], [], []
+        for j, k in enumerate(ks):
+            sel = [u.below(4) for _ in range(k)]
+            ds = [u.nonzero_fr() for _ in range(k)]
+            its.append(MsmItem(k + 1, len(idx), len(sc) // 32))
+            idx.extend(sel + [4 + j])
+            sc += b"".join(encode_fr(d) for d in ds) + encode_fr(1)
+            slots.append(4 + j)
+            state[j] = (state[j] + sum(d * wl.pw[p] for d, p in zip(ds, sel))) % R
+            exps.append(o.g1_to_bytes(o.g1_mul(state[j]) if state[j] else None))
+        d_idx = backend.to_device(np.array(idx, dtype=np.uint32).view(np.uint8))
+        d_sc = backend.to_device(sc)
+        print(f"case: slot round {rnd}", flush=True)
+        got = msm_batch_finalize(backend, base, d_sc, its, index=d_idx, slots=slots, slot_points=base)
+        assert got == exps, rnd
+    with pytest.raises(BackendError):                   # an empty MSM may not name a slot
+        msm_batch_finalize(backend, base, d_sc, [MsmItem(0, 0, 0)], index=d_idx, slots=[4], slot_points=base)
+
+
+def test_finalize_rejects_out_of_range_points(backend, wl):
+    base = backend.alloc(64 * 8)
+    base.upload(wl.srs.download(64 * 8))
+    before = base.download().tobytes()
+    sc = backend.to_device(encode_fr(5) * 4)
+    bad = backend.to_device(np.array([1, 2, 8, 3], dtype=np.uint32).view(np.uint8))   # 8 is out of range
+    print("case: finalize index out of range", flush=True)
+    with pytest.raises(BackendError):
+        msm_batch_finalize(backend, base, sc, [MsmItem(4, 0, 0)], index=bad, slots=[7], slot_points=base)
+    assert base.download().tobytes() == before          # no slot written
+    with pytest.raises(BackendError):                   # host-side range checks
+        msm_batch_finalize(backend, base, sc, [MsmItem(5, 0, 0)])
+    with pytest.raises(BackendError):
+        msm_batch_finalize(backend, base, sc, [MsmItem(2, 0, 0)], slots=[8], slot_points=base)
+    good = backend.to_device(np.array([1, 2, 7, 3], dtype=np.uint32).view(np.uint8))
+    e = 5 * (wl.pw[1] + wl.pw[2] + wl.pw[7] + wl.pw[3]) % R
+    assert msm_batch_finalize(backend, base, sc, [MsmItem(4, 0, 0)], index=good) == [o.g1_to_bytes(o.g1_mul(e))]

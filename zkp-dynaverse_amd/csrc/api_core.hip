@@ -19,6 +19,7 @@ int fail(int status, const char* message, char* error, size_t capacity) {
 Owner::Owner() {
     msm.attach(&quota);
     ntt.attach(&quota);
+    ntt_batch.attach(&quota);
     domain.fwd.quota = domain.inv.quota = &quota;
     stage_a.quota = stage_b.quota = stage_c.quota = &quota;
 }
@@ -36,6 +37,7 @@ void Owner::release_device() noexcept {
     }
     msm.release();
     ntt.release();
+    ntt_batch.release();
     domain.release();
     stage_a.release();
     stage_b.release();

@@ -53,10 +53,15 @@ struct MsmItem {
 
 // Per-MSM descriptor as the kernels read it (all bases are exclusive
 // prefix sums over the batch).
+constexpr uint32_t MSM_NO_SLOT = 0xffffffffu;
+
 struct MsmDesc {
     uint32_t n, c, windows, buckets;
     uint32_t point_base, scalar_base;
-    uint32_t thread_base, entry_base, bucket_base, chunk_base, window_base, pad;
+    uint32_t thread_base, entry_base, bucket_base, chunk_base, window_base;
+    uint32_t out_index;    // position of this MSM's result in the caller's list
+    uint32_t result_slot;  // device finalize: also store the result at this point index
+    uint32_t pad[3];
 };
 
 struct MsmBatchPlan {
@@ -65,6 +70,10 @@ struct MsmBatchPlan {
     size_t max_tasks = 0, max_heavy = 0;
     int chunk = 8;
     uint32_t task_points = 32;
+    // Indexed points at or beyond this limit are counted in the workspace's
+    // error word (total[3]) and read as point 0; only the finalize entry
+    // point, which knows the point count, sets it.
+    uint32_t point_limit = 0xffffffffu;
 };
 
 MsmBatchPlan msm_batch_plan(const std::vector<MsmItem>& items, const MsmConfig& cfg,
@@ -75,13 +84,13 @@ constexpr int MSM_PHASE_EVENTS = 8;
 // Scratch owned by one backend; grows to the largest batch seen.
 struct MsmWorkspace {
     DeviceBuffer digits, ranks, counts, offsets, sorted, tasks, partials, chunks, block_sums, total,
-        points, scalars, windows, heavy, descs;
+        points, scalars, windows, heavy, descs, finals;
     HostBuffer host_windows, host_descs;
     // Optional phase events (MSM_PHASE_EVENTS of them) recorded by the
     // enqueue: start, digits, scan, scatter, tasks, accumulate, reduce, tree.
     hipEvent_t* phase_events = nullptr;
-    DeviceBuffer* all[15] = {&digits, &ranks, &counts, &offsets, &sorted, &tasks, &partials, &chunks,
-                             &block_sums, &total, &points, &scalars, &windows, &heavy, &descs};
+    DeviceBuffer* all[16] = {&digits, &ranks, &counts, &offsets, &sorted, &tasks, &partials, &chunks,
+                             &block_sums, &total, &points, &scalars, &windows, &heavy, &descs, &finals};
     void attach(Quota* q) {
         for (DeviceBuffer* b : all) b->quota = q;
     }
@@ -108,6 +117,17 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
 // Host epilogue: for every MSM, sum_w 2^(w c) V_w then canonical affine.
 template <class F>
 void msm_batch_combine(const uint32_t* window_words, const MsmBatchPlan& plan, Affine<F>* out);
+
+// Device epilogue, one lane per MSM: the same sum and affine conversion,
+// written as canonical bytes to out + desc.out_index * CANON_BYTES and, where
+// desc.result_slot != MSM_NO_SLOT and slot_points is given, as a Montgomery
+// affine point at that index of slot_points.  Kernels only (capturable).
+template <class F>
+void msm_batch_finalize_enqueue(const uint32_t* window_words, const MsmBatchPlan& plan, const MsmWorkspace& ws,
+                                uint8_t* out, uint32_t* slot_points, hipStream_t stream);
+
+// Zero `words` 32-bit words of device memory with a kernel (capturable).
+void device_fill_words(uint32_t* p, size_t words, uint32_t value, hipStream_t stream);
 
 // ---- single MSM conveniences ------------------------------------------------
 

@@ -13,6 +13,12 @@ void msm_batch_prepare(const MsmBatchPlan& plan, int xyzz_words, MsmWorkspace& w
     msm_batch_prepare_impl(plan, xyzz_words, ws, stream);
 }
 
+void device_fill_words(uint32_t* p, size_t words, uint32_t value, hipStream_t stream) {
+    if (!words) return;
+    k_fill<<<unsigned(std::min<size_t>(grid_for(words, 256), 4096)), 256, 0, stream>>>(p, words, value);
+    HIP_CHECK_LAUNCH();
+}
+
 #define DVGPU_MSM_INSTANTIATE(F)                                                                              \
     template void msm_batch_enqueue<F>(const uint32_t*, const uint32_t*, const uint32_t*, const MsmBatchPlan&, \
                                        MsmWorkspace&, hipStream_t, uint32_t*);                                \
@@ -23,7 +29,9 @@ void msm_batch_prepare(const MsmBatchPlan& plan, int xyzz_words, MsmWorkspace& w
                                   const MsmConfig&);                                                          \
     template Affine<F> msm_combine_windows<F>(const uint32_t*, const MsmPlan&);                               \
     template void points_to_mont<F>(const uint8_t*, uint32_t*, size_t, hipStream_t);                          \
-    template void affine_to_bytes<F>(const Affine<F>&, uint8_t*);
+    template void affine_to_bytes<F>(const Affine<F>&, uint8_t*);                                            \
+    template void msm_batch_finalize_enqueue<F>(const uint32_t*, const MsmBatchPlan&, const MsmWorkspace&,       \
+                                                uint8_t*, uint32_t*, hipStream_t);
 
 DVGPU_MSM_INSTANTIATE(FqOps)
 

@@ -13,7 +13,9 @@ namespace dvgpu {
                                   const MsmConfig&);                                                          \
     template Affine<F> msm_combine_windows<F>(const uint32_t*, const MsmPlan&);                               \
     template void points_to_mont<F>(const uint8_t*, uint32_t*, size_t, hipStream_t);                          \
-    template void affine_to_bytes<F>(const Affine<F>&, uint8_t*);
+    template void affine_to_bytes<F>(const Affine<F>&, uint8_t*);                                            \
+    template void msm_batch_finalize_enqueue<F>(const uint32_t*, const MsmBatchPlan&, const MsmWorkspace&,       \
+                                                uint8_t*, uint32_t*, hipStream_t);
 
 DVGPU_MSM_INSTANTIATE(Fq2Ops)
 

@@ -169,7 +169,7 @@ static __global__ void k_digits(const MsmDesc* __restrict__ descs, uint32_t m, u
 static __global__ void k_scatter(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_entries,
                                  const uint32_t* __restrict__ digits, const uint32_t* __restrict__ ranks,
                                  const uint2* __restrict__ offsets, const uint32_t* __restrict__ point_index,
-                                 uint32_t* __restrict__ sorted) {
+                                 uint32_t point_limit, uint32_t* __restrict__ flags, uint32_t* __restrict__ sorted) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total_entries) return;
     const uint32_t dig = digits[e];
@@ -177,7 +177,11 @@ static __global__ void k_scatter(const MsmDesc* __restrict__ descs, uint32_t m, 
     const MsmDesc d = descs[find_desc<&MsmDesc::entry_base>(descs, m, e)];
     const uint32_t local = e - d.entry_base, w = local / d.n, i = local % d.n;
     const uint32_t pos = offsets[d.bucket_base + w * d.buckets + (dig >> 1)].x + ranks[e];
-    const uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
+    uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
+    if (pidx >= point_limit) {  // flagged for the host; point 0 keeps the access in bounds
+        flags[0] = 1u;
+        pidx = 0;
+    }
     sorted[pos] = pidx | ((dig & 1u) << 31);
 }
 
@@ -406,6 +410,50 @@ __global__ void __launch_bounds__(256) k_window_tree(const MsmDesc* __restrict__
     if (threadIdx.x == 0) Mem<F>::store_xyzz(out + size_t(gw) * XW, acc);
 }
 
+// Horner over one MSM's window values and conversion to affine, one lane per
+// MSM (the batch's many small MSMs finish in parallel on the device).
+template <class F>
+__global__ void __launch_bounds__(64) k_finalize(const MsmDesc* __restrict__ descs, uint32_t m,
+                                                 const uint32_t* __restrict__ windows, uint8_t* __restrict__ out,
+                                                 uint32_t* __restrict__ slot_points, const uint32_t* __restrict__ gate) {
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
+    constexpr int AW = PointLayout<F>::AFFINE_WORDS;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const MsmDesc d = descs[i];
+    const uint32_t* base = windows + size_t(d.window_base) * XW;
+    Xyzz<F> acc = Mem<F>::load_xyzz(base + size_t(d.windows - 1) * XW);
+    for (int w = int(d.windows) - 2; w >= 0; --w) {
+        for (uint32_t k = 0; k < d.c; ++k) acc = xyzz_dbl<F>(acc);
+        acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(base + size_t(w) * XW));
+    }
+    const Affine<F> a = xyzz_to_affine<F>(acc);
+    Fq comp[AW / 8];
+    if constexpr (AW == 16) {
+        comp[0] = a.x;
+        comp[1] = a.y;
+    } else {
+        comp[0] = a.x.c0;
+        comp[1] = a.x.c1;
+        comp[2] = a.y.c0;
+        comp[3] = a.y.c1;
+    }
+    uint32_t* o = reinterpret_cast<uint32_t*>(out) + size_t(d.out_index) * AW;
+    // A flagged batch (an indexed point out of range) leaves every slot as it was.
+    const bool store = slot_points && d.result_slot != MSM_NO_SLOT && gate[0] == 0;
+    uint32_t* sp = store ? slot_points + size_t(d.result_slot) * AW : nullptr;
+    for (int f = 0; f < AW / 8; ++f) {
+        uint32_t cw[8] = {0}, mw[8] = {0};
+        if (!a.inf) {
+            from_mont(comp[f], cw);
+            pack(canon(comp[f]), mw);
+        }
+        for (int k = 0; k < 8; ++k) o[f * 8 + k] = cw[k];
+        if (sp)
+            for (int k = 0; k < 8; ++k) sp[f * 8 + k] = mw[k];
+    }
+}
+
 // Canonical little-endian bytes -> Montgomery words; all-zero stays zero.
 template <class F>
 __global__ void k_to_mont(const uint8_t* __restrict__ canonical, uint32_t* __restrict__ mont, uint32_t n) {
@@ -466,6 +514,8 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
         d.bucket_base = uint32_t(buckets);
         d.chunk_base = uint32_t(chunks);
         d.window_base = uint32_t(windows);
+        d.out_index = uint32_t(p.descs.size());
+        d.result_slot = MSM_NO_SLOT;
         threads += it.n;
         entries += uint64_t(d.windows) * it.n;
         buckets += uint64_t(d.windows) * d.buckets;
@@ -552,7 +602,8 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
     HIP_CHECK_LAUNCH();
     mark(2);
     k_scatter<<<grid_for(plan.total_entries, 256), 256, 0, stream>>>(descs, m, plan.total_entries, digits, ranks,
-                                                                    offsets, point_index, sorted);
+                                                                    offsets, point_index, plan.point_limit,
+                                                                    total + 3, sorted);
     HIP_CHECK_LAUNCH();
     mark(3);
     k_tasks<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(counts, offsets, plan.total_buckets, tp, tasks,
@@ -571,6 +622,16 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
     k_window_tree<F><<<plan.total_windows, 256, 0, stream>>>(descs, m, plan.chunk, chunk_vals, window_out);
     HIP_CHECK_LAUNCH();
     mark(7);
+}
+
+template <class F>
+void msm_batch_finalize_enqueue(const uint32_t* window_words, const MsmBatchPlan& plan, const MsmWorkspace& ws,
+                                uint8_t* out, uint32_t* slot_points, hipStream_t stream) {
+    const uint32_t m = uint32_t(plan.descs.size());
+    require(ws.descs.bytes >= m * sizeof(MsmDesc), "MSM workspace not prepared for this batch");
+    k_finalize<F><<<grid_for(m, 64), 64, 0, stream>>>(static_cast<const MsmDesc*>(ws.descs.ptr), m, window_words,
+                                                      out, slot_points, static_cast<const uint32_t*>(ws.total.ptr) + 3);
+    HIP_CHECK_LAUNCH();
 }
 
 template <class F>

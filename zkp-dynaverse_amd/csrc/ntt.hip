@@ -10,7 +10,9 @@
 // (and applies 1/n on the inverse) with one product per element.
 #include "ntt.hpp"
 
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 namespace dvgpu {
 
@@ -36,17 +38,16 @@ __device__ __forceinline__ void st8(uint32_t* p, const Fr& v) {
     reinterpret_cast<uint4*>(p)[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
-// One DIF pass over index bits [a, a + S) of `rows` transforms of length 2^L.
-// CANON_IN: the input holds canonical integers (converted on load).
+// One DIF pass over index bits [a, a + S) of `rows` transforms of length 2^L;
+// `g` is the tile of this block.  CANON_IN: the input holds canonical
+// integers (converted on load).
 template <bool CANON_IN>
-__global__ void __launch_bounds__(THREADS) k_dif_pass(const uint32_t* __restrict__ in,
-                                                      uint32_t* __restrict__ out, int L, int a, int S,
-                                                      int C, const uint32_t* __restrict__ tw) {
-    __shared__ uint4 lds[TILE * 2];
+__device__ __forceinline__ void dif_tile(uint4* __restrict__ lds, const uint32_t* __restrict__ in,
+                                         uint32_t* __restrict__ out, int L, int a, int S, int C,
+                                         const uint32_t* __restrict__ tw, size_t g) {
     const size_t n = size_t(1) << L;
     const uint32_t tile_elems = (1u << S) * C;
     const size_t tiles_per_row = n / tile_elems;
-    const size_t g = blockIdx.x;
     const size_t row = g / tiles_per_row;
     const size_t t = g % tiles_per_row;
     const size_t lo_groups = (size_t(1) << a) / C;
@@ -98,6 +99,14 @@ __global__ void __launch_bounds__(THREADS) k_dif_pass(const uint32_t* __restrict
     }
 }
 
+template <bool CANON_IN>
+__global__ void __launch_bounds__(THREADS) k_dif_pass(const uint32_t* __restrict__ in,
+                                                      uint32_t* __restrict__ out, int L, int a, int S,
+                                                      int C, const uint32_t* __restrict__ tw) {
+    __shared__ uint4 lds[TILE * 2];
+    dif_tile<CANON_IN>(lds, in, out, L, a, S, C, tw, blockIdx.x);
+}
+
 // out[row][k] = x[row][rev_L(k)] * scale, scale given as raw limbs: a plain
 // integer leaves Montgomery form, a Montgomery value keeps it.
 template <bool CANON_OUT>
@@ -119,6 +128,60 @@ __global__ void k_twiddles(const uint32_t* __restrict__ lo, const uint32_t* __re
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     st8(tw + size_t(k) * 8, canon(mul(ld8(lo + (k & 1023) * 8), ld8(hi + (k >> 10) * 8))));
+}
+
+// ---- batches of transforms of several lengths -------------------------------
+
+// Entry `e` of a fused batch, with its pass schedule (the same schedule as a
+// single call of that length: pass p covers bits [a_p, a_p + S_p)).
+struct NttBatchDesc {
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t* work;
+    uint32_t L, rows, passes, inverse;
+    uint32_t a[NTT_BATCH_MAX_PASSES], S[NTT_BATCH_MAX_PASSES], C[NTT_BATCH_MAX_PASSES];
+    uint32_t block_base[NTT_BATCH_MAX_PASSES];  // first block of this entry in pass p
+    uint32_t elem_base;                          // first element in the bit-reversal launch
+    uint32_t scale[8];                           // raw limbs of the output scale
+};
+
+__device__ __forceinline__ uint32_t find_entry(const NttBatchDesc* __restrict__ d, uint32_t m, int p, uint32_t key) {
+    uint32_t lo = 0, hi = m - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        const uint32_t b = p < 0 ? d[mid].elem_base : d[mid].block_base[p];
+        if (b <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool CANON_IN>
+__global__ void __launch_bounds__(THREADS) k_dif_pass_batch(const NttBatchDesc* __restrict__ descs, uint32_t m,
+                                                            int p, const uint32_t* __restrict__ tw_fwd,
+                                                            const uint32_t* __restrict__ tw_inv) {
+    __shared__ uint4 lds[TILE * 2];
+    __shared__ uint32_t entry;
+    if (threadIdx.x == 0) entry = find_entry(descs, m, p, blockIdx.x);
+    __syncthreads();
+    const NttBatchDesc& d = descs[entry];
+    dif_tile<CANON_IN>(lds, CANON_IN ? d.in : d.work, d.work, int(d.L), int(d.a[p]), int(d.S[p]), int(d.C[p]),
+                       d.inverse ? tw_inv : tw_fwd, blockIdx.x - d.block_base[p]);
+}
+
+__global__ void __launch_bounds__(THREADS) k_bitrev_batch(const NttBatchDesc* __restrict__ descs, uint32_t m,
+                                                          uint32_t total) {
+    const uint32_t i = blockIdx.x * THREADS + threadIdx.x;
+    if (i >= total) return;
+    const NttBatchDesc& d = descs[find_entry(descs, m, -1, i)];
+    const uint32_t local = i - d.elem_base;
+    const uint32_t L = d.L;
+    const size_t n = size_t(1) << L;
+    const size_t row = local >> L, k = local & (n - 1);
+    const uint32_t src = L ? (__brev(uint32_t(k)) >> (32 - L)) : 0;
+    const uint32_t* from = d.passes ? d.work : d.in;
+    const Fr scale = unpack<FrParams>(d.scale);
+    st8(d.out + size_t(local) * 8, canon(mul(ld8(from + (row * n + src) * 8), scale)));
 }
 
 Fr fr_from_words(const uint32_t (&w)[8]) { return to_mont<FrParams>(w); }
@@ -182,6 +245,34 @@ void ntt_domain_prepare(NttDomain& d, const uint32_t (&root)[8], hipStream_t str
     d.ready = true;
 }
 
+// Scale applied by the bit-reversal pass to Montgomery work values:
+// canonical output wants a plain integer (1 or n^-1); Montgomery output wants
+// the Montgomery form of the same value.
+static Fr output_scale(int L, bool inverse, bool canonical) {
+    Fr scale = fp_zero<FrParams>();
+    if (canonical) {
+        if (!inverse) {
+            scale.l[0] = 1;
+        } else {
+            uint32_t nw[8] = {0};
+            nw[L / 32] = 1u << (L % 32);
+            const Fr ninv_m = inv(to_mont<FrParams>(nw));
+            uint32_t plain[8];
+            from_mont(ninv_m, plain);
+            scale = unpack<FrParams>(plain);
+        }
+    } else {
+        if (!inverse) {
+            scale = fp_one<FrParams>();
+        } else {
+            uint32_t nw[8] = {0};
+            nw[L / 32] = 1u << (L % 32);
+            scale = canon(inv(to_mont<FrParams>(nw)));
+        }
+    }
+    return scale;
+}
+
 static void run_rows(const NttDomain& d, const uint32_t* in, uint32_t* out, int L, size_t rows,
                      bool inverse, bool canonical, NttWorkspace& ws, hipStream_t stream) {
     require(d.ready, "NTT domain not prepared");
@@ -195,7 +286,8 @@ static void run_rows(const NttDomain& d, const uint32_t* in, uint32_t* out, int 
         HIP_CHECK(hipMemcpyAsync(out, in, total * 32, hipMemcpyDeviceToDevice, stream));
         return;
     }
-    const int passes = (L + MAX_STAGES - 1) / MAX_STAGES;
+    const int max_stages = d.pass_stages[L] >= 1 && d.pass_stages[L] <= MAX_STAGES ? d.pass_stages[L] : MAX_STAGES;
+    const int passes = (L + max_stages - 1) / max_stages;
     int top = L;
     bool first = true;
     for (int p = 0; p < passes; ++p) {
@@ -212,34 +304,103 @@ static void run_rows(const NttDomain& d, const uint32_t* in, uint32_t* out, int 
         first = false;
         top = a;
     }
-    // Scale: canonical output wants a plain integer (1 or n^-1); Montgomery
-    // output wants the Montgomery form of the same value.
-    Fr scale = fp_zero<FrParams>();
-    if (canonical) {
-        if (!inverse) {
-            scale.l[0] = 1;
-        } else {
-            uint32_t nw[8] = {0};
-            nw[L / 32] = 1u << (L % 32);
-            const Fr nm = to_mont<FrParams>(nw);
-            const Fr ninv_m = inv(nm);
-            uint32_t plain[8];
-            from_mont(ninv_m, plain);
-            scale = unpack<FrParams>(plain);
-        }
-    } else {
-        if (!inverse) {
-            scale = fp_one<FrParams>();
-        } else {
-            uint32_t nw[8] = {0};
-            nw[L / 32] = 1u << (L % 32);
-            scale = canon(inv(to_mont<FrParams>(nw)));
-        }
-    }
+    const Fr scale = output_scale(L, inverse, canonical);
     if (canonical)
         k_bitrev_scale<true><<<grid_for(total, THREADS), THREADS, 0, stream>>>(work, out, L, total, scale);
     else
         k_bitrev_scale<false><<<grid_for(total, THREADS), THREADS, 0, stream>>>(work, out, L, total, scale);
+    HIP_CHECK_LAUNCH();
+}
+
+void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& entries, NttBatchPlan& plan,
+                       hipStream_t stream) {
+    require(d.ready, "NTT domain not prepared");
+    require(!entries.empty(), "empty NTT batch");
+    // Outputs may not overlap any other entry's input or output (the entries
+    // run together); an entry may transform in place.
+    for (size_t i = 0; i < entries.size(); ++i) {
+        const NttBatchEntry& x = entries[i];
+        require(x.log_n >= 0 && x.log_n <= NTT_MAX_LOG && x.rows >= 1 && x.in && x.out, "invalid NTT batch entry");
+        const uintptr_t xo = reinterpret_cast<uintptr_t>(x.out), xb = (x.rows << x.log_n) * 32;
+        for (size_t j = 0; j < entries.size(); ++j) {
+            if (i == j) continue;
+            const NttBatchEntry& y = entries[j];
+            const uintptr_t yb = (y.rows << y.log_n) * 32;
+            for (uintptr_t yp : {reinterpret_cast<uintptr_t>(y.in), reinterpret_cast<uintptr_t>(y.out)})
+                require(xo + xb <= yp || yp + yb <= xo, "NTT batch entries overlap");
+        }
+    }
+    std::vector<NttBatchDesc> descs(entries.size());
+    uint64_t work = 0, elems = 0;
+    uint32_t blocks[NTT_BATCH_MAX_PASSES] = {0};
+    int max_passes = 0;
+    for (size_t i = 0; i < entries.size(); ++i) {
+        const NttBatchEntry& x = entries[i];
+        NttBatchDesc& e = descs[i];
+        const int L = x.log_n;
+        const size_t n = size_t(1) << L;
+        int stages = d.pass_stages[L] >= 1 && d.pass_stages[L] <= MAX_STAGES ? d.pass_stages[L] : MAX_STAGES;
+        stages = std::max(stages, (L + NTT_BATCH_MAX_PASSES - 1) / NTT_BATCH_MAX_PASSES);
+        const int passes = L ? (L + stages - 1) / stages : 0;
+        e.in = x.in;
+        e.out = x.out;
+        e.L = uint32_t(L);
+        e.rows = uint32_t(x.rows);
+        e.passes = uint32_t(passes);
+        e.inverse = x.inverse ? 1u : 0u;
+        e.work = reinterpret_cast<uint32_t*>(work);  // offset until the workspace is known
+        int top = L;
+        for (int p = 0; p < NTT_BATCH_MAX_PASSES; ++p) {
+            e.block_base[p] = blocks[p];
+            if (p >= passes) continue;
+            const int remaining = passes - p;
+            const int S = (top + remaining - 1) / remaining;
+            const int a = top - S;
+            const int C = int(std::min<size_t>(size_t(1) << a, size_t(TILE >> S)));
+            e.a[p] = uint32_t(a);
+            e.S[p] = uint32_t(S);
+            e.C[p] = uint32_t(C);
+            const uint64_t nb = uint64_t(x.rows) * (n / (size_t(1u << S) * C));
+            require(uint64_t(blocks[p]) + nb < (uint64_t(1) << 31), "NTT batch too large");
+            blocks[p] += uint32_t(nb);
+            top = a;
+        }
+        max_passes = std::max(max_passes, passes);
+        e.elem_base = uint32_t(elems);
+        elems += uint64_t(x.rows) << L;
+        require(elems < (uint64_t(1) << 31), "NTT batch too large");
+        if (passes) work += uint64_t(x.rows) << L;
+        Fr scale = passes ? output_scale(L, x.inverse, true) : fp_one<FrParams>();  // length 1: identity
+        pack(scale, e.scale);
+    }
+    uint32_t* wp = plan.work.as<uint32_t>(std::max<uint64_t>(work, 1) * 8);
+    for (auto& e : descs) e.work = wp + reinterpret_cast<uintptr_t>(e.work) * 8;
+    plan.count = uint32_t(entries.size());
+    plan.max_passes = max_passes;
+    plan.total_elems = uint32_t(elems);
+    for (int p = 0; p < NTT_BATCH_MAX_PASSES; ++p) plan.pass_blocks[p] = blocks[p];
+    // Earlier work on this stream may still read the pinned descriptors.
+    HIP_CHECK(hipStreamSynchronize(stream));
+    auto* host = plan.host_descs.as<NttBatchDesc>(descs.size());
+    std::memcpy(host, descs.data(), descs.size() * sizeof(NttBatchDesc));
+    auto* dev = plan.descs.as<NttBatchDesc>(descs.size());
+    HIP_CHECK(hipMemcpyAsync(dev, host, descs.size() * sizeof(NttBatchDesc), hipMemcpyHostToDevice, stream));
+}
+
+void ntt_batch_enqueue(const NttDomain& d, const NttBatchPlan& plan, hipStream_t stream) {
+    require(d.ready && plan.count, "NTT batch not prepared");
+    const auto* descs = static_cast<const NttBatchDesc*>(plan.descs.ptr);
+    const auto* fwd = static_cast<const uint32_t*>(d.fwd.ptr);
+    const auto* inv_tw = static_cast<const uint32_t*>(d.inv.ptr);
+    for (int p = 0; p < plan.max_passes; ++p) {
+        if (!plan.pass_blocks[p]) continue;
+        if (p == 0)
+            k_dif_pass_batch<true><<<plan.pass_blocks[p], THREADS, 0, stream>>>(descs, plan.count, p, fwd, inv_tw);
+        else
+            k_dif_pass_batch<false><<<plan.pass_blocks[p], THREADS, 0, stream>>>(descs, plan.count, p, fwd, inv_tw);
+        HIP_CHECK_LAUNCH();
+    }
+    k_bitrev_batch<<<grid_for(plan.total_elems, THREADS), THREADS, 0, stream>>>(descs, plan.count, plan.total_elems);
     HIP_CHECK_LAUNCH();
 }
 

@@ -1,5 +1,7 @@
 // Natural-order BN254 Fr NTT / INTT on gfx950.
 #pragma once
+#include <vector>
+
 #include "common.hpp"
 #include "field.hpp"
 
@@ -13,6 +15,8 @@ struct NttDomain {
     uint32_t root_words[8] = {0};  // canonical root this domain was built from
     bool ready = false;
     DeviceBuffer fwd, inv;         // 2^19 x 8 words each
+    // Butterfly stages per LDS pass by log2 length (0 = the maximum, 11).
+    int pass_stages[NTT_MAX_LOG + 1] = {0};
     void release() {
         fwd.release();
         inv.release();
@@ -42,5 +46,39 @@ void ntt_rows(const NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, 
 // Same, but input and output in Montgomery memory form (resident data).
 void ntt_rows_mont(const NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows,
                    bool inverse, NttWorkspace& ws, hipStream_t stream);
+
+// ---- fused batches of transforms of several lengths --------------------------
+// Every entry (rows x 2^log_n, canonical in and out) runs in the same launch
+// sequence: one launch per pass for all entries, then one bit-reversal
+// launch.  Entries must not overlap each other (each may run in place).
+constexpr int NTT_BATCH_MAX_PASSES = 6;
+
+struct NttBatchEntry {
+    const uint32_t* in;
+    uint32_t* out;
+    int log_n;
+    size_t rows;
+    bool inverse;
+};
+
+struct NttBatchPlan {
+    uint32_t count = 0, total_elems = 0;
+    int max_passes = 0;
+    uint32_t pass_blocks[NTT_BATCH_MAX_PASSES] = {0};
+    DeviceBuffer descs, work;
+    HostBuffer host_descs;
+    void attach(Quota* q) { descs.quota = work.quota = q; }
+    void release() {
+        descs.release();
+        work.release();
+        host_descs.release();
+    }
+};
+
+// Plan, size the workspace and upload the descriptors (synchronises the
+// stream first).  Must precede ntt_batch_enqueue; capture-safe afterwards.
+void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& entries, NttBatchPlan& plan,
+                       hipStream_t stream);
+void ntt_batch_enqueue(const NttDomain& d, const NttBatchPlan& plan, hipStream_t stream);
 
 }  // namespace dvgpu

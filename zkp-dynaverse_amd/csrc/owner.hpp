@@ -33,6 +33,7 @@ struct Owner {
     MsmWorkspace msm;
     NttWorkspace ntt;
     NttDomain domain;
+    NttBatchPlan ntt_batch;                  // fused NTT batches (dynaverse_hip_ext.h)
     DeviceBuffer stage_a, stage_b, stage_c;  // per-call device inputs/outputs
     HostBuffer pinned;                       // per-call host staging
     DvgpuBatchStats batch_stats{};
@@ -52,8 +53,10 @@ struct Owner {
     std::unordered_map<void*, size_t> ext_allocations;
     // Batches created through dynaverse_hip_ext.h (freed with the owner).
     std::vector<BatchObj*> batches;
-    // Measured window width per log2 MSM size (dvgpu_ext_set_window_table).
+    // Measured window width per log2 MSM size (dvgpu_ext_set_tuning); the
+    // NTT stages per pass live in `domain`, the task shape in msm_config.
     WindowTable window_table;
+    int msm_finalize_min = 0;  // 0 = built-in threshold for device-side window sums
 
     Owner();
     ~Owner();

@@ -877,6 +877,7 @@ int dvgpu_resident_configure(void* backend, uint64_t limit, uint64_t journal, co
             s->ntt.attach(&o->quota);
             for (DeviceBuffer* b : {&s->affine_ws, &s->scalar_ws, &s->g2_ws, &s->host_stage_dev}) b->quota = &o->quota;
             s->quota(journal, 0);
+            std::memcpy(s->domain.pass_stages, o->domain.pass_stages, sizeof(s->domain.pass_stages));
             ntt_domain_prepare(s->domain, rw, o->stream);
             s->stats.bytes_allocated = DOMAIN_BYTES;
             s->diag.resident_allocations = 2;

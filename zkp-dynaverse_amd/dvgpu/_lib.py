@@ -112,6 +112,12 @@ class DvgpuG2KernelResources(ctypes.Structure):
                 ("normalize_local_bytes", ctypes.c_uint64), ("normalize_shared_bytes", ctypes.c_uint64)]
 
 
+class DvgpuTuning(ctypes.Structure):
+    _fields_ = [("msm_window_bits", ctypes.c_int32 * 25), ("ntt_pass_stages", ctypes.c_int32 * 21),
+                ("msm_task_points", ctypes.c_int32), ("msm_chunk", ctypes.c_int32),
+                ("msm_device_finalize_min", ctypes.c_int32)]
+
+
 class DvgpuMsmItem(ctypes.Structure):
     _fields_ = [("n", ctypes.c_uint64), ("point_base", ctypes.c_uint32), ("scalar_base", ctypes.c_uint32),
                 ("window_bits", ctypes.c_int32), ("reserved", ctypes.c_int32)]
@@ -190,9 +196,15 @@ SIGNATURES = {
     "dvgpu_ext_copy_to_host": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, *_E]),
     "dvgpu_ext_stream": (_I, [c_void_p, ctypes.POINTER(c_void_p)]),
     "dvgpu_ext_device_copy": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, *_E]),
-    "dvgpu_ext_set_window_table": (_I, [c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "dvgpu_ext_set_tuning": (_I, [c_void_p, ctypes.POINTER(DvgpuTuning)]),
+    "dvgpu_ext_get_tuning": (_I, [c_void_p, ctypes.POINTER(DvgpuTuning)]),
     "dvgpu_ext_msm_batch_device": (_I, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(DvgpuMsmItem),
                                         c_size_t, _I, c_void_p, ctypes.POINTER(ctypes.c_float), *_E]),
+    "dvgpu_ext_msm_batch_finalize": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t,
+                                          ctypes.POINTER(DvgpuMsmItem), c_size_t, _I, c_void_p, c_void_p,
+                                          ctypes.POINTER(ctypes.c_uint32), c_void_p, c_size_t, *_E]),
+    "dvgpu_ext_ntt_batch_device": (_I, [c_void_p, ctypes.POINTER(DvgpuNttRows), c_size_t, c_void_p,
+                                        ctypes.POINTER(ctypes.c_float), *_E]),
     "dvgpu_ext_batch_create": (_I, [c_void_p, ctypes.POINTER(DvgpuBatchSpec), _I, ctypes.POINTER(c_void_p), *_E]),
     "dvgpu_ext_batch_run": (_I, [c_void_p, c_void_p, c_void_p, c_void_p,
                                  ctypes.POINTER(DvgpuG2ValidationReport), ctypes.POINTER(ctypes.c_float), *_E]),

@@ -1,7 +1,7 @@
 """Batching form: many uneven MSMs (G1, G2), NTT row batches and G2
 validation as one launch sequence, optionally one hipGraph replay.
 
-Window widths come from the measured table (``autotune``) unless given per
+Window widths come from the tuning table (``autotune``) unless given per
 item.  All inputs are device-resident (DeviceArray); results return to the
 host in the canonical encoding.
 """
@@ -20,6 +20,10 @@ class MsmItem:
     point_base: int = 0
     scalar_base: int = 0
     window_bits: int = 0
+
+
+def _addr(x) -> int:
+    return x.ptr if isinstance(x, DeviceArray) else int(x)
 
 
 def _items(items: Sequence[MsmItem]):
@@ -44,19 +48,50 @@ def msm_batch_device(backend: HipBackend, points: DeviceArray, scalars: DeviceAr
     return (res, list(ph)) if phases else res
 
 
-def set_window_table(backend: HipBackend, table: Sequence[int]) -> None:
-    arr = (ctypes.c_int32 * 25)(*([int(x) for x in table] + [0] * (25 - len(table)))[:25])
-    if backend.lib.dvgpu_ext_set_window_table(backend.handle, arr) != 0:
-        raise BackendError(-1, "invalid window table")
+def msm_batch_finalize(backend: HipBackend, points: DeviceArray, scalars: DeviceArray, items: Sequence[MsmItem],
+                       g2: bool = False, index: Optional[DeviceArray] = None,
+                       results: Optional[DeviceArray] = None, slots: Optional[Sequence[int]] = None,
+                       slot_points: Optional[DeviceArray] = None, host: bool = True):
+    """Window sums finished on the device.  `slots[i]` (or NO_SLOT) names the
+    point of `slot_points` that MSM i's result replaces.  Returns the
+    canonical results (host) when `host`."""
+    arr = _items(items)
+    width = 128 if g2 else 64
+    aw = width
+    out = ctypes.create_string_buffer(max(1, width * len(items))) if host else None
+    sl = (ctypes.c_uint32 * len(items))(*slots) if slots is not None else None
+    err = _err()
+    _check(backend.lib.dvgpu_ext_msm_batch_finalize(
+        backend.handle, points.ptr, points.nbytes // aw, index.ptr if index else None,
+        index.nbytes // 4 if index else 0, scalars.ptr, scalars.nbytes // 32, arr, len(items), int(g2),
+        results.ptr if results else None, out, sl, slot_points.ptr if slot_points else None,
+        slot_points.nbytes // aw if slot_points else 0, err, 256), err)
+    return [out.raw[width * i:width * (i + 1)] for i in range(len(items))] if host else None
+
+
+NO_SLOT = 0xFFFFFFFF
 
 
 @dataclass
 class NttRows:
-    inp: DeviceArray
-    out: DeviceArray
+    inp: object          # DeviceArray or a device address
+    out: object
     length: int
     rows: int
     inverse: bool = False
+
+
+def ntt_batch_device(backend: HipBackend, rows: Sequence[NttRows], timed: bool = False):
+    """All row batches in one fused launch sequence (entries must not overlap)."""
+    arr = (_lib.DvgpuNttRows * len(rows))()
+    for i, r in enumerate(rows):
+        arr[i] = _lib.DvgpuNttRows(_addr(r.inp), _addr(r.out), r.length, r.rows, int(r.inverse), 0)
+    root = ctypes.create_string_buffer(encode_fr(root_of_unity_2_20()))
+    ms = ctypes.c_float(0)
+    err = _err()
+    _check(backend.lib.dvgpu_ext_ntt_batch_device(backend.handle, arr, len(rows), root,
+                                                  ctypes.byref(ms) if timed else None, err, 256), err)
+    return ms.value if timed else None
 
 
 class DeviceBatch:
@@ -91,7 +126,7 @@ class DeviceBatch:
         if ntt:
             rows = (_lib.DvgpuNttRows * len(ntt))()
             for i, r in enumerate(ntt):
-                rows[i] = _lib.DvgpuNttRows(r.inp.ptr, r.out.ptr, r.length, r.rows, int(r.inverse), 0)
+                rows[i] = _lib.DvgpuNttRows(_addr(r.inp), _addr(r.out), r.length, r.rows, int(r.inverse), 0)
             self._keep.append(rows)
             spec.ntt, spec.ntt_count = rows, len(ntt)
         root = ctypes.create_string_buffer(encode_fr(root_of_unity_2_20()))
