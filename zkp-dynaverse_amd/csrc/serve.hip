@@ -1,0 +1,284 @@
+// Serving form: one call applies a batch of sparse updates to device-resident
+// session states and regenerates each session's commitment, opening and
+// evaluations.
+//
+// A session of size n = 2^L keeps, in two caller-owned device arenas:
+//   points: basis P_0..P_{n-1}, opening basis U_0..U_{n-1} (Montgomery affine)
+//           and two cached points, the commitment C and the opening W;
+//   fr:     coefficients f (canonical), evaluations NTT(f), the linear-term
+//           vector l (the opening functional, l_i = z^i) and the cached value
+//           y = <l, f>.
+// An update {(i_j, d_j)} becomes, for every session of the batch at once:
+//   C <- C + sum_j d_j P_{i_j}     (MSM over {P_{i_j}} u {C}, scalars {d_j} u {1})
+//   W <- W + sum_j d_j U_{i_j}     (same scalars over {U_{i_j}} u {W})
+//   f_{i_j} += d_j,  y += sum_j d_j l_{i_j},  evaluations <- NTT(f).
+// The two MSMs per session store their results back into the C and W slots
+// after every MSM of the batch has read its inputs (device finalize with
+// result slots), so no cached point travels through the host.
+#include <algorithm>
+#include <cstring>
+#include <iterator>
+#include <vector>
+
+#include "../../include/dynaverse_hip_ext.h"
+#include "owner.hpp"
+
+namespace dvgpu {
+
+using namespace bn254;
+
+namespace {
+
+Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
+
+// Per update, as the kernels read it.
+struct UpdDesc {
+    uint32_t entry_base;   // first changed entry of this update in the batch
+    uint32_t k;
+    uint32_t scalar_base;  // first delta in the batch's scalar array
+    uint32_t pad;
+    uint64_t coeff_offset, linear_offset, value_slot;
+};
+
+__device__ __forceinline__ uint32_t find_update(const UpdDesc* __restrict__ d, uint32_t m, uint32_t e) {
+    uint32_t lo = 0, hi = m - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (d[mid].entry_base <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ Fr ld_fr(const uint32_t* p) {
+    const uint4 a = reinterpret_cast<const uint4*>(p)[0];
+    const uint4 b = reinterpret_cast<const uint4*>(p)[1];
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    return unpack<FrParams>(w);
+}
+
+__device__ __forceinline__ void st_fr(uint32_t* p, const Fr& v) {
+    uint32_t w[8];
+    pack(canon(v), w);
+    reinterpret_cast<uint4*>(p)[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    reinterpret_cast<uint4*>(p)[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// f[coeff_offset + i_j] += d_j (canonical in, canonical out); indices of one
+// update are distinct and updates own disjoint coefficient ranges.
+__global__ void k_scatter_add(const UpdDesc* __restrict__ descs, uint32_t m, uint32_t entries,
+                              const uint32_t* __restrict__ local_index, const uint32_t* __restrict__ scalars,
+                              uint32_t* __restrict__ fr) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= entries) return;
+    const UpdDesc d = descs[find_update(descs, m, e)];
+    const uint32_t j = e - d.entry_base;
+    uint32_t* slot = fr + (d.coeff_offset + local_index[e]) * 8;
+    st_fr(slot, add(ld_fr(slot), ld_fr(scalars + size_t(d.scalar_base + j) * 8)));
+}
+
+// One block per update: y += sum_j d_j l_{i_j}; the new y goes to the value
+// slot and to out[r].
+__global__ void __launch_bounds__(64) k_linear_dot(const UpdDesc* __restrict__ descs,
+                                                   const uint32_t* __restrict__ local_index,
+                                                   const uint32_t* __restrict__ scalars, uint32_t* __restrict__ fr,
+                                                   uint32_t* __restrict__ out) {
+    __shared__ uint32_t part[64 * 8];
+    const UpdDesc d = descs[blockIdx.x];
+    Fr acc = fp_zero<FrParams>();
+    for (uint32_t j = threadIdx.x; j < d.k; j += 64) {
+        uint32_t dw[8];
+        const uint32_t* s = scalars + size_t(d.scalar_base + j) * 8;
+        for (int t = 0; t < 8; ++t) dw[t] = s[t];
+        const Fr l = ld_fr(fr + (d.linear_offset + local_index[d.entry_base + j]) * 8);
+        acc = add(acc, mul(to_mont<FrParams>(dw), l));  // (d R) l R^-1 = d l
+    }
+    uint32_t w[8];
+    pack(canon(acc), w);
+    for (int t = 0; t < 8; ++t) part[threadIdx.x * 8 + t] = w[t];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Fr y = ld_fr(fr + d.value_slot * 8);
+        for (int t = 0; t < 64; ++t) y = add(y, ld_fr(part + t * 8));
+        st_fr(fr + d.value_slot * 8, y);
+        st_fr(out + size_t(blockIdx.x) * 8, y);
+    }
+}
+
+struct Interval {
+    uint64_t lo, hi;  // [lo, hi)
+};
+
+// Sorts `v` by start; true if any two of its intervals intersect.
+bool sort_and_overlap(std::vector<Interval>& v) {
+    std::sort(v.begin(), v.end(), [](const Interval& a, const Interval& b) { return a.lo < b.lo; });
+    for (size_t i = 1; i < v.size(); ++i)
+        if (v[i].lo < v[i - 1].hi) return true;
+    return false;
+}
+
+// True if [lo, hi) meets one of the disjoint intervals of `sorted`.
+bool hits(const std::vector<Interval>& sorted, uint64_t lo, uint64_t hi) {
+    auto it = std::upper_bound(sorted.begin(), sorted.end(), lo,
+                               [](uint64_t x, const Interval& s) { return x < s.lo; });
+    if (it != sorted.begin() && std::prev(it)->hi > lo) return true;  // starts at or before lo
+    return it != sorted.end() && it->lo < hi;                         // starts inside [lo, hi)
+}
+
+}  // namespace
+}  // namespace dvgpu
+
+using namespace dvgpu;
+
+extern "C" {
+
+int dvgpu_ext_update_batch(void* owner, void* points, size_t point_count, void* fr, size_t fr_count,
+                           const DvgpuUpdate* updates, size_t count, const uint8_t* root, uint8_t* results,
+                           float* device_ms, char* error, size_t capacity) {
+    Owner* o = as_owner(owner);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(points && fr && updates && count && root && results && point_count <= 0xffffffffull &&
+                    count <= (1u << 20),
+                "invalid update batch arguments");
+        uint32_t rw[8];
+        std::memcpy(rw, root, 32);
+        require(ntt_root_has_order_2_20(rw), "NTT root must have order 2^20");
+        // ---- validation: ranges, distinct indices, no two updates sharing state
+        std::vector<Interval> written_fr, read_fr, written_pts, read_pts;
+        uint64_t entries = 0, scalars = 0;
+        for (size_t r = 0; r < count; ++r) {
+            const DvgpuUpdate& u = updates[r];
+            require(u.log_n <= uint32_t(NTT_MAX_LOG), "session size must be at most 2^20");
+            const uint64_t n = uint64_t(1) << u.log_n;
+            require(u.k <= n && (!u.k || (u.indices && u.deltas)), "invalid update entries");
+            require(uint64_t(u.srs_base) + n <= point_count && uint64_t(u.opening_base) + n <= point_count &&
+                        u.commitment_slot < point_count && u.opening_slot < point_count &&
+                        u.commitment_slot != u.opening_slot,
+                    "session points out of range");
+            require(u.coeff_offset + n <= fr_count && u.eval_offset + n <= fr_count &&
+                        u.linear_offset + n <= fr_count && u.value_slot < fr_count,
+                    "session field elements out of range");
+            std::vector<uint32_t> idx(u.indices, u.indices + u.k);
+            std::sort(idx.begin(), idx.end());
+            for (size_t j = 0; j < idx.size(); ++j)
+                require(idx[j] < n && (j == 0 || idx[j] != idx[j - 1]), "update indices must be distinct and < n");
+            written_fr.push_back({u.coeff_offset, u.coeff_offset + n});
+            if (u.eval_offset != u.coeff_offset) written_fr.push_back({u.eval_offset, u.eval_offset + n});
+            written_fr.push_back({u.value_slot, u.value_slot + 1});
+            read_fr.push_back({u.linear_offset, u.linear_offset + n});
+            written_pts.push_back({u.commitment_slot, uint64_t(u.commitment_slot) + 1});
+            written_pts.push_back({u.opening_slot, uint64_t(u.opening_slot) + 1});
+            read_pts.push_back({u.srs_base, u.srs_base + n});
+            read_pts.push_back({u.opening_base, u.opening_base + n});
+            entries += u.k;
+            scalars += u.k + 1;
+        }
+        require(!sort_and_overlap(written_fr) && !sort_and_overlap(written_pts),
+                "updates of one batch share session state");
+        for (const Interval& x : read_fr) require(!hits(written_fr, x.lo, x.hi), "linear terms overlap written state");
+        for (const Interval& x : read_pts) require(!hits(written_pts, x.lo, x.hi), "cached points overlap a basis");
+        require(entries < (1u << 30) && scalars < (1u << 30), "update batch too large");
+
+        // ---- host packing: [point index][local index][descs][scalars]
+        const size_t m = count;
+        const size_t idx_words = 2 * scalars, loc_words = entries;
+        const size_t desc_words = m * sizeof(UpdDesc) / 4, sc_words = scalars * 8;
+        const size_t off_loc = idx_words, off_desc = off_loc + loc_words;
+        const size_t off_desc_al = (off_desc + 3) & ~size_t(3);  // 16-byte alignment for descs and scalars
+        const size_t off_sc = (off_desc_al + desc_words + 3) & ~size_t(3);
+        const size_t total_words = off_sc + sc_words;
+        uint32_t* pin = o->pinned.as<uint32_t>(total_words);
+        std::vector<MsmItem> items;
+        items.reserve(2 * m);
+        std::vector<NttBatchEntry> ntt;
+        uint32_t* pidx = pin;
+        uint32_t* ploc = pin + off_loc;
+        auto* pdesc = reinterpret_cast<UpdDesc*>(pin + off_desc_al);
+        uint32_t* psc = pin + off_sc;
+        uint32_t ipos = 0, epos = 0, spos = 0;
+        uint32_t* frw = static_cast<uint32_t*>(fr);
+        for (size_t r = 0; r < m; ++r) {
+            const DvgpuUpdate& u = updates[r];
+            const uint32_t k = u.k;
+            // MSM over the basis, then over the opening basis; both end with the cached point.
+            for (uint32_t j = 0; j < k; ++j) pidx[ipos + j] = u.srs_base + u.indices[j];
+            pidx[ipos + k] = u.commitment_slot;
+            for (uint32_t j = 0; j < k; ++j) pidx[ipos + k + 1 + j] = u.opening_base + u.indices[j];
+            pidx[ipos + 2 * k + 1] = u.opening_slot;
+            items.push_back({size_t(k) + 1, ipos, spos, 0});
+            items.push_back({size_t(k) + 1, ipos + k + 1, spos, 0});
+            std::memcpy(ploc + epos, u.indices, size_t(k) * 4);
+            std::memcpy(psc + size_t(spos) * 8, u.deltas, size_t(k) * 32);
+            uint32_t one[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+            std::memcpy(psc + size_t(spos + k) * 8, one, 32);
+            pdesc[r] = {epos, k, spos, 0, u.coeff_offset, u.linear_offset, u.value_slot};
+            ntt.push_back({frw + u.coeff_offset * 8, frw + u.eval_offset * 8, int(u.log_n), 1, false});
+            ipos += 2 * (k + 1);
+            epos += k;
+            spos += k + 1;
+        }
+        for (uint32_t s = 0; s < spos; ++s)
+            require(is_canonical<FrParams>(*reinterpret_cast<const uint32_t(*)[8]>(psc + size_t(s) * 8)),
+                    "update deltas must be canonical Fr");
+
+        hipStream_t st = o->stream;
+        uint32_t* dev = o->stage_a.as<uint32_t>(total_words);
+        HIP_CHECK(hipMemcpyAsync(dev, pin, total_words * 4, hipMemcpyHostToDevice, st));
+        // Plans (each uploads its descriptors after synchronising the stream).
+        MsmBatchPlan plan = msm_batch_plan(items, o->msm_config, &o->window_table);
+        plan.point_limit = uint32_t(point_count);
+        for (size_t r = 0; r < m; ++r) {
+            plan.descs[2 * r].result_slot = updates[r].commitment_slot;
+            plan.descs[2 * r + 1].result_slot = updates[r].opening_slot;
+        }
+        msm_batch_prepare(plan, PointLayout<FqOps>::XYZZ_WORDS, o->msm, st);
+        ntt_domain_prepare(o->domain, rw, st);
+        ntt_batch_prepare(o->domain, ntt, o->ntt_batch, st);
+        uint32_t* win = o->msm.windows.as<uint32_t>(size_t(plan.total_windows) * PointLayout<FqOps>::XYZZ_WORDS);
+        uint8_t* fin = o->msm.finals.as<uint8_t>(2 * m * 64);
+        uint32_t* yout = o->stage_b.as<uint32_t>(m * 8);
+        const uint32_t* d_idx = dev;
+        const uint32_t* d_loc = dev + off_loc;
+        const auto* d_desc = reinterpret_cast<const UpdDesc*>(dev + off_desc_al);
+        const uint32_t* d_sc = dev + off_sc;
+
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (device_ms) {
+            HIP_CHECK(hipEventCreate(&e0));
+            HIP_CHECK(hipEventCreate(&e1));
+            HIP_CHECK(hipEventRecord(e0, st));
+        }
+        if (entries) {
+            k_scatter_add<<<grid_for(entries, 256), 256, 0, st>>>(d_desc, uint32_t(m), uint32_t(entries), d_loc, d_sc,
+                                                                  frw);
+            HIP_CHECK_LAUNCH();
+        }
+        k_linear_dot<<<unsigned(m), 64, 0, st>>>(d_desc, d_loc, d_sc, frw, yout);
+        HIP_CHECK_LAUNCH();
+        msm_batch_enqueue<FqOps>(static_cast<const uint32_t*>(points), d_idx, d_sc, plan, o->msm, st, win);
+        msm_batch_finalize_enqueue<FqOps>(win, plan, o->msm, fin, static_cast<uint32_t*>(points), st);
+        ntt_batch_enqueue(o->domain, o->ntt_batch, st);
+        if (device_ms) HIP_CHECK(hipEventRecord(e1, st));
+        uint8_t* host = o->msm.host_windows.as<uint8_t>(m * 160 + 4);
+        HIP_CHECK(hipMemcpyAsync(host, fin, 2 * m * 64, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(host + 2 * m * 64, yout, m * 32, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(host + m * 160, static_cast<uint32_t*>(o->msm.total.ptr) + 3, 4,
+                                 hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        if (device_ms) {
+            HIP_CHECK(hipEventElapsedTime(device_ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        uint32_t flag = 0;
+        std::memcpy(&flag, host + m * 160, 4);
+        require(flag == 0, "MSM point index out of range");  // not reachable after the range checks
+        for (size_t r = 0; r < m; ++r) {
+            std::memcpy(results + r * 160, host + r * 128, 128);
+            std::memcpy(results + r * 160 + 128, host + 2 * m * 64 + r * 32, 32);
+        }
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -139,6 +139,15 @@ class DvgpuBatchSpec(ctypes.Structure):
                 ("g2_masks", c_void_p), ("root", c_void_p)]
 
 
+class DvgpuUpdate(ctypes.Structure):
+    _fields_ = [("srs_base", ctypes.c_uint32), ("opening_base", ctypes.c_uint32),
+                ("commitment_slot", ctypes.c_uint32), ("opening_slot", ctypes.c_uint32),
+                ("coeff_offset", ctypes.c_uint64), ("eval_offset", ctypes.c_uint64),
+                ("linear_offset", ctypes.c_uint64), ("value_slot", ctypes.c_uint64),
+                ("log_n", ctypes.c_uint32), ("k", ctypes.c_uint32),
+                ("indices", c_void_p), ("deltas", c_void_p)]
+
+
 _I = ctypes.c_int
 _E = (ctypes.c_char_p, c_size_t)  # error, capacity
 
@@ -205,6 +214,8 @@ SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_uint32), c_void_p, c_size_t, *_E]),
     "dvgpu_ext_ntt_batch_device": (_I, [c_void_p, ctypes.POINTER(DvgpuNttRows), c_size_t, c_void_p,
                                         ctypes.POINTER(ctypes.c_float), *_E]),
+    "dvgpu_ext_update_batch": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, ctypes.POINTER(DvgpuUpdate),
+                                    c_size_t, c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float), *_E]),
     "dvgpu_ext_batch_create": (_I, [c_void_p, ctypes.POINTER(DvgpuBatchSpec), _I, ctypes.POINTER(c_void_p), *_E]),
     "dvgpu_ext_batch_run": (_I, [c_void_p, c_void_p, c_void_p, c_void_p,
                                  ctypes.POINTER(DvgpuG2ValidationReport), ctypes.POINTER(ctypes.c_float), *_E]),

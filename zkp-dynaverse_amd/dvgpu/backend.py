@@ -92,6 +92,28 @@ class DeviceArray:
                                                        self.ptr, n, err, 256), err)
         return out
 
+    def upload_at(self, offset: int, arr) -> None:
+        """Copy host bytes to [offset, offset + len) of this allocation."""
+        arr = np.ascontiguousarray(np.frombuffer(bytes(arr), dtype=np.uint8) if not isinstance(arr, np.ndarray)
+                                   else arr)
+        if offset < 0 or offset + arr.nbytes > self.nbytes:
+            raise ValueError("upload outside the allocation")
+        if not arr.nbytes:
+            return
+        err = _err()
+        _check(self.backend.lib.dvgpu_ext_copy_to_device(self.backend.handle, self.ptr + offset,
+                                                         ctypes.c_void_p(arr.ctypes.data), arr.nbytes,
+                                                         err, 256), err)
+
+    def download_at(self, offset: int, nbytes: int) -> np.ndarray:
+        if offset < 0 or offset + nbytes > self.nbytes:
+            raise ValueError("download outside the allocation")
+        out = np.empty(nbytes, dtype=np.uint8)
+        err = _err()
+        _check(self.backend.lib.dvgpu_ext_copy_to_host(self.backend.handle, ctypes.c_void_p(out.ctypes.data),
+                                                       self.ptr + offset, nbytes, err, 256), err)
+        return out
+
     def free(self) -> None:
         if self.ptr and self.backend.handle is not None and self.backend.handle.value:
             self.backend.lib.dvgpu_ext_free(self.backend.handle, self.ptr)

@@ -1,0 +1,479 @@
+"""Serving form: a request loop over device-resident session states with
+continuous batching and HBM-budget admission.
+
+A session (one Dynaverse state of size n) keeps everything an update needs in
+HBM, inside two arenas carved out of a fixed memory budget:
+
+* its own seeded SRS basis P_i = [tau^i] g1 and opening basis
+  U_i = [(tau^i - z^i) / (tau - z)] g1 (frozen, generated on the device by the
+  fixed-base kernel);
+* the cached commitment C = [f(tau)] g1 and opening W = [q(tau)] g1 with
+  q = (f - f(z)) / (X - z), stored as two points next to the bases;
+* coefficients f, evaluations NTT(f), the frozen linear term l_i = z^i and the
+  cached value y = f(z).
+
+An update request changes k coefficients; its proof is (C', W', y'), which
+satisfies the KZG opening relation f'(tau) - y' = q'(tau) (tau - z).  One
+``dvgpu_ext_update_batch`` call serves a whole batch of requests (two sparse
+MSMs per request with in-place result slots, scatter-add, linear term, a
+fused multi-length NTT).  The ``Server`` keeps ``lanes`` such calls in flight,
+each on its own owner and stream; a request that arrives while others run
+joins the next batch that a free lane takes (continuous batching).  Requests
+of one session are applied in arrival order, one at a time.
+
+``PlainSession`` is the one-call-at-a-time baseline through the reference's
+operator API (``msm_batch`` over host-gathered bases, ``ntt`` on host data),
+with the same state held on the host.  Both paths produce identical bytes.
+"""
+import collections
+import ctypes
+import threading
+import time
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from .backend import BackendError, DeviceArray, HipBackend, _check, _err
+from .codec import FR_MODULUS, encode_fr, root_of_unity_2_20
+from .rng import ChaCha20Stream
+
+R = FR_MODULUS
+PROOF_BYTES = 160   # C' (64) || W' (64) || y' (32)
+_ROOT = encode_fr(root_of_unity_2_20())
+
+
+def _le_rows(vals: Sequence[int]) -> np.ndarray:
+    return np.frombuffer(b"".join(int(v).to_bytes(32, "little") for v in vals), dtype=np.uint8).reshape(-1, 32)
+
+
+@dataclass
+class SessionSpec:
+    """Seeded definition of one session: SRS trapdoor tau, opening point
+    z = w_n^t (a domain point, so y = NTT(f)[t]) and initial coefficients."""
+    sid: int
+    n: int
+    seed: int
+
+    def derive(self):
+        rng = ChaCha20Stream(self.seed)
+        tau = rng.nonzero_fr()
+        t = rng.below(self.n)
+        w_n = pow(root_of_unity_2_20(), (1 << 20) // self.n, R)
+        z = pow(w_n, t, R)
+        coeffs = rng.uniform_fr_array(self.n)
+        return tau, t, z, coeffs
+
+
+def session_scalars(tau: int, z: int, n: int):
+    """(tau^i, (tau^i - z^i)/(tau - z), z^i) for i < n as (n, 32) arrays."""
+    inv = pow((tau - z) % R, R - 2, R)
+    tp, zp = 1, 1
+    a, b, c = [], [], []
+    for _ in range(n):
+        a.append(tp)
+        b.append((tp - zp) * inv % R)
+        c.append(zp)
+        tp = tp * tau % R
+        zp = zp * z % R
+    return _le_rows(a), _le_rows(b), _le_rows(c)
+
+
+class RangeAllocator:
+    """First-fit allocator of [0, capacity) (unit-agnostic)."""
+
+    def __init__(self, capacity: int):
+        self.capacity = capacity
+        self.free: List[Tuple[int, int]] = [(0, capacity)]     # sorted (start, size)
+
+    def alloc(self, size: int) -> Optional[int]:
+        for i, (s, z) in enumerate(self.free):
+            if z >= size:
+                if z == size:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (s + size, z - size)
+                return s
+        return None
+
+    def release(self, start: int, size: int) -> None:
+        self.free.append((start, size))
+        self.free.sort()
+        merged = []
+        for s, z in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == s:
+                merged[-1] = (merged[-1][0], merged[-1][1] + z)
+            else:
+                merged.append((s, z))
+        self.free = merged
+
+    @property
+    def free_units(self) -> int:
+        return sum(z for _, z in self.free)
+
+
+@dataclass
+class Session:
+    spec: SessionSpec
+    log_n: int
+    tau: int
+    t: int
+    z: int
+    pt_base: int          # points: [P (n)] [U (n)] [C] [W]
+    fr_base: int          # fr: [f (n)] [evals (n)] [l (n)] [y]
+    busy: bool = False
+
+    @property
+    def n(self):
+        return self.spec.n
+
+    def update_struct(self, idx: np.ndarray, deltas: bytes) -> _lib.DvgpuUpdate:
+        n = self.n
+        u = _lib.DvgpuUpdate()
+        u.srs_base, u.opening_base = self.pt_base, self.pt_base + n
+        u.commitment_slot, u.opening_slot = self.pt_base + 2 * n, self.pt_base + 2 * n + 1
+        u.coeff_offset, u.eval_offset = self.fr_base, self.fr_base + n
+        u.linear_offset, u.value_slot = self.fr_base + 2 * n, self.fr_base + 3 * n
+        u.log_n, u.k = self.log_n, len(idx)
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        u.indices = idx.ctypes.data if len(idx) else None
+        u.deltas = ctypes.cast(ctypes.c_char_p(deltas), ctypes.c_void_p).value if len(idx) else None
+        u._keep = (idx, deltas)       # the host arrays live as long as the struct
+        return u
+
+
+def session_points(n: int) -> int:
+    return 2 * n + 2
+
+
+def session_fr(n: int) -> int:
+    return 3 * n + 1
+
+
+def session_bytes(n: int) -> int:
+    return 64 * session_points(n) + 32 * session_fr(n)
+
+
+def merge_update(indices: Sequence[int], deltas: Sequence[int]) -> Tuple[np.ndarray, bytes, List[int]]:
+    """Repeated indices add their deltas: returns sorted distinct indices,
+    their canonical summed deltas (bytes) and the summed deltas (ints)."""
+    acc: Dict[int, int] = {}
+    for i, d in zip(indices, deltas):
+        acc[int(i)] = (acc.get(int(i), 0) + int(d)) % R
+    keys = sorted(acc)
+    vals = [acc[k] for k in keys]
+    return np.array(keys, dtype=np.uint32), b"".join(v.to_bytes(32, "little") for v in vals), vals
+
+
+class SessionStore:
+    """Session states in two HBM arenas under a fixed byte budget."""
+
+    def __init__(self, backend: HipBackend, budget_bytes: int, point_share: float = 0.7):
+        self.b = backend
+        self.budget = int(budget_bytes)
+        pts = int(self.budget * point_share) // 64
+        frs = (self.budget - pts * 64) // 32
+        if pts >= 1 << 32:
+            raise ValueError("point arena limited to 2^32 points")
+        self.points = backend.alloc(64 * pts)
+        self.fr = backend.alloc(32 * frs)
+        self.point_count, self.fr_count = pts, frs
+        self.pt_alloc, self.fr_alloc = RangeAllocator(pts), RangeAllocator(frs)
+        self.sessions: Dict[int, Session] = {}
+
+    def can_admit(self, n: int) -> bool:
+        return any(z >= session_points(n) for _, z in self.pt_alloc.free) and \
+            any(z >= session_fr(n) for _, z in self.fr_alloc.free)
+
+    def admit(self, spec: SessionSpec) -> Optional[Session]:
+        """Generate the session's bases on the device and prove its anchor
+        state; None when the budget has no room."""
+        n = spec.n
+        if n & (n - 1) or not 1 <= n <= 1 << 20:
+            raise ValueError("session size must be a power of two <= 2^20")
+        pb = self.pt_alloc.alloc(session_points(n))
+        if pb is None:
+            return None
+        fb = self.fr_alloc.alloc(session_fr(n))
+        if fb is None:
+            self.pt_alloc.release(pb, session_points(n))
+            return None
+        tau, t, z, coeffs = spec.derive()
+        s = Session(spec, n.bit_length() - 1, tau, t, z, pb, fb)
+        a, u, l = session_scalars(tau, z, n)
+        tmp = self.b.to_device(np.concatenate([a.reshape(-1), u.reshape(-1)]))
+        self.b.fixed_base_device(tmp.ptr, n, self.points.ptr + 64 * pb)
+        self.b.fixed_base_device(tmp.ptr + 32 * n, n, self.points.ptr + 64 * (pb + n))
+        tmp.free()
+        self.points.upload_at(64 * (pb + 2 * n), np.zeros(128, dtype=np.uint8))
+        self.fr.upload_at(32 * fb, np.zeros(32 * 2 * n, dtype=np.uint8))
+        self.fr.upload_at(32 * (fb + 2 * n), l.reshape(-1))
+        self.fr.upload_at(32 * (fb + 3 * n), np.zeros(32, dtype=np.uint8))
+        # Anchor: the initial coefficients enter as one dense update.
+        idx = np.arange(n, dtype=np.uint32)
+        deltas = coeffs.tobytes()
+        u_struct = s.update_struct(idx, deltas)
+        update_batch(self.b, self, [u_struct])
+        self.sessions[spec.sid] = s
+        return s
+
+    def release(self, s: Session) -> None:
+        self.pt_alloc.release(s.pt_base, session_points(s.n))
+        self.fr_alloc.release(s.fr_base, session_fr(s.n))
+        self.sessions.pop(s.spec.sid, None)
+
+    def evaluations(self, s: Session) -> bytes:
+        return self.fr.download_at(32 * (s.fr_base + s.n), 32 * s.n).tobytes()
+
+    def coefficients(self, s: Session) -> bytes:
+        return self.fr.download_at(32 * s.fr_base, 32 * s.n).tobytes()
+
+    def close(self):
+        self.points.free()
+        self.fr.free()
+
+
+def update_batch(backend: HipBackend, store: SessionStore, updates: Sequence[_lib.DvgpuUpdate],
+                 timed: bool = False):
+    """One native call for the batch: -> list of 160-byte proofs (and ms)."""
+    arr = (_lib.DvgpuUpdate * len(updates))(*updates)
+    out = ctypes.create_string_buffer(PROOF_BYTES * len(updates))
+    ms = ctypes.c_float(0)
+    err = _err()
+    _check(backend.lib.dvgpu_ext_update_batch(backend.handle, store.points.ptr, store.point_count, store.fr.ptr,
+                                              store.fr_count, arr, len(updates), _ROOT, out,
+                                              ctypes.byref(ms) if timed else None, err, 256), err)
+    proofs = [out.raw[PROOF_BYTES * i:PROOF_BYTES * (i + 1)] for i in range(len(updates))]
+    return (proofs, ms.value) if timed else proofs
+
+
+# ---- request loop --------------------------------------------------------------
+
+@dataclass
+class Request:
+    sid: int
+    indices: np.ndarray
+    deltas: bytes
+    arrival: float = 0.0
+    future: Future = field(default_factory=Future)
+    started: float = 0.0
+    done: float = 0.0
+
+
+def request_cost(n: int, k: int) -> int:
+    """Scratch bytes a request adds to a batch (MSM entries at c = 4 and the
+    NTT work row), for admission against the lanes' scratch budget."""
+    return 2 * (k + 1) * 64 * 16 + 64 * n + 4096
+
+
+class Server:
+    """Continuous-batching request loop over a SessionStore."""
+
+    def __init__(self, store: SessionStore, lanes: int = 2, max_batch: int = 256,
+                 scratch_budget: int = 1 << 30):
+        self.store = store
+        self.lanes = lanes
+        self.max_batch = max_batch
+        self.lane_budget = scratch_budget // max(1, lanes)
+        self.queue: collections.deque = collections.deque()
+        self.cv = threading.Condition()
+        self.stop_flag = False
+        self.free_lanes = lanes
+        self.batches = 0
+        self.batch_sizes: List[int] = []
+        self.device_ms: List[float] = []
+        self.backends = [HipBackend() for _ in range(lanes)]
+        self.work: List[collections.deque] = [collections.deque() for _ in range(lanes)]
+        self.idle = list(range(lanes))
+        self.threads = [threading.Thread(target=self._lane, args=(i,), daemon=True) for i in range(lanes)]
+        self.sched = threading.Thread(target=self._schedule, daemon=True)
+        for t in self.threads:
+            t.start()
+        self.sched.start()
+
+    def submit(self, sid: int, indices: np.ndarray, deltas: bytes, arrival: Optional[float] = None) -> Request:
+        if sid not in self.store.sessions:
+            raise KeyError("unknown session")
+        req = Request(sid, indices, deltas, arrival if arrival is not None else time.perf_counter())
+        with self.cv:
+            self.queue.append(req)
+            self.cv.notify_all()
+        return req
+
+    def _take_batch(self) -> List[Request]:
+        batch, cost, seen = [], 0, set()
+        keep = collections.deque()
+        while self.queue and len(batch) < self.max_batch:
+            req = self.queue.popleft()
+            s = self.store.sessions[req.sid]
+            c = request_cost(s.n, len(req.indices))
+            if s.busy or req.sid in seen or (batch and cost + c > self.lane_budget):
+                keep.append(req)          # stays queued, order kept per session
+                continue
+            seen.add(req.sid)
+            s.busy = True
+            batch.append(req)
+            cost += c
+        while keep:
+            self.queue.appendleft(keep.pop())
+        return batch
+
+    def _schedule(self):
+        while True:
+            with self.cv:
+                while not self.stop_flag and (not self.idle or not self._runnable()):
+                    self.cv.wait()
+                if self.stop_flag:
+                    return
+                batch = self._take_batch()
+                if not batch:
+                    continue
+                lane = self.idle.pop()
+                self.work[lane].append(batch)
+                self.cv.notify_all()
+
+    def _runnable(self) -> bool:
+        for req in self.queue:
+            if not self.store.sessions[req.sid].busy:
+                return True
+        return False
+
+    def _lane(self, i: int):
+        b = self.backends[i]
+        while True:
+            with self.cv:
+                while not self.stop_flag and not self.work[i]:
+                    self.cv.wait()
+                if self.stop_flag and not self.work[i]:
+                    return
+                batch = self.work[i].popleft()
+            t0 = time.perf_counter()
+            err = None
+            try:
+                ups = [self.store.sessions[r.sid].update_struct(r.indices, r.deltas) for r in batch]
+                proofs, ms = update_batch(b, self.store, ups, timed=True)
+            except Exception as e:          # delivered to every request of the batch
+                err, proofs, ms = e, None, 0.0
+            t1 = time.perf_counter()
+            with self.cv:
+                for k, r in enumerate(batch):
+                    r.started, r.done = t0, t1
+                    self.store.sessions[r.sid].busy = False
+                self.batches += 1
+                self.batch_sizes.append(len(batch))
+                self.device_ms.append(ms)
+                self.idle.append(i)
+                self.cv.notify_all()
+            for k, r in enumerate(batch):
+                if err is not None:
+                    r.future.set_exception(err)
+                else:
+                    r.future.set_result(proofs[k])
+
+    def close(self):
+        with self.cv:
+            self.stop_flag = True
+            self.cv.notify_all()
+        self.sched.join()
+        for t in self.threads:
+            t.join()
+        for b in self.backends:
+            b.close()
+
+
+# ---- one-call-at-a-time baseline ----------------------------------------------
+
+def _fixed_base_host(backend: HipBackend, scalars: np.ndarray) -> np.ndarray:
+    """[s_i] g1 as canonical (n, 64) bytes through the setup entry point, in
+    its 4096-point chunks (setup.rs:113-143)."""
+    n = scalars.shape[0]
+    out = [backend.fixed_base_encoded(scalars[c:c + 4096].tobytes(), min(4096, n - c), False)
+           for c in range(0, n, 4096)]
+    return np.frombuffer(b"".join(out), dtype=np.uint8).reshape(n, 64)
+
+
+class PlainSession:
+    """The same session state on the host, as the reference's plain path
+    keeps it (canonical bases, coefficients, cached points and value)."""
+
+    def __init__(self, backend: HipBackend, spec: SessionSpec):
+        n = spec.n
+        tau, t, z, coeffs = spec.derive()
+        self.spec, self.n, self.t, self.z, self.tau = spec, n, t, z, tau
+        a, u, l = session_scalars(tau, z, n)
+        self.P = _fixed_base_host(backend, a)
+        self.U = _fixed_base_host(backend, u)
+        self.l = [int.from_bytes(l[i].tobytes(), "little") for i in range(n)]
+        self.C = bytes(64)
+        self.W = bytes(64)
+        self.y = 0
+        self.evals = b""
+        idx = np.arange(n, dtype=np.uint32)
+        dvals = [int.from_bytes(coeffs[i].tobytes(), "little") for i in range(n)]
+        self.f = np.zeros_like(coeffs)
+        self.apply(backend, idx, coeffs.tobytes(), dvals)
+
+    def apply(self, backend: HipBackend, idx: np.ndarray, deltas: bytes, dvals: List[int]) -> bytes:
+        """msm_batch over the gathered bases (+ the cached point with scalar
+        1), the coefficient update on the host, ntt of the coefficients."""
+        k = len(idx)
+        pts = np.concatenate([self.P[idx].reshape(-1), np.frombuffer(self.C, dtype=np.uint8),
+                              self.U[idx].reshape(-1), np.frombuffer(self.W, dtype=np.uint8)]).tobytes()
+        sc = deltas + encode_fr(1)
+        out = backend.msm_batch_encoded(pts, sc + sc, k + 1, 2)
+        self.C, self.W = out[:64], out[64:128]
+        for i, d in zip(idx.tolist(), dvals):
+            cur = int.from_bytes(self.f[i].tobytes(), "little")
+            self.f[i] = np.frombuffer(((cur + d) % R).to_bytes(32, "little"), dtype=np.uint8)
+            self.y = (self.y + d * self.l[i]) % R
+        self.evals = backend.ntt_encoded(self.f.tobytes(), self.n, False)
+        return self.C + self.W + self.y.to_bytes(32, "little")
+
+
+# ---- seeded request streams ------------------------------------------------------
+
+SIZE_MIX = ((256, 0.4), (1024, 0.3), (4096, 0.2), (65536, 0.1))
+
+
+def session_pool(per_size: int, seed: int = 6) -> List[SessionSpec]:
+    specs, sid = [], 0
+    for n, _ in SIZE_MIX:
+        for _ in range(per_size):
+            specs.append(SessionSpec(sid, n, seed * 1_000_003 + sid))
+            sid += 1
+    return specs
+
+
+def request_stream(specs: Sequence[SessionSpec], count: int, qps: float, seed: int = 6, k_max: int = 64):
+    """Poisson arrivals at `qps`: (arrival offset s, sid, indices, deltas)."""
+    rng = ChaCha20Stream(seed * 7919 + 1)
+    by_size: Dict[int, List[SessionSpec]] = collections.defaultdict(list)
+    for s in specs:
+        by_size[s.n].append(s)
+    cum, acc = [], 0.0
+    for n, p in SIZE_MIX:
+        acc += p
+        cum.append((acc, n))
+    t = 0.0
+    out = []
+    for _ in range(count):
+        u = rng.below(1 << 30) / float(1 << 30)
+        t += -np.log(1.0 - u) / qps if qps > 0 else 0.0
+        v = rng.below(1 << 30) / float(1 << 30)
+        n = next(nn for c, nn in cum if v < c or c == cum[-1][0])
+        spec = by_size[n][rng.below(len(by_size[n]))]
+        k = 1 + rng.below(k_max)
+        idx = [rng.below(n) for _ in range(k)]
+        ds = rng.uniform_fr(k)
+        out.append((t, spec.sid, idx, ds))
+    return out
+
+
+def percentile(xs: Sequence[float], p: float) -> float:
+    if not xs:
+        return float("nan")
+    s = sorted(xs)
+    k = min(len(s) - 1, max(0, int(round(p / 100.0 * (len(s) - 1)))))
+    return s[k]
