@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--log-n", type=int, default=20)
     ap.add_argument("--window", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the batched, incremental and served workloads")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the plain copy kernel (traffic-counter calibration)")
     args = ap.parse_args()
@@ -89,7 +91,8 @@ def main():
     phases = [sum(p[k] for p in acc_ms) / len(acc_ms) for k in range(7)]
     from dvgpu._lib import load  # noqa: F401  (library already loaded)
     # window width actually used
-    c = args.window or (max(4, min(16, args.log_n - 3)))
+    from dvgpu.autotune import window_for
+    c = args.window or window_for(backend, n)
     windows = -(-255 // c)
     acc_macs = 136.0 * 10 * n * windows
     acc_s = phases[4] * 1e-3
@@ -130,6 +133,16 @@ def main():
         secondary[f"ntt_2p{log_t}_roundtrip_ok"] = bool((d_out.download() == vals).all())
         d_in.free()
         d_out.free()
+
+    if not args.no_serving:
+        from dvgpu import benchmarks
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        log("# batched launch")
+        secondary["batched_config"] = benchmarks.batched_launch(backend, log=log)
+        log("# incremental n=2^16")
+        secondary["incremental_2p16"] = benchmarks.incremental(backend, log=log)
+        log("# served stream")
+        secondary["served_stream"] = benchmarks.served_stream(backend, log=log)
 
     if args.calibrate_copy:
         nbytes = 256 << 20

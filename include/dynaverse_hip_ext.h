@@ -76,6 +76,9 @@ typedef struct DvgpuTuning {
                                         windows on the device (one lane per MSM) */
 } DvgpuTuning;
 int dvgpu_ext_set_tuning(void* owner, const DvgpuTuning* tuning);
+/* The window width the owner uses for an MSM of n points (positive), or a
+ * negative status. */
+int dvgpu_ext_msm_window(void* owner, uint64_t n);
 int dvgpu_ext_get_tuning(void* owner, DvgpuTuning* tuning);
 
 typedef struct DvgpuMsmItem {

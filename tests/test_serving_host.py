@@ -65,3 +65,12 @@ def test_request_stream_deterministic_and_mixed():
 def test_session_bytes_and_percentile():
     assert session_bytes(256) == 64 * 514 + 32 * 769
     assert percentile([1, 2, 3, 4, 5], 50) == 3 and percentile([1, 2, 3, 4, 5], 100) == 5
+
+
+def test_coset_point_constant_is_off_subgroup():
+    from dvgpu.benchmarks import COSET_POINT
+    from oracle import bn254 as o
+    from tests.g2corpus import twist_cofactor_point
+    T = twist_cofactor_point()
+    assert o.g2_to_bytes(T) == COSET_POINT
+    assert o.on_curve(T, o.FQ2) and o.mul(T, o.R, o.FQ2) is not None

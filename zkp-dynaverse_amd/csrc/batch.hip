@@ -191,6 +191,13 @@ int dvgpu_ext_set_tuning(void* owner, const DvgpuTuning* t) {
     return 0;
 }
 
+int dvgpu_ext_msm_window(void* owner, uint64_t n) {
+    Owner* o = as_owner(owner);
+    if (!o || o->retired) return -4;
+    if (!n) return -1;
+    return o->msm_config.window_bits ? o->msm_config.window_bits : msm_window_for(size_t(n), &o->window_table);
+}
+
 int dvgpu_ext_get_tuning(void* owner, DvgpuTuning* t) {
     Owner* o = as_owner(owner);
     if (!o || o->retired) return -4;

@@ -32,7 +32,7 @@ struct PointLayout<Fq2Ops> {
 
 struct MsmConfig {
     int window_bits = 0;  // c for every MSM of a call; 0 selects by size
-    int task_points = 32; // maximum points one accumulation thread adds
+    int task_points = 16; // maximum points one accumulation thread adds (measured best on MI355X)
     int chunk = 8;        // buckets per first-level reduction thread
 };
 

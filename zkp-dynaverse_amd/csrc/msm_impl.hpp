@@ -477,9 +477,14 @@ __global__ void k_to_mont(const uint8_t* __restrict__ canonical, uint32_t* __res
 // Host driver
 // ---------------------------------------------------------------------------
 
+// Window width by log2 size as measured on MI355X by the autotuning sweep
+// (batches of 2^(18 - L) MSMs of 2^L points); outside the swept range
+// c = L - 3 within [4, 16].
 inline int msm_default_window_impl(size_t n) {
+    static const int measured[21] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 7, 9, 9, 10, 10, 10, 13, 13, 15, 16, 15};
     int lg = 0;
     while ((size_t(1) << (lg + 1)) <= n) ++lg;
+    if (lg >= 10 && lg <= 20) return measured[lg];
     return std::max(4, std::min(16, lg - 3));
 }
 

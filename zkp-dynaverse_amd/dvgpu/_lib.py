@@ -207,6 +207,7 @@ SIGNATURES = {
     "dvgpu_ext_device_copy": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, *_E]),
     "dvgpu_ext_set_tuning": (_I, [c_void_p, ctypes.POINTER(DvgpuTuning)]),
     "dvgpu_ext_get_tuning": (_I, [c_void_p, ctypes.POINTER(DvgpuTuning)]),
+    "dvgpu_ext_msm_window": (_I, [c_void_p, ctypes.c_uint64]),
     "dvgpu_ext_msm_batch_device": (_I, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(DvgpuMsmItem),
                                         c_size_t, _I, c_void_p, ctypes.POINTER(ctypes.c_float), *_E]),
     "dvgpu_ext_msm_batch_finalize": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t,

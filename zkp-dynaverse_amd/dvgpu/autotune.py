@@ -41,7 +41,16 @@ def get(backend: HipBackend) -> dict:
         raise BackendError(-1, "tuning unavailable")
     return {"msm_window_bits": {i: t.msm_window_bits[i] for i in range(25) if t.msm_window_bits[i]},
             "ntt_pass_stages": {i: t.ntt_pass_stages[i] for i in range(21) if t.ntt_pass_stages[i]},
-            "msm_task_points": t.msm_task_points, "msm_chunk": t.msm_chunk}
+            "msm_task_points": t.msm_task_points, "msm_chunk": t.msm_chunk,
+            "msm_device_finalize_min": t.msm_device_finalize_min}
+
+
+def window_for(backend: HipBackend, n: int) -> int:
+    """The window width the owner uses for an n-point MSM."""
+    c = backend.lib.dvgpu_ext_msm_window(backend.handle, n)
+    if c <= 0:
+        raise BackendError(c, "window query failed")
+    return c
 
 
 def apply(backend: HipBackend, table: dict) -> None:
@@ -53,6 +62,7 @@ def apply(backend: HipBackend, table: dict) -> None:
         t.ntt_pass_stages[int(k)] = int(v)
     t.msm_task_points = int(table.get("msm_task_points", 0))
     t.msm_chunk = int(table.get("msm_chunk", 0))
+    t.msm_device_finalize_min = int(table.get("msm_device_finalize_min", 0))
     if backend.lib.dvgpu_ext_set_tuning(backend.handle, ctypes.byref(t)) != 0:
         raise BackendError(-1, "invalid tuning table")
 

@@ -1,0 +1,227 @@
+"""Measured workloads beside the single-MSM headline: the batched launch
+(64 uneven G1 MSMs, 8 G2 MSMs, 32 NTT rows, 4096 G2 checks), the incremental
+n = 2^16 session and the served mixed-n update stream against the
+one-call-at-a-time path.  Everything runs on seeded synthetic data; every
+served proof is compared with the plain path's bytes.
+"""
+import time
+from typing import Dict, List
+
+import numpy as np
+
+from .backend import HipBackend
+from .batching import DeviceBatch, MsmItem, NttRows, msm_batch_device
+from .rng import ChaCha20Stream
+from .serving import (PlainSession, Server, SessionSpec, SessionStore, merge_update, percentile, request_stream,
+                      session_pool, update_batch)
+from .workload import tau_powers
+
+PEAK_MAC_PER_S = 256 * 64 * 2.4e9   # v_mad_u64_u32 issue rate: CUs x lanes x clock
+
+
+def msm_macs(n: int, fq2: bool = False) -> float:
+    """136 MACs per Montgomery product times the Pippenger product count at
+    the best window (madd 10, add 14, dbl 9 products; Fq2 x3)."""
+    best = None
+    for c in range(4, 21):
+        w = -(-255 // c)
+        f = n * w * 10 + w * (1 << c) * 14 + w * (9 * c + 14)
+        best = f if best is None else min(best, f)
+    return 136.0 * best * (3 if fq2 else 1)
+
+
+def ntt_macs(n: int) -> float:
+    return 136.0 * (n // 2) * max(1, n.bit_length() - 1)
+
+
+def batched_launch(backend: HipBackend, replays: int = 10, log=print) -> Dict:
+    """The batched configuration as one hipGraph: batches/s and MAC rate."""
+    n_srs = 1 << 18
+    tau = ChaCha20Stream(4).nonzero_fr()
+    pw = tau_powers(tau, n_srs)
+    d_pw = backend.to_device(pw.reshape(-1))
+    srs = backend.alloc(64 * n_srs)
+    backend.fixed_base_device(d_pw.ptr, n_srs, srs.ptr)
+    srs2 = backend.alloc(128 * 1024)
+    backend.fixed_base_device(d_pw.ptr, 1024, srs2.ptr, g2=True)
+    d_pw.free()
+    sc = backend.to_device(ChaCha20Stream(40).uniform_fr_array(n_srs).reshape(-1))
+    u = ChaCha20Stream(41)
+    items = []
+    for _ in range(64):
+        m = int(round(2 ** (10 + 8 * u.below(1 << 20) / (1 << 20))))
+        m = max(1 << 10, min(n_srs, m))
+        items.append(MsmItem(m, u.below(n_srs - m + 1), u.below(n_srs - m + 1)))
+    items2 = [MsmItem(1024, 0, 4096 * k) for k in range(8)]
+    v12 = backend.to_device(ChaCha20Stream(42).uniform_fr_array(16 << 12).reshape(-1))
+    v16 = backend.to_device(ChaCha20Stream(43).uniform_fr_array(16 << 16).reshape(-1))
+    o12, o16 = backend.alloc(32 << 16), backend.alloc(32 << 20)
+    ntt = [NttRows(v12, o12, 1 << 12, 16), NttRows(v16, o16, 1 << 16, 16)]
+    # 4096 G2 points: [s_i] g2 on the device; odd ones get a cofactor-order
+    # twist point added (on the device), which leaves the subgroup.
+    g2sc = ChaCha20Stream(44).uniform_fr_array(4096).reshape(-1)
+    d_g2sc = backend.to_device(g2sc)
+    g2pts = backend.alloc(128 * 4096)
+    backend.fixed_base_device(d_g2sc.ptr, 4096, g2pts.ptr, g2=True)
+    wire = _coset_wire(backend, g2pts, 4096)
+    d_wire = backend.to_device(wire)
+    masks = backend.alloc(4 * 4096)
+    log("# batched workload ready")
+    eager = DeviceBatch(backend, g1=(srs, sc, items), g2=(srs2, sc, items2), ntt=ntt, g2_wire=d_wire,
+                        g2_wire_n=4096, g2_algorithm=3, g2_masks=masks, use_graph=False)
+    ref = eager.run()
+    eager.close()
+    graph = DeviceBatch(backend, g1=(srs, sc, items), g2=(srs2, sc, items2), ntt=ntt, g2_wire=d_wire,
+                        g2_wire_n=4096, g2_algorithm=3, g2_masks=masks, use_graph=True)
+    ms, same = [], True
+    t0 = time.perf_counter()
+    for _ in range(replays):
+        r = graph.run()
+        same &= r[0] == ref[0] and r[1] == ref[1] and r[2] == ref[2]
+        ms.append(r[3])
+    wall = (time.perf_counter() - t0) / replays
+    graph.close()
+    invalid = ref[2]["invalid_points"]
+    macs = sum(msm_macs(it.n) for it in items) + sum(msm_macs(it.n, True) for it in items2) \
+        + 16 * ntt_macs(1 << 12) + 16 * ntt_macs(1 << 16) + 4096 * 3 * 136 * (63 * 9 + 28 * 14 + 4 * 2 + 4 * 14 + 4)
+    dev = float(np.median(ms))
+    return {"batches_per_s": round(1e3 / (wall * 1e3), 2), "graph_device_ms": round(dev, 3),
+            "wall_ms": round(wall * 1e3, 3), "mac_frac": round(macs / (dev * 1e-3) / PEAK_MAC_PER_S, 4),
+            "g1_points": sum(it.n for it in items), "g2_invalid": invalid, "replays_match_eager": bool(same)}
+
+
+# A point of the twist E'(Fq2) of cofactor order: [r] T0 for the twist point
+# T0 with x = 1 (canonical wire bytes x.c0 || x.c1 || y.c0 || y.c1).  Adding
+# it to a subgroup point leaves the subgroup.
+COSET_POINT = bytes.fromhex(
+    "44f2242359f77b012b0f75db89f427f32aa85fe8c5695d6c2b326ea8ae37ef26"
+    "0732a8d2aaa02f947fe48530695cbf84901fc238b71a00a77a5e6ee19077f00d"
+    "57814d44f915fb58fd3f8b83232865f12d1479cee506486278c309f00836571e"
+    "c2ff043bf048e69c6e8eeafa83566132e733d5648c342bc63843ffd3c912e61c")
+
+
+def _coset_wire(backend: HipBackend, g2pts, count: int) -> bytes:
+    """Wire bytes of the device points g2pts (Montgomery), odd ones plus
+    COSET_POINT: one batch of G2 MSMs ([P_i] or [P_i, T] with scalars 1)."""
+    pts = backend.alloc(128 * (count + 1))
+    backend.device_copy(pts.ptr, g2pts.ptr, 128 * count)
+    t_canon = backend.to_device(COSET_POINT)
+    backend.points_to_mont_device(t_canon.ptr, pts.ptr + 128 * count, 1, g2=True)
+    idx, items = [], []
+    for i in range(count):
+        items.append(MsmItem(1 + (i & 1), len(idx), 0))
+        idx.extend([i, count] if i & 1 else [i])
+    d_idx = backend.to_device(np.array(idx, dtype=np.uint32).view(np.uint8))
+    ones = backend.to_device((1).to_bytes(32, "little") * 2)
+    out = msm_batch_device(backend, pts, ones, items, g2=True, index=d_idx)
+    for a in (pts, t_canon, d_idx, ones):
+        a.free()
+    return b"".join(out)
+
+
+def incremental(backend: HipBackend, log=print) -> Dict:
+    """n = 2^16 resident session, 16 updates with k cycling {1, 16, 256, 4096},
+    hotspot (a fixed 1% index subset) and spreading alternating; served path
+    (one update per call, the state on the device) against the plain path."""
+    spec = SessionSpec(0, 1 << 16, 5)
+    store = SessionStore(backend, 64 << 20)
+    s = store.admit(spec)
+    plain = PlainSession(backend, spec)
+    rng = ChaCha20Stream(50)
+    hot = [rng.below(spec.n) for _ in range(655)]
+    ups = []
+    for j in range(16):
+        k = (1, 16, 256, 4096)[j % 4]
+        idx = [hot[rng.below(len(hot))] if j % 2 == 0 else rng.below(spec.n) for _ in range(k)]
+        ups.append(merge_update(idx, rng.uniform_fr(k)))
+    served_ms, plain_ms, same = [], [], True
+    for idx, raw, dv in ups:
+        a = time.perf_counter()
+        p1 = update_batch(backend, store, [s.update_struct(idx, raw)])[0]
+        b = time.perf_counter()
+        p2 = plain.apply(backend, idx, raw, dv)
+        c = time.perf_counter()
+        served_ms.append((b - a) * 1e3)
+        plain_ms.append((c - b) * 1e3)
+        same &= p1 == p2
+    same &= store.evaluations(s) == plain.evals
+    store.close()
+    return {"served_updates_per_s": round(16 / (sum(served_ms) * 1e-3), 1),
+            "served_p50_ms": round(percentile(served_ms, 50), 3), "served_p99_ms": round(percentile(served_ms, 99), 3),
+            "plain_updates_per_s": round(16 / (sum(plain_ms) * 1e-3), 1),
+            "plain_p50_ms": round(percentile(plain_ms, 50), 3), "plain_p99_ms": round(percentile(plain_ms, 99), 3),
+            "identical": bool(same)}
+
+
+def _drive(srv: Server, stream, requests_prepared) -> List[float]:
+    """Open-loop driver: submit each request at its arrival offset."""
+    t0 = time.perf_counter() + 0.01
+    reqs = []
+    for (t, sid, _, _), (mi, raw) in zip(stream, requests_prepared):
+        now = time.perf_counter()
+        if t0 + t > now:
+            time.sleep(t0 + t - now)
+        reqs.append(srv.submit(sid, mi, raw, arrival=t0 + t))
+    for r in reqs:
+        r.future.result(timeout=300)
+    lat = [(r.done - r.arrival) * 1e3 for r in reqs]
+    span = max(r.done for r in reqs) - t0
+    return reqs, lat, span
+
+
+def served_stream(backend: HipBackend, per_size: int = 16, count: int = 3000, qps_points=(), plain_count: int = 600,
+                  lanes: int = 2, log=print) -> Dict:
+    """Poisson mixed-n update stream: served (continuous batching, resident
+    state) at several arrival rates and saturated, against the plain path
+    run back to back (its maximum rate).  Every proof is compared."""
+    specs = session_pool(per_size, seed=6)
+    store = SessionStore(backend, 32 << 30)
+    t_setup = time.perf_counter()
+    for sp in specs:
+        store.admit(sp)
+    log(f"# served: {len(specs)} sessions resident in {time.perf_counter() - t_setup:.1f}s")
+    # Plain reference run over the first plain_count requests (closed loop).
+    base_stream = request_stream(specs, count, 1.0, seed=6)
+    prepared = [merge_update(idx, ds) for _, _, idx, ds in base_stream]
+    plain = {sp.sid: PlainSession(backend, sp) for sp in specs}
+    p_lat, p_out = [], []
+    t0 = time.perf_counter()
+    for (t, sid, _, _), (mi, raw, dv) in zip(base_stream[:plain_count], prepared[:plain_count]):
+        a = time.perf_counter()
+        p_out.append(plain[sid].apply(backend, mi, raw, dv))
+        p_lat.append((time.perf_counter() - a) * 1e3)
+    plain_rate = plain_count / (time.perf_counter() - t0)
+    log(f"# plain path: {plain_rate:.0f} updates/s")
+    out = {"sessions": len(specs), "plain_updates_per_s": round(plain_rate, 1),
+           "plain_p50_ms": round(percentile(p_lat, 50), 3), "plain_p99_ms": round(percentile(p_lat, 99), 3)}
+    # Served, saturated: every request queued at t = 0 (throughput ceiling).
+    # The served sessions start from the same anchors, so the first
+    # plain_count proofs must match the plain run exactly.
+    srv = Server(store, lanes=lanes, max_batch=256)
+    sat_stream = [(0.0, sid, idx, ds) for _, sid, idx, ds in base_stream]
+    reqs, lat, span = _drive(srv, sat_stream, [(m, r) for m, r, _ in prepared])
+    same = [r.future.result() for r in reqs[:plain_count]] == p_out
+    sat_rate = count / span
+    out.update({"served_saturated_updates_per_s": round(sat_rate, 1),
+                "served_saturated_p50_ms": round(percentile(lat, 50), 3),
+                "served_saturated_p99_ms": round(percentile(lat, 99), 3),
+                "mean_batch": round(float(np.mean(srv.batch_sizes)), 1), "identical_to_plain": bool(same),
+                "speedup_vs_plain": round(sat_rate / plain_rate, 2)})
+    log(f"# served saturated: {sat_rate:.0f} updates/s (x{sat_rate / plain_rate:.1f}), mean batch "
+        f"{np.mean(srv.batch_sizes):.1f}")
+    srv.close()
+    # Open loop at set arrival rates (continuing the same sessions' states).
+    pts = []
+    for q in qps_points or (0.5 * sat_rate, 0.8 * sat_rate, plain_rate):
+        stream = request_stream(specs, count, q, seed=int(q) + 7)
+        prep = [merge_update(idx, ds)[:2] for _, _, idx, ds in stream]
+        srv = Server(store, lanes=lanes, max_batch=256)
+        reqs, lat, span = _drive(srv, stream, prep)
+        pts.append({"qps": round(q, 1), "updates_per_s": round(count / span, 1),
+                    "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
+                    "mean_batch": round(float(np.mean(srv.batch_sizes)), 1)})
+        log(f"# served at {q:.0f} QPS: p50 {pts[-1]['p50_ms']} ms, p99 {pts[-1]['p99_ms']} ms")
+        srv.close()
+    out["open_loop"] = pts
+    store.close()
+    return out
