@@ -156,15 +156,22 @@ int dvgpu_ext_batch_destroy(void* owner, void* batch);
 
 /* ---- serving: batched sparse updates of device-resident session states ---- */
 
-/* One sparse update of one session.  Point indices refer to the `points`
- * arena (Montgomery affine G1), field indices to the `fr` arena (canonical
- * Fr, 32 bytes each).  A session of size n = 2^log_n holds its basis
- * P_0..P_{n-1} at srs_base, its opening basis U_0..U_{n-1} at opening_base,
- * the cached commitment C and opening W at two slots, coefficients f,
- * evaluations NTT(f), the linear-term vector l and the cached value
+/* Window table of n resident bases for the serving path: row i receives the
+ * ceil(255 / window_bits) points [s_i 2^(c w)] g1, w = 0.. (Montgomery affine,
+ * 64 bytes each), from n device canonical scalars s_i. */
+int dvgpu_ext_window_table_device(void* owner, const void* scalars, size_t n, int window_bits, void* rows,
+                                  char* error, size_t error_capacity);
+
+/* One sparse update of one session.  Rows refer to the `points` arena viewed
+ * as rows of ceil(255 / window_bits) points; field indices to the `fr` arena
+ * (canonical Fr, 32 bytes each).  A session of size n = 2^log_n holds the
+ * window tables of its basis P (rows srs_row..+n) and opening basis U (rows
+ * opening_row..+n), the cached commitment C and opening W as XYZZ points
+ * (128 bytes at the start of their rows; all-zero = identity), coefficients
+ * f, evaluations NTT(f), the linear-term vector l and the cached value
  * y = <l, f>. */
 typedef struct DvgpuUpdate {
-    uint32_t srs_base, opening_base, commitment_slot, opening_slot;
+    uint32_t srs_row, opening_row, commitment_row, opening_cache_row;
     uint64_t coeff_offset, eval_offset, linear_offset, value_slot;
     uint32_t log_n;
     uint32_t k;               /* changed coefficients (0 allowed) */
@@ -173,13 +180,13 @@ typedef struct DvgpuUpdate {
 } DvgpuUpdate;
 
 /* Applies every update of the batch in one launch sequence:
- *   C += sum d_j P_{i_j},  W += sum d_j U_{i_j}   (results stored in the slots),
+ *   C += sum d_j P_{i_j},  W += sum d_j U_{i_j}   (in place, XYZZ),
  *   f_{i_j} += d_j,  y += sum d_j l_{i_j},  evaluations = NTT(f) (root given).
  * results: count x 160 host bytes = new C (64) || new W (64) || new y (32),
  * canonical.  Updates of one batch must not share session state (checked,
  * as are all ranges); device_ms (optional) is the kernel time. */
-int dvgpu_ext_update_batch(void* owner, void* points, size_t point_count, void* fr, size_t fr_count,
-                           const DvgpuUpdate* updates, size_t count, const uint8_t* root,
+int dvgpu_ext_update_batch(void* owner, void* points, size_t point_count, int window_bits, void* fr,
+                           size_t fr_count, const DvgpuUpdate* updates, size_t count, const uint8_t* root,
                            uint8_t* results, float* device_ms, char* error, size_t error_capacity);
 
 #ifdef __cplusplus

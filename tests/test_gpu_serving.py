@@ -62,7 +62,7 @@ def _check_session(store, s, mirror, proof):
 
 
 def test_anchor_and_batched_updates_closed_form(backend):
-    store = SessionStore(backend, 256 << 20)
+    store = SessionStore(backend, 1 << 30)
     specs = [SessionSpec(i, n, 900 + i) for i, n in enumerate((256, 1024, 4096, 65536, 256, 2))]
     sessions, mirrors = [], []
     for sp in specs:
@@ -146,7 +146,7 @@ def test_served_stream_equals_alone_and_plain(backend):
     specs = session_pool(2, seed=11)                    # 8 sessions over the size mix
     stream = request_stream(specs, 120, 0.0, seed=11)   # all requests at once
     # Served, continuous batching over two lanes.
-    store = SessionStore(backend, 512 << 20)
+    store = SessionStore(backend, 2 << 30)
     for sp in specs:
         store.admit(sp)
     print("case: served stream", flush=True)
@@ -161,7 +161,7 @@ def test_served_stream_equals_alone_and_plain(backend):
     ev_served = {sp.sid: store.evaluations(store.sessions[sp.sid]) for sp in specs}
     store.close()
     # Alone: a fresh store, every request in a batch of its own, in order.
-    store2 = SessionStore(backend, 512 << 20)
+    store2 = SessionStore(backend, 2 << 30)
     for sp in specs:
         store2.admit(sp)
     print("case: alone", flush=True)

@@ -63,7 +63,7 @@ def test_request_stream_deterministic_and_mixed():
 
 
 def test_session_bytes_and_percentile():
-    assert session_bytes(256) == 64 * 514 + 32 * 769
+    assert session_bytes(256) == 64 * 32 * 514 + 32 * 769
     assert percentile([1, 2, 3, 4, 5], 50) == 3 and percentile([1, 2, 3, 4, 5], 100) == 5
 
 

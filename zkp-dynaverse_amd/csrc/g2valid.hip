@@ -492,6 +492,50 @@ int dvgpu_g2_test_corrupt_next(void* backend, uint64_t index) {
     return 0;
 }
 
+// dynaverse_hip_ext.h: window tables of resident bases.  Entry (i, w) of the
+// scalar table is s_i 2^(c w) mod r; 2^(c w) < 2^253 < r for every c <= 20.
+}  // extern "C"
+
+namespace {
+__global__ void k_table_scalars(const uint32_t* __restrict__ scalars, size_t n, int c, uint32_t windows,
+                                uint32_t* __restrict__ out) {
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n * windows) return;
+    const size_t i = t / windows;
+    const uint32_t w = uint32_t(t % windows);
+    uint32_t sw[8], pw[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 8; ++k) sw[k] = scalars[i * 8 + k];
+    const uint32_t bit = uint32_t(c) * w;
+    pw[bit / 32] = 1u << (bit % 32);
+    // (s R) * 2^(cw) * R^-1 = s 2^(cw): the canonical product.
+    const Fr prod = canon(mul(to_mont<FrParams>(sw), unpack<FrParams>(pw)));
+    uint32_t o[8];
+    pack(prod, o);
+    for (int k = 0; k < 8; ++k) out[t * 8 + k] = o[k];
+}
+}  // namespace
+
+extern "C" {
+
+int dvgpu_ext_window_table_device(void* backend, const void* scalars, size_t n, int window_bits, void* rows,
+                                  char* error, size_t capacity) {
+    Owner* o = as_owner(backend);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(window_bits >= 4 && window_bits <= 20 && (!n || (scalars && rows)), "invalid window-table arguments");
+        if (!n) return 0;
+        const uint32_t windows = uint32_t((255 + window_bits - 1) / window_bits);
+        require(n * windows <= (size_t(1) << 30), "window table too large");
+        SetupState& st = setup_of(o);
+        uint32_t* tab = o->stage_c.as<uint32_t>(n * windows * 8);
+        k_table_scalars<<<grid_for(n * windows, 256), 256, 0, o->stream>>>(static_cast<const uint32_t*>(scalars), n,
+                                                                          window_bits, windows, tab);
+        HIP_CHECK_LAUNCH();
+        fixed_base_mont_enqueue<FqOps>(st, tab, n * windows, static_cast<uint32_t*>(rows), o->stream);
+        HIP_CHECK(hipStreamSynchronize(o->stream));
+        return 0;
+    });
+}
+
 // dynaverse_hip_ext.h: device-resident SRS generation.
 int dvgpu_ext_fixed_base_device(void* backend, const void* scalars, size_t n, int g2, void* points,
                                 char* error, size_t capacity) {

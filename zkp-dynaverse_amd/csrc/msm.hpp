@@ -49,6 +49,11 @@ struct MsmItem {
     uint32_t point_base;   // first point (or first entry of the index array)
     uint32_t scalar_base;  // first scalar
     int c;                 // 0 = by size
+    // Window-table mode (0 = off): the points are rows of `table_windows`
+    // precomputed multiples [2^(c w)] B, so every digit of every window lands
+    // in one bucket set and no doublings combine windows.  Point indices then
+    // name rows; row r, window w is point r * table_windows + w.
+    uint32_t table_windows = 0;
 };
 
 // Per-MSM descriptor as the kernels read it (all bases are exclusive
@@ -61,7 +66,9 @@ struct MsmDesc {
     uint32_t thread_base, entry_base, bucket_base, chunk_base, window_base;
     uint32_t out_index;    // position of this MSM's result in the caller's list
     uint32_t result_slot;  // device finalize: also store the result at this point index
-    uint32_t pad[3];
+    uint32_t digit_windows;  // digits per scalar (= windows, or table_windows in table mode)
+    uint32_t table;          // 1 in window-table mode (then windows == 1)
+    uint32_t pad;
 };
 
 struct MsmBatchPlan {

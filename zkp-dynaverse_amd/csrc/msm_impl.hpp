@@ -145,7 +145,7 @@ static __global__ void k_digits(const MsmDesc* __restrict__ descs, uint32_t m, u
     const int c = int(d.c);
     const uint32_t full = 1u << c;
     uint32_t carry = 0;
-    for (uint32_t w = 0; w < d.windows; ++w) {
+    for (uint32_t w = 0; w < d.digit_windows; ++w) {
         uint32_t v = scalar_bits(s, int(w) * c, c) + carry;
         uint32_t sign = 0;
         if (v > d.buckets) {
@@ -160,7 +160,8 @@ static __global__ void k_digits(const MsmDesc* __restrict__ descs, uint32_t m, u
             digits[slot] = NO_DIGIT;
         } else {
             const uint32_t bucket = v - 1;
-            ranks[slot] = atomicAdd(&counts[d.bucket_base + w * d.buckets + bucket], 1u);
+            const uint32_t row = d.table ? 0u : w;  // table mode: one bucket set for all windows
+            ranks[slot] = atomicAdd(&counts[d.bucket_base + row * d.buckets + bucket], 1u);
             digits[slot] = (bucket << 1) | sign;
         }
     }
@@ -176,8 +177,12 @@ static __global__ void k_scatter(const MsmDesc* __restrict__ descs, uint32_t m, 
     if (dig == NO_DIGIT) return;
     const MsmDesc d = descs[find_desc<&MsmDesc::entry_base>(descs, m, e)];
     const uint32_t local = e - d.entry_base, w = local / d.n, i = local % d.n;
-    const uint32_t pos = offsets[d.bucket_base + w * d.buckets + (dig >> 1)].x + ranks[e];
+    const uint32_t pos = offsets[d.bucket_base + (d.table ? 0u : w) * d.buckets + (dig >> 1)].x + ranks[e];
     uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
+    if (d.table) {
+        const uint64_t p = uint64_t(pidx) * d.digit_windows + w;  // row, window -> table point
+        pidx = p < point_limit ? uint32_t(p) : point_limit;
+    }
     if (pidx >= point_limit) {  // flagged for the host; point 0 keeps the access in bounds
         flags[0] = 1u;
         pidx = 0;
@@ -511,6 +516,12 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
         d.n = uint32_t(it.n);
         d.c = uint32_t(c);
         d.windows = uint32_t((255 + c - 1) / c);
+        d.digit_windows = d.windows;
+        if (it.table_windows) {
+            require(it.table_windows == d.windows, "window-table rows must hold ceil(255 / c) points");
+            d.table = 1;
+            d.windows = 1;
+        }
         d.buckets = 1u << (c - 1);
         d.point_base = it.point_base;
         d.scalar_base = it.scalar_base;
@@ -522,7 +533,7 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
         d.out_index = uint32_t(p.descs.size());
         d.result_slot = MSM_NO_SLOT;
         threads += it.n;
-        entries += uint64_t(d.windows) * it.n;
+        entries += uint64_t(d.digit_windows) * it.n;
         buckets += uint64_t(d.windows) * d.buckets;
         chunks += uint64_t(d.windows) * (d.buckets / uint32_t(cfg.chunk));
         windows += d.windows;

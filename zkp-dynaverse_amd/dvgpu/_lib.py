@@ -140,8 +140,8 @@ class DvgpuBatchSpec(ctypes.Structure):
 
 
 class DvgpuUpdate(ctypes.Structure):
-    _fields_ = [("srs_base", ctypes.c_uint32), ("opening_base", ctypes.c_uint32),
-                ("commitment_slot", ctypes.c_uint32), ("opening_slot", ctypes.c_uint32),
+    _fields_ = [("srs_row", ctypes.c_uint32), ("opening_row", ctypes.c_uint32),
+                ("commitment_row", ctypes.c_uint32), ("opening_cache_row", ctypes.c_uint32),
                 ("coeff_offset", ctypes.c_uint64), ("eval_offset", ctypes.c_uint64),
                 ("linear_offset", ctypes.c_uint64), ("value_slot", ctypes.c_uint64),
                 ("log_n", ctypes.c_uint32), ("k", ctypes.c_uint32),
@@ -215,8 +215,10 @@ SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_uint32), c_void_p, c_size_t, *_E]),
     "dvgpu_ext_ntt_batch_device": (_I, [c_void_p, ctypes.POINTER(DvgpuNttRows), c_size_t, c_void_p,
                                         ctypes.POINTER(ctypes.c_float), *_E]),
-    "dvgpu_ext_update_batch": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, ctypes.POINTER(DvgpuUpdate),
-                                    c_size_t, c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float), *_E]),
+    "dvgpu_ext_update_batch": (_I, [c_void_p, c_void_p, c_size_t, _I, c_void_p, c_size_t,
+                                    ctypes.POINTER(DvgpuUpdate), c_size_t, c_void_p, c_void_p,
+                                    ctypes.POINTER(ctypes.c_float), *_E]),
+    "dvgpu_ext_window_table_device": (_I, [c_void_p, c_void_p, c_size_t, _I, c_void_p, *_E]),
     "dvgpu_ext_batch_create": (_I, [c_void_p, ctypes.POINTER(DvgpuBatchSpec), _I, ctypes.POINTER(c_void_p), *_E]),
     "dvgpu_ext_batch_run": (_I, [c_void_p, c_void_p, c_void_p, c_void_p,
                                  ctypes.POINTER(DvgpuG2ValidationReport), ctypes.POINTER(ctypes.c_float), *_E]),

@@ -124,7 +124,7 @@ def incremental(backend: HipBackend, log=print) -> Dict:
     hotspot (a fixed 1% index subset) and spreading alternating; served path
     (one update per call, the state on the device) against the plain path."""
     spec = SessionSpec(0, 1 << 16, 5)
-    store = SessionStore(backend, 64 << 20)
+    store = SessionStore(backend, 1 << 30)
     s = store.admit(spec)
     plain = PlainSession(backend, spec)
     rng = ChaCha20Stream(50)

@@ -5,21 +5,22 @@ A session (one Dynaverse state of size n) keeps everything an update needs in
 HBM, inside two arenas carved out of a fixed memory budget:
 
 * its own seeded SRS basis P_i = [tau^i] g1 and opening basis
-  U_i = [(tau^i - z^i) / (tau - z)] g1 (frozen, generated on the device by the
-  fixed-base kernel);
+  U_i = [(tau^i - z^i) / (tau - z)] g1, frozen, as window tables: row i holds
+  [2^(c w)] P_i for w < ceil(255 / c) (built on the device by the fixed-base
+  kernel), so an update's MSM needs no doublings between windows;
 * the cached commitment C = [f(tau)] g1 and opening W = [q(tau)] g1 with
-  q = (f - f(z)) / (X - z), stored as two points next to the bases;
+  q = (f - f(z)) / (X - z), as XYZZ points updated in place;
 * coefficients f, evaluations NTT(f), the frozen linear term l_i = z^i and the
   cached value y = f(z).
 
 An update request changes k coefficients; its proof is (C', W', y'), which
 satisfies the KZG opening relation f'(tau) - y' = q'(tau) (tau - z).  One
 ``dvgpu_ext_update_batch`` call serves a whole batch of requests (two sparse
-MSMs per request with in-place result slots, scatter-add, linear term, a
-fused multi-length NTT).  The ``Server`` keeps ``lanes`` such calls in flight,
-each on its own owner and stream; a request that arrives while others run
-joins the next batch that a free lane takes (continuous batching).  Requests
-of one session are applied in arrival order, one at a time.
+window-table MSMs per request, scatter-add, linear term, a fused
+multi-length NTT).  The ``Server`` keeps ``lanes`` such calls in flight, each
+on its own owner and stream; a request that arrives while others run joins
+the next batch that a free lane takes (continuous batching).  Requests of one
+session are applied in arrival order, one at a time.
 
 ``PlainSession`` is the one-call-at-a-time baseline through the reference's
 operator API (``msm_batch`` over host-gathered bases, ``ntt`` on host data),
@@ -114,6 +115,10 @@ class RangeAllocator:
         return sum(z for _, z in self.free)
 
 
+TABLE_BITS = 8                               # window width of the resident tables
+TABLE_ROW = -(-255 // TABLE_BITS)            # points per table row
+
+
 @dataclass
 class Session:
     spec: SessionSpec
@@ -121,7 +126,7 @@ class Session:
     tau: int
     t: int
     z: int
-    pt_base: int          # points: [P (n)] [U (n)] [C] [W]
+    row: int              # points arena rows: [P table (n)] [U table (n)] [C] [W]
     fr_base: int          # fr: [f (n)] [evals (n)] [l (n)] [y]
     busy: bool = False
 
@@ -132,19 +137,19 @@ class Session:
     def update_struct(self, idx: np.ndarray, deltas: bytes) -> _lib.DvgpuUpdate:
         n = self.n
         u = _lib.DvgpuUpdate()
-        u.srs_base, u.opening_base = self.pt_base, self.pt_base + n
-        u.commitment_slot, u.opening_slot = self.pt_base + 2 * n, self.pt_base + 2 * n + 1
+        u.srs_row, u.opening_row = self.row, self.row + n
+        u.commitment_row, u.opening_cache_row = self.row + 2 * n, self.row + 2 * n + 1
         u.coeff_offset, u.eval_offset = self.fr_base, self.fr_base + n
         u.linear_offset, u.value_slot = self.fr_base + 2 * n, self.fr_base + 3 * n
-        u.log_n, u.k = self.log_n, len(idx)
         idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        u.log_n, u.k = self.log_n, len(idx)
         u.indices = idx.ctypes.data if len(idx) else None
         u.deltas = ctypes.cast(ctypes.c_char_p(deltas), ctypes.c_void_p).value if len(idx) else None
         u._keep = (idx, deltas)       # the host arrays live as long as the struct
         return u
 
 
-def session_points(n: int) -> int:
+def session_rows(n: int) -> int:
     return 2 * n + 2
 
 
@@ -153,7 +158,7 @@ def session_fr(n: int) -> int:
 
 
 def session_bytes(n: int) -> int:
-    return 64 * session_points(n) + 32 * session_fr(n)
+    return 64 * TABLE_ROW * session_rows(n) + 32 * session_fr(n)
 
 
 def merge_update(indices: Sequence[int], deltas: Sequence[int]) -> Tuple[np.ndarray, bytes, List[int]]:
@@ -170,57 +175,59 @@ def merge_update(indices: Sequence[int], deltas: Sequence[int]) -> Tuple[np.ndar
 class SessionStore:
     """Session states in two HBM arenas under a fixed byte budget."""
 
-    def __init__(self, backend: HipBackend, budget_bytes: int, point_share: float = 0.7):
+    def __init__(self, backend: HipBackend, budget_bytes: int, point_share: float = 0.97):
         self.b = backend
         self.budget = int(budget_bytes)
-        pts = int(self.budget * point_share) // 64
-        frs = (self.budget - pts * 64) // 32
-        if pts >= 1 << 32:
+        rows = int(self.budget * point_share) // (64 * TABLE_ROW)
+        frs = (self.budget - rows * 64 * TABLE_ROW) // 32
+        if rows * TABLE_ROW >= 1 << 32:
             raise ValueError("point arena limited to 2^32 points")
-        self.points = backend.alloc(64 * pts)
+        self.points = backend.alloc(64 * TABLE_ROW * rows)
         self.fr = backend.alloc(32 * frs)
-        self.point_count, self.fr_count = pts, frs
-        self.pt_alloc, self.fr_alloc = RangeAllocator(pts), RangeAllocator(frs)
+        self.point_count, self.fr_count = rows * TABLE_ROW, frs
+        self.row_alloc, self.fr_alloc = RangeAllocator(rows), RangeAllocator(frs)
         self.sessions: Dict[int, Session] = {}
 
     def can_admit(self, n: int) -> bool:
-        return any(z >= session_points(n) for _, z in self.pt_alloc.free) and \
+        return any(z >= session_rows(n) for _, z in self.row_alloc.free) and \
             any(z >= session_fr(n) for _, z in self.fr_alloc.free)
 
     def admit(self, spec: SessionSpec) -> Optional[Session]:
-        """Generate the session's bases on the device and prove its anchor
-        state; None when the budget has no room."""
+        """Build the session's window tables on the device and prove its
+        anchor state; None when the budget has no room."""
         n = spec.n
         if n & (n - 1) or not 1 <= n <= 1 << 20:
             raise ValueError("session size must be a power of two <= 2^20")
-        pb = self.pt_alloc.alloc(session_points(n))
-        if pb is None:
+        row = self.row_alloc.alloc(session_rows(n))
+        if row is None:
             return None
         fb = self.fr_alloc.alloc(session_fr(n))
         if fb is None:
-            self.pt_alloc.release(pb, session_points(n))
+            self.row_alloc.release(row, session_rows(n))
             return None
         tau, t, z, coeffs = spec.derive()
-        s = Session(spec, n.bit_length() - 1, tau, t, z, pb, fb)
+        s = Session(spec, n.bit_length() - 1, tau, t, z, row, fb)
         a, u, l = session_scalars(tau, z, n)
         tmp = self.b.to_device(np.concatenate([a.reshape(-1), u.reshape(-1)]))
-        self.b.fixed_base_device(tmp.ptr, n, self.points.ptr + 64 * pb)
-        self.b.fixed_base_device(tmp.ptr + 32 * n, n, self.points.ptr + 64 * (pb + n))
+        rb = 64 * TABLE_ROW
+        for k, base in ((0, row), (1, row + n)):
+            err = _err()
+            _check(self.b.lib.dvgpu_ext_window_table_device(self.b.handle, tmp.ptr + 32 * n * k, n, TABLE_BITS,
+                                                            self.points.ptr + rb * base, err, 256), err)
         tmp.free()
-        self.points.upload_at(64 * (pb + 2 * n), np.zeros(128, dtype=np.uint8))
+        for r in (row + 2 * n, row + 2 * n + 1):
+            self.points.upload_at(rb * r, np.zeros(128, dtype=np.uint8))
         self.fr.upload_at(32 * fb, np.zeros(32 * 2 * n, dtype=np.uint8))
         self.fr.upload_at(32 * (fb + 2 * n), l.reshape(-1))
         self.fr.upload_at(32 * (fb + 3 * n), np.zeros(32, dtype=np.uint8))
         # Anchor: the initial coefficients enter as one dense update.
-        idx = np.arange(n, dtype=np.uint32)
-        deltas = coeffs.tobytes()
-        u_struct = s.update_struct(idx, deltas)
+        u_struct = s.update_struct(np.arange(n, dtype=np.uint32), coeffs.tobytes())
         update_batch(self.b, self, [u_struct])
         self.sessions[spec.sid] = s
         return s
 
     def release(self, s: Session) -> None:
-        self.pt_alloc.release(s.pt_base, session_points(s.n))
+        self.row_alloc.release(s.row, session_rows(s.n))
         self.fr_alloc.release(s.fr_base, session_fr(s.n))
         self.sessions.pop(s.spec.sid, None)
 
@@ -242,8 +249,8 @@ def update_batch(backend: HipBackend, store: SessionStore, updates: Sequence[_li
     out = ctypes.create_string_buffer(PROOF_BYTES * len(updates))
     ms = ctypes.c_float(0)
     err = _err()
-    _check(backend.lib.dvgpu_ext_update_batch(backend.handle, store.points.ptr, store.point_count, store.fr.ptr,
-                                              store.fr_count, arr, len(updates), _ROOT, out,
+    _check(backend.lib.dvgpu_ext_update_batch(backend.handle, store.points.ptr, store.point_count, TABLE_BITS,
+                                              store.fr.ptr, store.fr_count, arr, len(updates), _ROOT, out,
                                               ctypes.byref(ms) if timed else None, err, 256), err)
     proofs = [out.raw[PROOF_BYTES * i:PROOF_BYTES * (i + 1)] for i in range(len(updates))]
     return (proofs, ms.value) if timed else proofs
