@@ -189,6 +189,13 @@ int dvgpu_ext_update_batch(void* owner, void* points, size_t point_count, int wi
                            size_t fr_count, const DvgpuUpdate* updates, size_t count, const uint8_t* root,
                            uint8_t* results, float* device_ms, char* error, size_t error_capacity);
 
+/* Sizes every workspace of the owner for update batches of up to
+ * max_updates updates with max_entries changed coefficients and
+ * max_elements NTT elements in total, so that later batches within those
+ * bounds allocate (and free) nothing. */
+int dvgpu_ext_update_reserve(void* owner, size_t max_updates, size_t max_entries, size_t max_elements,
+                             int window_bits, const uint8_t* root, char* error, size_t error_capacity);
+
 #ifdef __cplusplus
 }
 #endif

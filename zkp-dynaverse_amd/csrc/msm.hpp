@@ -68,12 +68,25 @@ struct MsmDesc {
     uint32_t result_slot;  // device finalize: also store the result at this point index
     uint32_t digit_windows;  // digits per scalar (= windows, or table_windows in table mode)
     uint32_t table;          // 1 in window-table mode (then windows == 1)
+    uint32_t chunks;           // histogram blocks per window: ceil(n / 2^chunk_shift)
+    uint32_t hist_block_base;  // first histogram block (chunks x digit_windows per MSM)
+    uint32_t hist_base;        // first counter of its histograms (one bucket set per block)
+    uint32_t chunk_shift;      // log2 of the points per histogram block
     uint32_t pad;
 };
+
+// Digits are ranked per block of 2^chunk_shift points and one window in LDS
+// when every bucket set of a batch has at most MSM_LDS_BUCKETS buckets.  A
+// block's histogram costs one counter per bucket, so blocks hold at least
+// twice as many points as buckets (and 4096..32768 points).
+constexpr uint32_t MSM_LDS_BUCKETS = 16384;
+constexpr uint32_t MSM_DIGIT_THREADS = 1024;
 
 struct MsmBatchPlan {
     std::vector<MsmDesc> descs;
     uint32_t total_threads = 0, total_entries = 0, total_buckets = 0, total_chunks = 0, total_windows = 0;
+    uint32_t total_hist_blocks = 0, total_hist = 0;
+    bool lds_digits = false;  // every bucket set fits the LDS histogram
     size_t max_tasks = 0, max_heavy = 0;
     int chunk = 8;
     uint32_t task_points = 32;
@@ -91,13 +104,13 @@ constexpr int MSM_PHASE_EVENTS = 8;
 // Scratch owned by one backend; grows to the largest batch seen.
 struct MsmWorkspace {
     DeviceBuffer digits, ranks, counts, offsets, sorted, tasks, partials, chunks, block_sums, total,
-        points, scalars, windows, heavy, descs, finals;
+        points, scalars, windows, heavy, descs, finals, hist;
     HostBuffer host_windows, host_descs;
     // Optional phase events (MSM_PHASE_EVENTS of them) recorded by the
     // enqueue: start, digits, scan, scatter, tasks, accumulate, reduce, tree.
     hipEvent_t* phase_events = nullptr;
-    DeviceBuffer* all[16] = {&digits, &ranks, &counts, &offsets, &sorted, &tasks, &partials, &chunks,
-                             &block_sums, &total, &points, &scalars, &windows, &heavy, &descs, &finals};
+    DeviceBuffer* all[17] = {&digits, &ranks, &counts, &offsets, &sorted, &tasks, &partials, &chunks,
+                             &block_sums, &total, &points, &scalars, &windows, &heavy, &descs, &finals, &hist};
     void attach(Quota* q) {
         for (DeviceBuffer* b : all) b->quota = q;
     }

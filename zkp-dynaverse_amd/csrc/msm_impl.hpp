@@ -167,17 +167,106 @@ static __global__ void k_digits(const MsmDesc* __restrict__ descs, uint32_t m, u
     }
 }
 
+// Blocks of the LDS digit path map to XCDs round-robin; consecutive logical
+// blocks (the windows of one chunk of points) are placed on one XCD so that
+// its L2 serves the chunk's scalars to all of them.
+__device__ __forceinline__ uint32_t xcd_grouped_block(uint32_t bid, uint32_t grid) {
+    constexpr uint32_t XCDS = 8;
+    const uint32_t q = grid / XCDS, r = grid % XCDS, x = bid % XCDS, pos = bid / XCDS;
+    return x < r ? x * (q + 1) + pos : r * (q + 1) + (x - r) * q + pos;
+}
+
+// One block per (MSM, chunk of points, window): the signed digit of every
+// point of the chunk in that window (the carry in is recomputed from the
+// lower windows), ranked by an LDS atomic in the block's histogram, which
+// then goes to global memory for the per-bucket prefix over blocks.
+static __global__ void __launch_bounds__(MSM_DIGIT_THREADS) k_digits_lds(const MsmDesc* __restrict__ descs, uint32_t m,
+                                                           uint32_t grid, const uint32_t* __restrict__ scalars,
+                                                           uint32_t* __restrict__ hist, uint32_t* __restrict__ digits,
+                                                           uint32_t* __restrict__ ranks) {
+    __shared__ uint32_t counts[MSM_LDS_BUCKETS];
+    const uint32_t g = xcd_grouped_block(blockIdx.x, grid);
+    const MsmDesc d = descs[find_desc<&MsmDesc::hist_block_base>(descs, m, g)];
+    const uint32_t local = g - d.hist_block_base;
+    const uint32_t j = local / d.digit_windows, w = local % d.digit_windows;
+    for (uint32_t b = threadIdx.x; b < d.buckets; b += MSM_DIGIT_THREADS) counts[b] = 0;
+    __syncthreads();
+    const int c = int(d.c);
+    const uint32_t full = 1u << c;
+    const uint32_t i0 = j << d.chunk_shift, i1 = min(d.n, i0 + (1u << d.chunk_shift));
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += MSM_DIGIT_THREADS) {
+        const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(d.scalar_base + i) * 8);
+        const uint4 a = sp[0], b = sp[1];
+        const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        uint32_t carry = 0;
+        for (uint32_t k = 0; k < w; ++k) carry = scalar_bits(s, int(k) * c, c) + carry > d.buckets ? 1u : 0u;
+        uint32_t v = scalar_bits(s, int(w) * c, c) + carry;
+        uint32_t sign = 0;
+        if (v > d.buckets) {
+            v = full - v;
+            sign = 1;
+        }
+        const size_t slot = size_t(d.entry_base) + size_t(w) * d.n + i;
+        if (v == 0) {
+            digits[slot] = NO_DIGIT;
+        } else {
+            ranks[slot] = atomicAdd(&counts[v - 1], 1u);
+            digits[slot] = ((v - 1) << 1) | sign;
+        }
+    }
+    __syncthreads();
+    uint32_t* out = hist + d.hist_base + size_t(local) * d.buckets;
+    for (uint32_t b = threadIdx.x; b < d.buckets; b += MSM_DIGIT_THREADS) out[b] = counts[b];
+}
+
+// One thread per bucket: exclusive prefix of the bucket's per-block counts
+// over the blocks feeding it (in place), and the bucket's total.
+static __global__ void k_hist_prefix(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_buckets,
+                                     uint32_t* __restrict__ hist, uint32_t* __restrict__ counts) {
+    const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gb >= total_buckets) return;
+    const MsmDesc d = descs[find_desc<&MsmDesc::bucket_base>(descs, m, gb)];
+    const uint32_t local = gb - d.bucket_base, row = local / d.buckets, b = local % d.buckets;
+    uint32_t run = 0;
+    // Normal mode: bucket row w is fed by blocks (chunk j, window w); table
+    // mode: the single row is fed by every block of the MSM.
+    const uint32_t nblk = d.table ? d.chunks * d.digit_windows : d.chunks;
+    for (uint32_t k = 0; k < nblk; ++k) {
+        const uint32_t blk = d.table ? k : k * d.digit_windows + row;
+        uint32_t* p = hist + d.hist_base + size_t(blk) * d.buckets + b;
+        const uint32_t v = *p;
+        *p = run;
+        run += v;
+    }
+    counts[gb] = run;
+}
+
+// After the bucket scan: every block's prefix becomes an absolute position
+// (bucket start + earlier blocks' entries), so the scatter reads one word.
+static __global__ void k_hist_absolute(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_hist,
+                                       const uint2* __restrict__ offsets, uint32_t* __restrict__ hist) {
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= total_hist) return;
+    const MsmDesc d = descs[find_desc<&MsmDesc::hist_base>(descs, m, h)];
+    const uint32_t local = h - d.hist_base, blk = local / d.buckets, b = local % d.buckets;
+    const uint32_t row = d.table ? 0u : blk % d.digit_windows;
+    hist[h] += offsets[d.bucket_base + row * d.buckets + b].x;
+}
+
 static __global__ void k_scatter(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_entries,
                                  const uint32_t* __restrict__ digits, const uint32_t* __restrict__ ranks,
                                  const uint2* __restrict__ offsets, const uint32_t* __restrict__ point_index,
-                                 uint32_t point_limit, uint32_t* __restrict__ flags, uint32_t* __restrict__ sorted) {
+                                 uint32_t point_limit, uint32_t* __restrict__ flags,
+                                 const uint32_t* __restrict__ hist, uint32_t* __restrict__ sorted) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total_entries) return;
     const uint32_t dig = digits[e];
     if (dig == NO_DIGIT) return;
     const MsmDesc d = descs[find_desc<&MsmDesc::entry_base>(descs, m, e)];
     const uint32_t local = e - d.entry_base, w = local / d.n, i = local % d.n;
-    const uint32_t pos = offsets[d.bucket_base + (d.table ? 0u : w) * d.buckets + (dig >> 1)].x + ranks[e];
+    const uint32_t pos =
+        ranks[e] + (hist ? hist[d.hist_base + size_t((i >> d.chunk_shift) * d.digit_windows + w) * d.buckets + (dig >> 1)]
+                         : offsets[d.bucket_base + (d.table ? 0u : w) * d.buckets + (dig >> 1)].x);
     uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
     if (d.table) {
         const uint64_t p = uint64_t(pidx) * d.digit_windows + w;  // row, window -> table point
@@ -507,7 +596,8 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
     MsmBatchPlan p;
     p.chunk = cfg.chunk;
     p.task_points = uint32_t(cfg.task_points);
-    uint64_t threads = 0, entries = 0, buckets = 0, chunks = 0, windows = 0;
+    uint64_t threads = 0, entries = 0, buckets = 0, chunks = 0, windows = 0, hist_blocks = 0, hist_words = 0;
+    bool lds = true;
     for (const MsmItem& it : items) {
         require(it.n > 0 && it.n <= (size_t(1) << 24), "MSM length must be within 1..2^24");
         const int c = it.c ? it.c : (cfg.window_bits ? cfg.window_bits : msm_window_for_impl(it.n, table));
@@ -532,6 +622,14 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
         d.window_base = uint32_t(windows);
         d.out_index = uint32_t(p.descs.size());
         d.result_slot = MSM_NO_SLOT;
+        d.chunk_shift = 12;
+        while (d.chunk_shift < 15 && (1u << d.chunk_shift) < 2 * d.buckets) ++d.chunk_shift;
+        d.chunks = uint32_t((it.n + (size_t(1) << d.chunk_shift) - 1) >> d.chunk_shift);
+        d.hist_block_base = uint32_t(hist_blocks);
+        d.hist_base = uint32_t(hist_words);
+        hist_blocks += uint64_t(d.chunks) * d.digit_windows;
+        hist_words += uint64_t(d.chunks) * d.digit_windows * d.buckets;
+        lds &= d.buckets <= MSM_LDS_BUCKETS;
         threads += it.n;
         entries += uint64_t(d.digit_windows) * it.n;
         buckets += uint64_t(d.windows) * d.buckets;
@@ -539,7 +637,11 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
         windows += d.windows;
         p.descs.push_back(d);
     }
-    require(entries < (uint64_t(1) << 31) && buckets < (uint64_t(1) << 31), "MSM batch too large");
+    require(entries < (uint64_t(1) << 31) && buckets < (uint64_t(1) << 31) && hist_words < (uint64_t(1) << 31),
+            "MSM batch too large");
+    p.total_hist_blocks = uint32_t(hist_blocks);
+    p.total_hist = uint32_t(hist_words);
+    p.lds_digits = lds && hist_words <= uint64_t(entries) * 2 + (uint64_t(1) << 20);
     p.total_threads = uint32_t(threads);
     p.total_entries = uint32_t(entries);
     p.total_buckets = uint32_t(buckets);
@@ -566,6 +668,7 @@ inline void msm_batch_prepare_impl(const MsmBatchPlan& plan, int xw, MsmWorkspac
     ws.partials.as<uint32_t>(plan.max_tasks * xw);
     ws.heavy.as<uint32_t>(plan.max_heavy);
     ws.chunks.as<uint32_t>(size_t(plan.total_chunks) * xw);
+    if (plan.lds_digits) ws.hist.as<uint32_t>(plan.total_hist);
     // Earlier work on this stream may still read the pinned staging area.
     HIP_CHECK(hipStreamSynchronize(stream));
     MsmDesc* host = ws.host_descs.as<MsmDesc>(m);
@@ -604,22 +707,35 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
     auto mark = [&](int k) {
         if (ev) HIP_CHECK(hipEventRecord(ev[k], stream));
     };
+    uint32_t* hist = plan.lds_digits ? static_cast<uint32_t*>(ws.hist.ptr) : nullptr;
+    require(!plan.lds_digits || ws.hist.bytes >= size_t(plan.total_hist) * 4, "MSM workspace not prepared");
     mark(0);
-    k_fill<<<unsigned(std::min<size_t>(grid_for(plan.total_buckets, 256), 4096)), 256, 0, stream>>>(
-        counts, plan.total_buckets, 0u);
     k_fill<<<1, 64, 0, stream>>>(total, 4, 0u);
-    k_digits<<<grid_for(plan.total_threads, 256), 256, 0, stream>>>(descs, m, plan.total_threads, scalars, counts,
-                                                                   digits, ranks);
+    if (plan.lds_digits) {
+        k_digits_lds<<<plan.total_hist_blocks, MSM_DIGIT_THREADS, 0, stream>>>(descs, m, plan.total_hist_blocks,
+                                                                               scalars, hist, digits, ranks);
+        k_hist_prefix<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(descs, m, plan.total_buckets, hist,
+                                                                           counts);
+    } else {
+        k_fill<<<unsigned(std::min<size_t>(grid_for(plan.total_buckets, 256), 4096)), 256, 0, stream>>>(
+            counts, plan.total_buckets, 0u);
+        k_digits<<<grid_for(plan.total_threads, 256), 256, 0, stream>>>(descs, m, plan.total_threads, scalars,
+                                                                       counts, digits, ranks);
+    }
     HIP_CHECK_LAUNCH();
     mark(1);
     k_scan_tiles<<<tiles, SCAN_THREADS, 0, stream>>>(counts, plan.total_buckets, tp, tile_sums);
     k_scan_sums<<<1, SCAN_THREADS, 0, stream>>>(tile_sums, tiles, total);
     k_scan_apply<<<tiles, SCAN_THREADS, 0, stream>>>(counts, plan.total_buckets, tp, tile_sums, offsets);
     HIP_CHECK_LAUNCH();
+    if (plan.lds_digits) {
+        k_hist_absolute<<<grid_for(plan.total_hist, 256), 256, 0, stream>>>(descs, m, plan.total_hist, offsets, hist);
+        HIP_CHECK_LAUNCH();
+    }
     mark(2);
     k_scatter<<<grid_for(plan.total_entries, 256), 256, 0, stream>>>(descs, m, plan.total_entries, digits, ranks,
                                                                     offsets, point_index, plan.point_limit,
-                                                                    total + 3, sorted);
+                                                                    total + 3, hist, sorted);
     HIP_CHECK_LAUNCH();
     mark(3);
     k_tasks<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(counts, offsets, plan.total_buckets, tp, tasks,

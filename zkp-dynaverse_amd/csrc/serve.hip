@@ -375,4 +375,39 @@ int dvgpu_ext_update_batch(void* owner, void* points, size_t point_count, int wi
     });
 }
 
+int dvgpu_ext_update_reserve(void* owner, size_t max_updates, size_t max_entries, size_t max_elements,
+                             int window_bits, const uint8_t* root, char* error, size_t capacity) {
+    Owner* o = as_owner(owner);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(max_updates && max_updates <= (1u << 20) && window_bits >= 4 && window_bits <= 20 && root &&
+                    max_elements <= (size_t(1) << 31),
+                "invalid reservation");
+        uint32_t rw[8];
+        std::memcpy(rw, root, 32);
+        require(ntt_root_has_order_2_20(rw), "NTT root must have order 2^20");
+        const size_t m = max_updates, e = std::max(max_entries, m);
+        const uint32_t W = uint32_t((255 + window_bits - 1) / window_bits);
+        // The largest plan a batch within these bounds can produce: 2m MSMs
+        // over e points each side (the counts, not the split, size the buffers).
+        std::vector<MsmItem> items(2 * m, MsmItem{1, 0, 0, window_bits, W});
+        items[0].n = items[1].n = e - (m - 1);
+        MsmBatchPlan plan = msm_batch_plan(items, o->msm_config, &o->window_table);
+        hipStream_t st = o->stream;
+        msm_batch_prepare(plan, PointLayout<FqOps>::XYZZ_WORDS, o->msm, st);
+        o->msm.windows.as<uint32_t>(size_t(plan.total_windows) * PointLayout<FqOps>::XYZZ_WORDS);
+        o->msm.finals.as<uint32_t>(2 * m * 32);
+        o->msm.host_windows.as<uint8_t>(2 * m * 128 + m * 32 + 4);
+        const size_t words = 2 * e + e + m * sizeof(UpdDesc) / 4 + e * 8 + 16;
+        o->pinned.as<uint32_t>(words);
+        o->stage_a.as<uint32_t>(words);
+        o->stage_b.as<uint32_t>(m * 8);
+        ntt_domain_prepare(o->domain, rw, st);
+        o->ntt_batch.work.as<uint32_t>(std::max<size_t>(max_elements, 1) * 8);
+        o->ntt_batch.descs.as<uint8_t>(m * 256);
+        o->ntt_batch.host_descs.as<uint8_t>(m * 256);
+        HIP_CHECK(hipStreamSynchronize(st));
+        return 0;
+    });
+}
+
 }  // extern "C"

@@ -218,6 +218,7 @@ SIGNATURES = {
     "dvgpu_ext_update_batch": (_I, [c_void_p, c_void_p, c_size_t, _I, c_void_p, c_size_t,
                                     ctypes.POINTER(DvgpuUpdate), c_size_t, c_void_p, c_void_p,
                                     ctypes.POINTER(ctypes.c_float), *_E]),
+    "dvgpu_ext_update_reserve": (_I, [c_void_p, c_size_t, c_size_t, c_size_t, _I, c_void_p, *_E]),
     "dvgpu_ext_window_table_device": (_I, [c_void_p, c_void_p, c_size_t, _I, c_void_p, *_E]),
     "dvgpu_ext_batch_create": (_I, [c_void_p, ctypes.POINTER(DvgpuBatchSpec), _I, ctypes.POINTER(c_void_p), *_E]),
     "dvgpu_ext_batch_run": (_I, [c_void_p, c_void_p, c_void_p, c_void_p,

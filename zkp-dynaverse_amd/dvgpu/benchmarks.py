@@ -169,12 +169,15 @@ def _drive(srv: Server, stream, requests_prepared) -> List[float]:
     return reqs, lat, span
 
 
-def served_stream(backend: HipBackend, per_size: int = 16, count: int = 3000, qps_points=(), plain_count: int = 600,
-                  lanes: int = 2, log=print) -> Dict:
+def served_stream(backend: HipBackend, counts=(96, 72, 48, 24), count: int = 4000, qps_points=(),
+                  plain_count: int = 600, lanes: int = 2, log=print) -> Dict:
     """Poisson mixed-n update stream: served (continuous batching, resident
     state) at several arrival rates and saturated, against the plain path
-    run back to back (its maximum rate).  Every proof is compared."""
-    specs = session_pool(per_size, seed=6)
+    run back to back (its maximum rate).  Every proof is compared.  Sessions
+    per size follow the request mix (a session's updates apply in order, so
+    the pool, not the batch limit, bounds how many requests can share a
+    batch)."""
+    specs = session_pool(counts=counts, seed=6)
     store = SessionStore(backend, 32 << 30)
     t_setup = time.perf_counter()
     for sp in specs:

@@ -279,7 +279,7 @@ class Server:
     """Continuous-batching request loop over a SessionStore."""
 
     def __init__(self, store: SessionStore, lanes: int = 2, max_batch: int = 256,
-                 scratch_budget: int = 1 << 30):
+                 scratch_budget: int = 1 << 30, max_k: int = 64):
         self.store = store
         self.lanes = lanes
         self.max_batch = max_batch
@@ -292,6 +292,15 @@ class Server:
         self.batch_sizes: List[int] = []
         self.device_ms: List[float] = []
         self.backends = [HipBackend() for _ in range(lanes)]
+        # Every lane's workspaces are sized up front for the largest batch
+        # (one update per resident session at most), so serving allocates
+        # nothing: a growing device buffer would free the old one, and that
+        # synchronises the whole device, stalling every lane.
+        elements = sum(sess.n for sess in store.sessions.values()) or 1
+        for b in self.backends:
+            err = _err()
+            _check(b.lib.dvgpu_ext_update_reserve(b.handle, max_batch, max_batch * max_k, elements, TABLE_BITS,
+                                                  _ROOT, err, 256), err)
         self.work: List[collections.deque] = [collections.deque() for _ in range(lanes)]
         self.idle = list(range(lanes))
         self.threads = [threading.Thread(target=self._lane, args=(i,), daemon=True) for i in range(lanes)]
@@ -444,10 +453,12 @@ class PlainSession:
 SIZE_MIX = ((256, 0.4), (1024, 0.3), (4096, 0.2), (65536, 0.1))
 
 
-def session_pool(per_size: int, seed: int = 6) -> List[SessionSpec]:
+def session_pool(per_size: int = 0, seed: int = 6, counts: Optional[Sequence[int]] = None) -> List[SessionSpec]:
+    """Seeded sessions over the size mix: `per_size` of every size, or
+    `counts[i]` of size SIZE_MIX[i]."""
     specs, sid = [], 0
-    for n, _ in SIZE_MIX:
-        for _ in range(per_size):
+    for j, (n, _) in enumerate(SIZE_MIX):
+        for _ in range(counts[j] if counts is not None else per_size):
             specs.append(SessionSpec(sid, n, seed * 1_000_003 + sid))
             sid += 1
     return specs
