@@ -4,6 +4,7 @@ n = 2^16 session and the served mixed-n update stream against the
 one-call-at-a-time path.  Everything runs on seeded synthetic data; every
 served proof is compared with the plain path's bytes.
 """
+import gc
 import time
 from typing import Dict, List
 
@@ -153,20 +154,37 @@ def incremental(backend: HipBackend, log=print) -> Dict:
             "identical": bool(same)}
 
 
-def _drive(srv: Server, stream, requests_prepared) -> List[float]:
-    """Open-loop driver: submit each request at its arrival offset."""
-    t0 = time.perf_counter() + 0.01
-    reqs = []
-    for (t, sid, _, _), (mi, raw) in zip(stream, requests_prepared):
-        now = time.perf_counter()
-        if t0 + t > now:
-            time.sleep(t0 + t - now)
-        reqs.append(srv.submit(sid, mi, raw, arrival=t0 + t))
-    for r in reqs:
-        r.future.result(timeout=300)
+def _drive(srv: Server, stream, requests_prepared):
+    """Open-loop driver: submit each request at its arrival offset.  The
+    cyclic garbage collector is paused meanwhile (its pauses would land in
+    the tail latency of whichever requests are in flight)."""
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter() + 0.01
+        reqs = []
+        for (t, sid, _, _), (mi, raw) in zip(stream, requests_prepared):
+            now = time.perf_counter()
+            if t0 + t > now + 2e-4:
+                time.sleep(t0 + t - now)
+            reqs.append(srv.submit(sid, mi, raw, arrival=t0 + t))
+        for r in reqs:
+            r.future.result(timeout=300)
+    finally:
+        gc.enable()
     lat = [(r.done - r.arrival) * 1e3 for r in reqs]
     span = max(r.done for r in reqs) - t0
     return reqs, lat, span
+
+
+def _parts(reqs) -> Dict:
+    """p99 of the latency components: submission lag behind the scheduled
+    arrival, queueing until a batch starts, and the batch's service time."""
+    lag = [(r.submitted - r.arrival) * 1e3 for r in reqs]
+    wait = [(r.started - r.submitted) * 1e3 for r in reqs]
+    svc = [(r.done - r.started) * 1e3 for r in reqs]
+    return {"p99_submit_lag_ms": round(percentile(lag, 99), 3), "p99_queue_ms": round(percentile(wait, 99), 3),
+            "p99_batch_ms": round(percentile(svc, 99), 3), "p50_batch_ms": round(percentile(svc, 50), 3)}
 
 
 def served_stream(backend: HipBackend, counts=(96, 72, 48, 24), count: int = 4000, qps_points=(),
@@ -222,7 +240,7 @@ def served_stream(backend: HipBackend, counts=(96, 72, 48, 24), count: int = 400
         reqs, lat, span = _drive(srv, stream, prep)
         pts.append({"qps": round(q, 1), "updates_per_s": round(count / span, 1),
                     "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
-                    "mean_batch": round(float(np.mean(srv.batch_sizes)), 1)})
+                    "mean_batch": round(float(np.mean(srv.batch_sizes)), 1), **_parts(reqs)})
         log(f"# served at {q:.0f} QPS: p50 {pts[-1]['p50_ms']} ms, p99 {pts[-1]['p99_ms']} ms")
         srv.close()
     out["open_loop"] = pts

@@ -265,6 +265,7 @@ class Request:
     deltas: bytes
     arrival: float = 0.0
     future: Future = field(default_factory=Future)
+    submitted: float = 0.0
     started: float = 0.0
     done: float = 0.0
 
@@ -312,7 +313,9 @@ class Server:
     def submit(self, sid: int, indices: np.ndarray, deltas: bytes, arrival: Optional[float] = None) -> Request:
         if sid not in self.store.sessions:
             raise KeyError("unknown session")
-        req = Request(sid, indices, deltas, arrival if arrival is not None else time.perf_counter())
+        now = time.perf_counter()
+        req = Request(sid, indices, deltas, arrival if arrival is not None else now)
+        req.submitted = now
         with self.cv:
             self.queue.append(req)
             self.cv.notify_all()
