@@ -105,6 +105,47 @@ struct Mem<Fq2Ops> {
 constexpr uint32_t NO_DIGIT = 0xffffffffu;
 
 // bits [bit, bit + c) of a 256-bit little-endian scalar (c <= 24)
+// Signed c-bit digits in [-(2^(c-1) - 1), 2^(c-1)] by one addition: with
+// K = sum_k (2^(c-1) - 1) 2^(kc), digit k of s is window k of t = s + K minus
+// (2^(c-1) - 1) (the representation is unique, so this equals the
+// carry-propagating recoding).  t needs nine words once W c exceeds 256.
+__device__ __forceinline__ void recode_constant(int c, uint32_t windows, uint32_t (&k)[9]) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) k[i] = 0;
+    const uint64_t low = (uint64_t(1) << (c - 1)) - 1;
+    for (uint32_t w = 0; w < windows; ++w) {
+        const uint32_t bit = w * uint32_t(c), q = bit >> 5, sh = bit & 31;
+        const uint64_t v = low << sh;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            if (uint32_t(i) == q) k[i] |= uint32_t(v);
+            if (uint32_t(i) == q + 1) k[i] |= uint32_t(v >> 32);
+        }
+    }
+}
+
+__device__ __forceinline__ void recode_add(const uint32_t (&s)[8], const uint32_t (&k)[9], uint32_t (&t)[9]) {
+    uint64_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const uint64_t x = uint64_t(i < 8 ? s[i] : 0u) + k[i] + carry;
+        t[i] = uint32_t(x);
+        carry = x >> 32;
+    }
+}
+
+// Window `w` of the nine-word t (no dynamic register indexing: selects).
+__device__ __forceinline__ uint32_t window_of(const uint32_t (&t)[9], uint32_t bit, int c) {
+    const uint32_t q = bit >> 5, sh = bit & 31;
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        if (uint32_t(i) == q) lo = t[i];
+        if (uint32_t(i) == q + 1) hi = t[i];
+    }
+    return uint32_t(((uint64_t(hi) << 32) | lo) >> sh) & ((1u << c) - 1u);
+}
+
 __device__ __forceinline__ uint32_t scalar_bits(const uint32_t (&s)[8], int bit, int c) {
     const int j = bit >> 5, sh = bit & 31;
     uint64_t v = 0;
@@ -176,47 +217,58 @@ __device__ __forceinline__ uint32_t xcd_grouped_block(uint32_t bid, uint32_t gri
     return x < r ? x * (q + 1) + pos : r * (q + 1) + (x - r) * q + pos;
 }
 
-// One block per (MSM, chunk of points, window): the signed digit of every
-// point of the chunk in that window (the carry in is recomputed from the
-// lower windows), ranked by an LDS atomic in the block's histogram, which
-// then goes to global memory for the per-bucket prefix over blocks.
-static __global__ void __launch_bounds__(MSM_DIGIT_THREADS) k_digits_lds(const MsmDesc* __restrict__ descs, uint32_t m,
-                                                           uint32_t grid, const uint32_t* __restrict__ scalars,
-                                                           uint32_t* __restrict__ hist, uint32_t* __restrict__ digits,
-                                                           uint32_t* __restrict__ ranks) {
+// One block per (MSM, chunk of points, window), run twice.  Counting pass:
+// the signed digit of every point of the chunk in that window (the carry in
+// is recomputed from the lower windows) is counted in an LDS histogram, which
+// goes to global memory for the per-bucket prefix over blocks.  Placing pass:
+// the block loads its absolute start per bucket, recomputes the digits and
+// places every point with an LDS atomic on that start.  (The order within a
+// bucket is arbitrary; a bucket's sum does not depend on it.)
+template <bool PLACE>
+static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
+    k_digits_lds(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t grid, const uint32_t* __restrict__ scalars,
+                 uint32_t* __restrict__ hist, const uint32_t* __restrict__ point_index, uint32_t point_limit,
+                 uint32_t* __restrict__ flags, uint32_t* __restrict__ sorted) {
     __shared__ uint32_t counts[MSM_LDS_BUCKETS];
     const uint32_t g = xcd_grouped_block(blockIdx.x, grid);
     const MsmDesc d = descs[find_desc<&MsmDesc::hist_block_base>(descs, m, g)];
     const uint32_t local = g - d.hist_block_base;
     const uint32_t j = local / d.digit_windows, w = local % d.digit_windows;
-    for (uint32_t b = threadIdx.x; b < d.buckets; b += MSM_DIGIT_THREADS) counts[b] = 0;
+    uint32_t* mine = hist + d.hist_base + size_t(local) * d.buckets;
+    for (uint32_t b = threadIdx.x; b < d.buckets; b += MSM_DIGIT_THREADS) counts[b] = PLACE ? mine[b] : 0u;
     __syncthreads();
     const int c = int(d.c);
-    const uint32_t full = 1u << c;
     const uint32_t i0 = j << d.chunk_shift, i1 = min(d.n, i0 + (1u << d.chunk_shift));
+    uint32_t kconst[9];
+    recode_constant(c, d.digit_windows, kconst);
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += MSM_DIGIT_THREADS) {
         const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(d.scalar_base + i) * 8);
         const uint4 a = sp[0], b = sp[1];
         const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        uint32_t carry = 0;
-        for (uint32_t k = 0; k < w; ++k) carry = scalar_bits(s, int(k) * c, c) + carry > d.buckets ? 1u : 0u;
-        uint32_t v = scalar_bits(s, int(w) * c, c) + carry;
-        uint32_t sign = 0;
-        if (v > d.buckets) {
-            v = full - v;
-            sign = 1;
-        }
-        const size_t slot = size_t(d.entry_base) + size_t(w) * d.n + i;
-        if (v == 0) {
-            digits[slot] = NO_DIGIT;
-        } else {
-            ranks[slot] = atomicAdd(&counts[v - 1], 1u);
-            digits[slot] = ((v - 1) << 1) | sign;
+        uint32_t t[9];
+        recode_add(s, kconst, t);
+        const int32_t digit = int32_t(window_of(t, w * uint32_t(c), c)) - int32_t(d.buckets - 1);
+        if (digit == 0) continue;
+        const uint32_t sign = digit < 0 ? 1u : 0u;
+        const uint32_t v = uint32_t(digit < 0 ? -digit : digit);
+        const uint32_t pos = atomicAdd(&counts[v - 1], 1u);
+        if (PLACE) {
+            uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
+            if (d.table) {
+                const uint64_t p = uint64_t(pidx) * d.digit_windows + w;  // row, window -> table point
+                pidx = p < point_limit ? uint32_t(p) : point_limit;
+            }
+            if (pidx >= point_limit) {  // flagged for the host; point 0 keeps the access in bounds
+                flags[0] = 1u;
+                pidx = 0;
+            }
+            sorted[pos] = pidx | (sign << 31);
         }
     }
-    __syncthreads();
-    uint32_t* out = hist + d.hist_base + size_t(local) * d.buckets;
-    for (uint32_t b = threadIdx.x; b < d.buckets; b += MSM_DIGIT_THREADS) out[b] = counts[b];
+    if (!PLACE) {
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < d.buckets; b += MSM_DIGIT_THREADS) mine[b] = counts[b];
+    }
 }
 
 // One thread per bucket: exclusive prefix of the bucket's per-block counts
@@ -256,17 +308,14 @@ static __global__ void k_hist_absolute(const MsmDesc* __restrict__ descs, uint32
 static __global__ void k_scatter(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_entries,
                                  const uint32_t* __restrict__ digits, const uint32_t* __restrict__ ranks,
                                  const uint2* __restrict__ offsets, const uint32_t* __restrict__ point_index,
-                                 uint32_t point_limit, uint32_t* __restrict__ flags,
-                                 const uint32_t* __restrict__ hist, uint32_t* __restrict__ sorted) {
+                                 uint32_t point_limit, uint32_t* __restrict__ flags, uint32_t* __restrict__ sorted) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total_entries) return;
     const uint32_t dig = digits[e];
     if (dig == NO_DIGIT) return;
     const MsmDesc d = descs[find_desc<&MsmDesc::entry_base>(descs, m, e)];
     const uint32_t local = e - d.entry_base, w = local / d.n, i = local % d.n;
-    const uint32_t pos =
-        ranks[e] + (hist ? hist[d.hist_base + size_t((i >> d.chunk_shift) * d.digit_windows + w) * d.buckets + (dig >> 1)]
-                         : offsets[d.bucket_base + (d.table ? 0u : w) * d.buckets + (dig >> 1)].x);
+    const uint32_t pos = ranks[e] + offsets[d.bucket_base + (d.table ? 0u : w) * d.buckets + (dig >> 1)].x;
     uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
     if (d.table) {
         const uint64_t p = uint64_t(pidx) * d.digit_windows + w;  // row, window -> table point
@@ -393,13 +442,36 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t* __restrict__
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total[1]) return;
     const uint2 task = tasks[t];
+    constexpr int AW = PointLayout<F>::AFFINE_WORDS;
     Xyzz<F> acc = xyzz_inf<F>();
+    // Software pipeline: the next point's gather is in flight while the
+    // current point is added.
+    uint32_t entry = task.y ? sorted[task.x] : 0u;
+    uint4 raw[AW / 4];
+    if (task.y) {
+        const uint4* q = reinterpret_cast<const uint4*>(points + size_t(entry & 0x7fffffffu) * AW);
+#pragma unroll
+        for (int k = 0; k < AW / 4; ++k) raw[k] = q[k];
+    }
     for (uint32_t e = 0; e < task.y; ++e) {
-        const uint32_t entry = sorted[task.x + e];
-        const uint32_t idx = entry & 0x7fffffffu;
-        Affine<F> p = Mem<F>::load_affine(points + size_t(idx) * PointLayout<F>::AFFINE_WORDS);
+        uint32_t cur[AW];
+#pragma unroll
+        for (int k = 0; k < AW / 4; ++k) {
+            cur[4 * k] = raw[k].x;
+            cur[4 * k + 1] = raw[k].y;
+            cur[4 * k + 2] = raw[k].z;
+            cur[4 * k + 3] = raw[k].w;
+        }
+        const uint32_t sign = entry >> 31;
+        if (e + 1 < task.y) {
+            entry = sorted[task.x + e + 1];
+            const uint4* q = reinterpret_cast<const uint4*>(points + size_t(entry & 0x7fffffffu) * AW);
+#pragma unroll
+            for (int k = 0; k < AW / 4; ++k) raw[k] = q[k];
+        }
+        Affine<F> p = Mem<F>::load_affine(cur);
         if (p.inf) continue;
-        if (entry >> 31) p.y = F::neg(p.y);
+        if (sign) p.y = F::neg(p.y);
         acc = xyzz_madd<F>(acc, p.x, p.y);
     }
     Mem<F>::store_xyzz(partials + size_t(t) * PointLayout<F>::XYZZ_WORDS, acc);
@@ -439,6 +511,24 @@ __global__ void __launch_bounds__(256) k_heavy(const uint32_t* __restrict__ heav
 }
 
 
+// One thread per bucket of 2..HEAVY_TASKS tasks: the bucket's task partials
+// summed into its first task's partial (heavier buckets are summed by
+// k_heavy), so the chunk reduction reads one value per bucket.
+template <class F>
+__global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* __restrict__ counts,
+                                                     const uint2* __restrict__ offsets, uint32_t total_buckets,
+                                                     uint32_t task_points, uint32_t* __restrict__ partials) {
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
+    const uint32_t wb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (wb >= total_buckets) return;
+    const uint32_t nt = (counts[wb] + task_points - 1) / task_points;
+    if (nt < 2 || nt > HEAVY_TASKS) return;
+    const uint32_t t0 = offsets[wb].y;
+    Xyzz<F> s = Mem<F>::load_xyzz(partials + size_t(t0) * XW);
+    for (uint32_t k = 1; k < nt; ++k) s = xyzz_add<F>(s, Mem<F>::load_xyzz(partials + size_t(t0 + k) * XW));
+    Mem<F>::store_xyzz(partials + size_t(t0) * XW, s);
+}
+
 // One thread per chunk of L buckets of one window:
 //   out = sum_i (i + 1) S_{jL+i} + (jL) * sum_i S_{jL+i}
 // where S_b is the sum of bucket b's task partials.
@@ -457,15 +547,8 @@ __global__ void __launch_bounds__(256) k_reduce_chunks(const MsmDesc* __restrict
     for (int i = chunk - 1; i >= 0; --i) {
         const size_t wb = size_t(d.bucket_base) + size_t(w) * d.buckets + size_t(j) * chunk + i;
         const uint32_t cnt = counts[wb];
-        if (cnt) {
-            const uint32_t t0 = offsets[wb].y;
-            uint32_t nt = (cnt + task_points - 1) / task_points;
-            if (nt > HEAVY_TASKS) nt = 1;  // combined by k_heavy
-            Xyzz<F> s = Mem<F>::load_xyzz(partials + size_t(t0) * PointLayout<F>::XYZZ_WORDS);
-            for (uint32_t k = 1; k < nt; ++k)
-                s = xyzz_add<F>(s, Mem<F>::load_xyzz(partials + size_t(t0 + k) * PointLayout<F>::XYZZ_WORDS));
-            run = xyzz_add<F>(run, s);
-        }
+        if (cnt)  // the bucket total sits in its first task's partial (k_bucket_sums, k_heavy)
+            run = xyzz_add<F>(run, Mem<F>::load_xyzz(partials + size_t(offsets[wb].y) * PointLayout<F>::XYZZ_WORDS));
         acc = xyzz_add<F>(acc, run);
     }
     const uint32_t offset = j * uint32_t(chunk);
@@ -659,8 +742,10 @@ inline void msm_batch_prepare_impl(const MsmBatchPlan& plan, int xw, MsmWorkspac
     const uint32_t tiles = uint32_t((plan.total_buckets + SCAN_TILE - 1) / SCAN_TILE);
     ws.counts.as<uint32_t>(plan.total_buckets);
     ws.offsets.as<uint2>(plan.total_buckets);
-    ws.digits.as<uint32_t>(plan.total_entries);
-    ws.ranks.as<uint32_t>(plan.total_entries);
+    if (!plan.lds_digits) {
+        ws.digits.as<uint32_t>(plan.total_entries);
+        ws.ranks.as<uint32_t>(plan.total_entries);
+    }
     ws.sorted.as<uint32_t>(plan.total_entries);
     ws.block_sums.as<uint2>(tiles);
     ws.total.as<uint32_t>(4);
@@ -712,8 +797,8 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
     mark(0);
     k_fill<<<1, 64, 0, stream>>>(total, 4, 0u);
     if (plan.lds_digits) {
-        k_digits_lds<<<plan.total_hist_blocks, MSM_DIGIT_THREADS, 0, stream>>>(descs, m, plan.total_hist_blocks,
-                                                                               scalars, hist, digits, ranks);
+        k_digits_lds<false><<<plan.total_hist_blocks, MSM_DIGIT_THREADS, 0, stream>>>(
+            descs, m, plan.total_hist_blocks, scalars, hist, point_index, plan.point_limit, total + 3, sorted);
         k_hist_prefix<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(descs, m, plan.total_buckets, hist,
                                                                            counts);
     } else {
@@ -733,9 +818,13 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
         HIP_CHECK_LAUNCH();
     }
     mark(2);
-    k_scatter<<<grid_for(plan.total_entries, 256), 256, 0, stream>>>(descs, m, plan.total_entries, digits, ranks,
-                                                                    offsets, point_index, plan.point_limit,
-                                                                    total + 3, hist, sorted);
+    if (plan.lds_digits)
+        k_digits_lds<true><<<plan.total_hist_blocks, MSM_DIGIT_THREADS, 0, stream>>>(
+            descs, m, plan.total_hist_blocks, scalars, hist, point_index, plan.point_limit, total + 3, sorted);
+    else
+        k_scatter<<<grid_for(plan.total_entries, 256), 256, 0, stream>>>(descs, m, plan.total_entries, digits, ranks,
+                                                                        offsets, point_index, plan.point_limit,
+                                                                        total + 3, sorted);
     HIP_CHECK_LAUNCH();
     mark(3);
     k_tasks<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(counts, offsets, plan.total_buckets, tp, tasks,
@@ -747,6 +836,8 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
     mark(5);
     k_heavy<F><<<unsigned(std::min<size_t>(plan.max_heavy, 1024)), 256, 0, stream>>>(heavy, total + 2, counts,
                                                                                      offsets, tp, partials);
+    k_bucket_sums<F><<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(counts, offsets, plan.total_buckets, tp,
+                                                                           partials);
     k_reduce_chunks<F><<<grid_for(plan.total_chunks, 256), 256, 0, stream>>>(
         descs, m, plan.total_chunks, counts, offsets, partials, plan.chunk, tp, chunk_vals);
     HIP_CHECK_LAUNCH();

@@ -21,7 +21,13 @@ using namespace bn254;
 namespace {
 
 constexpr int TILE = 2048;  // elements per LDS tile (64 KiB)
-constexpr int THREADS = 256;
+constexpr int THREADS = 256;        // element-wise kernels
+constexpr int PASS_THREADS = 1024;  // DIF passes: one butterfly per thread and stage on a full tile
+
+// Threads for a DIF pass over tiles of `tile` elements (one per butterfly).
+inline unsigned pass_threads(uint32_t tile) {
+    return unsigned(std::min<uint32_t>(PASS_THREADS, std::max<uint32_t>(64, tile / 2)));
+}
 constexpr int MAX_STAGES = 11;
 
 __device__ __forceinline__ Fr ld8(const uint32_t* p) {
@@ -55,7 +61,7 @@ __device__ __forceinline__ void dif_tile(uint4* __restrict__ lds, const uint32_t
     const size_t lo0 = (t % lo_groups) * C;
     const size_t base = row * n + (hi << (a + S)) + lo0;
 
-    for (uint32_t e = threadIdx.x; e < tile_elems; e += THREADS) {
+    for (uint32_t e = threadIdx.x; e < tile_elems; e += blockDim.x) {
         const uint32_t mid = e / C, c = e % C;
         const uint32_t* src = in + (base + (size_t(mid) << a) + c) * 8;
         uint4 v0 = reinterpret_cast<const uint4*>(src)[0];
@@ -76,7 +82,7 @@ __device__ __forceinline__ void dif_tile(uint4* __restrict__ lds, const uint32_t
     for (int sl = S - 1; sl >= 0; --sl) {
         const int s = a + sl;
         const uint32_t h = 1u << sl;
-        for (uint32_t b = threadIdx.x; b < half; b += THREADS) {
+        for (uint32_t b = threadIdx.x; b < half; b += blockDim.x) {
             const uint32_t c = b % C, q = b / C;
             const uint32_t m1 = ((q >> sl) << (sl + 1)) | (q & (h - 1));
             const uint32_t m2 = m1 + h;
@@ -91,7 +97,7 @@ __device__ __forceinline__ void dif_tile(uint4* __restrict__ lds, const uint32_t
         __syncthreads();
     }
 
-    for (uint32_t e = threadIdx.x; e < tile_elems; e += THREADS) {
+    for (uint32_t e = threadIdx.x; e < tile_elems; e += blockDim.x) {
         const uint32_t mid = e / C, c = e % C;
         uint32_t* dst = out + (base + (size_t(mid) << a) + c) * 8;
         reinterpret_cast<uint4*>(dst)[0] = lds[2 * e];
@@ -100,7 +106,7 @@ __device__ __forceinline__ void dif_tile(uint4* __restrict__ lds, const uint32_t
 }
 
 template <bool CANON_IN>
-__global__ void __launch_bounds__(THREADS) k_dif_pass(const uint32_t* __restrict__ in,
+__global__ void __launch_bounds__(PASS_THREADS) k_dif_pass(const uint32_t* __restrict__ in,
                                                       uint32_t* __restrict__ out, int L, int a, int S,
                                                       int C, const uint32_t* __restrict__ tw) {
     __shared__ uint4 lds[TILE * 2];
@@ -157,7 +163,7 @@ __device__ __forceinline__ uint32_t find_entry(const NttBatchDesc* __restrict__ 
 }
 
 template <bool CANON_IN>
-__global__ void __launch_bounds__(THREADS) k_dif_pass_batch(const NttBatchDesc* __restrict__ descs, uint32_t m,
+__global__ void __launch_bounds__(PASS_THREADS) k_dif_pass_batch(const NttBatchDesc* __restrict__ descs, uint32_t m,
                                                             int p, const uint32_t* __restrict__ tw_fwd,
                                                             const uint32_t* __restrict__ tw_inv) {
     __shared__ uint4 lds[TILE * 2];
@@ -297,9 +303,10 @@ static void run_rows(const NttDomain& d, const uint32_t* in, uint32_t* out, int 
         const int C = int(std::min<size_t>(size_t(1) << a, size_t(TILE >> S)));
         const size_t blocks = rows * (n / (size_t(1u << S) * C));
         if (first && canonical)
-            k_dif_pass<true><<<unsigned(blocks), THREADS, 0, stream>>>(in, work, L, a, S, C, tw);
+            k_dif_pass<true><<<unsigned(blocks), pass_threads((1u << S) * C), 0, stream>>>(in, work, L, a, S, C, tw);
         else
-            k_dif_pass<false><<<unsigned(blocks), THREADS, 0, stream>>>(first ? in : work, work, L, a, S, C, tw);
+            k_dif_pass<false><<<unsigned(blocks), pass_threads((1u << S) * C), 0, stream>>>(first ? in : work, work, L,
+                                                                                           a, S, C, tw);
         HIP_CHECK_LAUNCH();
         first = false;
         top = a;
@@ -332,7 +339,7 @@ void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& ent
     }
     std::vector<NttBatchDesc> descs(entries.size());
     uint64_t work = 0, elems = 0;
-    uint32_t blocks[NTT_BATCH_MAX_PASSES] = {0};
+    uint32_t blocks[NTT_BATCH_MAX_PASSES] = {0}, tile_max[NTT_BATCH_MAX_PASSES] = {0};
     int max_passes = 0;
     for (size_t i = 0; i < entries.size(); ++i) {
         const NttBatchEntry& x = entries[i];
@@ -361,6 +368,7 @@ void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& ent
             e.S[p] = uint32_t(S);
             e.C[p] = uint32_t(C);
             const uint64_t nb = uint64_t(x.rows) * (n / (size_t(1u << S) * C));
+            tile_max[p] = std::max<uint32_t>(tile_max[p], (1u << S) * uint32_t(C));
             require(uint64_t(blocks[p]) + nb < (uint64_t(1) << 31), "NTT batch too large");
             blocks[p] += uint32_t(nb);
             top = a;
@@ -378,7 +386,10 @@ void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& ent
     plan.count = uint32_t(entries.size());
     plan.max_passes = max_passes;
     plan.total_elems = uint32_t(elems);
-    for (int p = 0; p < NTT_BATCH_MAX_PASSES; ++p) plan.pass_blocks[p] = blocks[p];
+    for (int p = 0; p < NTT_BATCH_MAX_PASSES; ++p) {
+        plan.pass_blocks[p] = blocks[p];
+        plan.pass_threads[p] = pass_threads(tile_max[p]);
+    }
     // Earlier work on this stream may still read the pinned descriptors.
     HIP_CHECK(hipStreamSynchronize(stream));
     auto* host = plan.host_descs.as<NttBatchDesc>(descs.size());
@@ -395,9 +406,11 @@ void ntt_batch_enqueue(const NttDomain& d, const NttBatchPlan& plan, hipStream_t
     for (int p = 0; p < plan.max_passes; ++p) {
         if (!plan.pass_blocks[p]) continue;
         if (p == 0)
-            k_dif_pass_batch<true><<<plan.pass_blocks[p], THREADS, 0, stream>>>(descs, plan.count, p, fwd, inv_tw);
+            k_dif_pass_batch<true><<<plan.pass_blocks[p], plan.pass_threads[p], 0, stream>>>(descs, plan.count, p, fwd,
+                                                                                              inv_tw);
         else
-            k_dif_pass_batch<false><<<plan.pass_blocks[p], THREADS, 0, stream>>>(descs, plan.count, p, fwd, inv_tw);
+            k_dif_pass_batch<false><<<plan.pass_blocks[p], plan.pass_threads[p], 0, stream>>>(descs, plan.count, p,
+                                                                                               fwd, inv_tw);
         HIP_CHECK_LAUNCH();
     }
     k_bitrev_batch<<<grid_for(plan.total_elems, THREADS), THREADS, 0, stream>>>(descs, plan.count, plan.total_elems);

@@ -65,6 +65,7 @@ struct NttBatchPlan {
     uint32_t count = 0, total_elems = 0;
     int max_passes = 0;
     uint32_t pass_blocks[NTT_BATCH_MAX_PASSES] = {0};
+    uint32_t pass_threads[NTT_BATCH_MAX_PASSES] = {0};
     DeviceBuffer descs, work;
     HostBuffer host_descs;
     void attach(Quota* q) { descs.quota = work.quota = q; }
