@@ -123,7 +123,7 @@ MsmBatchPlan plan_items(const DvgpuMsmItem* items, size_t count, const Owner* o,
 }
 
 // Batches of at least this many MSMs finish their window sums on the device.
-size_t finalize_threshold(const Owner* o) { return o->msm_finalize_min > 0 ? size_t(o->msm_finalize_min) : 8; }
+size_t finalize_threshold(const Owner* o) { return o->msm_finalize_min > 0 ? size_t(o->msm_finalize_min) : 4096; }
 
 // Enqueue the device epilogue: zero every result (MSMs skipped as empty stay
 // the identity), then one lane per planned MSM.
