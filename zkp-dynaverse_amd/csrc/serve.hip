@@ -185,6 +185,16 @@ void batch_to_affine(const uint32_t* xyzz, size_t count, uint8_t* out, size_t ou
     }
 }
 
+// Update MSMs are small and latency-bound (tens of points per bucket set of
+// 128): short accumulation tasks and one bucket per reduction thread shorten
+// the serial chains; k_heavy and the bucket sums absorb the extra partials.
+MsmConfig update_msm_config(const Owner* o) {
+    MsmConfig cfg = o->msm_config;
+    cfg.task_points = 4;
+    cfg.chunk = 1;
+    return cfg;
+}
+
 struct Interval {
     uint64_t lo, hi;  // [lo, hi)
 };
@@ -311,7 +321,7 @@ int dvgpu_ext_update_batch(void* owner, void* points, size_t point_count, int wi
         uint32_t* dev = o->stage_a.as<uint32_t>(total_words);
         HIP_CHECK(hipMemcpyAsync(dev, pin, total_words * 4, hipMemcpyHostToDevice, st));
         // Plans (each uploads its descriptors after synchronising the stream).
-        MsmBatchPlan plan = msm_batch_plan(items, o->msm_config, &o->window_table);
+        MsmBatchPlan plan = msm_batch_plan(items, update_msm_config(o), &o->window_table);
         plan.point_limit = uint32_t(point_count);
         for (size_t r = 0; r < m; ++r) {
             plan.descs[2 * r].result_slot = uint32_t(updates[r].commitment_row * W);
@@ -391,7 +401,7 @@ int dvgpu_ext_update_reserve(void* owner, size_t max_updates, size_t max_entries
         // over e points each side (the counts, not the split, size the buffers).
         std::vector<MsmItem> items(2 * m, MsmItem{1, 0, 0, window_bits, W});
         items[0].n = items[1].n = e - (m - 1);
-        MsmBatchPlan plan = msm_batch_plan(items, o->msm_config, &o->window_table);
+        MsmBatchPlan plan = msm_batch_plan(items, update_msm_config(o), &o->window_table);
         hipStream_t st = o->stream;
         msm_batch_prepare(plan, PointLayout<FqOps>::XYZZ_WORDS, o->msm, st);
         o->msm.windows.as<uint32_t>(size_t(plan.total_windows) * PointLayout<FqOps>::XYZZ_WORDS);
