@@ -74,6 +74,8 @@ typedef struct DvgpuTuning {
     int32_t msm_chunk;            /* buckets per first-level reduction thread: 1, 2, 4, 8 */
     int32_t msm_device_finalize_min; /* batches of at least this many MSMs sum their
                                         windows on the device (one lane per MSM) */
+    int32_t msm_chunk_by_log2[25];  /* buckets per reduction thread by log2 MSM size
+                                       (0 = msm_chunk) */
 } DvgpuTuning;
 int dvgpu_ext_set_tuning(void* owner, const DvgpuTuning* tuning);
 /* The window width the owner uses for an MSM of n points (positive), or a

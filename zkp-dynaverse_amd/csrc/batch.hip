@@ -183,7 +183,10 @@ int dvgpu_ext_set_tuning(void* owner, const DvgpuTuning* t) {
     if (t->msm_device_finalize_min < 0) return -1;
     if (t->msm_chunk != 0 && t->msm_chunk != 1 && t->msm_chunk != 2 && t->msm_chunk != 4 && t->msm_chunk != 8)
         return -1;
+    for (int32_t ch : t->msm_chunk_by_log2)
+        if (ch != 0 && ch != 1 && ch != 2 && ch != 4 && ch != 8) return -1;
     for (int i = 0; i < 25; ++i) o->window_table.c[i] = t->msm_window_bits[i];
+    for (int i = 0; i < 25; ++i) o->window_table.chunk[i] = t->msm_chunk_by_log2[i];
     for (int i = 0; i <= NTT_MAX_LOG; ++i) o->domain.pass_stages[i] = t->ntt_pass_stages[i];
     o->msm_config.task_points = t->msm_task_points ? t->msm_task_points : MsmConfig{}.task_points;
     o->msm_config.chunk = t->msm_chunk ? t->msm_chunk : MsmConfig{}.chunk;
@@ -203,6 +206,7 @@ int dvgpu_ext_get_tuning(void* owner, DvgpuTuning* t) {
     if (!o || o->retired) return -4;
     if (!t) return -1;
     for (int i = 0; i < 25; ++i) t->msm_window_bits[i] = o->window_table.c[i];
+    for (int i = 0; i < 25; ++i) t->msm_chunk_by_log2[i] = o->window_table.chunk[i];
     for (int i = 0; i <= NTT_MAX_LOG; ++i) t->ntt_pass_stages[i] = o->domain.pass_stages[i];
     t->msm_task_points = o->msm_config.task_points;
     t->msm_chunk = o->msm_config.chunk;

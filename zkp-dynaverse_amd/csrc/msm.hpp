@@ -40,6 +40,7 @@ struct MsmConfig {
 // entry of 0 falls back to the built-in rule.
 struct WindowTable {
     int c[25] = {0};
+    int chunk[25] = {0};  // buckets per reduction thread by log2 size (0 = the config's)
 };
 int msm_window_for(size_t n, const WindowTable* table);
 
@@ -72,7 +73,7 @@ struct MsmDesc {
     uint32_t hist_block_base;  // first histogram block (chunks x digit_windows per MSM)
     uint32_t hist_base;        // first counter of its histograms (one bucket set per block)
     uint32_t chunk_shift;      // log2 of the points per histogram block
-    uint32_t pad;
+    uint32_t chunk;            // buckets per first-level reduction thread
 };
 
 // Digits are ranked per block of 2^chunk_shift points and one window in LDS

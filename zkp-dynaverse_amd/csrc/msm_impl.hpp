@@ -537,11 +537,12 @@ template <class F>
 __global__ void __launch_bounds__(256) k_reduce_chunks(const MsmDesc* __restrict__ descs, uint32_t m,
                                                        uint32_t total_chunks, const uint32_t* __restrict__ counts,
                                                        const uint2* __restrict__ offsets,
-                                                       const uint32_t* __restrict__ partials, int chunk,
+                                                       const uint32_t* __restrict__ partials, int /*unused*/,
                                                        uint32_t task_points, uint32_t* __restrict__ out) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total_chunks) return;
     const MsmDesc d = descs[find_desc<&MsmDesc::chunk_base>(descs, m, g)];
+    const int chunk = int(d.chunk);
     const uint32_t per_window = d.buckets / uint32_t(chunk);
     const uint32_t local = g - d.chunk_base, w = local / per_window, j = local % per_window;
     Xyzz<F> run = xyzz_inf<F>(), acc = xyzz_inf<F>();
@@ -563,7 +564,7 @@ __global__ void __launch_bounds__(256) k_reduce_chunks(const MsmDesc* __restrict
 // One block per window of the batch: strided sums of the window's chunk
 // values, then an LDS tree; thread 0 writes the window value.
 template <class F>
-__global__ void __launch_bounds__(256) k_window_tree(const MsmDesc* __restrict__ descs, uint32_t m, int chunk,
+__global__ void __launch_bounds__(256) k_window_tree(const MsmDesc* __restrict__ descs, uint32_t m, int /*unused*/,
                                                      const uint32_t* __restrict__ chunk_vals,
                                                      uint32_t* __restrict__ out) {
     constexpr int XW = PointLayout<F>::XYZZ_WORDS;
@@ -571,7 +572,7 @@ __global__ void __launch_bounds__(256) k_window_tree(const MsmDesc* __restrict__
     uint32_t* slot = reinterpret_cast<uint32_t*>(lds) + threadIdx.x * XW;
     const uint32_t gw = blockIdx.x;
     const MsmDesc d = descs[find_desc<&MsmDesc::window_base>(descs, m, gw)];
-    const uint32_t w = gw - d.window_base, per_window = d.buckets / uint32_t(chunk);
+    const uint32_t w = gw - d.window_base, per_window = d.buckets / d.chunk;
     const size_t base = size_t(d.chunk_base) + size_t(w) * per_window;
     Xyzz<F> acc = xyzz_inf<F>();
     for (uint32_t k = threadIdx.x; k < per_window; k += 256)
@@ -659,7 +660,7 @@ __global__ void k_to_mont(const uint8_t* __restrict__ canonical, uint32_t* __res
 // (batches of 2^(18 - L) MSMs of 2^L points); outside the swept range
 // c = L - 3 within [4, 16].
 inline int msm_default_window_impl(size_t n) {
-    static const int measured[21] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 8, 8, 8, 11, 10, 10, 13, 13, 15, 15, 15};
+    static const int measured[21] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 8, 8, 8, 9, 10, 12, 13, 13, 15, 15, 15};
     int lg = 0;
     while ((size_t(1) << (lg + 1)) <= n) ++lg;
     if (lg >= 10 && lg <= 20) return measured[lg];
@@ -697,6 +698,13 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
             d.windows = 1;
         }
         d.buckets = 1u << (c - 1);
+        {
+            int lg = 0;
+            while ((size_t(1) << (lg + 1)) <= it.n) ++lg;
+            const int ch = table && lg < 25 && table->chunk[lg] ? table->chunk[lg] : cfg.chunk;
+            require(ch >= 1 && ch <= 8 && (ch & (ch - 1)) == 0, "invalid reduction chunk");
+            d.chunk = uint32_t(std::min<int>(ch, int(d.buckets)));
+        }
         d.point_base = it.point_base;
         d.scalar_base = it.scalar_base;
         d.thread_base = uint32_t(threads);
@@ -717,7 +725,7 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
         threads += it.n;
         entries += uint64_t(d.digit_windows) * it.n;
         buckets += uint64_t(d.windows) * d.buckets;
-        chunks += uint64_t(d.windows) * (d.buckets / uint32_t(cfg.chunk));
+        chunks += uint64_t(d.windows) * (d.buckets / d.chunk);
         windows += d.windows;
         p.descs.push_back(d);
     }

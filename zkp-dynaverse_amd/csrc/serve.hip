@@ -195,6 +195,13 @@ MsmConfig update_msm_config(const Owner* o) {
     return cfg;
 }
 
+// The update shape wins over per-size reduction chunks tuned for large MSMs.
+WindowTable update_window_table(const Owner* o) {
+    WindowTable t = o->window_table;
+    for (int& ch : t.chunk) ch = 0;
+    return t;
+}
+
 struct Interval {
     uint64_t lo, hi;  // [lo, hi)
 };
@@ -321,7 +328,8 @@ int dvgpu_ext_update_batch(void* owner, void* points, size_t point_count, int wi
         uint32_t* dev = o->stage_a.as<uint32_t>(total_words);
         HIP_CHECK(hipMemcpyAsync(dev, pin, total_words * 4, hipMemcpyHostToDevice, st));
         // Plans (each uploads its descriptors after synchronising the stream).
-        MsmBatchPlan plan = msm_batch_plan(items, update_msm_config(o), &o->window_table);
+        const WindowTable wt = update_window_table(o);
+        MsmBatchPlan plan = msm_batch_plan(items, update_msm_config(o), &wt);
         plan.point_limit = uint32_t(point_count);
         for (size_t r = 0; r < m; ++r) {
             plan.descs[2 * r].result_slot = uint32_t(updates[r].commitment_row * W);
@@ -401,7 +409,8 @@ int dvgpu_ext_update_reserve(void* owner, size_t max_updates, size_t max_entries
         // over e points each side (the counts, not the split, size the buffers).
         std::vector<MsmItem> items(2 * m, MsmItem{1, 0, 0, window_bits, W});
         items[0].n = items[1].n = e - (m - 1);
-        MsmBatchPlan plan = msm_batch_plan(items, update_msm_config(o), &o->window_table);
+        const WindowTable wt = update_window_table(o);
+        MsmBatchPlan plan = msm_batch_plan(items, update_msm_config(o), &wt);
         hipStream_t st = o->stream;
         msm_batch_prepare(plan, PointLayout<FqOps>::XYZZ_WORDS, o->msm, st);
         o->msm.windows.as<uint32_t>(size_t(plan.total_windows) * PointLayout<FqOps>::XYZZ_WORDS);

@@ -115,7 +115,7 @@ class DvgpuG2KernelResources(ctypes.Structure):
 class DvgpuTuning(ctypes.Structure):
     _fields_ = [("msm_window_bits", ctypes.c_int32 * 25), ("ntt_pass_stages", ctypes.c_int32 * 21),
                 ("msm_task_points", ctypes.c_int32), ("msm_chunk", ctypes.c_int32),
-                ("msm_device_finalize_min", ctypes.c_int32)]
+                ("msm_device_finalize_min", ctypes.c_int32), ("msm_chunk_by_log2", ctypes.c_int32 * 25)]
 
 
 class DvgpuMsmItem(ctypes.Structure):

@@ -42,7 +42,8 @@ def get(backend: HipBackend) -> dict:
     return {"msm_window_bits": {i: t.msm_window_bits[i] for i in range(25) if t.msm_window_bits[i]},
             "ntt_pass_stages": {i: t.ntt_pass_stages[i] for i in range(21) if t.ntt_pass_stages[i]},
             "msm_task_points": t.msm_task_points, "msm_chunk": t.msm_chunk,
-            "msm_device_finalize_min": t.msm_device_finalize_min}
+            "msm_device_finalize_min": t.msm_device_finalize_min,
+            "msm_chunk_by_log2": {i: t.msm_chunk_by_log2[i] for i in range(25) if t.msm_chunk_by_log2[i]}}
 
 
 def window_for(backend: HipBackend, n: int) -> int:
@@ -63,6 +64,8 @@ def apply(backend: HipBackend, table: dict) -> None:
     t.msm_task_points = int(table.get("msm_task_points", 0))
     t.msm_chunk = int(table.get("msm_chunk", 0))
     t.msm_device_finalize_min = int(table.get("msm_device_finalize_min", 0))
+    for k, v in table.get("msm_chunk_by_log2", {}).items():
+        t.msm_chunk_by_log2[int(k)] = int(v)
     if backend.lib.dvgpu_ext_set_tuning(backend.handle, ctypes.byref(t)) != 0:
         raise BackendError(-1, "invalid tuning table")
 
@@ -121,6 +124,23 @@ def tune_msm(backend, inp: _Inputs, logs, reps: int, log=print) -> Dict[int, dic
                   "mpoints_s": round(k * (1 << L) / (times[best] * 1e-3) / 1e6, 2)}
         log(f"msm 2^{L}: best c={best} {times[best]:.3f} ms for {k} MSMs "
             f"({out[L]['mpoints_s']} Mpoints/s); " + " ".join(f"{c}:{t:.2f}" for c, t in times.items()))
+    return out
+
+
+def tune_chunk(backend, inp: _Inputs, windows: Dict[int, int], logs, reps: int, log=print) -> Dict[int, int]:
+    """Buckets per reduction thread, per log2 size, at the chosen window."""
+    log_max = inp.n.bit_length() - 1
+    base = get(backend)
+    out = {}
+    for L in logs:
+        items = _msm_items(L, log_max, windows[L])
+        times = {}
+        for ch in (1, 2, 4, 8):
+            apply(backend, dict(base, msm_chunk_by_log2={L: ch}))
+            times[ch] = _median_ms(lambda: msm_batch_device(backend, inp.points, inp.d_sc2, items), reps)
+        out[L] = min(times, key=times.get)
+        log(f"chunk 2^{L}: best {out[L]}; " + " ".join(f"{c}:{t:.3f}" for c, t in times.items()))
+    apply(backend, base)
     return out
 
 
@@ -191,9 +211,12 @@ def tune(backend: Optional[HipBackend] = None, log_max: int = 20, reps: int = 5,
         windows = {L: v["c"] for L, v in msm.items()}
         apply(b, {"msm_window_bits": windows})
         shape = tune_shape(b, inp, windows, [12, 16, log_max], reps, log)
+        apply(b, {"msm_window_bits": windows, "msm_task_points": shape["task_points"]})
+        chunks = tune_chunk(b, inp, windows, logs, reps, log)
         ntt = tune_ntt(b, logs, reps, log)
         table = {"device": b.device_identity(), "msm_window_bits": {str(k): v for k, v in windows.items()},
                  "msm_task_points": shape["task_points"], "msm_chunk": shape["chunk"],
+                 "msm_chunk_by_log2": {str(k): v for k, v in chunks.items()},
                  "ntt_pass_stages": {str(k): v["stages"] for k, v in ntt.items()},
                  "sweep": {"msm": {str(k): v for k, v in msm.items()}, "shape": shape,
                            "ntt": {str(k): v for k, v in ntt.items()}}}
