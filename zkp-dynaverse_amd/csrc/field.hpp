@@ -57,6 +57,19 @@ HD Fp<C> fp_zero() {
 template <class C>
 HD Fp<C> fp_one() { return fp_const<C>(C::ONE); }
 
+// Low 32 bits of a*b.  On the device through v_mad_u64_u32, which issues at
+// full rate (the 32-bit v_mul_lo_u32 is quarter rate): the Montgomery digit
+// of every column is on the serial chain of a product.
+HD uint32_t mul_lo32(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t r, carry;
+    asm("v_mad_u64_u32 %0, %1, %2, %3, 0" : "=v"(r), "=s"(carry) : "v"(a), "v"(b));
+    return uint32_t(r);
+#else
+    return a * b;
+#endif
+}
+
 // Montgomery product a*b/R mod m, FIPS (product scanning) order.
 template <class C>
 HD Fp<C> mul(const Fp<C>& a, const Fp<C>& b) {
@@ -71,7 +84,7 @@ HD Fp<C> mul(const Fp<C>& a, const Fp<C>& b) {
             acc += uint64_t(m[i]) * C::MOD[k - i];
         }
         acc += uint64_t(a.l[k]) * b.l[0];
-        m[k] = (uint32_t(acc) * C::INV) & LIMB_MASK;
+        m[k] = mul_lo32(uint32_t(acc), C::INV) & LIMB_MASK;
         acc += uint64_t(m[k]) * C::MOD[0];
         acc >>= LIMB_BITS;
     }
@@ -108,7 +121,7 @@ HD Fp<C> sqr(const Fp<C>& a) {
         for (int i = (k < LIMBS ? 0 : k - LIMBS + 1); i < (k < LIMBS ? k : LIMBS); ++i)
             acc += uint64_t(m[i]) * C::MOD[k - i];
         if (k < LIMBS) {
-            m[k] = (uint32_t(acc) * C::INV) & LIMB_MASK;
+            m[k] = mul_lo32(uint32_t(acc), C::INV) & LIMB_MASK;
             acc += uint64_t(m[k]) * C::MOD[0];
         } else {
             out.l[k - LIMBS] = uint32_t(acc) & LIMB_MASK;
