@@ -89,8 +89,7 @@ struct MsmBatchPlan {
     uint32_t total_hist_blocks = 0, total_hist = 0;
     bool lds_digits = false;  // every bucket set fits the LDS histogram
     size_t max_tasks = 0, max_heavy = 0;
-    int chunk = 8;
-    uint32_t task_points = 32;
+    uint32_t task_points = 16;
     // Indexed points at or beyond this limit are counted in the workspace's
     // error word (total[3]) and read as point 0; only the finalize entry
     // point, which knows the point count, sets it.

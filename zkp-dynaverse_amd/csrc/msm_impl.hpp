@@ -537,8 +537,8 @@ template <class F>
 __global__ void __launch_bounds__(256) k_reduce_chunks(const MsmDesc* __restrict__ descs, uint32_t m,
                                                        uint32_t total_chunks, const uint32_t* __restrict__ counts,
                                                        const uint2* __restrict__ offsets,
-                                                       const uint32_t* __restrict__ partials, int /*unused*/,
-                                                       uint32_t task_points, uint32_t* __restrict__ out) {
+                                                       const uint32_t* __restrict__ partials,
+                                                       uint32_t* __restrict__ out) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total_chunks) return;
     const MsmDesc d = descs[find_desc<&MsmDesc::chunk_base>(descs, m, g)];
@@ -564,22 +564,28 @@ __global__ void __launch_bounds__(256) k_reduce_chunks(const MsmDesc* __restrict
 // One block per window of the batch: strided sums of the window's chunk
 // values, then an LDS tree; thread 0 writes the window value.
 template <class F>
-__global__ void __launch_bounds__(256) k_window_tree(const MsmDesc* __restrict__ descs, uint32_t m, int /*unused*/,
-                                                     const uint32_t* __restrict__ chunk_vals,
-                                                     uint32_t* __restrict__ out) {
-    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
-    __shared__ uint4 lds[256 * XW / 4];
+struct TreeThreads {
+    // 64 KiB of LDS partials: 512 G1 or 256 G2 XYZZ points per block.
+    static constexpr int value = 65536 / (PointLayout<F>::XYZZ_WORDS * 4);
+};
+
+template <class F>
+__global__ void __launch_bounds__(TreeThreads<F>::value)
+    k_window_tree(const MsmDesc* __restrict__ descs, uint32_t m, const uint32_t* __restrict__ chunk_vals,
+                  uint32_t* __restrict__ out) {
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS, T = TreeThreads<F>::value;
+    __shared__ uint4 lds[T * XW / 4];
     uint32_t* slot = reinterpret_cast<uint32_t*>(lds) + threadIdx.x * XW;
     const uint32_t gw = blockIdx.x;
     const MsmDesc d = descs[find_desc<&MsmDesc::window_base>(descs, m, gw)];
     const uint32_t w = gw - d.window_base, per_window = d.buckets / d.chunk;
     const size_t base = size_t(d.chunk_base) + size_t(w) * per_window;
     Xyzz<F> acc = xyzz_inf<F>();
-    for (uint32_t k = threadIdx.x; k < per_window; k += 256)
+    for (uint32_t k = threadIdx.x; k < per_window; k += T)
         acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(chunk_vals + (base + k) * XW));
     Mem<F>::store_xyzz(slot, acc);
     __syncthreads();
-    for (uint32_t s = 128; s > 0; s >>= 1) {
+    for (uint32_t s = T / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(slot + s * XW));
             Mem<F>::store_xyzz(slot, acc);
@@ -679,7 +685,6 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
     require(cfg.chunk >= 1 && cfg.chunk <= 8 && (cfg.chunk & (cfg.chunk - 1)) == 0 && cfg.task_points >= 1,
             "invalid MSM config");
     MsmBatchPlan p;
-    p.chunk = cfg.chunk;
     p.task_points = uint32_t(cfg.task_points);
     uint64_t threads = 0, entries = 0, buckets = 0, chunks = 0, windows = 0, hist_blocks = 0, hist_words = 0;
     bool lds = true;
@@ -848,10 +853,10 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
     k_bucket_sums<F><<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(counts, offsets, plan.total_buckets, tp,
                                                                            partials);
     k_reduce_chunks<F><<<grid_for(plan.total_chunks, 256), 256, 0, stream>>>(
-        descs, m, plan.total_chunks, counts, offsets, partials, plan.chunk, tp, chunk_vals);
+        descs, m, plan.total_chunks, counts, offsets, partials, chunk_vals);
     HIP_CHECK_LAUNCH();
     mark(6);
-    k_window_tree<F><<<plan.total_windows, 256, 0, stream>>>(descs, m, plan.chunk, chunk_vals, window_out);
+    k_window_tree<F><<<plan.total_windows, TreeThreads<F>::value, 0, stream>>>(descs, m, chunk_vals, window_out);
     HIP_CHECK_LAUNCH();
     mark(7);
 }
