@@ -20,7 +20,7 @@ using namespace bn254;
 
 namespace {
 
-constexpr int TILE = 2048;  // elements per LDS tile (64 KiB)
+constexpr int TILE = 2048;  // elements per LDS tile (72 KiB as nine limb planes)
 constexpr int THREADS = 256;        // element-wise kernels
 constexpr int PASS_THREADS = 1024;  // DIF passes: one butterfly per thread and stage on a full tile
 
@@ -47,8 +47,23 @@ __device__ __forceinline__ void st8(uint32_t* p, const Fr& v) {
 // One DIF pass over index bits [a, a + S) of `rows` transforms of length 2^L;
 // `g` is the tile of this block.  CANON_IN: the input holds canonical
 // integers (converted on load).
+// The tile lives in LDS as nine limb planes (limb k of element e at
+// lds[k * TILE + e]): butterflies read and write limbs directly, with no
+// packing, and consecutive lanes touch consecutive words.
+__device__ __forceinline__ Fr lds_ld(const uint32_t* __restrict__ lds, uint32_t e) {
+    Fr x;
+#pragma unroll
+    for (int k = 0; k < LIMBS; ++k) x.l[k] = lds[k * TILE + e];
+    return x;
+}
+
+__device__ __forceinline__ void lds_st(uint32_t* __restrict__ lds, uint32_t e, const Fr& x) {
+#pragma unroll
+    for (int k = 0; k < LIMBS; ++k) lds[k * TILE + e] = x.l[k];
+}
+
 template <bool CANON_IN>
-__device__ __forceinline__ void dif_tile(uint4* __restrict__ lds, const uint32_t* __restrict__ in,
+__device__ __forceinline__ void dif_tile(uint32_t* __restrict__ lds, const uint32_t* __restrict__ in,
                                          uint32_t* __restrict__ out, int L, int a, int S, int C,
                                          const uint32_t* __restrict__ tw, size_t g) {
     const size_t n = size_t(1) << L;
@@ -64,17 +79,10 @@ __device__ __forceinline__ void dif_tile(uint4* __restrict__ lds, const uint32_t
     for (uint32_t e = threadIdx.x; e < tile_elems; e += blockDim.x) {
         const uint32_t mid = e / C, c = e % C;
         const uint32_t* src = in + (base + (size_t(mid) << a) + c) * 8;
-        uint4 v0 = reinterpret_cast<const uint4*>(src)[0];
-        uint4 v1 = reinterpret_cast<const uint4*>(src)[1];
-        if (CANON_IN) {
-            const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-            uint32_t o[8];
-            pack(to_mont<FrParams>(w), o);
-            v0 = make_uint4(o[0], o[1], o[2], o[3]);
-            v1 = make_uint4(o[4], o[5], o[6], o[7]);
-        }
-        lds[2 * e] = v0;
-        lds[2 * e + 1] = v1;
+        const uint4 v0 = reinterpret_cast<const uint4*>(src)[0];
+        const uint4 v1 = reinterpret_cast<const uint4*>(src)[1];
+        const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+        lds_st(lds, e, CANON_IN ? to_mont<FrParams>(w) : unpack<FrParams>(w));
     }
     __syncthreads();
 
@@ -88,20 +96,17 @@ __device__ __forceinline__ void dif_tile(uint4* __restrict__ lds, const uint32_t
             const uint32_t m2 = m1 + h;
             const size_t j = (size_t(q & (h - 1)) << a) + lo0 + c;
             const Fr w = ld8(tw + (j << (NTT_MAX_LOG - 1 - s)) * 8);
-            uint32_t* p1 = reinterpret_cast<uint32_t*>(&lds[2 * (m1 * C + c)]);
-            uint32_t* p2 = reinterpret_cast<uint32_t*>(&lds[2 * (m2 * C + c)]);
-            const Fr x = ld8(p1), y = ld8(p2);
-            st8(p1, reduce(norm(add_w(x, y))));
-            st8(p2, mul(sub_w(x, y), w));
+            const uint32_t e1 = m1 * C + c, e2 = m2 * C + c;
+            const Fr x = lds_ld(lds, e1), y = lds_ld(lds, e2);
+            lds_st(lds, e1, reduce(norm(add_w(x, y))));
+            lds_st(lds, e2, mul(sub_w(x, y), w));
         }
         __syncthreads();
     }
 
     for (uint32_t e = threadIdx.x; e < tile_elems; e += blockDim.x) {
         const uint32_t mid = e / C, c = e % C;
-        uint32_t* dst = out + (base + (size_t(mid) << a) + c) * 8;
-        reinterpret_cast<uint4*>(dst)[0] = lds[2 * e];
-        reinterpret_cast<uint4*>(dst)[1] = lds[2 * e + 1];
+        st8(out + (base + (size_t(mid) << a) + c) * 8, lds_ld(lds, e));
     }
 }
 
@@ -109,7 +114,7 @@ template <bool CANON_IN>
 __global__ void __launch_bounds__(PASS_THREADS) k_dif_pass(const uint32_t* __restrict__ in,
                                                       uint32_t* __restrict__ out, int L, int a, int S,
                                                       int C, const uint32_t* __restrict__ tw) {
-    __shared__ uint4 lds[TILE * 2];
+    __shared__ uint32_t lds[LIMBS * TILE];
     dif_tile<CANON_IN>(lds, in, out, L, a, S, C, tw, blockIdx.x);
 }
 
@@ -166,7 +171,7 @@ template <bool CANON_IN>
 __global__ void __launch_bounds__(PASS_THREADS) k_dif_pass_batch(const NttBatchDesc* __restrict__ descs, uint32_t m,
                                                             int p, const uint32_t* __restrict__ tw_fwd,
                                                             const uint32_t* __restrict__ tw_inv) {
-    __shared__ uint4 lds[TILE * 2];
+    __shared__ uint32_t lds[LIMBS * TILE];
     __shared__ uint32_t entry;
     if (threadIdx.x == 0) entry = find_entry(descs, m, p, blockIdx.x);
     __syncthreads();
