@@ -74,3 +74,98 @@ def test_coset_point_constant_is_off_subgroup():
     T = twist_cofactor_point()
     assert o.g2_to_bytes(T) == COSET_POINT
     assert o.on_curve(T, o.FQ2) and o.mul(T, o.R, o.FQ2) is not None
+
+
+# ---- the request loop with stand-in lanes (no device) --------------------------
+
+import threading
+import time
+from types import SimpleNamespace
+
+from dvgpu.serving import Server, request_cost
+
+
+class FakeStore:
+    def __init__(self, sizes):
+        self.sessions = {sid: SimpleNamespace(n=n, busy=False) for sid, n in enumerate(sizes)}
+
+
+class FakeLane:
+    """Applies updates to a per-session log; the 'proof' is the log length."""
+    log = None
+    lock = threading.Lock()
+
+    def __init__(self, delay=0.002, poison=None):
+        self.delay, self.poison = delay, poison
+
+    def run(self, store, batch):
+        sids = [r.sid for r in batch]
+        assert len(set(sids)) == len(sids), "a session twice in one batch"
+        if self.poison is not None and any(int(r.indices[0]) == self.poison for r in batch):
+            raise RuntimeError("poisoned batch")
+        time.sleep(self.delay)
+        out = []
+        with FakeLane.lock:
+            for r in batch:
+                FakeLane.log.setdefault(r.sid, []).append(int(r.indices[0]))
+                out.append(len(FakeLane.log[r.sid]).to_bytes(4, "little"))
+        FakeLane.batches.append(len(batch))
+        return out, 0.0
+
+    def close(self):
+        pass
+
+
+def _serve(store, reqs, **kw):
+    FakeLane.log, FakeLane.batches = {}, []
+    lane_kw = kw.pop("lane", {})
+    srv = Server(store, lane_factory=lambda: FakeLane(**lane_kw), **kw)
+    try:
+        handles = [srv.submit(sid, np.array([seq], dtype=np.uint32), b"") for sid, seq in reqs]
+        results = []
+        for h in handles:
+            try:
+                results.append(h.future.result(timeout=30))
+            except RuntimeError as e:
+                results.append(e)
+        return srv, handles, results
+    finally:
+        srv.close()
+
+
+def test_server_keeps_session_order_and_shares_batches():
+    store = FakeStore([256] * 40)
+    rng = np.random.default_rng(3)
+    reqs = [(int(rng.integers(0, 40)), i) for i in range(600)]
+    srv, handles, results = _serve(store, reqs, lanes=2, max_batch=64)
+    for sid in range(40):
+        mine = [seq for s, seq in reqs if s == sid]
+        assert FakeLane.log.get(sid, []) == mine               # arrival order per session
+    # Each proof is its request's position in its session's sequence.
+    pos = {}
+    for (sid, _), res in zip(reqs, results):
+        pos[sid] = pos.get(sid, 0) + 1
+        assert int.from_bytes(res, "little") == pos[sid]
+    assert srv.batches < len(reqs) and max(FakeLane.batches) > 1   # continuous batching
+    assert all(h.done >= h.started >= h.submitted for h in handles)
+
+
+def test_server_batch_limit_and_scratch_budget():
+    store = FakeStore([65536] * 30)
+    reqs = [(i % 30, i) for i in range(300)]
+    budget = 3 * request_cost(65536, 1)                      # three large requests per lane batch
+    srv, _, _ = _serve(store, reqs, lanes=2, max_batch=64, scratch_budget=2 * budget)
+    assert max(FakeLane.batches) <= 3
+    srv2, _, _ = _serve(FakeStore([256] * 100), [(i, i) for i in range(100)], lanes=1, max_batch=8,
+                        lane={"delay": 0.01})
+    assert max(FakeLane.batches) <= 8
+
+
+def test_server_delivers_batch_errors_and_keeps_serving():
+    store = FakeStore([256] * 8)
+    reqs = [(i % 8, i) for i in range(80)]
+    srv, handles, results = _serve(store, reqs, lanes=2, max_batch=16, lane={"poison": 13})
+    failed = {seq for (sid, seq), r in zip(reqs, results) if isinstance(r, RuntimeError)}
+    assert 13 in failed and 1 <= len(failed) <= 8             # one batch failed, with all its requests
+    assert all(not isinstance(r, RuntimeError) for (sid, seq), r in zip(reqs, results) if seq > 40)
+    assert not any(s.busy for s in store.sessions.values())   # sessions released after the error

@@ -276,11 +276,37 @@ def request_cost(n: int, k: int) -> int:
     return 2 * (k + 1) * 64 * 16 + 64 * n + 4096
 
 
+class HipLane:
+    """One worker lane of the Server: its own owner (and so its own stream),
+    workspaces sized up front for the largest batch (one update per resident
+    session at most), so serving allocates nothing: a growing device buffer
+    would free the old one, and that synchronises the whole device, stalling
+    every lane."""
+
+    def __init__(self, store: SessionStore, max_batch: int, max_k: int):
+        self.b = HipBackend()
+        elements = sum(sess.n for sess in store.sessions.values()) or 1
+        err = _err()
+        _check(self.b.lib.dvgpu_ext_update_reserve(self.b.handle, max_batch, max_batch * max_k, elements,
+                                                   TABLE_BITS, _ROOT, err, 256), err)
+
+    def run(self, store: SessionStore, batch: Sequence["Request"]):
+        """-> (one 160-byte proof per request, device ms)."""
+        ups = [store.sessions[r.sid].update_struct(r.indices, r.deltas) for r in batch]
+        return update_batch(self.b, store, ups, timed=True)
+
+    def close(self):
+        self.b.close()
+
+
 class Server:
-    """Continuous-batching request loop over a SessionStore."""
+    """Continuous-batching request loop over a SessionStore.
+
+    `lane_factory()` makes one worker lane (an object with run(store, batch)
+    and close()); the default is a HipLane."""
 
     def __init__(self, store: SessionStore, lanes: int = 2, max_batch: int = 256,
-                 scratch_budget: int = 1 << 30, max_k: int = 64):
+                 scratch_budget: int = 1 << 30, max_k: int = 64, lane_factory=None):
         self.store = store
         self.lanes = lanes
         self.max_batch = max_batch
@@ -288,20 +314,11 @@ class Server:
         self.queue: collections.deque = collections.deque()
         self.cv = threading.Condition()
         self.stop_flag = False
-        self.free_lanes = lanes
         self.batches = 0
         self.batch_sizes: List[int] = []
         self.device_ms: List[float] = []
-        self.backends = [HipBackend() for _ in range(lanes)]
-        # Every lane's workspaces are sized up front for the largest batch
-        # (one update per resident session at most), so serving allocates
-        # nothing: a growing device buffer would free the old one, and that
-        # synchronises the whole device, stalling every lane.
-        elements = sum(sess.n for sess in store.sessions.values()) or 1
-        for b in self.backends:
-            err = _err()
-            _check(b.lib.dvgpu_ext_update_reserve(b.handle, max_batch, max_batch * max_k, elements, TABLE_BITS,
-                                                  _ROOT, err, 256), err)
+        factory = lane_factory or (lambda: HipLane(store, max_batch, max_k))
+        self.workers = [factory() for _ in range(lanes)]
         self.work: List[collections.deque] = [collections.deque() for _ in range(lanes)]
         self.idle = list(range(lanes))
         self.threads = [threading.Thread(target=self._lane, args=(i,), daemon=True) for i in range(lanes)]
@@ -360,7 +377,7 @@ class Server:
         return False
 
     def _lane(self, i: int):
-        b = self.backends[i]
+        worker = self.workers[i]
         while True:
             with self.cv:
                 while not self.stop_flag and not self.work[i]:
@@ -371,8 +388,7 @@ class Server:
             t0 = time.perf_counter()
             err = None
             try:
-                ups = [self.store.sessions[r.sid].update_struct(r.indices, r.deltas) for r in batch]
-                proofs, ms = update_batch(b, self.store, ups, timed=True)
+                proofs, ms = worker.run(self.store, batch)
             except Exception as e:          # delivered to every request of the batch
                 err, proofs, ms = e, None, 0.0
             t1 = time.perf_counter()
@@ -398,8 +414,8 @@ class Server:
         self.sched.join()
         for t in self.threads:
             t.join()
-        for b in self.backends:
-            b.close()
+        for w in self.workers:
+            w.close()
 
 
 # ---- one-call-at-a-time baseline ----------------------------------------------
