@@ -69,7 +69,7 @@ struct MsmDesc {
     uint32_t result_slot;  // device finalize: also store the result at this point index
     uint32_t digit_windows;  // digits per scalar (= windows, or table_windows in table mode)
     uint32_t table;          // 1 in window-table mode (then windows == 1)
-    uint32_t chunks;           // histogram blocks per window: ceil(n / 2^chunk_shift)
+    uint32_t chunks;           // chunks of 2^chunk_shift points (histogram blocks per window; per MSM in table mode)
     uint32_t hist_block_base;  // first histogram block (chunks x digit_windows per MSM)
     uint32_t hist_base;        // first counter of its histograms (one bucket set per block)
     uint32_t chunk_shift;      // log2 of the points per histogram block
@@ -86,7 +86,7 @@ constexpr uint32_t MSM_DIGIT_THREADS = 1024;
 struct MsmBatchPlan {
     std::vector<MsmDesc> descs;
     uint32_t total_threads = 0, total_entries = 0, total_buckets = 0, total_chunks = 0, total_windows = 0;
-    uint32_t total_hist_blocks = 0, total_hist = 0;
+    uint32_t total_hist_blocks = 0, total_hist = 0, digit_threads = MSM_DIGIT_THREADS, max_window_chunks = 0;
     bool lds_digits = false;  // every bucket set fits the LDS histogram
     size_t max_tasks = 0, max_heavy = 0;
     uint32_t task_points = 16;

@@ -234,41 +234,46 @@ static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
     const uint32_t g = xcd_grouped_block(blockIdx.x, grid);
     const MsmDesc d = descs[find_desc<&MsmDesc::hist_block_base>(descs, m, g)];
     const uint32_t local = g - d.hist_block_base;
-    const uint32_t j = local / d.digit_windows, w = local % d.digit_windows;
+    // Normal mode: block = (chunk j, window w).  Table mode: every window
+    // shares the bucket set, so one block covers all windows of its chunk.
+    const uint32_t j = d.table ? local : local / d.digit_windows;
+    const uint32_t w0 = d.table ? 0u : local % d.digit_windows, w1 = d.table ? d.digit_windows : w0 + 1;
     uint32_t* mine = hist + d.hist_base + size_t(local) * d.buckets;
-    for (uint32_t b = threadIdx.x; b < d.buckets; b += MSM_DIGIT_THREADS) counts[b] = PLACE ? mine[b] : 0u;
+    for (uint32_t b = threadIdx.x; b < d.buckets; b += blockDim.x) counts[b] = PLACE ? mine[b] : 0u;
     __syncthreads();
     const int c = int(d.c);
     const uint32_t i0 = j << d.chunk_shift, i1 = min(d.n, i0 + (1u << d.chunk_shift));
     uint32_t kconst[9];
     recode_constant(c, d.digit_windows, kconst);
-    for (uint32_t i = i0 + threadIdx.x; i < i1; i += MSM_DIGIT_THREADS) {
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(d.scalar_base + i) * 8);
         const uint4 a = sp[0], b = sp[1];
         const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
         uint32_t t[9];
         recode_add(s, kconst, t);
-        const int32_t digit = int32_t(window_of(t, w * uint32_t(c), c)) - int32_t(d.buckets - 1);
-        if (digit == 0) continue;
-        const uint32_t sign = digit < 0 ? 1u : 0u;
-        const uint32_t v = uint32_t(digit < 0 ? -digit : digit);
-        const uint32_t pos = atomicAdd(&counts[v - 1], 1u);
-        if (PLACE) {
-            uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
-            if (d.table) {
-                const uint64_t p = uint64_t(pidx) * d.digit_windows + w;  // row, window -> table point
-                pidx = p < point_limit ? uint32_t(p) : point_limit;
+        for (uint32_t w = w0; w < w1; ++w) {
+            const int32_t digit = int32_t(window_of(t, w * uint32_t(c), c)) - int32_t(d.buckets - 1);
+            if (digit == 0) continue;
+            const uint32_t sign = digit < 0 ? 1u : 0u;
+            const uint32_t v = uint32_t(digit < 0 ? -digit : digit);
+            const uint32_t pos = atomicAdd(&counts[v - 1], 1u);
+            if (PLACE) {
+                uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
+                if (d.table) {
+                    const uint64_t p = uint64_t(pidx) * d.digit_windows + w;  // row, window -> table point
+                    pidx = p < point_limit ? uint32_t(p) : point_limit;
+                }
+                if (pidx >= point_limit) {  // flagged for the host; point 0 keeps the access in bounds
+                    flags[0] = 1u;
+                    pidx = 0;
+                }
+                sorted[pos] = pidx | (sign << 31);
             }
-            if (pidx >= point_limit) {  // flagged for the host; point 0 keeps the access in bounds
-                flags[0] = 1u;
-                pidx = 0;
-            }
-            sorted[pos] = pidx | (sign << 31);
         }
     }
     if (!PLACE) {
         __syncthreads();
-        for (uint32_t b = threadIdx.x; b < d.buckets; b += MSM_DIGIT_THREADS) mine[b] = counts[b];
+        for (uint32_t b = threadIdx.x; b < d.buckets; b += blockDim.x) mine[b] = counts[b];
     }
 }
 
@@ -283,7 +288,7 @@ static __global__ void k_hist_prefix(const MsmDesc* __restrict__ descs, uint32_t
     uint32_t run = 0;
     // Normal mode: bucket row w is fed by blocks (chunk j, window w); table
     // mode: the single row is fed by every block of the MSM.
-    const uint32_t nblk = d.table ? d.chunks * d.digit_windows : d.chunks;
+    const uint32_t nblk = d.chunks;
     for (uint32_t k = 0; k < nblk; ++k) {
         const uint32_t blk = d.table ? k : k * d.digit_windows + row;
         uint32_t* p = hist + d.hist_base + size_t(blk) * d.buckets + b;
@@ -569,11 +574,11 @@ struct TreeThreads {
     static constexpr int value = 65536 / (PointLayout<F>::XYZZ_WORDS * 4);
 };
 
-template <class F>
-__global__ void __launch_bounds__(TreeThreads<F>::value)
+template <class F, int T>
+__global__ void __launch_bounds__(T)
     k_window_tree(const MsmDesc* __restrict__ descs, uint32_t m, const uint32_t* __restrict__ chunk_vals,
                   uint32_t* __restrict__ out) {
-    constexpr int XW = PointLayout<F>::XYZZ_WORDS, T = TreeThreads<F>::value;
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
     __shared__ uint4 lds[T * XW / 4];
     uint32_t* slot = reinterpret_cast<uint32_t*>(lds) + threadIdx.x * XW;
     const uint32_t gw = blockIdx.x;
@@ -687,6 +692,7 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
     MsmBatchPlan p;
     p.task_points = uint32_t(cfg.task_points);
     uint64_t threads = 0, entries = 0, buckets = 0, chunks = 0, windows = 0, hist_blocks = 0, hist_words = 0;
+    uint64_t digit_threads = 0, max_window_chunks = 0;
     bool lds = true;
     for (const MsmItem& it : items) {
         require(it.n > 0 && it.n <= (size_t(1) << 24), "MSM length must be within 1..2^24");
@@ -724,19 +730,25 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
         d.chunks = uint32_t((it.n + (size_t(1) << d.chunk_shift) - 1) >> d.chunk_shift);
         d.hist_block_base = uint32_t(hist_blocks);
         d.hist_base = uint32_t(hist_words);
-        hist_blocks += uint64_t(d.chunks) * d.digit_windows;
-        hist_words += uint64_t(d.chunks) * d.digit_windows * d.buckets;
+        const uint64_t blocks = uint64_t(d.chunks) * (d.table ? 1u : d.digit_windows);
+        hist_blocks += blocks;
+        hist_words += blocks * d.buckets;
+        digit_threads = std::max<uint64_t>(digit_threads, std::min<uint64_t>(it.n, uint64_t(1) << d.chunk_shift));
         lds &= d.buckets <= MSM_LDS_BUCKETS;
         threads += it.n;
         entries += uint64_t(d.digit_windows) * it.n;
         buckets += uint64_t(d.windows) * d.buckets;
         chunks += uint64_t(d.windows) * (d.buckets / d.chunk);
+        max_window_chunks = std::max<uint64_t>(max_window_chunks, d.buckets / d.chunk);
         windows += d.windows;
         p.descs.push_back(d);
     }
     require(entries < (uint64_t(1) << 31) && buckets < (uint64_t(1) << 31) && hist_words < (uint64_t(1) << 31),
             "MSM batch too large");
     p.total_hist_blocks = uint32_t(hist_blocks);
+    p.max_window_chunks = uint32_t(max_window_chunks);
+    // Digit blocks need at most one thread per point of their chunk.
+    p.digit_threads = uint32_t(std::min<uint64_t>(MSM_DIGIT_THREADS, std::max<uint64_t>(64, (digit_threads + 63) / 64 * 64)));
     p.total_hist = uint32_t(hist_words);
     p.lds_digits = lds && hist_words <= uint64_t(entries) * 2 + (uint64_t(1) << 20);
     p.total_threads = uint32_t(threads);
@@ -811,7 +823,7 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
     mark(0);
     k_fill<<<1, 64, 0, stream>>>(total, 4, 0u);
     if (plan.lds_digits) {
-        k_digits_lds<false><<<plan.total_hist_blocks, MSM_DIGIT_THREADS, 0, stream>>>(
+        k_digits_lds<false><<<plan.total_hist_blocks, plan.digit_threads, 0, stream>>>(
             descs, m, plan.total_hist_blocks, scalars, hist, point_index, plan.point_limit, total + 3, sorted);
         k_hist_prefix<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(descs, m, plan.total_buckets, hist,
                                                                            counts);
@@ -833,7 +845,7 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
     }
     mark(2);
     if (plan.lds_digits)
-        k_digits_lds<true><<<plan.total_hist_blocks, MSM_DIGIT_THREADS, 0, stream>>>(
+        k_digits_lds<true><<<plan.total_hist_blocks, plan.digit_threads, 0, stream>>>(
             descs, m, plan.total_hist_blocks, scalars, hist, point_index, plan.point_limit, total + 3, sorted);
     else
         k_scatter<<<grid_for(plan.total_entries, 256), 256, 0, stream>>>(descs, m, plan.total_entries, digits, ranks,
@@ -856,7 +868,11 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
         descs, m, plan.total_chunks, counts, offsets, partials, chunk_vals);
     HIP_CHECK_LAUNCH();
     mark(6);
-    k_window_tree<F><<<plan.total_windows, TreeThreads<F>::value, 0, stream>>>(descs, m, chunk_vals, window_out);
+    if (plan.max_window_chunks <= 128)  // small bucket sets (update batches): one value per thread
+        k_window_tree<F, 128><<<plan.total_windows, 128, 0, stream>>>(descs, m, chunk_vals, window_out);
+    else
+        k_window_tree<F, TreeThreads<F>::value><<<plan.total_windows, TreeThreads<F>::value, 0, stream>>>(
+            descs, m, chunk_vals, window_out);
     HIP_CHECK_LAUNCH();
     mark(7);
 }
