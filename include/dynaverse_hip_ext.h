@@ -93,7 +93,10 @@ typedef struct DvgpuMsmItem {
 
 /* count MSMs over device-resident Montgomery points (G1 or G2) and canonical
  * scalars in one kernel sequence; point_index (optional, device u32) selects
- * points for sparse MSMs.  results: count x 64 (G1) / 128 (G2) host bytes. */
+ * points for sparse MSMs.  results: count x 64 (G1) / 128 (G2) host bytes.
+ * Ranges are the caller's to keep (raw device pointers carry no length):
+ * every item's points, indices and scalars must lie inside their buffers;
+ * dvgpu_ext_msm_batch_finalize takes the lengths and checks them. */
 int dvgpu_ext_msm_batch_device(void* owner, const void* points, const void* point_index,
                                const void* scalars, const DvgpuMsmItem* items, size_t count, int g2,
                                uint8_t* results, float* phase_ms, char* error, size_t error_capacity);

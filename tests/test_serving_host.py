@@ -169,3 +169,12 @@ def test_server_delivers_batch_errors_and_keeps_serving():
     assert 13 in failed and 1 <= len(failed) <= 8             # one batch failed, with all its requests
     assert all(not isinstance(r, RuntimeError) for (sid, seq), r in zip(reqs, results) if seq > 40)
     assert not any(s.busy for s in store.sessions.values())   # sessions released after the error
+
+
+def test_update_record_layout_matches_struct():
+    import ctypes
+    from dvgpu import _lib
+    from dvgpu.serving import UPDATE_DTYPE
+    assert UPDATE_DTYPE.itemsize == ctypes.sizeof(_lib.DvgpuUpdate)
+    for name, _ in _lib.DvgpuUpdate._fields_:
+        assert UPDATE_DTYPE.fields[name][1] == getattr(_lib.DvgpuUpdate, name).offset, name

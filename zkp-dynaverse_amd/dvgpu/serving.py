@@ -129,6 +129,7 @@ class Session:
     row: int              # points arena rows: [P table (n)] [U table (n)] [C] [W]
     fr_base: int          # fr: [f (n)] [evals (n)] [l (n)] [y]
     busy: bool = False
+    record: tuple = ()
 
     @property
     def n(self):
@@ -207,6 +208,7 @@ class SessionStore:
             return None
         tau, t, z, coeffs = spec.derive()
         s = Session(spec, n.bit_length() - 1, tau, t, z, row, fb)
+        s.record = session_record(s)
         a, u, l = session_scalars(tau, z, n)
         tmp = self.b.to_device(np.concatenate([a.reshape(-1), u.reshape(-1)]))
         rb = 64 * TABLE_ROW
@@ -246,14 +248,58 @@ def update_batch(backend: HipBackend, store: SessionStore, updates: Sequence[_li
                  timed: bool = False):
     """One native call for the batch: -> list of 160-byte proofs (and ms)."""
     arr = (_lib.DvgpuUpdate * len(updates))(*updates)
-    out = ctypes.create_string_buffer(PROOF_BYTES * len(updates))
+    return _update_call(backend, store, ctypes.cast(arr, ctypes.POINTER(_lib.DvgpuUpdate)), len(updates), timed)
+
+
+def _update_call(backend, store, ptr, count, timed):
+    out = ctypes.create_string_buffer(PROOF_BYTES * count)
     ms = ctypes.c_float(0)
     err = _err()
     _check(backend.lib.dvgpu_ext_update_batch(backend.handle, store.points.ptr, store.point_count, TABLE_BITS,
-                                              store.fr.ptr, store.fr_count, arr, len(updates), _ROOT, out,
+                                              store.fr.ptr, store.fr_count, ptr, count, _ROOT, out,
                                               ctypes.byref(ms) if timed else None, err, 256), err)
-    proofs = [out.raw[PROOF_BYTES * i:PROOF_BYTES * (i + 1)] for i in range(len(updates))]
+    raw = out.raw
+    proofs = [raw[PROOF_BYTES * i:PROOF_BYTES * (i + 1)] for i in range(count)]
     return (proofs, ms.value) if timed else proofs
+
+
+# DvgpuUpdate as a numpy record, for building a whole batch without one
+# ctypes object per request (the layout is asserted against the struct).
+UPDATE_DTYPE = np.dtype([("srs_row", "<u4"), ("opening_row", "<u4"), ("commitment_row", "<u4"),
+                         ("opening_cache_row", "<u4"), ("coeff_offset", "<u8"), ("eval_offset", "<u8"),
+                         ("linear_offset", "<u8"), ("value_slot", "<u8"), ("log_n", "<u4"), ("k", "<u4"),
+                         ("indices", "<u8"), ("deltas", "<u8")])
+
+
+def session_record(s: "Session") -> tuple:
+    n = s.n
+    return (s.row, s.row + n, s.row + 2 * n, s.row + 2 * n + 1, s.fr_base, s.fr_base + n, s.fr_base + 2 * n,
+            s.fr_base + 3 * n, s.log_n, 0, 0, 0)
+
+
+def update_batch_records(backend: HipBackend, store: SessionStore, batch: Sequence["Request"], timed: bool = False):
+    """update_batch for queued requests, built as one numpy record array."""
+    m = len(batch)
+    rec = np.empty(m, dtype=UPDATE_DTYPE)
+    sessions = store.sessions
+    rec[:] = [sessions[r.sid].record for r in batch]
+    keep = []
+    ks = np.empty(m, dtype=np.uint32)
+    ip = np.zeros(m, dtype=np.uint64)
+    dp = np.zeros(m, dtype=np.uint64)
+    for i, r in enumerate(batch):
+        k = len(r.indices)
+        ks[i] = k
+        if k:
+            idx = r.indices if r.indices.dtype == np.uint32 and r.indices.flags.c_contiguous else \
+                np.ascontiguousarray(r.indices, dtype=np.uint32)
+            keep.append(idx)
+            ip[i] = idx.ctypes.data
+            dp[i] = ctypes.cast(ctypes.c_char_p(r.deltas), ctypes.c_void_p).value
+    rec["k"] = ks
+    rec["indices"] = ip
+    rec["deltas"] = dp
+    return _update_call(backend, store, rec.ctypes.data_as(ctypes.POINTER(_lib.DvgpuUpdate)), m, timed)
 
 
 # ---- request loop --------------------------------------------------------------
@@ -292,8 +338,7 @@ class HipLane:
 
     def run(self, store: SessionStore, batch: Sequence["Request"]):
         """-> (one 160-byte proof per request, device ms)."""
-        ups = [store.sessions[r.sid].update_struct(r.indices, r.deltas) for r in batch]
-        return update_batch(self.b, store, ups, timed=True)
+        return update_batch_records(self.b, store, batch, timed=True)
 
     def close(self):
         self.b.close()
@@ -311,7 +356,10 @@ class Server:
         self.lanes = lanes
         self.max_batch = max_batch
         self.lane_budget = scratch_budget // max(1, lanes)
-        self.queue: collections.deque = collections.deque()
+        # Per-session FIFOs and the sessions ready to run (idle, with a
+        # pending request), so forming a batch costs O(batch), not O(queue).
+        self.pending: Dict[int, collections.deque] = collections.defaultdict(collections.deque)
+        self.ready: collections.deque = collections.deque()
         self.cv = threading.Condition()
         self.stop_flag = False
         self.batches = 0
@@ -334,26 +382,28 @@ class Server:
         req = Request(sid, indices, deltas, arrival if arrival is not None else now)
         req.submitted = now
         with self.cv:
-            self.queue.append(req)
+            q = self.pending[sid]
+            q.append(req)
+            if len(q) == 1 and not self.store.sessions[sid].busy:
+                self.ready.append(sid)
             self.cv.notify_all()
         return req
 
     def _take_batch(self) -> List[Request]:
-        batch, cost, seen = [], 0, set()
-        keep = collections.deque()
-        while self.queue and len(batch) < self.max_batch:
-            req = self.queue.popleft()
-            s = self.store.sessions[req.sid]
-            c = request_cost(s.n, len(req.indices))
-            if s.busy or req.sid in seen or (batch and cost + c > self.lane_budget):
-                keep.append(req)          # stays queued, order kept per session
-                continue
-            seen.add(req.sid)
-            s.busy = True
+        """One request from each of up to max_batch ready sessions, in the
+        order they became ready, within the lane's scratch budget."""
+        batch, cost = [], 0
+        while self.ready and len(batch) < self.max_batch:
+            sid = self.ready[0]
+            req = self.pending[sid][0]
+            c = request_cost(self.store.sessions[sid].n, len(req.indices))
+            if batch and cost + c > self.lane_budget:
+                break
+            self.ready.popleft()
+            self.pending[sid].popleft()
+            self.store.sessions[sid].busy = True
             batch.append(req)
             cost += c
-        while keep:
-            self.queue.appendleft(keep.pop())
         return batch
 
     def _schedule(self):
@@ -371,10 +421,7 @@ class Server:
                 self.cv.notify_all()
 
     def _runnable(self) -> bool:
-        for req in self.queue:
-            if not self.store.sessions[req.sid].busy:
-                return True
-        return False
+        return bool(self.ready)
 
     def _lane(self, i: int):
         worker = self.workers[i]
@@ -393,9 +440,11 @@ class Server:
                 err, proofs, ms = e, None, 0.0
             t1 = time.perf_counter()
             with self.cv:
-                for k, r in enumerate(batch):
+                for r in batch:
                     r.started, r.done = t0, t1
                     self.store.sessions[r.sid].busy = False
+                    if self.pending[r.sid]:
+                        self.ready.append(r.sid)
                 self.batches += 1
                 self.batch_sizes.append(len(batch))
                 self.device_ms.append(ms)
