@@ -2,6 +2,7 @@
 // Entry-point semantics follow native/src/icicle_msm.cu (cited per function).
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "owner.hpp"
 
@@ -312,14 +313,22 @@ int dvgpu_bn254_msm_batch(void* backend, const uint8_t* points, const uint8_t* s
         HIP_CHECK(hipMemcpyAsync(dpts, points, total * 64, hipMemcpyHostToDevice, o->stream));
         HIP_CHECK(hipMemcpyAsync(dsc, scalars, total * 32, hipMemcpyHostToDevice, o->stream));
         points_to_mont<FqOps>(dpts, mont, total, o->stream);
+        // All sets in one batched kernel sequence; windows combined on the host pool.
+        std::vector<MsmItem> items;
+        for (size_t i = 0; i < count; ++i)
+            items.push_back({length, uint32_t(i * length), uint32_t(i * length), 0});
+        const MsmBatchPlan plan = msm_batch_plan(items, o->msm_config, &o->window_table);
+        constexpr int XW = PointLayout<FqOps>::XYZZ_WORDS;
+        msm_batch_prepare(plan, XW, o->msm, o->stream);
+        uint32_t* win = o->msm.windows.as<uint32_t>(size_t(plan.total_windows) * XW);
+        msm_batch_enqueue<FqOps>(mont, nullptr, dsc, plan, o->msm, o->stream, win);
+        uint32_t* host = o->msm.host_windows.as<uint32_t>(size_t(plan.total_windows) * XW);
+        HIP_CHECK(hipMemcpyAsync(host, win, size_t(plan.total_windows) * XW * 4, hipMemcpyDeviceToHost, o->stream));
+        HIP_CHECK(hipStreamSynchronize(o->stream));
+        std::vector<Affine<FqOps>> res(count);
+        msm_batch_combine<FqOps>(host, plan, res.data());
         uint8_t encoded[DVGPU_MAX_MSM_BATCH_SIZE * 64];
-        for (size_t i = 0; i < count; ++i) {
-            MsmConfig cfg = o->msm_config;
-            if (!cfg.window_bits) cfg.window_bits = msm_window_for(length, &o->window_table);
-            const Affine<FqOps> r = msm_run<FqOps>(mont + i * length * 16, dsc + i * length * 8, length,
-                                                   o->msm, o->stream, cfg);
-            affine_to_bytes<FqOps>(r, encoded + i * 64);
-        }
+        for (size_t i = 0; i < count; ++i) affine_to_bytes<FqOps>(res[i], encoded + i * 64);
         std::memcpy(results, encoded, count * 64);
         return 0;
     });
