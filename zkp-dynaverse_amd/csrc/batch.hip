@@ -54,25 +54,45 @@ struct BatchObj {
                       MsmWorkspace& w, DeviceBuffer& win, DeviceBuffer& fin, HostBuffer& host, bool dev_fin,
                       size_t count, hipStream_t st);
 
+    // NTT rows and G2 validation share no buffer with the MSMs: they run on a
+    // second stream forked from and joined back into the owner's stream (in a
+    // captured graph: a parallel branch), so the latency-bound subgroup
+    // checks overlap the MSM kernels.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+
     void enqueue(hipStream_t st) {
+        const bool branch = wire_n || !ntt_groups.empty();
+        if (branch) {
+            HIP_CHECK(hipEventRecord(fork, st));
+            HIP_CHECK(hipStreamWaitEvent(side, fork, 0));
+            for (const auto& g : ntt_groups) ntt_batch_enqueue(domain, *g, side);
+            if (wire_n) {
+                auto* rep = static_cast<DvgpuG2ValidationReport*>(report_dev.ptr);
+                g2_validate_enqueue(wire, wire_n, algorithm, masks, rep, owner->g2_kernel.threads, side, nullptr,
+                                    nullptr);
+                HIP_CHECK(hipMemcpyAsync(report_host.ptr, rep, sizeof(DvgpuG2ValidationReport),
+                                         hipMemcpyDeviceToHost, side));
+            }
+            HIP_CHECK(hipEventRecord(join, side));
+        }
         if (!p1.descs.empty())
             enqueue_msms<FqOps>(g1_points, g1_index, g1_scalars, p1, w1, win1, fin1, host1, dev_fin1, count1, st);
         if (!p2.descs.empty())
             enqueue_msms<Fq2Ops>(g2_points, g2_index, g2_scalars, p2, w2, win2, fin2, host2, dev_fin2, count2, st);
-        for (const auto& g : ntt_groups) ntt_batch_enqueue(domain, *g, st);
-        if (wire_n) {
-            auto* rep = static_cast<DvgpuG2ValidationReport*>(report_dev.ptr);
-            g2_validate_enqueue(wire, wire_n, algorithm, masks, rep, owner->g2_kernel.threads, st, nullptr, nullptr);
-            HIP_CHECK(hipMemcpyAsync(report_host.ptr, rep, sizeof(DvgpuG2ValidationReport), hipMemcpyDeviceToHost, st));
-        }
+        if (branch) HIP_CHECK(hipStreamWaitEvent(st, join, 0));
     }
 
     ~BatchObj() {
         if (owner && owner->stream) (void)hipStreamSynchronize(owner->stream);
+        if (side) (void)hipStreamSynchronize(side);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
         w1.release();
         w2.release();
         for (auto& g : ntt_groups) g->release();
@@ -451,6 +471,9 @@ int dvgpu_ext_batch_create(void* owner, const DvgpuBatchSpec* spec, int use_grap
         }
         HIP_CHECK(hipEventCreate(&b->e0));
         HIP_CHECK(hipEventCreate(&b->e1));
+        HIP_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
+        HIP_CHECK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
         // Warm-up run: sizes every remaining workspace before any capture.
         b->enqueue(st);
         HIP_CHECK(hipStreamSynchronize(st));
