@@ -124,7 +124,19 @@ namespace {
 Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
 
 // Plan the non-empty items; map[i] is the caller's index of planned MSM i.
-MsmBatchPlan plan_items(const DvgpuMsmItem* items, size_t count, const Owner* o, std::vector<size_t>& map) {
+// G2 MSMs: every group operation costs about three G1 ones, so the serial
+// bucket-reduction chains are cut short (two buckets per reduction thread,
+// whatever the G1-tuned per-size table says).
+MsmConfig g2_config(const Owner* o, WindowTable& table) {
+    MsmConfig cfg = o->msm_config;
+    cfg.chunk = std::min(cfg.chunk, 2);
+    table = o->window_table;
+    for (int& ch : table.chunk) ch = 0;
+    return cfg;
+}
+
+MsmBatchPlan plan_items(const DvgpuMsmItem* items, size_t count, const Owner* o, std::vector<size_t>& map,
+                        bool g2) {
     std::vector<MsmItem> v;
     for (size_t i = 0; i < count; ++i) {
         require(items[i].n <= DVGPU_MAX_MSM_SIZE * 16ull, "batched MSM length out of range");
@@ -137,7 +149,9 @@ MsmBatchPlan plan_items(const DvgpuMsmItem* items, size_t count, const Owner* o,
                                           : msm_window_for(size_t(items[i].n), &o->window_table)});
     }
     if (v.empty()) return MsmBatchPlan{};
-    MsmBatchPlan plan = msm_batch_plan(v, o->msm_config, &o->window_table);
+    WindowTable g2_table;
+    const MsmConfig cfg = g2 ? g2_config(o, g2_table) : o->msm_config;
+    MsmBatchPlan plan = msm_batch_plan(v, cfg, g2 ? &g2_table : &o->window_table);
     for (size_t k = 0; k < plan.descs.size(); ++k) plan.descs[k].out_index = uint32_t(map[k]);
     return plan;
 }
@@ -241,7 +255,7 @@ int dvgpu_ext_msm_batch_device(void* owner, const void* points, const void* poin
     return guarded(o, error, capacity, [&]() -> int {
         require(items && results && count && (g2 == 0 || g2 == 1) && points && scalars, "invalid batch arguments");
         std::vector<size_t> map;
-        const MsmBatchPlan plan = plan_items(items, count, o, map);
+        const MsmBatchPlan plan = plan_items(items, count, o, map, g2 != 0);
         const int xw = g2 ? PointLayout<Fq2Ops>::XYZZ_WORDS : PointLayout<FqOps>::XYZZ_WORDS;
         if (plan.descs.empty()) {
             std::memset(results, 0, count * (g2 ? 128 : 64));
@@ -317,7 +331,7 @@ int dvgpu_ext_msm_batch_finalize(void* owner, const void* points, size_t point_c
             }
         }
         std::vector<size_t> map;
-        MsmBatchPlan plan = plan_items(items, count, o, map);
+        MsmBatchPlan plan = plan_items(items, count, o, map, g2 != 0);
         plan.point_limit = uint32_t(point_count);
         const size_t pb = g2 ? 128 : 64;
         uint8_t* out = results_device ? static_cast<uint8_t*>(results_device) : o->msm.finals.as<uint8_t>(count * pb);
@@ -396,7 +410,7 @@ int dvgpu_ext_batch_create(void* owner, const DvgpuBatchSpec* spec, int use_grap
         hipStream_t st = o->stream;
         if (spec->g1_count) {
             require(spec->g1_items && spec->g1_points && spec->g1_scalars, "missing G1 batch buffers");
-            b->p1 = plan_items(spec->g1_items, spec->g1_count, o, b->map1);
+            b->p1 = plan_items(spec->g1_items, spec->g1_count, o, b->map1, false);
             b->count1 = spec->g1_count;
             b->g1_points = static_cast<const uint32_t*>(spec->g1_points);
             b->g1_index = static_cast<const uint32_t*>(spec->g1_index);
@@ -415,7 +429,7 @@ int dvgpu_ext_batch_create(void* owner, const DvgpuBatchSpec* spec, int use_grap
         }
         if (spec->g2_count) {
             require(spec->g2_items && spec->g2_points && spec->g2_scalars, "missing G2 batch buffers");
-            b->p2 = plan_items(spec->g2_items, spec->g2_count, o, b->map2);
+            b->p2 = plan_items(spec->g2_items, spec->g2_count, o, b->map2, true);
             b->count2 = spec->g2_count;
             b->g2_points = static_cast<const uint32_t*>(spec->g2_points);
             b->g2_index = static_cast<const uint32_t*>(spec->g2_index);

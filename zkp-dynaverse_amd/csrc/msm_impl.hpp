@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "host_pool.hpp"
 
@@ -868,7 +869,9 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
         descs, m, plan.total_chunks, counts, offsets, partials, chunk_vals);
     HIP_CHECK_LAUNCH();
     mark(6);
-    if (plan.max_window_chunks <= 128)  // small bucket sets (update batches): one value per thread
+    if (plan.max_window_chunks <= 32)  // small bucket sets: one value per thread, fewest levels
+        k_window_tree<F, 32><<<plan.total_windows, 32, 0, stream>>>(descs, m, chunk_vals, window_out);
+    else if (plan.max_window_chunks <= 128)
         k_window_tree<F, 128><<<plan.total_windows, 128, 0, stream>>>(descs, m, chunk_vals, window_out);
     else
         k_window_tree<F, TreeThreads<F>::value><<<plan.total_windows, TreeThreads<F>::value, 0, stream>>>(
@@ -939,7 +942,9 @@ inline MsmPlan msm_plan_impl(size_t n, const MsmConfig& cfg) {
 template <class F>
 MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, MsmWorkspace& ws,
                     hipStream_t stream, const MsmConfig& cfg, uint32_t* window_out) {
-    const MsmBatchPlan plan = msm_batch_plan_impl({MsmItem{n, 0, 0, cfg.window_bits}}, cfg, nullptr);
+    MsmConfig c = cfg;
+    if constexpr (std::is_same<F, Fq2Ops>::value) c.chunk = std::min(c.chunk, 2);  // G2: short reduction chains
+    const MsmBatchPlan plan = msm_batch_plan_impl({MsmItem{n, 0, 0, cfg.window_bits}}, c, nullptr);
     msm_batch_prepare_impl(plan, PointLayout<F>::XYZZ_WORDS, ws, stream);
     msm_batch_enqueue<F>(points, nullptr, scalars, plan, ws, stream, window_out);
     MsmPlan p;
