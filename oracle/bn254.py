@@ -581,3 +581,74 @@ def ecntt(points, inverse=False):
             acc = add(acc, mul(p, pow(w, j * k, R) * ninv % R))
         out.append(acc)
     return out
+
+
+def ecntt_fft(points, inverse=False):
+    """The same transform as ``ecntt`` by the iterative radix-2 DIT form of
+    ``ntt`` with group additions and twiddle scalar multiplications."""
+    n = len(points)
+    assert n >= 1 and n & (n - 1) == 0
+    a = list(points)
+    if n == 1:
+        return a
+    logn = n.bit_length() - 1
+    a = [a[int(bin(i)[2:].zfill(logn)[::-1], 2)] for i in range(n)]
+    w_n = fr_root_of_unity(n)
+    if inverse:
+        w_n = pow(w_n, R - 2, R)
+    h = 1
+    while h < n:
+        w = pow(w_n, n // (2 * h), R)
+        for start in range(0, n, 2 * h):
+            t = 1
+            for j in range(h):
+                u, v = a[start + j], mul(a[start + j + h], t)
+                a[start + j], a[start + j + h] = add(u, v), add(u, neg(v))
+                t = t * w % R
+        h *= 2
+    if inverse:
+        ninv = pow(n, R - 2, R)
+        a = [mul(p, ninv) for p in a]
+    return a
+
+
+# --------------------------------------------------------------------------
+# KZG opening cache (crates/protocol/src/kzg.rs:103-127, 217-275)
+# --------------------------------------------------------------------------
+
+
+def kzg_srs(tau: int, max_degree: int):
+    """KzgParams::from_tau's G1 powers [tau^i] G1, i <= max_degree (kzg.rs:107-127)."""
+    return [g1_mul(pow(tau, i, R)) for i in range(max_degree + 1)]
+
+
+def kzg_opening_closed_form(tau: int, coefficients, z: int):
+    """The single-point opening witness [q(tau)] G1 of q = (f - f(z)) / (X - z)
+    (KzgParams::open_single, kzg.rs:202-210) from the trapdoor, tau != z."""
+    def ev(x):
+        acc = 0
+        for c in reversed(coefficients):
+            acc = (acc * x + c) % R
+        return acc
+    return g1_mul((ev(tau) - ev(z)) * pow((tau - z) % R, R - 2, R) % R)
+
+
+def kzg_openings(powers, coefficients, n: int, offset=None):
+    """KzgParams::precompute_single_openings (kzg.rs:217-275): openings at
+    offset * w_n^i (offset None: the subgroup itself), i < n, in order."""
+    if len(coefficients) > n or len(coefficients) > len(powers):
+        raise ValueError("polynomial exceeds the domain or the SRS")
+    degree = max(len(coefficients) - 1, 0)
+    if degree == 0:
+        return [None] * n
+    m = 1
+    while m < 2 * degree - 1:
+        m *= 2
+    reversed_c = list(reversed(coefficients[1:])) + [0] * (m - degree)
+    scalar_fft = ntt(reversed_c, False)
+    prepared = ecntt_fft(list(powers[:degree]) + [None] * (m - degree), False)
+    conv = ecntt_fft([mul(p, s) for p, s in zip(prepared, scalar_fft)], True)
+    h = list(reversed(conv[:degree])) + [None] * (n - degree)
+    if offset is not None:
+        h = [mul(p, pow(offset, j, R)) for j, p in enumerate(h)]
+    return ecntt_fft(h, False)

@@ -12,7 +12,10 @@
  *   - the natural-order radix-2 NTT / INTT over <w_n>, w_n = root^(2^20/n)
  *     with root the caller's 2^20-th root (crates/gpu/src/icicle.rs:429-456,
  *     arkworks Radix2EvaluationDomain), inverse scaled by 1/n;
- *   - fixed-base [s_i] G1 (crates/protocol/src/setup.rs:13-22).
+ *   - fixed-base [s_i] G1 (crates/protocol/src/setup.rs:13-22);
+ *   - the KZG opening cache (crates/protocol/src/kzg.rs:217-275): the same
+ *     convolution and group DFTs, in Jacobian coordinates with 254-bit
+ *     double-and-add for every twiddle.
  * Pinned by tests/test_oracle.py against the Python oracle and closed forms.
  */
 #include <pthread.h>
@@ -434,5 +437,182 @@ int bn254ref_ntt(const uint8_t* in, size_t n, int inverse, const uint8_t* root, 
     free(tw);
     free(tids);
     free(jobs);
+    return 0;
+}
+
+/* ---- KZG opening cache (crates/protocol/src/kzg.rs:217-275) ------------- */
+/* [k]P by double-and-add over the 254-bit canonical scalar k. */
+static jac jmul(jac p, const uint64_t k[4]) {
+    jac acc = JINF;
+    for (int b = 253; b >= 0; --b) {
+        acc = jdbl(acc);
+        if ((k[b >> 6] >> (b & 63)) & 1) acc = jadd(acc, p);
+    }
+    return acc;
+}
+
+static void fr_canon(fe x, uint64_t k[4]) {
+    uint8_t b[32];
+    from_mont(&FR, x, b);
+    memcpy(k, b, 32);
+}
+
+typedef struct {
+    jac* a;
+    size_t h, lo, hi;        /* butterflies [lo, hi) of the stage with half-size h */
+    const uint64_t* tw;      /* canonical w_{2h}^j, 4 words each */
+} gfft_job;
+
+static void* gfft_worker(void* arg) {
+    gfft_job* j = (gfft_job*)arg;
+    for (size_t b = j->lo; b < j->hi; ++b) {
+        size_t blk = b / j->h, k = b % j->h, i = blk * 2 * j->h + k;
+        jac u = j->a[i], v = j->a[i + j->h];
+        if (k) v = jmul(v, j->tw + 4 * k);
+        j->a[i] = jadd(u, v);
+        jac nv = v;
+        nv.Y = fsub(&FQ, (fe){{0}}, v.Y);
+        j->a[i + j->h] = jadd(u, nv);
+    }
+    return NULL;
+}
+
+/* Natural-order DFT over <w_n> in G1 (radix-2 DIT, the textbook iterative
+ * form of Radix2EvaluationDomain::fft); inverse uses w^-1 and scales 1/n. */
+static void gfft(jac* a, size_t n, fe w_n, int inverse, int threads) {
+    int logn = 0;
+    while (((size_t)1 << logn) < n) ++logn;
+    for (size_t i = 0; i < n; ++i) {
+        size_t r = 0;
+        for (int b = 0; b < logn; ++b) r |= ((i >> b) & 1) << (logn - 1 - b);
+        if (r > i) { jac t = a[i]; a[i] = a[r]; a[r] = t; }
+    }
+    fe w = inverse ? finv(&FR, w_n) : w_n;
+    uint64_t* tw = (uint64_t*)malloc(sizeof(uint64_t) * 4 * (n / 2 + 1));
+    if (threads < 1) threads = 1;
+    pthread_t* tids = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+    gfft_job* jobs = (gfft_job*)calloc((size_t)threads, sizeof(gfft_job));
+    for (size_t h = 1; h < n; h <<= 1) {
+        fe wh = w;
+        for (size_t s = h; s < n / 2; s <<= 1) wh = fmul(&FR, wh, wh); /* w_{2h} */
+        fe t = FR.one;
+        for (size_t k = 0; k < h; ++k) {
+            fr_canon(t, tw + 4 * k);
+            t = fmul(&FR, t, wh);
+        }
+        size_t total = n / 2, per = (total + threads - 1) / threads;
+        for (int q = 0; q < threads; ++q) {
+            jobs[q].a = a;
+            jobs[q].h = h;
+            jobs[q].tw = tw;
+            jobs[q].lo = q * per < total ? q * per : total;
+            jobs[q].hi = (q + 1) * per < total ? (q + 1) * per : total;
+            if (threads == 1) gfft_worker(&jobs[q]);
+            else pthread_create(&tids[q], NULL, gfft_worker, &jobs[q]);
+        }
+        if (threads > 1)
+            for (int q = 0; q < threads; ++q) pthread_join(tids[q], NULL);
+    }
+    if (inverse && n > 1) {
+        uint8_t nb[32] = {0};
+        for (int k = 0; k < 8; ++k) nb[k] = (uint8_t)(n >> (8 * k));
+        uint64_t s[4];
+        fr_canon(finv(&FR, to_mont(&FR, nb)), s);
+        for (size_t i = 0; i < n; ++i) a[i] = jmul(a[i], s);
+    }
+    free(tw);
+    free(tids);
+    free(jobs);
+}
+
+static fe root_of(const uint8_t* root_2_20, size_t n) {
+    fe w = to_mont(&FR, root_2_20);
+    int logn = 0;
+    while (((size_t)1 << logn) < n) ++logn;
+    for (int k = logn; k < 20; ++k) w = fmul(&FR, w, w);
+    return w;
+}
+
+typedef struct { jac* a; const fe* s; size_t lo, hi; } scale_job;
+
+static void* scale_worker(void* arg) {
+    scale_job* j = (scale_job*)arg;
+    for (size_t i = j->lo; i < j->hi; ++i) {
+        uint64_t k[4];
+        fr_canon(j->s[i], k);
+        j->a[i] = jmul(j->a[i], k);
+    }
+    return NULL;
+}
+
+static void scale_all(jac* a, const fe* s, size_t n, int threads) {
+    if (threads < 1) threads = 1;
+    pthread_t* tids = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+    scale_job* jobs = (scale_job*)calloc((size_t)threads, sizeof(scale_job));
+    size_t per = (n + threads - 1) / threads;
+    for (int q = 0; q < threads; ++q) {
+        jobs[q].a = a;
+        jobs[q].s = s;
+        jobs[q].lo = q * per < n ? q * per : n;
+        jobs[q].hi = (q + 1) * per < n ? (q + 1) * per : n;
+        pthread_create(&tids[q], NULL, scale_worker, &jobs[q]);
+    }
+    for (int q = 0; q < threads; ++q) pthread_join(tids[q], NULL);
+    free(tids);
+    free(jobs);
+}
+
+/* precompute_single_openings: powers (count x 64 B affine), coefficients
+ * (ncoeff x 32 B), domain of n points (coset by `offset` when not NULL).
+ * out: n x 64 B affine.  Returns -1 for the reference's error cases. */
+int bn254ref_kzg_openings(const uint8_t* powers, size_t count, const uint8_t* coeffs, size_t ncoeff, size_t n,
+                          const uint8_t* offset, const uint8_t* root_2_20, int threads, uint8_t* out) {
+    init_fields();
+    if (!n || (n & (n - 1)) || n > (1u << 20) || ncoeff > n || ncoeff > count) return -1;
+    size_t degree = ncoeff ? ncoeff - 1 : 0;
+    if (!degree) {
+        memset(out, 0, 64 * n);
+        return 0;
+    }
+    size_t M = 1;
+    while (M < 2 * degree - 1) M <<= 1;
+    if (M > (1u << 20)) return -1;
+    /* scalar_fft = DFT_M(f_d, ..., f_1) */
+    uint8_t* rev = (uint8_t*)calloc(M, 32);
+    for (size_t p = 0; p < degree; ++p) memcpy(rev + 32 * p, coeffs + 32 * (degree - p), 32);
+    uint8_t* sf = (uint8_t*)malloc(32 * M);
+    bn254ref_ntt(rev, M, 0, root_2_20, threads, sf);
+    /* prepared = DFT_M([tau^0], ..., [tau^(d-1)]) */
+    jac* conv = (jac*)malloc(sizeof(jac) * M);
+    for (size_t p = 0; p < M; ++p) {
+        conv[p] = JINF;
+        if (p < degree) {
+            aff a = aff_from_bytes(powers + 64 * p);
+            if (!a.inf) { conv[p].X = a.x; conv[p].Y = a.y; conv[p].Z = FQ.one; }
+        }
+    }
+    gfft(conv, M, root_of(root_2_20, M), 0, threads);
+    fe* s = (fe*)malloc(sizeof(fe) * (M > n ? M : n));
+    for (size_t p = 0; p < M; ++p) s[p] = to_mont(&FR, sf + 32 * p);
+    scale_all(conv, s, M, threads);
+    gfft(conv, M, root_of(root_2_20, M), 1, threads);
+    /* truncate(degree), reverse, zero-pad to n */
+    jac* h = (jac*)malloc(sizeof(jac) * n);
+    for (size_t j = 0; j < n; ++j) h[j] = j < degree ? conv[degree - 1 - j] : JINF;
+    if (offset) { /* coset fft: coefficients times offset^j */
+        fe g = to_mont(&FR, offset), t = FR.one;
+        for (size_t j = 0; j < n; ++j) {
+            s[j] = t;
+            t = fmul(&FR, t, g);
+        }
+        scale_all(h, s, n, threads);
+    }
+    gfft(h, n, root_of(root_2_20, n), 0, threads);
+    for (size_t j = 0; j < n; ++j) jac_to_bytes(h[j], out + 64 * j);
+    free(rev);
+    free(sf);
+    free(conv);
+    free(s);
+    free(h);
     return 0;
 }

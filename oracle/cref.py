@@ -43,3 +43,16 @@ def fixed_base_g1(scalars: bytes, n: int) -> bytes:
     out = ctypes.create_string_buffer(64 * n)
     load().bn254ref_fixed_base_g1(_p(scalars), n, out)
     return out.raw
+
+
+def kzg_openings(powers: bytes, count: int, coefficients: bytes, ncoeff: int, n: int, root: bytes,
+                 offset: bytes = None, threads: int = 1) -> bytes:
+    lib = load()
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    lib.bn254ref_kzg_openings.argtypes = [vp, sz, vp, sz, sz, vp, vp, ctypes.c_int, vp]
+    out = ctypes.create_string_buffer(64 * n)
+    rc = lib.bn254ref_kzg_openings(_p(powers or b"\0"), count, _p(coefficients or b"\0"), ncoeff, n,
+                                   _p(offset) if offset is not None else None, _p(root), threads, out)
+    if rc != 0:
+        raise ValueError("invalid opening-cache arguments")
+    return out.raw

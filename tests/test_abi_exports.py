@@ -10,7 +10,9 @@ LIB = os.path.join(ROOT, "zkp-dynaverse_amd", "dvgpu", "libdynaverse_hip.so")
 
 def declared_functions():
     names = set()
-    for h in ("dynaverse_gpu.h", "dynaverse_hip_ext.h"):
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(?:int|void)\s+(dvgpu_\w+)\s*\(", text):

@@ -46,6 +46,8 @@ void Owner::release_device() noexcept {
     pinned.release();
     for (BatchObj* b : batches) batch_destroy(b);
     batches.clear();
+    for (KzgSrs* z : kzg) kzg_destroy(z);
+    kzg.clear();
     for (auto& kv : ext_allocations) {
         (void)hipFree(kv.first);
         quota.release(kv.second);

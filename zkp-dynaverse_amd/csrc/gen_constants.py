@@ -91,6 +91,63 @@ def f2_pow(a, e):
     return out
 
 
+def _g1_mul(k):
+    """[k]G1 in affine coordinates (plain double-and-add; None = infinity)."""
+    def add(p, q):
+        if p is None:
+            return q
+        if q is None:
+            return p
+        if p[0] == q[0] and (p[1] + q[1]) % Q == 0:
+            return None
+        if p == q:
+            m = 3 * p[0] * p[0] * pow(2 * p[1], Q - 2, Q) % Q
+        else:
+            m = (q[1] - p[1]) * pow(q[0] - p[0], Q - 2, Q) % Q
+        x = (m * m - p[0] - q[0]) % Q
+        return x, (m * (p[0] - x) - p[1]) % Q
+    acc, base = None, (1, 2)
+    while k:
+        if k & 1:
+            acc = add(acc, base)
+        base = add(base, base)
+        k >>= 1
+    return acc
+
+
+def glv_constants():
+    """GLV endomorphism phi(x, y) = (beta x, y) = [lambda] on G1, with the short
+    lattice basis of {(a, b) : a + b lambda = 0 mod r} from the extended
+    Euclidean algorithm on (r, lambda) (Gallant-Lambert-Vanstone 2001, sec. 4)
+    and the reciprocals that round b2 k / r and -b1 k / r for a split
+    k = k1 + k2 lambda with |k1|, |k2| < 2^126."""
+    beta = next(c for c in (pow(g, (Q - 1) // 3, Q) for g in range(2, 50)) if c != 1)
+    lam = next(c for c in (pow(g, (R - 1) // 3, R) for g in range(2, 50)) if c != 1)
+    g1 = _g1_mul(1)
+    if _g1_mul(lam) != (beta * g1[0] % Q, g1[1]):
+        lam = lam * lam % R
+    assert _g1_mul(lam) == (beta * g1[0] % Q, g1[1])
+    seq = [(R, 0), (lam, 1)]
+    while seq[-1][0]:
+        (ra, ta), (rb, tb) = seq[-2], seq[-1]
+        seq.append((ra - ra // rb * rb, ta - ra // rb * tb))
+    m = max(i for i, (r_, _) in enumerate(seq) if r_ * r_ >= R)
+    a1, b1 = seq[m + 1][0], -seq[m + 1][1]
+    a2, b2 = min(((seq[m][0], -seq[m][1]), (seq[m + 2][0], -seq[m + 2][1])), key=lambda v: v[0] ** 2 + v[1] ** 2)
+    assert (a1 + b1 * lam) % R == 0 and (a2 + b2 * lam) % R == 0 and a1 * b2 - a2 * b1 == R
+    assert b1 == -a2 and a1 > 0 and a2 > 0 and b2 > 0
+    rc1, rc2 = (b2 << 384) // R, (a2 << 384) // R
+
+    def words(name, v, n):
+        assert 0 <= v < 1 << (32 * n)
+        return "constexpr uint32_t %s[%d] = {%s};" % (
+            name, n, ", ".join("0x%08xu" % ((v >> (32 * i)) & 0xFFFFFFFF) for i in range(n)))
+    return ["// GLV: phi(x, y) = (GLV_BETA x, y) = [GLV_LAMBDA](x, y); basis (a1, -a2), (a2, b2).",
+            "constexpr uint32_t GLV_BETA[%d] = {%s};" % (NL, ", ".join("0x%08xu" % x for x in mont(beta, Q))),
+            words("GLV_LAMBDA", lam, 8), words("GLV_A1", a1, 4), words("GLV_A2", a2, 2),
+            words("GLV_B2", b2, 4), words("GLV_RC1", rc1, 9), words("GLV_RC2", rc2, 7)]
+
+
 def main():
     xi = (9, 1)
     n = (81 + 1) % Q
@@ -125,6 +182,7 @@ def main():
     lam = 6 * 4965661367192848881 ** 2
     out.append("constexpr uint32_t G2_LAMBDA[4] = {%s};" % ", ".join(
         "0x%08xu" % ((lam >> (32 * i)) & 0xFFFFFFFF) for i in range(4)))
+    out.extend(glv_constants())
     out.append("}  // namespace bn254")
     print("\n".join(out))
 

@@ -23,6 +23,7 @@ namespace dvgpu {
 struct ResidentState;  // resident.hip
 struct BatchObj;       // batch.hip
 struct SetupState;     // setup.hip
+struct KzgSrs;         // kzg.hip
 
 struct Owner {
     hipStream_t stream = nullptr;
@@ -53,6 +54,8 @@ struct Owner {
     std::unordered_map<void*, size_t> ext_allocations;
     // Batches created through dynaverse_hip_ext.h (freed with the owner).
     std::vector<BatchObj*> batches;
+    // KZG SRS objects created through dynaverse_kzg.h (freed with the owner).
+    std::vector<KzgSrs*> kzg;
     // Measured window width per log2 MSM size (dvgpu_ext_set_tuning); the
     // NTT stages per pass live in `domain`, the task shape in msm_config.
     WindowTable window_table;
@@ -98,5 +101,7 @@ DvgpuResidentDiagnostics resident_diagnostics(const ResidentState* state);
 void batch_destroy(BatchObj* b) noexcept;
 // Setup hooks (setup.hip).
 void setup_destroy(SetupState* state) noexcept;
+// KZG hooks (kzg.hip).
+void kzg_destroy(KzgSrs* srs) noexcept;
 
 }  // namespace dvgpu

@@ -148,6 +148,11 @@ class DvgpuUpdate(ctypes.Structure):
                 ("indices", c_void_p), ("deltas", c_void_p)]
 
 
+class DvgpuKzgTimings(ctypes.Structure):
+    _fields_ = [("scalar_and_basis_ms", ctypes.c_float), ("convolution_ms", ctypes.c_float),
+                ("domain_transform_ms", ctypes.c_float)]
+
+
 _I = ctypes.c_int
 _E = (ctypes.c_char_p, c_size_t)  # error, capacity
 
@@ -220,6 +225,13 @@ SIGNATURES = {
                                     ctypes.POINTER(ctypes.c_float), *_E]),
     "dvgpu_ext_update_reserve": (_I, [c_void_p, c_size_t, c_size_t, c_size_t, _I, c_void_p, *_E]),
     "dvgpu_ext_window_table_device": (_I, [c_void_p, c_void_p, c_size_t, _I, c_void_p, *_E]),
+    # dynaverse_kzg.h
+    "dvgpu_kzg_srs_create": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, ctypes.POINTER(c_void_p), *_E]),
+    "dvgpu_kzg_srs_destroy": (_I, [c_void_p, c_void_p]),
+    "dvgpu_kzg_srs_basis": (_I, [c_void_p, c_void_p, c_u64p, c_u64p]),
+    "dvgpu_kzg_precompute_single_openings": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_size_t,
+                                                  c_void_p, c_void_p, ctypes.POINTER(DvgpuKzgTimings), *_E]),
+    "dvgpu_ext_glv_split": (_I, [c_void_p, c_size_t, c_void_p]),
     "dvgpu_ext_batch_create": (_I, [c_void_p, ctypes.POINTER(DvgpuBatchSpec), _I, ctypes.POINTER(c_void_p), *_E]),
     "dvgpu_ext_batch_run": (_I, [c_void_p, c_void_p, c_void_p, c_void_p,
                                  ctypes.POINTER(DvgpuG2ValidationReport), ctypes.POINTER(ctypes.c_float), *_E]),
