@@ -48,6 +48,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the batched, incremental and served workloads")
+    ap.add_argument("--no-kzg", action="store_true", help="skip the opening-cache precompute workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the plain copy kernel (traffic-counter calibration)")
     args = ap.parse_args()
@@ -144,6 +145,13 @@ def main():
         log("# served stream")
         secondary["served_stream"] = benchmarks.served_stream(backend, log=log)
 
+    if not args.no_kzg:
+        from dvgpu import benchmarks
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        log("# opening caches n=2^16")
+        secondary["opening_cache_2p16"] = benchmarks.opening_cache(backend, log=log)
+        secondary["opening_cache_2p13_x1"] = benchmarks.opening_cache(backend, log_n=13, rows=1, log=log)
+
     if args.calibrate_copy:
         nbytes = 256 << 20
         a = backend.alloc(nbytes)
@@ -179,6 +187,22 @@ def main():
                "sample": f"one G1 MSM over all 2^{ns.bit_length() - 1} SRS points with the same "
                          f"scalars, C Pippenger (oracle/c, not arkworks), {dt:.1f}s, "
                          f"result {'matches' if cpu_ok else 'DIFFERS FROM'} [f(tau)]g1"}
+
+    if cpu is not None and not args.no_kzg:
+        # Opening-cache sample: one cache of 2^13 coefficients over 2^13 points
+        # (cold: the SRS transform included), the C restatement of kzg.rs:217-275.
+        from dvgpu.codec import root_of_unity_2_20
+        nk = 1 << 13
+        srs_k = b"".join(backend.fixed_base_encoded(powers[i:i + 4096].tobytes(), min(4096, nk - i), False)
+                         for i in range(0, nk, 4096))
+        a = time.perf_counter()
+        cref.kzg_openings(srs_k, nk, scal[:nk].tobytes(), nk, nk, encode_fr(root_of_unity_2_20()), threads=threads)
+        dt = time.perf_counter() - a
+        gpu_ms = secondary.get("opening_cache_2p13_x1", {}).get("cold_ms")
+        secondary["opening_cache_cpu_sample"] = {
+            "cpu_ms": round(dt * 1e3, 1), "cores": threads, "kind": "port", "gpu_cold_ms": gpu_ms,
+            "sample": "one opening cache, 2^13 coefficients over 2^13 points, SRS transform included, "
+                      "C oracle (Jacobian double-and-add twiddles), not arkworks"}
 
     out = {
         "metric": "G1 MSM Mpoints/s at 2^16/2^20; update-proof/s + p50/p99 ms, served vs plain",

@@ -68,6 +68,13 @@ int dvgpu_kzg_precompute_single_openings(void* backend, void* srs, const uint8_t
  * non-canonical scalar.  No counterpart in the reference (test access). */
 int dvgpu_ext_glv_split(const uint8_t* scalars, size_t count, uint8_t* out);
 
+/* Joint sparse form of (|k1|, |k2|) of the same split, as the transforms use
+ * it when a wave shares the twiddle: out 64 B per scalar, four 128-bit masks
+ * (digit of |k1| nonzero, negative; digit of |k2| nonzero, negative), bit l
+ * for position l.  Host computation; -1 for a non-canonical scalar.  No
+ * counterpart in the reference (test access). */
+int dvgpu_ext_glv_jsf(const uint8_t* scalars, size_t count, uint8_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -138,3 +138,29 @@ def test_glv_split_rejects_non_canonical():
     lib.dvgpu_ext_glv_split.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
     out = ctypes.create_string_buffer(40)
     assert lib.dvgpu_ext_glv_split(ctypes.c_char_p(o.R.to_bytes(32, "little")), 1, out) == -1
+
+
+def test_joint_sparse_form_of_the_split():
+    """The recoding the shared-twiddle stages use: digits in {-1, 0, 1} that
+    sum back to |k1| and |k2|, within 127 positions, about half the columns
+    nonzero, and no two adjacent nonzero columns among any three."""
+    lib = ctypes.CDLL(LIB)
+    lib.dvgpu_ext_glv_jsf.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    rng = random.Random(13)
+    ks = [0, 1, o.R - 1, 1 << 200] + [rng.randrange(o.R) for _ in range(1500)]
+    data = b"".join(o.fr_to_bytes(k) for k in ks)
+    out = ctypes.create_string_buffer(64 * len(ks))
+    assert lib.dvgpu_ext_glv_jsf(ctypes.c_char_p(data), len(ks), out) == 0
+    nonzero = 0
+    for i, (k, (k1, k2)) in enumerate(zip(ks, _split(ks))):
+        m = [int.from_bytes(out.raw[64 * i + 16 * q:64 * i + 16 * q + 16], "little") for q in range(4)]
+        assert not (m[1] & ~m[0]) and not (m[3] & ~m[2])          # signs only on nonzero digits
+        assert m[0] >> 127 == 0 and m[2] >> 127 == 0
+        v1 = sum((-1 if m[1] >> l & 1 else 1) << l for l in range(128) if m[0] >> l & 1)
+        v2 = sum((-1 if m[3] >> l & 1 else 1) << l for l in range(128) if m[2] >> l & 1)
+        assert (v1, v2) == (abs(k1), abs(k2)), k
+        cols = m[0] | m[2]
+        nonzero += bin(cols).count("1")
+        for l in range(126):                                       # JSF: of 3 consecutive columns one is zero
+            assert (cols >> l) & 7 != 7, k
+    assert nonzero / len(ks) < 0.55 * 127

@@ -12,7 +12,8 @@
 // floor(2^384 a2 / r); their error is below 2^-130, so each |c - exact| is
 // at most 1/2 + 2^-130 and the bound 2^126 holds for every k < r.
 // [k]P is then one joint double-and-add over 126 bit positions on the table
-// {P1, P2, P1 + P2} with P1 = [sign k1]P, P2 = [sign k2]phi(P).
+// {P1, P2, P1 + P2} with P1 = [sign k1]P, P2 = [sign k2]phi(P), or, for a
+// scalar shared by the wave, over the joint sparse form of (|k1|, |k2|).
 #pragma once
 #include "curve.hpp"
 
@@ -102,6 +103,47 @@ HD GlvSplit glv_split(const uint32_t (&k)[8]) {
     return s;
 }
 
+// Joint sparse form of (a, b), both < 2^126 (Solinas, "Low-weight binary
+// representations for pairs of integers", 2001; Hankerson-Menezes-Vanstone,
+// Guide to ECC, Algorithm 3.50): digits u1, u2 in {-1, 0, 1} with
+// a = sum u1_l 2^l, b = sum u2_l 2^l, at most 127 positions, and on average
+// half of the columns (u1_l, u2_l) nonzero.  Bit l of nz/sg marks digit l
+// nonzero / negative.  a and b are consumed.
+struct Jsf {
+    uint32_t nz1[4], sg1[4], nz2[4], sg2[4];
+};
+
+HD Jsf jsf_recode(uint32_t (&a)[4], uint32_t (&b)[4]) {
+    Jsf j = {};
+    uint32_t d1 = 0, d2 = 0;
+    for (int l = 0; l < 4 * 32; ++l) {
+        const uint32_t l1 = (a[0] + d1) & 7, l2 = (b[0] + d2) & 7;
+        int u1 = 0, u2 = 0;
+        if (l1 & 1) {
+            u1 = (l1 & 3) == 1 ? 1 : -1;
+            if ((l1 == 3 || l1 == 5) && (l2 & 3) == 2) u1 = -u1;
+        }
+        if (l2 & 1) {
+            u2 = (l2 & 3) == 1 ? 1 : -1;
+            if ((l2 == 3 || l2 == 5) && (l1 & 3) == 2) u2 = -u2;
+        }
+        const uint32_t bit = 1u << (l & 31);
+        if (u1) j.nz1[l >> 5] |= bit;
+        if (u1 < 0) j.sg1[l >> 5] |= bit;
+        if (u2) j.nz2[l >> 5] |= bit;
+        if (u2 < 0) j.sg2[l >> 5] |= bit;
+        if (2 * int(d1) == 1 + u1) d1 = 1 - d1;
+        if (2 * int(d2) == 1 + u2) d2 = 1 - d2;
+        for (int w = 0; w < 3; ++w) {
+            a[w] = (a[w] >> 1) | (a[w + 1] << 31);
+            b[w] = (b[w] >> 1) | (b[w + 1] << 31);
+        }
+        a[3] >>= 1;
+        b[3] >>= 1;
+    }
+    return j;
+}
+
 }  // namespace bn254
 
 namespace dvgpu {
@@ -125,15 +167,15 @@ __device__ inline bn254::Xyzz<bn254::FqOps> g1_mul_glv(const bn254::Xyzz<bn254::
     for (int b = GLV_BITS - 1; b >= 0; --b) {
         acc = xyzz_dbl<FqOps>(acc);
         const uint32_t d = glv_digit(s.k1, s.k2, b);
-        const P t = d == 3 ? sum : (d == 2 ? p2 : p1);
-        const P r = xyzz_add<FqOps>(acc, t);
-        if (d) acc = r;
+        if (d) acc = xyzz_add<FqOps>(acc, d == 3 ? sum : (d == 2 ? p2 : p1));
     }
     return acc;
 }
 
-// Same result for a split that is identical in every active lane: the digit
-// is a scalar value, so steps with a zero digit only double.
+// Same result for a split that is identical in every active lane: the split
+// is recoded to joint sparse form in scalar registers and the table grows by
+// P1 - P2, so about half of the 127 steps add (three quarters with plain
+// joint bits) and every branch is on a scalar digit.
 __device__ inline bn254::Xyzz<bn254::FqOps> g1_mul_glv_uniform(const bn254::Xyzz<bn254::FqOps>& p,
                                                                 const GlvSplit& s) {
     using namespace bn254;
@@ -144,16 +186,33 @@ __device__ inline bn254::Xyzz<bn254::FqOps> g1_mul_glv_uniform(const bn254::Xyzz
         k2[i] = __builtin_amdgcn_readfirstlane(s.k2[i]);
     }
     const uint32_t n1 = __builtin_amdgcn_readfirstlane(s.neg1), n2 = __builtin_amdgcn_readfirstlane(s.neg2);
+    const Jsf j = jsf_recode(k1, k2);
     const Fq bx = mul(p.X, fp_const<FqParams>(GLV_BETA));
     const Fq ny = neg(p.Y);
-    const P p1 = {p.X, n1 ? ny : p.Y, p.ZZ, p.ZZZ};
-    const P p2 = {bx, n2 ? ny : p.Y, p.ZZ, p.ZZZ};
-    const P sum = xyzz_add<FqOps>(p1, p2);
+    const Fq y1 = n1 ? ny : p.Y, y2 = n2 ? ny : p.Y;
+    const P sum = xyzz_add<FqOps>({p.X, y1, p.ZZ, p.ZZZ}, {bx, y2, p.ZZ, p.ZZZ});
+    const P dif = xyzz_add<FqOps>({p.X, y1, p.ZZ, p.ZZZ}, {bx, neg(y2), p.ZZ, p.ZZZ});
     P acc = xyzz_inf<FqOps>();
-    for (int b = GLV_BITS - 1; b >= 0; --b) {
+    for (int l = 4 * 32 - 1; l >= 0; --l) {
         acc = xyzz_dbl<FqOps>(acc);
-        const uint32_t d = glv_digit(k1, k2, b);
-        if (d) acc = xyzz_add<FqOps>(acc, d == 3 ? sum : (d == 2 ? p2 : p1));
+        const uint32_t w = l >> 5, bit = 1u << (l & 31);
+        const bool a1 = j.nz1[w] & bit, a2 = j.nz2[w] & bit;
+        if (!(a1 || a2)) continue;
+        const bool s1 = j.sg1[w] & bit, s2 = j.sg2[w] & bit;
+        P t;
+        bool negate;
+        if (a1 && a2) {
+            t = s1 == s2 ? sum : dif;  // u1 P1 + u2 P2 = +-(P1 + P2) or +-(P1 - P2)
+            negate = s1;
+        } else if (a1) {
+            t = {p.X, y1, p.ZZ, p.ZZZ};
+            negate = s1;
+        } else {
+            t = {bx, y2, p.ZZ, p.ZZZ};
+            negate = s2;
+        }
+        if (negate) t.Y = neg(t.Y);
+        acc = xyzz_add<FqOps>(acc, t);
     }
     return acc;
 }

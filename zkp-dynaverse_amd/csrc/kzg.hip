@@ -81,13 +81,14 @@ __global__ void __launch_bounds__(GFFT_THREADS) k_gfft_stage(uint32_t* __restric
     const uint64_t i = (row << L) | low | (e << lo) | (high << (s + 1)), j = i + (uint64_t(1) << s);
     const uint32_t x = uint32_t(e << (NTT_MAX_LOG - 1 - s + lo));  // omega_(2^20)^x
     const bool uniform = __all(x == __builtin_amdgcn_readfirstlane(x));
-    const G1 a = ldg1(data + i * 32);
-    G1 b = ldg1(data + j * 32);
     if (DIT) {
+        G1 b = ldg1(data + j * 32);
         if (x) b = twiddle_mul(b, tw, x, uniform);
+        const G1 a = ldg1(data + i * 32);  // loaded after the product: fewer live registers
         stg1(data + i * 32, xyzz_add<FqOps>(a, b));
         stg1(data + j * 32, xyzz_add<FqOps>(a, xyzz_neg<FqOps>(b)));
     } else {
+        const G1 a = ldg1(data + i * 32), b = ldg1(data + j * 32);
         const G1 d = xyzz_add<FqOps>(a, xyzz_neg<FqOps>(b));
         stg1(data + i * 32, xyzz_add<FqOps>(a, b));
         stg1(data + j * 32, x ? twiddle_mul(d, tw, x, uniform) : d);
@@ -442,6 +443,22 @@ int dvgpu_ext_glv_split(const uint8_t* scalars, size_t count, uint8_t* out) {
         std::memcpy(out + 40 * i + 16, s.k2, 16);
         std::memcpy(out + 40 * i + 32, &s.neg1, 4);
         std::memcpy(out + 40 * i + 36, &s.neg2, 4);
+    }
+    return 0;
+}
+
+int dvgpu_ext_glv_jsf(const uint8_t* scalars, size_t count, uint8_t* out) {
+    if (count && (!scalars || !out)) return -1;
+    for (size_t i = 0; i < count; ++i) {
+        uint32_t k[8];
+        std::memcpy(k, scalars + 32 * i, 32);
+        if (!is_canonical<FrParams>(k)) return -1;
+        GlvSplit s = bn254::glv_split(k);
+        const bn254::Jsf j = bn254::jsf_recode(s.k1, s.k2);
+        std::memcpy(out + 64 * i, j.nz1, 16);
+        std::memcpy(out + 64 * i + 16, j.sg1, 16);
+        std::memcpy(out + 64 * i + 32, j.nz2, 16);
+        std::memcpy(out + 64 * i + 48, j.sg2, 16);
     }
     return 0;
 }

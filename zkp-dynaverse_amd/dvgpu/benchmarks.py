@@ -246,3 +246,115 @@ def served_stream(backend: HipBackend, counts=(96, 72, 48, 24), count: int = 400
     out["open_loop"] = pts
     store.close()
     return out
+
+
+# ---- opening-cache precompute (crates/protocol/src/kzg.rs:217-275) ----------
+
+GLV_PRODUCTS_UNIFORM = 126 * 9 + 0.75 * 126 * 14 + 15   # shared twiddle: adds only on set digits
+GLV_PRODUCTS_LANE = 126 * (9 + 14) + 15                  # per-lane scalar: every step adds
+
+
+def _gfft_products(L: int, rows: int) -> float:
+    """Montgomery products of one rows x 2^L G1 transform under the stage
+    plan of kzg.hip: 2 additions per butterfly, a GLV scalar product for
+    every non-unit twiddle (stage twiddles wave-shared, the diagonal per lane)."""
+    if L == 0:
+        return 0.0
+    k = 0 if rows >= 64 else (6 if L >= 12 else L // 2)
+    m = L - k
+    half = rows << (L - 1)
+    prods = 0.0
+    for s in range(L):
+        lo = m if s >= m else 0
+        nontrivial = half * (1 - 2.0 ** -(s - lo))
+        prods += half * 28 + nontrivial * GLV_PRODUCTS_UNIFORM
+    if k and m:
+        prods += (rows << L) * (1 - 2.0 ** -k) * (1 - 2.0 ** -m) * GLV_PRODUCTS_LANE
+    return prods
+
+
+def opening_cache_macs(log_n: int, rows: int, with_basis: bool) -> float:
+    n = 1 << log_n
+    d = n - 1
+    LM = max(0, (2 * d - 2).bit_length()) if d > 1 else 0
+    prods = _gfft_products(LM, rows) + _gfft_products(log_n, rows) + (rows << LM) * GLV_PRODUCTS_LANE
+    if with_basis:
+        prods += _gfft_products(LM, 1)
+    return 136.0 * prods
+
+
+def opening_cache(backend: HipBackend, log_n: int = 16, rows: int = 3, reps: int = 3, log=print) -> Dict:
+    """The anchor opening caches q4, q5, q6 (dynaverse.rs:303-314): three
+    polynomials of n coefficients over the n-point domain, one call.  The
+    first call builds the retained SRS transform; later calls reuse it.
+    Sampled openings are checked against the trapdoor form
+    [(f(tau) - f(w^i)) / (tau - w^i)] g1 computed with the fixed-base kernel."""
+    from .codec import FR_MODULUS, encode_fr, root_of_unity_2_20
+    from .kzg import KzgSrs
+    from .workload import poly_eval, srs_scalars
+    n = 1 << log_n
+    r = FR_MODULUS
+    tau, powers = srs_scalars(2024, n)
+    srs = b"".join(backend.fixed_base_encoded(powers[i:i + 4096].tobytes(), min(4096, n - i), False)
+                   for i in range(0, n, 4096))
+    coeffs = ChaCha20Stream(2025).uniform_fr_array(rows * n)
+    data = coeffs.tobytes()
+    with KzgSrs(backend, encoded=srs) as z:
+        a = time.perf_counter()
+        raw = z.precompute_bytes(data, n, rows, n)
+        cold_ms = (time.perf_counter() - a) * 1e3
+        cold_t = dict(z.last_timings)
+        warm, phases = [], []
+        for _ in range(reps):
+            a = time.perf_counter()
+            again = z.precompute_bytes(data, n, rows, n)
+            warm.append((time.perf_counter() - a) * 1e3)
+            phases.append(dict(z.last_timings))
+        same = again == raw
+    w = pow(root_of_unity_2_20(), 1 << (20 - log_n), r)
+    ok = True
+    for row in range(rows):
+        f = coeffs[row * n:(row + 1) * n]
+        ft = poly_eval(f, tau)
+        for i in (0, 1, n // 3, n - 1):
+            z_i = pow(w, i, r)
+            q = (ft - poly_eval(f, z_i)) * pow((tau - z_i) % r, r - 2, r) % r
+            exp = backend.fixed_base_encoded(encode_fr(q), 1, False)
+            ok &= raw[64 * (row * n + i):64 * (row * n + i + 1)] == exp
+    warm_ms = sorted(warm)[len(warm) // 2]
+    dev_warm = min(sum(p.values()) for p in phases)
+    macs = opening_cache_macs(log_n, rows, False)
+    out = {"log_n": log_n, "caches": rows, "cold_ms": round(cold_ms, 3), "warm_ms": round(warm_ms, 3),
+           "warm_device_ms": round(dev_warm, 3), "cold_phases_ms": {k: round(v, 3) for k, v in cold_t.items()},
+           "warm_phases_ms": {k: round(v, 3) for k, v in min(phases, key=lambda p: sum(p.values())).items()},
+           "caches_per_s": round(rows / (warm_ms * 1e-3), 2),
+           "mac_frac": round(macs / (dev_warm * 1e-3) / PEAK_MAC_PER_S, 4),
+           "verified_sampled_against_trapdoor": bool(ok), "repeat_identical": bool(same)}
+    log(f"opening cache 2^{log_n} x{rows}: cold {cold_ms:.1f} ms, warm {warm_ms:.1f} ms "
+        f"(device {dev_warm:.1f} ms, {out['mac_frac']:.3f} of MAC peak), ok={ok}")
+    return out
+
+
+def main(argv=None):
+    """One workload by name, for profiling: python -m dvgpu.benchmarks opening_cache --log-n 16"""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("workload", choices=["opening_cache", "batched_launch", "incremental", "served_stream"])
+    ap.add_argument("--log-n", type=int, default=16)
+    ap.add_argument("--rows", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=3)
+    a = ap.parse_args(argv)
+    b = HipBackend()
+    try:
+        if a.workload == "opening_cache":
+            res = opening_cache(b, a.log_n, a.rows, a.reps)
+        else:
+            res = globals()[a.workload](b)
+        print(json.dumps(res), flush=True)
+    finally:
+        b.close()
+
+
+if __name__ == "__main__":
+    main()
