@@ -1,0 +1,45 @@
+/*
+ * dynaverse_pairing.h -- batched BN254 pairing-product checks on the device.
+ *
+ * Replaces the host pairing checks of the opening-cache validation
+ * (crates/protocol/src/dynaverse/rebuilding/machine.rs:255-390): every cache
+ * point pi_i of a commitment C is accepted when
+ *     e(C - [v_i] g1, g2) == e(pi_i, [tau]g2 - [z_i]g2),
+ * which the reference's PairingChecks::equal evaluates in Product mode as one
+ * two-term Miller product and one final exponentiation per point
+ * (machine.rs:333, validate_initial_caches machine.rs:357-382).  Here each
+ * check is a group of `terms` pairs (P, Q) and the answer is whether
+ * prod e(P_j, Q_j) is one in GT; the caller negates one side (e(A, B) ==
+ * e(C, D) is the group (A, B), (-C, D)).  The pairing is the optimal ate
+ * pairing with the final exponent (q^12 - 1)/r; accept/reject equals that of
+ * any non-degenerate bilinear pairing, arkworks' included.
+ *
+ * Encodings as in dynaverse_gpu.h: G1 64 B x || y, G2 128 B x.c0 || x.c1 ||
+ * y.c0 || y.c1, little-endian canonical, all-zero = identity (the pair then
+ * contributes one).  Points must lie on E (G1) and on the twist E' (G2) or
+ * the call fails with -1 before any device work; G2 points are taken to be in
+ * the r-torsion (callers validate them, as for dvgpu_bn254_g2_msm).  Status
+ * codes as in dynaverse_gpu.h.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* count groups of `terms` pairs: g1 count x terms x 64 B, g2 count x terms x
+ * 128 B (pair j of group i at index i*terms + j).  results: count bytes, 1
+ * when the group's product is one.  gt (optional): count x 384 B reduced
+ * products, Fq12 = Fq6[w]/(w^2 - v), Fq6 = Fq2[v]/(v^3 - (9 + u)), written
+ * c0.a0, c0.a1, c0.a2, c1.a0, c1.a1, c1.a2 (Fq2 each: real, imaginary),
+ * canonical LE32 Fq.  kernel_ms (optional): event-timed device duration.
+ * terms in 1..16; count up to 2^24.  No counterpart in the reference ABI. */
+int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const uint8_t* g2, size_t terms, size_t count,
+                              uint8_t* results, uint8_t* gt, float* kernel_ms, char* error,
+                              size_t error_capacity);
+
+#ifdef __cplusplus
+}
+#endif

@@ -652,3 +652,106 @@ def kzg_openings(powers, coefficients, n: int, offset=None):
     if offset is not None:
         h = [mul(p, pow(offset, j, R)) for j, p in enumerate(h)]
     return ecntt_fft(h, False)
+
+
+# --------------------------------------------------------------------------
+# Optimal ate pairing (what arkworks' Bn254::multi_pairing computes; the
+# reference's cache checks, crates/protocol/src/dynaverse/rebuilding/
+# machine.rs:255-390, compare two such products)
+# --------------------------------------------------------------------------
+# Fq12 as Fq[w]/(w^12 - 18 w^6 + 82): w^6 = xi = 9 + u, so u = w^6 - 9 and
+# u^2 = -1.  Elements are tuples of 12 coefficients (w^0 first).  This flat
+# form is independent of the Fq2/Fq6/Fq12 tower the device uses.
+
+F12_ONE = (1,) + (0,) * 11
+ATE_LOOP = 6 * BN_SEED + 2
+
+
+def f12_mul(a, b):
+    t = [0] * 23
+    for i, x in enumerate(a):
+        if x:
+            for j, y in enumerate(b):
+                t[i + j] += x * y
+    for k in range(22, 11, -1):       # w^k = 18 w^(k-6) - 82 w^(k-12)
+        c = t[k]
+        if c:
+            t[k - 6] += 18 * c
+            t[k - 12] -= 82 * c
+    return tuple(v % Q for v in t[:12])
+
+
+def f12_pow(a, e):
+    out = F12_ONE
+    while e:
+        if e & 1:
+            out = f12_mul(out, a)
+        a = f12_mul(a, a)
+        e >>= 1
+    return out
+
+
+def f12_from_f2(a, k):
+    """The Fq2 element a times w^k (0 <= k < 6) in flat form."""
+    out = [0] * 12
+    out[k] = (a[0] - 9 * a[1]) % Q
+    out[k + 6] = a[1] % Q
+    return out
+
+
+def _line(t, q, p):
+    """Line through the twist points t, q (tangent when equal) evaluated at
+    the untwisted P = (xp, yp): with x' = x w^2, y' = y w^3, slope' = m w,
+    l = yp - (m xp) w + (m x_t - y_t) w^3."""
+    (x1, y1), (x2, y2) = t, q
+    if x1 != x2:
+        m = f2_mul(f2_sub(y2, y1), f2_inv(f2_sub(x2, x1)))
+    else:
+        assert y1 == y2, "vertical line"
+        m = f2_mul(f2_mul((3, 0), f2_mul(x1, x1)), f2_inv(f2_add(y1, y1)))
+    xp, yp = p
+    c1 = f2_neg(f2_mul(m, (xp, 0)))
+    c3 = f2_sub(f2_mul(m, x1), y1)
+    out = [0] * 12
+    out[0] = yp
+    for k, c in ((1, c1), (3, c3)):
+        e = f12_from_f2(c, k)
+        out = [(u + v) % Q for u, v in zip(out, e)]
+    return tuple(out)
+
+
+def miller_loop(p, q):
+    """f_{6x+2,Q}(P) l_{T,pi(Q)}(P) l_{T',-pi^2(Q)}(P); 1 if either is infinity."""
+    if p is None or q is None:
+        return F12_ONE
+    f, t = F12_ONE, q
+    for i in range(ATE_LOOP.bit_length() - 2, -1, -1):
+        f = f12_mul(f12_mul(f, f), _line(t, t, p))
+        t = double(t, FQ2)
+        if (ATE_LOOP >> i) & 1:
+            f = f12_mul(f, _line(t, q, p))
+            t = add(t, q, FQ2)
+    q1 = psi(q)
+    q2 = neg(psi(q1), FQ2)
+    f = f12_mul(f, _line(t, q1, p))
+    t = add(t, q1, FQ2)
+    return f12_mul(f, _line(t, q2, p))
+
+
+FINAL_EXP = (Q ** 12 - 1) // R
+
+
+def final_exponentiation(f):
+    return f12_pow(f, FINAL_EXP)
+
+
+def pairing(p, q):
+    return final_exponentiation(miller_loop(p, q))
+
+
+def pairing_product(pairs):
+    """prod e(P_i, Q_i) with one final exponentiation."""
+    f = F12_ONE
+    for p, q in pairs:
+        f = f12_mul(f, miller_loop(p, q))
+    return final_exponentiation(f)

@@ -148,6 +148,29 @@ def glv_constants():
             words("GLV_B2", b2, 4), words("GLV_RC1", rc1, 9), words("GLV_RC2", rc2, 7)]
 
 
+def pairing_constants(xi):
+    """Optimal ate pairing: loop count 6x + 2, Frobenius coefficients
+    gamma_k = xi^(k (q - 1) / 6) of w^k (k = 1..5) for Fq12 = Fq2[w]/(w^6 - xi),
+    and the hard part (q^4 - q^2 + 1) / r of the final exponent."""
+    x = 4965661367192848881
+    ate = 6 * x + 2
+    hard = (Q ** 4 - Q ** 2 + 1) // R
+    assert hard * R == Q ** 4 - Q ** 2 + 1
+    out = ["// Optimal ate pairing: PAIRING_ATE = 6x + 2, FROB_Ck = xi^(k(q-1)/6), hard exponent words.",
+           "constexpr uint64_t PAIRING_ATE_LO = 0x%016xull;" % (ate & (2 ** 64 - 1)),
+           "constexpr uint32_t PAIRING_ATE_HI = %du;" % (ate >> 64),
+           "constexpr int PAIRING_ATE_BITS = %d;" % ate.bit_length()]
+    for k in range(1, 6):
+        g = f2_pow(xi, k * (Q - 1) // 6)
+        out.append("constexpr uint32_t FROB_C%d_0[%d] = {%s};" % (k, NL, ", ".join("0x%08xu" % v for v in mont(g[0], Q))))
+        out.append("constexpr uint32_t FROB_C%d_1[%d] = {%s};" % (k, NL, ", ".join("0x%08xu" % v for v in mont(g[1], Q))))
+    nw = (hard.bit_length() + 31) // 32
+    out.append("constexpr int HARD_EXP_BITS = %d;" % hard.bit_length())
+    out.append("constexpr uint32_t HARD_EXP[%d] = {%s};" % (
+        nw, ", ".join("0x%08xu" % ((hard >> (32 * i)) & 0xFFFFFFFF) for i in range(nw))))
+    return out
+
+
 def main():
     xi = (9, 1)
     n = (81 + 1) % Q
@@ -183,6 +206,7 @@ def main():
     out.append("constexpr uint32_t G2_LAMBDA[4] = {%s};" % ", ".join(
         "0x%08xu" % ((lam >> (32 * i)) & 0xFFFFFFFF) for i in range(4)))
     out.extend(glv_constants())
+    out.extend(pairing_constants(xi))
     out.append("}  // namespace bn254")
     print("\n".join(out))
 

@@ -1,0 +1,200 @@
+// Optimal ate pairing on BN254 (host and device), for batched pairing-product
+// checks: prod_i e(P_i, Q_i) == 1.
+//
+// Tower: Fq2 = Fq[u]/(u^2 + 1), Fq6 = Fq2[v]/(v^3 - xi), Fq12 = Fq6[w]/(w^2 - v),
+// xi = 9 + u, so w^6 = xi and an Fq12 element c0 + c1 w (c_j = a0 + a1 v +
+// a2 v^2) has the Fq2 coefficient (c_j)_i at w^(2i + j).
+//
+// Miller loop f_{6x+2,Q}(P) l_{T,pi(Q)}(P) l_{T',-pi^2(Q)}(P) over the
+// binary digits of 6x + 2, with T in homogeneous projective coordinates on
+// the twist E': y^2 = x^3 + 3/xi and the lines of the untwisted points
+// (x w^2, y w^3) evaluated at P = (xp, yp).  With slope m the line is
+// yp - (m xp) w + (m x_T - y_T) w^3; every line is scaled by a factor in Fq2,
+// which the final exponentiation removes (Fq2 lies in Fq6, and a^(q^6-1) = 1
+// there):
+//   tangent at T = (X, Y, Z):  (2 Y Z^2) yp - (3 X^2 Z) xp w + (3 X^3 - 2 Y^2 Z) w^3
+//   chord T, Q = (x2, y2):     (x2 Z - X) yp - (y2 Z - Y) xp w + ((y2 Z - Y) x2 - (x2 Z - X) y2) w^3
+// Point steps: dbl-2007-bl and madd-1998-cmo (EFD, homogeneous, a = 0).
+// Final exponentiation: f^((q^6 - 1)(q^2 + 1)) by conjugation, inversion and
+// the q^2-power Frobenius, then the hard part (q^4 - q^2 + 1)/r by
+// square-and-multiply over its 761 bits.
+#pragma once
+#include "curve.hpp"
+
+// The large tower and Miller-step functions stay out of line on the device:
+// inlined, one check would be a single body of several hundred thousand
+// instructions (minutes of compile time, the whole register file spilled).
+#if defined(__HIPCC__)
+#define HD_CALL __host__ __device__ inline __attribute__((noinline))
+#else
+#define HD_CALL inline
+#endif
+
+namespace bn254 {
+
+struct Fq6 {
+    Fq2 a0, a1, a2;
+};
+struct Fq12 {
+    Fq6 c0, c1;
+};
+
+HD Fq2 f2_mul_xi(const Fq2& a) {  // (a0 + a1 u)(9 + u) = (9 a0 - a1) + (a0 + 9 a1) u
+    const Fq a0x9 = add(dbl(dbl(dbl(a.c0))), a.c0), a1x9 = add(dbl(dbl(dbl(a.c1))), a.c1);
+    return {sub(a0x9, a.c1), add(a.c0, a1x9)};
+}
+
+HD Fq6 f6_zero() { return {f2_zero(), f2_zero(), f2_zero()}; }
+HD Fq6 f6_one() { return {f2_one(), f2_zero(), f2_zero()}; }
+HD Fq6 f6_add(const Fq6& a, const Fq6& b) { return {f2_add(a.a0, b.a0), f2_add(a.a1, b.a1), f2_add(a.a2, b.a2)}; }
+HD Fq6 f6_sub(const Fq6& a, const Fq6& b) { return {f2_sub(a.a0, b.a0), f2_sub(a.a1, b.a1), f2_sub(a.a2, b.a2)}; }
+HD Fq6 f6_neg(const Fq6& a) { return {f2_neg(a.a0), f2_neg(a.a1), f2_neg(a.a2)}; }
+HD Fq6 f6_mul_v(const Fq6& a) { return {f2_mul_xi(a.a2), a.a0, a.a1}; }  // a v
+
+HD_CALL Fq6 f6_mul(const Fq6& a, const Fq6& b) {
+    // Karatsuba over the three coefficients (6 Fq2 products).
+    const Fq2 t0 = f2_mul(a.a0, b.a0), t1 = f2_mul(a.a1, b.a1), t2 = f2_mul(a.a2, b.a2);
+    const Fq2 m12 = f2_sub(f2_sub(f2_mul(f2_add(a.a1, a.a2), f2_add(b.a1, b.a2)), t1), t2);
+    const Fq2 m01 = f2_sub(f2_sub(f2_mul(f2_add(a.a0, a.a1), f2_add(b.a0, b.a1)), t0), t1);
+    const Fq2 m02 = f2_sub(f2_sub(f2_mul(f2_add(a.a0, a.a2), f2_add(b.a0, b.a2)), t0), t2);
+    return {f2_add(t0, f2_mul_xi(m12)), f2_add(m01, f2_mul_xi(t2)), f2_add(m02, t1)};
+}
+
+HD Fq6 f6_mul_f2(const Fq6& a, const Fq2& b) { return {f2_mul(a.a0, b), f2_mul(a.a1, b), f2_mul(a.a2, b)}; }
+
+HD_CALL Fq6 f6_inv(const Fq6& a) {
+    const Fq2 t0 = f2_sub(f2_sqr(a.a0), f2_mul_xi(f2_mul(a.a1, a.a2)));
+    const Fq2 t1 = f2_sub(f2_mul_xi(f2_sqr(a.a2)), f2_mul(a.a0, a.a1));
+    const Fq2 t2 = f2_sub(f2_sqr(a.a1), f2_mul(a.a0, a.a2));
+    const Fq2 den = f2_add(f2_mul(a.a0, t0), f2_mul_xi(f2_add(f2_mul(a.a2, t1), f2_mul(a.a1, t2))));
+    const Fq2 di = f2_inv(den);
+    return {f2_mul(t0, di), f2_mul(t1, di), f2_mul(t2, di)};
+}
+
+HD Fq12 f12_one() { return {f6_one(), f6_zero()}; }
+
+HD_CALL Fq12 f12_mul(const Fq12& a, const Fq12& b) {
+    const Fq6 t0 = f6_mul(a.c0, b.c0), t1 = f6_mul(a.c1, b.c1);
+    const Fq6 m = f6_sub(f6_sub(f6_mul(f6_add(a.c0, a.c1), f6_add(b.c0, b.c1)), t0), t1);
+    return {f6_add(t0, f6_mul_v(t1)), m};
+}
+
+HD_CALL Fq12 f12_sqr(const Fq12& a) {
+    // (c0 + c1 w)^2 = c0^2 + v c1^2 + 2 c0 c1 w, as (c0 + c1)(c0 + v c1) - ...
+    const Fq6 p = f6_mul(a.c0, a.c1);
+    const Fq6 s = f6_mul(f6_add(a.c0, a.c1), f6_add(a.c0, f6_mul_v(a.c1)));
+    return {f6_sub(f6_sub(s, p), f6_mul_v(p)), f6_add(p, p)};
+}
+
+HD Fq12 f12_conj(const Fq12& a) { return {a.c0, f6_neg(a.c1)}; }
+
+HD_CALL Fq12 f12_inv(const Fq12& a) {
+    const Fq6 den = f6_sub(f6_mul(a.c0, a.c0), f6_mul_v(f6_mul(a.c1, a.c1)));
+    const Fq6 di = f6_inv(den);
+    return {f6_mul(a.c0, di), f6_neg(f6_mul(a.c1, di))};
+}
+
+// a^q: conjugate every Fq2 coefficient and scale the coefficient of w^k by
+// xi^(k(q-1)/6).
+HD Fq2 frob_c(int k) {
+    switch (k) {
+        case 1: return {fp_const<FqParams>(FROB_C1_0), fp_const<FqParams>(FROB_C1_1)};
+        case 2: return {fp_const<FqParams>(FROB_C2_0), fp_const<FqParams>(FROB_C2_1)};
+        case 3: return {fp_const<FqParams>(FROB_C3_0), fp_const<FqParams>(FROB_C3_1)};
+        case 4: return {fp_const<FqParams>(FROB_C4_0), fp_const<FqParams>(FROB_C4_1)};
+        default: return {fp_const<FqParams>(FROB_C5_0), fp_const<FqParams>(FROB_C5_1)};
+    }
+}
+
+HD_CALL Fq12 f12_frobenius(const Fq12& a) {
+    // w^(2i + j): c_j's coefficient i
+    return {{f2_conj(a.c0.a0), f2_mul(f2_conj(a.c0.a1), frob_c(2)), f2_mul(f2_conj(a.c0.a2), frob_c(4))},
+            {f2_mul(f2_conj(a.c1.a0), frob_c(1)), f2_mul(f2_conj(a.c1.a1), frob_c(3)),
+             f2_mul(f2_conj(a.c1.a2), frob_c(5))}};
+}
+
+HD bool f12_is_one(const Fq12& a) {
+    return eq(a.c0.a0.c0, fp_one<FqParams>()) && is_zero(a.c0.a0.c1) && f2_is_zero(a.c0.a1) &&
+           f2_is_zero(a.c0.a2) && f2_is_zero(a.c1.a0) && f2_is_zero(a.c1.a1) && f2_is_zero(a.c1.a2);
+}
+
+struct G2Proj {
+    Fq2 X, Y, Z;
+};
+
+// Tangent line at T (evaluated at (xp, yp)) and T <- 2T.  Lines are dense
+// Fq12 values with zero coefficients at w^2, w^4, w^5.
+HD_CALL Fq12 miller_dbl(G2Proj& t, const Fq& xp, const Fq& yp) {
+    const Fq2 xx = f2_sqr(t.X), yy = f2_sqr(t.Y);
+    const Fq2 yz = f2_mul(t.Y, t.Z);
+    const Fq2 three_xx = f2_add(f2_dbl(xx), xx);
+    // line: (2 Y Z^2) yp - (3 X^2 Z) xp w + (3 X^3 - 2 Y^2 Z) w^3
+    const Fq2 c0 = f2_dbl(f2_mul(yz, t.Z));
+    const Fq2 c1 = f2_neg(f2_mul_fq(f2_mul(three_xx, t.Z), xp));
+    const Fq2 c3 = f2_sub(f2_mul(three_xx, t.X), f2_dbl(f2_mul(yy, t.Z)));
+    // dbl-2007-bl, a = 0: w = 3XX, s = 2YZ, R = Y s, B = (X + R)^2 - XX - RR,
+    // h = w^2 - 2B; X3 = h s, Y3 = w (B - h) - 2 RR, Z3 = s^3.
+    const Fq2 s = f2_dbl(yz);
+    const Fq2 ss = f2_sqr(s);
+    const Fq2 r = f2_mul(t.Y, s);
+    const Fq2 rr = f2_sqr(r);
+    const Fq2 b = f2_sub(f2_sub(f2_sqr(f2_add(t.X, r)), xx), rr);
+    const Fq2 h = f2_sub(f2_sqr(three_xx), f2_dbl(b));
+    t = {f2_mul(h, s), f2_sub(f2_mul(three_xx, f2_sub(b, h)), f2_dbl(rr)), f2_mul(s, ss)};
+    return {{f2_mul_fq(c0, yp), f2_zero(), f2_zero()}, {c1, c3, f2_zero()}};
+}
+
+// Chord through T and the affine Q = (x2, y2) (evaluated at (xp, yp)) and
+// T <- T + Q.
+HD_CALL Fq12 miller_add(G2Proj& t, const Fq2& x2, const Fq2& y2, const Fq& xp, const Fq& yp) {
+    const Fq2 theta = f2_sub(f2_mul(y2, t.Z), t.Y);  // y2 Z - Y
+    const Fq2 lambda = f2_sub(f2_mul(x2, t.Z), t.X); // x2 Z - X
+    const Fq2 c0 = f2_mul_fq(lambda, yp);
+    const Fq2 c1 = f2_neg(f2_mul_fq(theta, xp));
+    const Fq2 c3 = f2_sub(f2_mul(theta, x2), f2_mul(lambda, y2));
+    // madd-1998-cmo: uu = u^2, vv = v^2, vvv = v vv, R = vv X,
+    // A = uu Z - vvv - 2R; X3 = v A, Y3 = u (R - A) - vvv Y, Z3 = vvv Z.
+    const Fq2 uu = f2_sqr(theta), vv = f2_sqr(lambda);
+    const Fq2 vvv = f2_mul(lambda, vv);
+    const Fq2 r = f2_mul(vv, t.X);
+    const Fq2 a = f2_sub(f2_sub(f2_mul(uu, t.Z), vvv), f2_dbl(r));
+    t = {f2_mul(lambda, a), f2_sub(f2_mul(theta, f2_sub(r, a)), f2_mul(vvv, t.Y)), f2_mul(vvv, t.Z)};
+    return {{c0, f2_zero(), f2_zero()}, {c1, c3, f2_zero()}};
+}
+
+// pi(Q) on the twist: (conj(x) xi^((q-1)/3), conj(y) xi^((q-1)/2)).
+HD void twist_frobenius(const Fq2& x, const Fq2& y, Fq2& ox, Fq2& oy) {
+    ox = f2_mul(f2_conj(x), frob_c(2));
+    oy = f2_mul(f2_conj(y), frob_c(3));
+}
+
+// Miller loop of one pair; P = (xp, yp) and Q = (x2, y2) finite, affine,
+// Montgomery.  The caller multiplies the results of several pairs.
+HD_CALL Fq12 miller_loop(const Fq& xp, const Fq& yp, const Fq2& x2, const Fq2& y2) {
+    G2Proj t = {x2, y2, f2_one()};
+    Fq12 f = f12_one();
+    for (int i = PAIRING_ATE_BITS - 2; i >= 0; --i) {
+        f = f12_mul(f12_sqr(f), miller_dbl(t, xp, yp));
+        const uint32_t bit = i >= 64 ? (PAIRING_ATE_HI >> (i - 64)) & 1u : uint32_t(PAIRING_ATE_LO >> i) & 1u;
+        if (bit) f = f12_mul(f, miller_add(t, x2, y2, xp, yp));
+    }
+    Fq2 q1x, q1y, q2x, q2y;
+    twist_frobenius(x2, y2, q1x, q1y);
+    twist_frobenius(q1x, q1y, q2x, q2y);
+    f = f12_mul(f, miller_add(t, q1x, q1y, xp, yp));
+    f = f12_mul(f, miller_add(t, q2x, f2_neg(q2y), xp, yp));
+    return f;
+}
+
+HD_CALL Fq12 final_exponentiation(const Fq12& f) {
+    Fq12 a = f12_mul(f12_conj(f), f12_inv(f));      // f^(q^6 - 1)
+    a = f12_mul(f12_frobenius(f12_frobenius(a)), a);  // ^(q^2 + 1)
+    Fq12 r = f12_one();
+    for (int i = HARD_EXP_BITS - 1; i >= 0; --i) {
+        r = f12_sqr(r);
+        if ((HARD_EXP[i >> 5] >> (i & 31)) & 1u) r = f12_mul(r, a);
+    }
+    return r;
+}
+
+}  // namespace bn254

@@ -1,0 +1,64 @@
+"""Batched pairing-product checks on the device (include/dynaverse_pairing.h):
+the host mirror of the opening-cache checks of
+crates/protocol/src/dynaverse/rebuilding/machine.rs:255-390, where every
+cache point pi_i of a commitment C must satisfy
+e(C - [v_i] g1, g2) == e(pi_i, [tau]g2 - [z_i]g2).
+
+    ok = pairing_check(backend, [[(P1, Q1), (P2, Q2)], ...])   # prod == 1 per group
+    groups = cache_check_groups(differences, g2, openings, divisors)
+
+Points are affine int tuples or None, as everywhere in this package.
+"""
+import ctypes
+from typing import List, Sequence, Tuple
+
+from .backend import HipBackend, _check, _err
+from .codec import FQ_MODULUS, encode_g1, encode_g2
+
+
+def _encode(groups):
+    terms = len(groups[0])
+    if any(len(g) != terms for g in groups):
+        raise ValueError("every group needs the same number of pairs")
+    g1 = b"".join(encode_g1(p) for g in groups for p, _ in g)
+    g2 = b"".join(encode_g2(q) for g in groups for _, q in g)
+    return terms, g1, g2
+
+
+def pairing_check_encoded(backend: HipBackend, g1: bytes, g2: bytes, terms: int, count: int,
+                          want_gt: bool = False):
+    """Encoded form; returns (results bytes, gt bytes or None, kernel ms)."""
+    res = ctypes.create_string_buffer(max(1, count))
+    gt = ctypes.create_string_buffer(384 * count) if want_gt else None
+    ms = ctypes.c_float(0)
+    err = _err()
+    _check(backend.lib.dvgpu_bn254_pairing_check(backend.handle, ctypes.c_char_p(g1 or b"\0"),
+                                                 ctypes.c_char_p(g2 or b"\0"), terms, count, res, gt,
+                                                 ctypes.byref(ms), err, 256), err)
+    return res.raw[:count], (gt.raw if gt is not None else None), ms.value
+
+
+def pairing_check(backend: HipBackend, groups: Sequence[Sequence[Tuple]]) -> List[bool]:
+    """For every group of (P, Q) pairs: is prod e(P, Q) one?"""
+    if not groups:
+        return []
+    terms, g1, g2 = _encode(groups)
+    res, _, _ = pairing_check_encoded(backend, g1, g2, terms, len(groups))
+    return [b == 1 for b in res]
+
+
+def pairing_products(backend: HipBackend, groups) -> List[Tuple[int, ...]]:
+    """The reduced products in tower order (12 canonical Fq ints each)."""
+    terms, g1, g2 = _encode(groups)
+    _, gt, _ = pairing_check_encoded(backend, g1, g2, terms, len(groups), want_gt=True)
+    return [tuple(int.from_bytes(gt[384 * i + 32 * k:384 * i + 32 * k + 32], "little") for k in range(12))
+            for i in range(len(groups))]
+
+
+def _neg_g1(p):
+    return None if p is None else (p[0], (-p[1]) % FQ_MODULUS)
+
+
+def cache_check_groups(commitment_minus_values: Sequence, g2_gen, openings: Sequence, divisors: Sequence):
+    """The two-term groups of machine.rs:333: (C - [v_i]g1, g2), (-pi_i, [tau - z_i]g2)."""
+    return [[(d, g2_gen), (_neg_g1(pi), q)] for d, pi, q in zip(commitment_minus_values, openings, divisors)]
