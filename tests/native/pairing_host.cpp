@@ -59,12 +59,12 @@ int main() {
             wr12(miller_loop(rd(a[0]), rd(a[1]), rd2(a + 2), rd2(a + 4)));
         } else if (!strcmp(op, "product")) {
             const int k = atoi(a[0]);
-            Fq12 f = f12_one();
+            std::vector<PairingInput> in;
             for (int i = 0; i < k; ++i) {
                 char** p = a + 1 + 6 * i;
-                f = f12_mul(f, miller_loop(rd(p[0]), rd(p[1]), rd2(p + 2), rd2(p + 4)));
+                in.push_back({rd(p[0]), rd(p[1]), rd2(p + 2), rd2(p + 4)});
             }
-            wr12(final_exponentiation(f));
+            wr12(final_exponentiation(miller_product(in.data(), k)));
         } else if (!strcmp(op, "fexp")) {
             wr12(final_exponentiation(rd12(a)));
         } else if (!strcmp(op, "mul")) {

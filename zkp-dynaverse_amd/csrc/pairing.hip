@@ -35,16 +35,16 @@ __global__ void __launch_bounds__(64) k_pairing_check(const uint32_t* __restrict
                                                       uint32_t* __restrict__ gt) {
     const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= count) return;
-    Fq12 f = f12_one();
+    PairingInput in[PAIRING_MAX_TERMS];
+    int k = 0;
     for (uint32_t j = 0; j < terms; ++j) {
         const uint32_t* p = g1 + (g * terms + j) * 16;
         const uint32_t* q = g2 + (g * terms + j) * 32;
-        if (all_zero(p, 16) || all_zero(q, 32)) continue;
-        const Fq xp = ld_canon_fq(p), yp = ld_canon_fq(p + 8);
-        const Fq2 x2 = {ld_canon_fq(q), ld_canon_fq(q + 8)}, y2 = {ld_canon_fq(q + 16), ld_canon_fq(q + 24)};
-        f = f12_mul(f, miller_loop(xp, yp, x2, y2));
+        if (all_zero(p, 16) || all_zero(q, 32)) continue;  // e(O, Q) = e(P, O) = 1
+        in[k++] = {ld_canon_fq(p), ld_canon_fq(p + 8), {ld_canon_fq(q), ld_canon_fq(q + 8)},
+                   {ld_canon_fq(q + 16), ld_canon_fq(q + 24)}};
     }
-    const Fq12 r = final_exponentiation(f);
+    const Fq12 r = final_exponentiation(k ? miller_product(in, k) : f12_one());
     ok[g] = f12_is_one(r) ? 1 : 0;
     if (gt) {
         uint32_t* o = gt + g * 96;
@@ -102,7 +102,7 @@ extern "C" int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const
                                          char* error, size_t capacity) {
     Owner* o = as_owner(backend);
     return guarded(o, error, capacity, [&]() -> int {
-        require(terms >= 1 && terms <= 16, "terms must be 1..16");
+        require(terms >= 1 && terms <= PAIRING_MAX_TERMS, "terms must be 1..16");
         require(count <= (size_t(1) << 24), "too many pairing checks");
         if (!count) return 0;
         require(g1 && g2 && results, "null pairing buffers");

@@ -16,8 +16,9 @@
 //   chord T, Q = (x2, y2):     (x2 Z - X) yp - (y2 Z - Y) xp w + ((y2 Z - Y) x2 - (x2 Z - X) y2) w^3
 // Point steps: dbl-2007-bl and madd-1998-cmo (EFD, homogeneous, a = 0).
 // Final exponentiation: f^((q^6 - 1)(q^2 + 1)) by conjugation, inversion and
-// the q^2-power Frobenius, then the hard part (q^4 - q^2 + 1)/r by
-// square-and-multiply over its 761 bits.
+// the q^2-power Frobenius, then the hard part (q^4 - q^2 + 1)/r through
+// three exponentiations by x and the q-power Frobenius.  Several pairs share
+// one Miller loop (one squaring chain) and one final exponentiation.
 #pragma once
 #include "curve.hpp"
 
@@ -122,16 +123,35 @@ struct G2Proj {
     Fq2 X, Y, Z;
 };
 
-// Tangent line at T (evaluated at (xp, yp)) and T <- 2T.  Lines are dense
-// Fq12 values with zero coefficients at w^2, w^4, w^5.
-HD_CALL Fq12 miller_dbl(G2Proj& t, const Fq& xp, const Fq& yp) {
+// A line l0 + l1 w + l3 w^3 (Fq2 coefficients; w^1 and w^3 are c1's first
+// two coefficients in the tower).
+struct Line {
+    Fq2 l0, l1, l3;
+};
+
+// f * line: (f0 + f1 w)(c0 + c1 w) with c0 = l0, c1 = l1 + l3 v; 15 Fq2 products.
+HD_CALL Fq12 f12_mul_line(const Fq12& f, const Line& l) {
+    const Fq6 t0 = f6_mul_f2(f.c0, l.l0);
+    // f1 (l1 + l3 v) = (b0 l1 + xi b2 l3) + (b0 l3 + b1 l1) v + (b1 l3 + b2 l1) v^2
+    const Fq6& b = f.c1;
+    const Fq6 t1 = {f2_add(f2_mul(b.a0, l.l1), f2_mul_xi(f2_mul(b.a2, l.l3))),
+                    f2_add(f2_mul(b.a0, l.l3), f2_mul(b.a1, l.l1)), f2_add(f2_mul(b.a1, l.l3), f2_mul(b.a2, l.l1))};
+    // (f0 + f1)((l0 + l1) + l3 v)
+    const Fq6 s = f6_add(f.c0, f.c1);
+    const Fq2 m0 = f2_add(l.l0, l.l1);
+    const Fq6 t2 = {f2_add(f2_mul(s.a0, m0), f2_mul_xi(f2_mul(s.a2, l.l3))),
+                    f2_add(f2_mul(s.a0, l.l3), f2_mul(s.a1, m0)), f2_add(f2_mul(s.a1, l.l3), f2_mul(s.a2, m0))};
+    return {f6_add(t0, f6_mul_v(t1)), f6_sub(f6_sub(t2, t0), t1)};
+}
+
+// Tangent line at T (evaluated at (xp, yp)) and T <- 2T.
+HD_CALL Line miller_dbl(G2Proj& t, const Fq& xp, const Fq& yp) {
     const Fq2 xx = f2_sqr(t.X), yy = f2_sqr(t.Y);
     const Fq2 yz = f2_mul(t.Y, t.Z);
     const Fq2 three_xx = f2_add(f2_dbl(xx), xx);
     // line: (2 Y Z^2) yp - (3 X^2 Z) xp w + (3 X^3 - 2 Y^2 Z) w^3
-    const Fq2 c0 = f2_dbl(f2_mul(yz, t.Z));
-    const Fq2 c1 = f2_neg(f2_mul_fq(f2_mul(three_xx, t.Z), xp));
-    const Fq2 c3 = f2_sub(f2_mul(three_xx, t.X), f2_dbl(f2_mul(yy, t.Z)));
+    const Line l = {f2_mul_fq(f2_dbl(f2_mul(yz, t.Z)), yp), f2_neg(f2_mul_fq(f2_mul(three_xx, t.Z), xp)),
+                    f2_sub(f2_mul(three_xx, t.X), f2_dbl(f2_mul(yy, t.Z)))};
     // dbl-2007-bl, a = 0: w = 3XX, s = 2YZ, R = Y s, B = (X + R)^2 - XX - RR,
     // h = w^2 - 2B; X3 = h s, Y3 = w (B - h) - 2 RR, Z3 = s^3.
     const Fq2 s = f2_dbl(yz);
@@ -141,17 +161,16 @@ HD_CALL Fq12 miller_dbl(G2Proj& t, const Fq& xp, const Fq& yp) {
     const Fq2 b = f2_sub(f2_sub(f2_sqr(f2_add(t.X, r)), xx), rr);
     const Fq2 h = f2_sub(f2_sqr(three_xx), f2_dbl(b));
     t = {f2_mul(h, s), f2_sub(f2_mul(three_xx, f2_sub(b, h)), f2_dbl(rr)), f2_mul(s, ss)};
-    return {{f2_mul_fq(c0, yp), f2_zero(), f2_zero()}, {c1, c3, f2_zero()}};
+    return l;
 }
 
 // Chord through T and the affine Q = (x2, y2) (evaluated at (xp, yp)) and
 // T <- T + Q.
-HD_CALL Fq12 miller_add(G2Proj& t, const Fq2& x2, const Fq2& y2, const Fq& xp, const Fq& yp) {
+HD_CALL Line miller_add(G2Proj& t, const Fq2& x2, const Fq2& y2, const Fq& xp, const Fq& yp) {
     const Fq2 theta = f2_sub(f2_mul(y2, t.Z), t.Y);  // y2 Z - Y
     const Fq2 lambda = f2_sub(f2_mul(x2, t.Z), t.X); // x2 Z - X
-    const Fq2 c0 = f2_mul_fq(lambda, yp);
-    const Fq2 c1 = f2_neg(f2_mul_fq(theta, xp));
-    const Fq2 c3 = f2_sub(f2_mul(theta, x2), f2_mul(lambda, y2));
+    const Line l = {f2_mul_fq(lambda, yp), f2_neg(f2_mul_fq(theta, xp)),
+                    f2_sub(f2_mul(theta, x2), f2_mul(lambda, y2))};
     // madd-1998-cmo: uu = u^2, vv = v^2, vvv = v vv, R = vv X,
     // A = uu Z - vvv - 2R; X3 = v A, Y3 = u (R - A) - vvv Y, Z3 = vvv Z.
     const Fq2 uu = f2_sqr(theta), vv = f2_sqr(lambda);
@@ -159,7 +178,7 @@ HD_CALL Fq12 miller_add(G2Proj& t, const Fq2& x2, const Fq2& y2, const Fq& xp, c
     const Fq2 r = f2_mul(vv, t.X);
     const Fq2 a = f2_sub(f2_sub(f2_mul(uu, t.Z), vvv), f2_dbl(r));
     t = {f2_mul(lambda, a), f2_sub(f2_mul(theta, f2_sub(r, a)), f2_mul(vvv, t.Y)), f2_mul(vvv, t.Z)};
-    return {{c0, f2_zero(), f2_zero()}, {c1, c3, f2_zero()}};
+    return l;
 }
 
 // pi(Q) on the twist: (conj(x) xi^((q-1)/3), conj(y) xi^((q-1)/2)).
@@ -168,33 +187,81 @@ HD void twist_frobenius(const Fq2& x, const Fq2& y, Fq2& ox, Fq2& oy) {
     oy = f2_mul(f2_conj(y), frob_c(3));
 }
 
-// Miller loop of one pair; P = (xp, yp) and Q = (x2, y2) finite, affine,
-// Montgomery.  The caller multiplies the results of several pairs.
-HD_CALL Fq12 miller_loop(const Fq& xp, const Fq& yp, const Fq2& x2, const Fq2& y2) {
-    G2Proj t = {x2, y2, f2_one()};
+struct PairingInput {
+    Fq xp, yp;   // P, affine, finite
+    Fq2 x2, y2;  // Q on the twist, affine, finite
+};
+
+constexpr int PAIRING_MAX_TERMS = 16;
+
+// prod_j f_j for k finite pairs, sharing the squaring of the accumulator
+// (one Miller loop over all pairs).
+HD_CALL Fq12 miller_product(const PairingInput* in, int k) {
+    G2Proj t[PAIRING_MAX_TERMS];
+    for (int j = 0; j < k; ++j) t[j] = {in[j].x2, in[j].y2, f2_one()};
     Fq12 f = f12_one();
     for (int i = PAIRING_ATE_BITS - 2; i >= 0; --i) {
-        f = f12_mul(f12_sqr(f), miller_dbl(t, xp, yp));
+        if (i != PAIRING_ATE_BITS - 2) f = f12_sqr(f);
+        for (int j = 0; j < k; ++j) f = f12_mul_line(f, miller_dbl(t[j], in[j].xp, in[j].yp));
         const uint32_t bit = i >= 64 ? (PAIRING_ATE_HI >> (i - 64)) & 1u : uint32_t(PAIRING_ATE_LO >> i) & 1u;
-        if (bit) f = f12_mul(f, miller_add(t, x2, y2, xp, yp));
+        if (bit)
+            for (int j = 0; j < k; ++j) f = f12_mul_line(f, miller_add(t[j], in[j].x2, in[j].y2, in[j].xp, in[j].yp));
     }
-    Fq2 q1x, q1y, q2x, q2y;
-    twist_frobenius(x2, y2, q1x, q1y);
-    twist_frobenius(q1x, q1y, q2x, q2y);
-    f = f12_mul(f, miller_add(t, q1x, q1y, xp, yp));
-    f = f12_mul(f, miller_add(t, q2x, f2_neg(q2y), xp, yp));
+    for (int j = 0; j < k; ++j) {
+        Fq2 q1x, q1y, q2x, q2y;
+        twist_frobenius(in[j].x2, in[j].y2, q1x, q1y);
+        twist_frobenius(q1x, q1y, q2x, q2y);
+        f = f12_mul_line(f, miller_add(t[j], q1x, q1y, in[j].xp, in[j].yp));
+        f = f12_mul_line(f, miller_add(t[j], q2x, f2_neg(q2y), in[j].xp, in[j].yp));
+    }
     return f;
 }
 
-HD_CALL Fq12 final_exponentiation(const Fq12& f) {
-    Fq12 a = f12_mul(f12_conj(f), f12_inv(f));      // f^(q^6 - 1)
-    a = f12_mul(f12_frobenius(f12_frobenius(a)), a);  // ^(q^2 + 1)
-    Fq12 r = f12_one();
-    for (int i = HARD_EXP_BITS - 1; i >= 0; --i) {
+// Miller value of one pair.
+HD_CALL Fq12 miller_loop(const Fq& xp, const Fq& yp, const Fq2& x2, const Fq2& y2) {
+    const PairingInput in = {xp, yp, x2, y2};
+    return miller_product(&in, 1);
+}
+
+// f^x for the BN parameter x (63 bits), square-and-multiply.
+HD_CALL Fq12 f12_pow_x(const Fq12& f) {
+    Fq12 r = f;
+    for (int i = 61; i >= 0; --i) {
         r = f12_sqr(r);
-        if ((HARD_EXP[i >> 5] >> (i & 31)) & 1u) r = f12_mul(r, a);
+        if ((BN_SEED >> i) & 1u) r = f12_mul(r, f);
     }
     return r;
+}
+
+// f^((q^12 - 1)/r).  Easy part f^((q^6 - 1)(q^2 + 1)); the result a lies in
+// the cyclotomic subgroup, where a^-1 = conj(a).  Hard part
+// (q^4 - q^2 + 1)/r = l0 + l1 q + l2 q^2 + l3 q^3 with
+//   l3 = 1, l2 = 6x^2 + 1, l1 = -36x^3 - 18x^2 - 12x + 1,
+//   l0 = -36x^3 - 30x^2 - 18x - 2
+// (Scott, Benger, Charlemagne, Dominguez Perez, Kachisa, "On the final
+// exponentiation for calculating pairings on ordinary elliptic curves",
+// 2009), from a^x, a^(x^2), a^(x^3) and the q-power Frobenius.
+HD_CALL Fq12 final_exponentiation(const Fq12& f) {
+    Fq12 a = f12_mul(f12_conj(f), f12_inv(f));
+    a = f12_mul(f12_frobenius(f12_frobenius(a)), a);
+    const Fq12 x1 = f12_pow_x(a), x2 = f12_pow_x(x1), x3 = f12_pow_x(x2);
+    const Fq12 x3_4 = f12_sqr(f12_sqr(x3));
+    const Fq12 x3_36 = f12_mul(f12_sqr(f12_sqr(f12_sqr(x3_4))), x3_4);
+    const Fq12 x2_2 = f12_sqr(x2);
+    const Fq12 x2_6 = f12_mul(f12_sqr(x2_2), x2_2);
+    const Fq12 x2_12 = f12_sqr(x2_6);
+    const Fq12 x2_18 = f12_mul(x2_12, x2_6);
+    const Fq12 x2_30 = f12_mul(x2_18, x2_12);
+    const Fq12 x1_2 = f12_sqr(x1);
+    const Fq12 x1_6 = f12_mul(f12_sqr(x1_2), x1_2);
+    const Fq12 x1_12 = f12_sqr(x1_6);
+    const Fq12 x1_18 = f12_mul(x1_12, x1_6);
+    const Fq12 p0 = f12_conj(f12_mul(f12_mul(x3_36, x2_30), f12_mul(x1_18, f12_sqr(a))));
+    const Fq12 p1 = f12_mul(f12_conj(f12_mul(f12_mul(x3_36, x2_18), x1_12)), a);
+    const Fq12 p2 = f12_mul(x2_6, a);
+    const Fq12 q2 = f12_frobenius(f12_frobenius(p2));
+    const Fq12 q3 = f12_frobenius(f12_frobenius(f12_frobenius(a)));
+    return f12_mul(f12_mul(p0, f12_frobenius(p1)), f12_mul(q2, q3));
 }
 
 }  // namespace bn254
