@@ -48,7 +48,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the batched, incremental and served workloads")
-    ap.add_argument("--no-kzg", action="store_true", help="skip the opening-cache precompute workload")
+    ap.add_argument("--no-kzg", action="store_true",
+                    help="skip the opening-cache precompute and pairing-check workloads")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the plain copy kernel (traffic-counter calibration)")
     args = ap.parse_args()
@@ -151,6 +152,8 @@ def main():
         log("# opening caches n=2^16")
         secondary["opening_cache_2p16"] = benchmarks.opening_cache(backend, log=log)
         secondary["opening_cache_2p13_x1"] = benchmarks.opening_cache(backend, log_n=13, rows=1, log=log)
+        log("# opening-cache validation: 3 x 2^16 pairing-product checks")
+        secondary["pairing_cache_check_2p16"] = benchmarks.pairing_cache_check(backend, log=log)
 
     if args.calibrate_copy:
         nbytes = 256 << 20
@@ -199,6 +202,17 @@ def main():
         cref.kzg_openings(srs_k, nk, scal[:nk].tobytes(), nk, nk, encode_fr(root_of_unity_2_20()), threads=threads)
         dt = time.perf_counter() - a
         gpu_ms = secondary.get("opening_cache_2p13_x1", {}).get("cold_ms")
+        # Pairing-product sample: the oracle's optimal ate pairing (Python
+        # big integers, one core) on four two-term checks.
+        from oracle import bn254 as ob
+        a = time.perf_counter()
+        for k in range(4):
+            p1 = ob.g1_mul(k + 2)
+            ob.pairing_product([(p1, ob.G2_GEN), (ob.neg(ob.G1_GEN), ob.mul(ob.G2_GEN, k + 2, ob.FQ2))])
+        dt_p = time.perf_counter() - a
+        secondary["pairing_cpu_sample"] = {
+            "checks_per_s": round(4 / dt_p, 2), "cores": 1, "kind": "port",
+            "sample": "four two-term pairing products, Python oracle (not arkworks)"}
         secondary["opening_cache_cpu_sample"] = {
             "cpu_ms": round(dt * 1e3, 1), "cores": threads, "kind": "port", "gpu_cold_ms": gpu_ms,
             "sample": "one opening cache, 2^13 coefficients over 2^13 points, SRS transform included, "

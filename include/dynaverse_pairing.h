@@ -16,10 +16,11 @@
  *
  * Encodings as in dynaverse_gpu.h: G1 64 B x || y, G2 128 B x.c0 || x.c1 ||
  * y.c0 || y.c1, little-endian canonical, all-zero = identity (the pair then
- * contributes one).  Points must lie on E (G1) and on the twist E' (G2) or
- * the call fails with -1 before any device work; G2 points are taken to be in
- * the r-torsion (callers validate them, as for dvgpu_bn254_g2_msm).  Status
- * codes as in dynaverse_gpu.h.
+ * contributes one).  Points must be canonical and lie on E (G1) and on the
+ * twist E' (G2), checked on the device; otherwise the call fails with -1 and
+ * zeroes results and gt.  G2 points are taken to be in the r-torsion
+ * (callers validate them, as for dvgpu_bn254_g2_msm).  Status codes as in
+ * dynaverse_gpu.h.
  */
 #pragma once
 #include <stddef.h>

@@ -2,6 +2,7 @@
 // group of pairs; Miller loops multiplied into one Fq12, one final
 // exponentiation, the product compared with one.
 #include <cstring>
+#include <string>
 
 #include "../../include/dynaverse_pairing.h"
 #include "owner.hpp"
@@ -30,6 +31,12 @@ __device__ __forceinline__ bool all_zero(const uint32_t* p, int n) {
     return !any;
 }
 
+__device__ __forceinline__ bool canonical_fq(const uint32_t* p) {
+    uint32_t w[8];
+    for (int i = 0; i < 8; ++i) w[i] = p[i];
+    return is_canonical<FqParams>(w);
+}
+
 __global__ void __launch_bounds__(64) k_pairing_check(const uint32_t* __restrict__ g1, const uint32_t* __restrict__ g2,
                                                       uint32_t terms, uint64_t count, uint8_t* __restrict__ ok,
                                                       uint32_t* __restrict__ gt) {
@@ -37,12 +44,29 @@ __global__ void __launch_bounds__(64) k_pairing_check(const uint32_t* __restrict
     if (g >= count) return;
     PairingInput in[PAIRING_MAX_TERMS];
     int k = 0;
+    bool valid = true;
     for (uint32_t j = 0; j < terms; ++j) {
         const uint32_t* p = g1 + (g * terms + j) * 16;
         const uint32_t* q = g2 + (g * terms + j) * 32;
-        if (all_zero(p, 16) || all_zero(q, 32)) continue;  // e(O, Q) = e(P, O) = 1
-        in[k++] = {ld_canon_fq(p), ld_canon_fq(p + 8), {ld_canon_fq(q), ld_canon_fq(q + 8)},
-                   {ld_canon_fq(q + 16), ld_canon_fq(q + 24)}};
+        const bool p0 = all_zero(p, 16), q0 = all_zero(q, 32);
+        bool pv = p0, qv = q0;
+        PairingInput e;
+        if (!p0 && canonical_fq(p) && canonical_fq(p + 8)) {
+            e.xp = ld_canon_fq(p);
+            e.yp = ld_canon_fq(p + 8);
+            pv = affine_on_curve<FqOps>({e.xp, e.yp, false});
+        }
+        if (!q0 && canonical_fq(q) && canonical_fq(q + 8) && canonical_fq(q + 16) && canonical_fq(q + 24)) {
+            e.x2 = {ld_canon_fq(q), ld_canon_fq(q + 8)};
+            e.y2 = {ld_canon_fq(q + 16), ld_canon_fq(q + 24)};
+            qv = affine_on_curve<Fq2Ops>({e.x2, e.y2, false});
+        }
+        valid = valid && pv && qv;
+        if (!p0 && !q0) in[k++] = e;  // e(O, Q) = e(P, O) = 1
+    }
+    if (!valid) {  // reported to the host, which fails the call
+        ok[g] = 2;
+        return;
     }
     const Fq12 r = final_exponentiation(k ? miller_product(in, k) : f12_one());
     ok[g] = f12_is_one(r) ? 1 : 0;
@@ -61,37 +85,6 @@ __global__ void __launch_bounds__(64) k_pairing_check(const uint32_t* __restrict
 
 Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
 
-bool host_all_zero(const uint8_t* p, size_t n) {
-    for (size_t i = 0; i < n; ++i)
-        if (p[i]) return false;
-    return true;
-}
-
-void validate_points(const uint8_t* g1, const uint8_t* g2, size_t n) {
-    for (size_t i = 0; i < n; ++i) {
-        const uint8_t* p = g1 + 64 * i;
-        if (!host_all_zero(p, 64)) {
-            uint32_t x[8], y[8];
-            std::memcpy(x, p, 32);
-            std::memcpy(y, p + 32, 32);
-            require(is_canonical<FqParams>(x) && is_canonical<FqParams>(y), "G1 coordinate not canonical");
-            const Affine<FqOps> a = {to_mont<FqParams>(x), to_mont<FqParams>(y), false};
-            require(affine_on_curve<FqOps>(a), "G1 point is not on the curve");
-        }
-        const uint8_t* q = g2 + 128 * i;
-        if (!host_all_zero(q, 128)) {
-            uint32_t w[4][8];
-            for (int k = 0; k < 4; ++k) {
-                std::memcpy(w[k], q + 32 * k, 32);
-                require(is_canonical<FqParams>(w[k]), "G2 coordinate not canonical");
-            }
-            const Affine<Fq2Ops> a = {{to_mont<FqParams>(w[0]), to_mont<FqParams>(w[1])},
-                                      {to_mont<FqParams>(w[2]), to_mont<FqParams>(w[3])}, false};
-            require(affine_on_curve<Fq2Ops>(a), "G2 point is not on the twist");
-        }
-    }
-}
-
 }  // namespace
 }  // namespace dvgpu
 
@@ -107,7 +100,6 @@ extern "C" int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const
         if (!count) return 0;
         require(g1 && g2 && results, "null pairing buffers");
         const size_t n = terms * count;
-        validate_points(g1, g2, n);
         hipStream_t st = o->stream;
         // one device allocation: g1 | g2 | results | gt
         const size_t b1 = n * 64, b2 = n * 128, br = (count + 255) & ~size_t(255), bg = gt ? count * 384 : 0;
@@ -130,6 +122,14 @@ extern "C" int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const
         if (kernel_ms) {
             HIP_CHECK(hipEventElapsedTime(kernel_ms, ev[0], ev[1]));
             for (auto& e : ev) (void)hipEventDestroy(e);
+        }
+        const void* bad = std::memchr(results, 2, count);
+        if (bad) {
+            const size_t i = static_cast<const uint8_t*>(bad) - results;
+            std::memset(results, 0, count);
+            if (gt) std::memset(gt, 0, count * 384);
+            throw InvalidArgument(("pairing check " + std::to_string(i) +
+                                   ": a point is not canonical or not on its curve").c_str());
         }
         return 0;
     });

@@ -335,12 +335,86 @@ def opening_cache(backend: HipBackend, log_n: int = 16, rows: int = 3, reps: int
     return out
 
 
+# ---- opening-cache validation (rebuilding/machine.rs:357-382) ---------------
+
+def pairing_cache_check(backend: HipBackend, log_n: int = 16, caches: int = 3, reps: int = 3, log=print) -> Dict:
+    """validate_initial_caches at n = 2^log_n: every point of the three caches
+    checked by e(C - [v_i]g1, g2) == e(pi_i, [tau]g2 - [z_i]g2), i.e.
+    caches x n two-term pairing products.  The points come from a 2^10-point
+    cache computed on the device and verified, tiled to the full count (the
+    per-check work does not depend on the values), with every 97th check
+    corrupted so both outcomes occur."""
+    from .codec import FQ_MODULUS, FR_MODULUS, encode_g1, encode_g2
+    from .kzg import KzgSrs
+    from .pairing import pairing_check_encoded
+    from .workload import srs_scalars
+    r = FR_MODULUS
+    m = 1 << 10
+    tau, powers = srs_scalars(2026, m)
+    srs = backend.fixed_base_encoded(powers.tobytes(), m, False)
+    coeffs = ChaCha20Stream(2027).uniform_fr(m)
+    with KzgSrs(backend, encoded=srs) as z:
+        raw = z.precompute_bytes(b"".join(c.to_bytes(32, "little") for c in coeffs), m, 1, m)
+    w = 1
+    from .codec import root_of_unity_2_20, decode_g1
+    w_m = pow(root_of_unity_2_20(), 1 << (20 - 10), r)
+
+    def ev(x):
+        acc = 0
+        for c in reversed(coeffs):
+            acc = (acc * x + c) % r
+        return acc
+    f_tau = ev(tau)
+    # C - [f(z)]g1 = [f(tau) - f(z)]g1 and [tau - z]g2 through the fixed-base kernels
+    diffs, divs = [], []
+    for i in range(m):
+        zi = w
+        diffs.append(((f_tau - ev(zi)) % r).to_bytes(32, "little"))
+        divs.append(((tau - zi) % r).to_bytes(32, "little"))
+        w = w * w_m % r
+    d1 = b"".join(backend.fixed_base_encoded(b"".join(diffs[i:i + 4096]), len(diffs[i:i + 4096]), False)
+                  for i in range(0, m, 4096))
+    d2, _ = backend.fixed_base_g2_checked_encoded(b"".join(divs), m, 3)
+    g2gen = backend.fixed_base_g2_checked_encoded((1).to_bytes(32, "little"), 1, 3)[0]
+    g1, g2 = bytearray(), bytearray()
+    for i in range(m):
+        op = decode_g1(raw[64 * i:64 * i + 64])
+        neg = encode_g1(None if op is None else (op[0], (-op[1]) % FQ_MODULUS))
+        g1 += d1[64 * i:64 * i + 64] + neg
+        g2 += g2gen + d2[128 * i:128 * i + 128]
+    total = caches << log_n
+    reps_tile = total // m
+    g1 = bytearray(bytes(g1) * reps_tile)
+    g2 = bytes(g2) * reps_tile
+    bad = list(range(5, total, 97))
+    for i in bad:                      # corrupt the opening: pi -> pi + g1
+        g1[128 * i + 64:128 * i + 128] = encode_g1((1, 2))
+    g1 = bytes(g1)
+    res, _, ms0 = pairing_check_encoded(backend, g1, g2, 2, total)
+    expect_bad = set(bad)
+    ok = all((res[i] == 0) == (i in expect_bad) for i in range(total))
+    times, walls = [], []
+    for _ in range(reps):
+        a = time.perf_counter()
+        again, _, ms = pairing_check_encoded(backend, g1, g2, 2, total)
+        walls.append((time.perf_counter() - a) * 1e3)
+        times.append(ms)
+    kms = sorted(times)[len(times) // 2]
+    out = {"checks": total, "terms": 2, "kernel_ms": round(kms, 3),
+           "wall_ms": round(sorted(walls)[len(walls) // 2], 3),
+           "checks_per_s": round(total / (kms * 1e-3)), "accepts_and_rejects_correct": bool(ok and again == res),
+           "rejected": len(bad)}
+    log(f"pairing cache check {total} x 2 terms: kernel {kms:.1f} ms ({out['checks_per_s']:.0f} checks/s), ok={ok}")
+    return out
+
+
 def main(argv=None):
     """One workload by name, for profiling: python -m dvgpu.benchmarks opening_cache --log-n 16"""
     import argparse
     import json
     ap = argparse.ArgumentParser(description=main.__doc__)
-    ap.add_argument("workload", choices=["opening_cache", "batched_launch", "incremental", "served_stream"])
+    ap.add_argument("workload", choices=["opening_cache", "pairing_cache_check", "batched_launch", "incremental",
+                                         "served_stream"])
     ap.add_argument("--log-n", type=int, default=16)
     ap.add_argument("--rows", type=int, default=3)
     ap.add_argument("--reps", type=int, default=3)
@@ -349,6 +423,8 @@ def main(argv=None):
     try:
         if a.workload == "opening_cache":
             res = opening_cache(b, a.log_n, a.rows, a.reps)
+        elif a.workload == "pairing_cache_check":
+            res = pairing_cache_check(b, a.log_n, a.rows, a.reps)
         else:
             res = globals()[a.workload](b)
         print(json.dumps(res), flush=True)
