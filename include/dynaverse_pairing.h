@@ -41,6 +41,26 @@ int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const uint8_t* g
                               uint8_t* results, uint8_t* gt, float* kernel_ms, char* error,
                               size_t error_capacity);
 
+/* The whole opening-cache validation of validate_initial_caches
+ * (machine.rs:357-382) and validate_cache_block_prepared (machine.rs:
+ * 276-336, Product mode) on the device: for every wire w < wires and point
+ * i < n, result[w*n + i] = 1 when
+ *     e(C_w - [v_wi] g1_0, g2_0) == e(pi_wi, g2_1 - [z_i] g2_0),
+ * with commitments C_w (wires x 64 B, the anchor's s4, s5, s6),
+ * g1_0 = g1_powers[0], g2_0 = g2_powers[0], g2_1 = g2_powers[1] (the
+ * rebuild SRS powers), points z_i (n x 32 B Fr: the domain's small points),
+ * values v_wi (wires x n x 32 B: the witness wires 3..5) and caches pi_wi
+ * (wires x n x 64 B: q4, q5, q6).  The divisor of point i is formed once and
+ * shared by every wire, as machine.rs:362-368 does.  The reference stops at
+ * the first failing (wire, i) in the order of its 4-point blocks; callers
+ * scan the results in that order for the same error.  Inputs are checked
+ * (canonical, on their curves); a bad one fails the call with -1.
+ * wires in 1..16, n up to 2^22.  No counterpart in the reference ABI. */
+int dvgpu_bn254_opening_cache_check(void* backend, const uint8_t* commitments, size_t wires, const uint8_t* g1_0,
+                                    const uint8_t* g2_0, const uint8_t* g2_1, const uint8_t* points, size_t n,
+                                    const uint8_t* values, const uint8_t* caches, uint8_t* results,
+                                    float* kernel_ms, char* error, size_t error_capacity);
+
 #ifdef __cplusplus
 }
 #endif

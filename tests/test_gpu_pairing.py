@@ -118,3 +118,44 @@ def test_large_batch(backend):
     res, _, ms = pairing_check_encoded(backend, g1 * reps, g2b * reps, 2, 64 * reps)
     print(f"65536 two-term checks: {ms:.2f} ms", flush=True)
     assert [b == 1 for b in res] == expect * reps
+
+
+def test_device_opening_cache_check(backend):
+    """validate_initial_caches on three device-computed caches of one anchor:
+    all pass; a changed cache point, a changed witness value and a swapped
+    commitment each fail at their own (wire, index), found first in the
+    reference's block order."""
+    from dvgpu.pairing import first_failure, opening_cache_check
+    rng = random.Random(35)
+    tau, n = rng.randrange(2, o.R), 16
+    srs = o.kzg_srs(tau, n)
+    w = o.fr_root_of_unity(n)
+    zs = [pow(w, i, o.R) for i in range(n)]
+    polys = [[rng.randrange(o.R) for _ in range(n)] for _ in range(3)]
+    with KzgSrs(backend, srs) as z:
+        caches = z.precompute_batch(polys, n)
+
+    def ev(c, x):
+        acc = 0
+        for a in reversed(c):
+            acc = (acc * x + a) % o.R
+        return acc
+    values = [[ev(c, zi) for zi in zs] for c in polys]
+    commits = [o.straight_msm(srs[:n], c) for c in polys]
+    g2_1 = g2(tau)
+    res = opening_cache_check(backend, commits, o.G1_GEN, o.G2_GEN, g2_1, zs, values, caches)
+    assert res == [[True] * n for _ in range(3)] and first_failure(res) is None
+    bad_caches = [list(c) for c in caches]
+    bad_caches[2][9] = o.add(bad_caches[2][9], o.G1_GEN)
+    bad_values = [list(v) for v in values]
+    bad_values[0][13] = (bad_values[0][13] + 1) % o.R
+    res = opening_cache_check(backend, commits, o.G1_GEN, o.G2_GEN, g2_1, zs, bad_values, bad_caches)
+    fails = [(wi, i) for wi in range(3) for i in range(n) if not res[wi][i]]
+    assert fails == [(0, 13), (2, 9)]
+    assert first_failure(res) == (2, 9)                 # block 8..11 precedes block 12..15
+    res = opening_cache_check(backend, [commits[1], commits[0], commits[2]], o.G1_GEN, o.G2_GEN, g2_1, zs,
+                              values, caches)
+    assert [all(r) for r in res] == [False, False, True]
+    with pytest.raises(BackendError):
+        opening_cache_check(backend, commits, o.G1_GEN, o.G2_GEN, g2_1, zs, values,
+                            [[(1, 3)] + list(c[1:]) for c in caches])

@@ -3,8 +3,11 @@
 // exponentiation, the product compared with one.
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/dynaverse_pairing.h"
+#include "g1mem.hpp"
+#include "glv.hpp"
 #include "owner.hpp"
 #include "pairing.hpp"
 
@@ -83,7 +86,113 @@ __global__ void __launch_bounds__(64) k_pairing_check(const uint32_t* __restrict
     }
 }
 
+// ---- opening-cache check (rebuilding/machine.rs:357-382) --------------------
+
+__device__ __forceinline__ bool canonical_fr(const uint32_t* p) {
+    uint32_t w[8];
+    for (int i = 0; i < 8; ++i) w[i] = p[i];
+    return is_canonical<FrParams>(w);
+}
+
+// Divisor D_i = [tau]g2 - [z_i] g2_0 as affine Montgomery words (x.c0, x.c1,
+// y.c0, y.c1; all-zero words = infinity).  flags[i] = 2 for a non-canonical z_i.
+__global__ void __launch_bounds__(64) k_cache_divisors(const uint32_t* __restrict__ z, uint64_t n, Fq2 g0x, Fq2 g0y,
+                                                       Fq2 g1x, Fq2 g1y, uint32_t* __restrict__ div,
+                                                       uint8_t* __restrict__ flags) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k[8];
+    for (int q = 0; q < 8; ++q) k[q] = z[i * 8 + q];
+    uint32_t* o = div + i * 32;
+    if (!is_canonical<FrParams>(k)) {
+        flags[i] = 2;
+        for (int q = 0; q < 32; ++q) o[q] = 0;
+        return;
+    }
+    flags[i] = 0;
+    const Xyzz<Fq2Ops> g0 = {g0x, g0y, f2_one(), f2_one()};
+    const Xyzz<Fq2Ops> zg = xyzz_mul_words<Fq2Ops, 8>(g0, k, 254);
+    const Affine<Fq2Ops> d = xyzz_to_affine<Fq2Ops>(xyzz_add<Fq2Ops>({g1x, g1y, f2_one(), f2_one()},
+                                                                       xyzz_neg<Fq2Ops>(zg)));
+    const Fq* c[4] = {&d.x.c0, &d.x.c1, &d.y.c0, &d.y.c1};
+    for (int q = 0; q < 4; ++q) {
+        uint32_t w[8];
+        pack(d.inf ? fp_zero<FqParams>() : *c[q], w);
+        for (int u = 0; u < 8; ++u) o[q * 8 + u] = w[u];
+    }
+}
+
+// Check (w, i): e(C_w - [v_wi] g1_0, g2_0) == e(pi_wi, D_i), as the product
+// e(C_w - [v_wi] g1_0, g2_0) e(-pi_wi, D_i) == 1.
+__global__ void __launch_bounds__(64) k_cache_check(const uint32_t* __restrict__ values, const uint32_t* __restrict__ caches,
+                                                    const uint32_t* __restrict__ div, const uint8_t* __restrict__ flags,
+                                                    const uint32_t* __restrict__ commitments, uint32_t wires, uint64_t n,
+                                                    Fq g1x, Fq g1y, Fq2 g0x, Fq2 g0y, uint8_t* __restrict__ ok) {
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= uint64_t(wires) * n) return;
+    const uint64_t w = t / n, i = t % n;
+    const uint32_t* v = values + t * 8;
+    const uint32_t* pi = caches + t * 16;
+    bool valid = flags[i] == 0 && canonical_fr(v);
+    const bool pi0 = all_zero(pi, 16);
+    Fq px = fp_zero<FqParams>(), py = fp_zero<FqParams>();
+    if (!pi0) {
+        valid = valid && canonical_fq(pi) && canonical_fq(pi + 8);
+        if (valid) {
+            px = ld_canon_fq(pi);
+            py = ld_canon_fq(pi + 8);
+            valid = affine_on_curve<FqOps>({px, py, false});
+        }
+    }
+    if (!valid) {
+        ok[t] = 2;
+        return;
+    }
+    uint32_t k[8];
+    for (int q = 0; q < 8; ++q) k[q] = v[q];
+    const uint32_t* cw = commitments + w * 16;  // Montgomery affine, all-zero = infinity
+    const G1 c = all_zero(cw, 16) ? xyzz_inf<FqOps>()
+                                  : G1{ldfq(cw), ldfq(cw + 8), fp_one<FqParams>(), fp_one<FqParams>()};
+    const G1 vg = g1_mul_scalar({g1x, g1y, fp_one<FqParams>(), fp_one<FqParams>()}, k, false);
+    const Affine<FqOps> d = xyzz_to_affine<FqOps>(xyzz_add<FqOps>(c, xyzz_neg<FqOps>(vg)));
+    const uint32_t* dv = div + i * 32;
+    const bool dinf = all_zero(dv, 32);
+    PairingInput in[2];
+    int m = 0;
+    if (!d.inf) in[m++] = {d.x, d.y, g0x, g0y};
+    if (!pi0 && !dinf) in[m++] = {px, neg(py), {ldfq(dv), ldfq(dv + 8)}, {ldfq(dv + 16), ldfq(dv + 24)}};
+    ok[t] = f12_is_one(final_exponentiation(m ? miller_product(in, m) : f12_one())) ? 1 : 0;
+}
+
 Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
+
+Affine<FqOps> host_g1(const uint8_t* b, const char* what) {
+    uint32_t x[8], y[8];
+    std::memcpy(x, b, 32);
+    std::memcpy(y, b + 32, 32);
+    bool any = false;
+    for (int q = 0; q < 8; ++q) any |= (x[q] | y[q]) != 0;
+    if (!any) return {fp_zero<FqParams>(), fp_zero<FqParams>(), true};
+    require(is_canonical<FqParams>(x) && is_canonical<FqParams>(y), what);
+    const Affine<FqOps> a = {to_mont<FqParams>(x), to_mont<FqParams>(y), false};
+    require(affine_on_curve<FqOps>(a), what);
+    return a;
+}
+
+Affine<Fq2Ops> host_g2(const uint8_t* b, const char* what) {
+    uint32_t w[4][8];
+    bool any = false;
+    for (int k = 0; k < 4; ++k) {
+        std::memcpy(w[k], b + 32 * k, 32);
+        for (int q = 0; q < 8; ++q) any |= w[k][q] != 0;
+    }
+    require(any, what);
+    for (int k = 0; k < 4; ++k) require(is_canonical<FqParams>(w[k]), what);
+    const Affine<Fq2Ops> a = {{to_mont<FqParams>(w[0]), to_mont<FqParams>(w[1])},
+                              {to_mont<FqParams>(w[2]), to_mont<FqParams>(w[3])}, false};
+    require(affine_on_curve<Fq2Ops>(a), what);
+    return a;
+}
 
 }  // namespace
 }  // namespace dvgpu
@@ -130,6 +239,73 @@ extern "C" int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const
             if (gt) std::memset(gt, 0, count * 384);
             throw InvalidArgument(("pairing check " + std::to_string(i) +
                                    ": a point is not canonical or not on its curve").c_str());
+        }
+        return 0;
+    });
+}
+
+extern "C" int dvgpu_bn254_opening_cache_check(void* backend, const uint8_t* commitments, size_t wires,
+                                               const uint8_t* g1_0, const uint8_t* g2_0, const uint8_t* g2_1,
+                                               const uint8_t* points, size_t n, const uint8_t* values,
+                                               const uint8_t* caches, uint8_t* results, float* kernel_ms,
+                                               char* error, size_t capacity) {
+    Owner* o = as_owner(backend);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(wires >= 1 && wires <= 16, "wires must be 1..16");
+        require(n <= (size_t(1) << 22), "too many cache points");
+        if (!n) return 0;
+        require(commitments && g1_0 && g2_0 && g2_1 && points && values && caches && results,
+                "null cache-check buffers");
+        const Affine<FqOps> g1 = host_g1(g1_0, "g1 power is not a canonical curve point");
+        require(!g1.inf, "g1 power is the identity");
+        const Affine<Fq2Ops> h0 = host_g2(g2_0, "g2 power 0 is not a canonical twist point");
+        const Affine<Fq2Ops> h1 = host_g2(g2_1, "g2 power 1 is not a canonical twist point");
+        std::vector<uint32_t> cm(wires * 16, 0u);
+        for (size_t w = 0; w < wires; ++w) {
+            const Affine<FqOps> c = host_g1(commitments + 64 * w, "commitment is not a canonical curve point");
+            if (c.inf) continue;
+            uint32_t a[8], b[8];
+            pack(c.x, a);
+            pack(c.y, b);
+            std::memcpy(cm.data() + 16 * w, a, 32);
+            std::memcpy(cm.data() + 16 * w + 8, b, 32);
+        }
+        hipStream_t st = o->stream;
+        const size_t total = wires * n;
+        const size_t bz = n * 32, bv = total * 32, bc = total * 64, bd = n * 128, bf = (n + 255) & ~size_t(255),
+                     bm = wires * 64, br = (total + 255) & ~size_t(255);
+        auto* base = static_cast<uint8_t*>(o->stage_a.ensure(bz + bv + bc + bd + bf + bm + br));
+        uint8_t *dz = base, *dv = dz + bz, *dc = dv + bv, *dd = dc + bc, *df = dd + bd, *dm = df + bf, *dr = dm + bm;
+        HIP_CHECK(hipMemcpyAsync(dz, points, bz, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(dv, values, bv, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(dc, caches, bc, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(dm, cm.data(), bm, hipMemcpyHostToDevice, st));
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        if (kernel_ms) {
+            for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+            HIP_CHECK(hipEventRecord(ev[0], st));
+        }
+        k_cache_divisors<<<grid_for(n, 64), 64, 0, st>>>(reinterpret_cast<const uint32_t*>(dz), n, h0.x, h0.y, h1.x,
+                                                          h1.y, reinterpret_cast<uint32_t*>(dd), df);
+        HIP_CHECK_LAUNCH();
+        k_cache_check<<<grid_for(total, 64), 64, 0, st>>>(
+            reinterpret_cast<const uint32_t*>(dv), reinterpret_cast<const uint32_t*>(dc),
+            reinterpret_cast<const uint32_t*>(dd), df, reinterpret_cast<const uint32_t*>(dm), uint32_t(wires), n,
+            g1.x, g1.y, h0.x, h0.y, dr);
+        HIP_CHECK_LAUNCH();
+        if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
+        HIP_CHECK(hipMemcpyAsync(results, dr, total, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        if (kernel_ms) {
+            HIP_CHECK(hipEventElapsedTime(kernel_ms, ev[0], ev[1]));
+            for (auto& e : ev) (void)hipEventDestroy(e);
+        }
+        const void* bad = std::memchr(results, 2, total);
+        if (bad) {
+            const size_t t = static_cast<const uint8_t*>(bad) - results;
+            std::memset(results, 0, total);
+            throw InvalidArgument(("cache check wire " + std::to_string(t / n) + " index " + std::to_string(t % n) +
+                                   ": a point, value or cache point is not canonical or not on its curve").c_str());
         }
         return 0;
     });

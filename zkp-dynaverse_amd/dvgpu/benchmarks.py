@@ -338,73 +338,57 @@ def opening_cache(backend: HipBackend, log_n: int = 16, rows: int = 3, reps: int
 # ---- opening-cache validation (rebuilding/machine.rs:357-382) ---------------
 
 def pairing_cache_check(backend: HipBackend, log_n: int = 16, caches: int = 3, reps: int = 3, log=print) -> Dict:
-    """validate_initial_caches at n = 2^log_n: every point of the three caches
-    checked by e(C - [v_i]g1, g2) == e(pi_i, [tau]g2 - [z_i]g2), i.e.
-    caches x n two-term pairing products.  The points come from a 2^10-point
-    cache computed on the device and verified, tiled to the full count (the
-    per-check work does not depend on the values), with every 97th check
-    corrupted so both outcomes occur."""
-    from .codec import FQ_MODULUS, FR_MODULUS, encode_g1, encode_g2
+    """validate_initial_caches (machine.rs:357-382) at n = 2^log_n, end to end
+    on the device: three caches of one anchor computed by the opening-cache
+    kernels, their commitments [f(tau)] g1, the witness values f(w^i) by an Fr
+    NTT, then every (wire, i) checked by e(C - [v]g1, g2) == e(pi, [tau - z]g2)
+    with the divisors formed on the device once per point.  A second call
+    plants one wrong cache point per 97 checks, all of which must be found."""
+    from .codec import FR_MODULUS, encode_fr, root_of_unity_2_20
     from .kzg import KzgSrs
-    from .pairing import pairing_check_encoded
-    from .workload import srs_scalars
+    from .pairing import opening_cache_check_encoded
+    from .workload import poly_eval, srs_scalars
     r = FR_MODULUS
-    m = 1 << 10
-    tau, powers = srs_scalars(2026, m)
-    srs = backend.fixed_base_encoded(powers.tobytes(), m, False)
-    coeffs = ChaCha20Stream(2027).uniform_fr(m)
+    n = 1 << log_n
+    tau, powers = srs_scalars(2026, n)
+    srs = b"".join(backend.fixed_base_encoded(powers[i:i + 4096].tobytes(), min(4096, n - i), False)
+                   for i in range(0, n, 4096))
+    coeffs = ChaCha20Stream(2027).uniform_fr_array(caches * n)
     with KzgSrs(backend, encoded=srs) as z:
-        raw = z.precompute_bytes(b"".join(c.to_bytes(32, "little") for c in coeffs), m, 1, m)
-    w = 1
-    from .codec import root_of_unity_2_20, decode_g1
-    w_m = pow(root_of_unity_2_20(), 1 << (20 - 10), r)
-
-    def ev(x):
-        acc = 0
-        for c in reversed(coeffs):
-            acc = (acc * x + c) % r
-        return acc
-    f_tau = ev(tau)
-    # C - [f(z)]g1 = [f(tau) - f(z)]g1 and [tau - z]g2 through the fixed-base kernels
-    diffs, divs = [], []
-    for i in range(m):
-        zi = w
-        diffs.append(((f_tau - ev(zi)) % r).to_bytes(32, "little"))
-        divs.append(((tau - zi) % r).to_bytes(32, "little"))
-        w = w * w_m % r
-    d1 = b"".join(backend.fixed_base_encoded(b"".join(diffs[i:i + 4096]), len(diffs[i:i + 4096]), False)
-                  for i in range(0, m, 4096))
-    d2, _ = backend.fixed_base_g2_checked_encoded(b"".join(divs), m, 3)
-    g2gen = backend.fixed_base_g2_checked_encoded((1).to_bytes(32, "little"), 1, 3)[0]
-    g1, g2 = bytearray(), bytearray()
-    for i in range(m):
-        op = decode_g1(raw[64 * i:64 * i + 64])
-        neg = encode_g1(None if op is None else (op[0], (-op[1]) % FQ_MODULUS))
-        g1 += d1[64 * i:64 * i + 64] + neg
-        g2 += g2gen + d2[128 * i:128 * i + 128]
-    total = caches << log_n
-    reps_tile = total // m
-    g1 = bytearray(bytes(g1) * reps_tile)
-    g2 = bytes(g2) * reps_tile
-    bad = list(range(5, total, 97))
-    for i in bad:                      # corrupt the opening: pi -> pi + g1
-        g1[128 * i + 64:128 * i + 128] = encode_g1((1, 2))
-    g1 = bytes(g1)
-    res, _, ms0 = pairing_check_encoded(backend, g1, g2, 2, total)
-    expect_bad = set(bad)
-    ok = all((res[i] == 0) == (i in expect_bad) for i in range(total))
+        cache_bytes = z.precompute_bytes(coeffs.tobytes(), n, caches, n)
+    values = b"".join(backend.ntt_encoded(coeffs[w * n:(w + 1) * n].tobytes(), n, False) for w in range(caches))
+    commits = b"".join(backend.fixed_base_encoded(encode_fr(poly_eval(coeffs[w * n:(w + 1) * n], tau)), 1, False)
+                       for w in range(caches))
+    w_n = pow(root_of_unity_2_20(), 1 << (20 - log_n), r)
+    pts, x = bytearray(), 1
+    for _ in range(n):
+        pts += x.to_bytes(32, "little")
+        x = x * w_n % r
+    g1_0 = srs[:64]
+    g2_0, _ = backend.fixed_base_g2_checked_encoded(encode_fr(1), 1, 3)
+    g2_1, _ = backend.fixed_base_g2_checked_encoded(encode_fr(tau), 1, 3)
+    args = (commits, caches, g1_0, g2_0, g2_1, bytes(pts), n, values)
     times, walls = [], []
     for _ in range(reps):
         a = time.perf_counter()
-        again, _, ms = pairing_check_encoded(backend, g1, g2, 2, total)
+        res, ms = opening_cache_check_encoded(backend, *args, cache_bytes)
         walls.append((time.perf_counter() - a) * 1e3)
         times.append(ms)
+    all_pass = res == b"\x01" * (caches * n)
+    bad = bytearray(cache_bytes)
+    planted = list(range(5, caches * n, 97))
+    for t in planted:                      # pi -> identity (g1 where pi is the identity)
+        bad[64 * t:64 * t + 64] = (1).to_bytes(32, "little") + (2).to_bytes(32, "little") \
+            if cache_bytes[64 * t:64 * t + 64] == bytes(64) else bytes(64)
+    res_bad, _ = opening_cache_check_encoded(backend, *args, bytes(bad))
+    found = [t for t in range(caches * n) if res_bad[t] != 1]
     kms = sorted(times)[len(times) // 2]
-    out = {"checks": total, "terms": 2, "kernel_ms": round(kms, 3),
-           "wall_ms": round(sorted(walls)[len(walls) // 2], 3),
-           "checks_per_s": round(total / (kms * 1e-3)), "accepts_and_rejects_correct": bool(ok and again == res),
-           "rejected": len(bad)}
-    log(f"pairing cache check {total} x 2 terms: kernel {kms:.1f} ms ({out['checks_per_s']:.0f} checks/s), ok={ok}")
+    out = {"log_n": log_n, "wires": caches, "checks": caches * n, "kernel_ms": round(kms, 3),
+           "wall_ms": round(sorted(walls)[len(walls) // 2], 3), "checks_per_s": round(caches * n / (kms * 1e-3)),
+           "all_valid_accepted": bool(all_pass), "planted_rejections": len(planted),
+           "planted_found_exactly": found == planted}
+    log(f"opening-cache check n=2^{log_n} x{caches}: kernel {kms:.1f} ms, wall {out['wall_ms']:.1f} ms "
+        f"({out['checks_per_s']:.0f} checks/s), valid={all_pass}, planted found={found == planted}")
     return out
 
 

@@ -62,3 +62,45 @@ def _neg_g1(p):
 def cache_check_groups(commitment_minus_values: Sequence, g2_gen, openings: Sequence, divisors: Sequence):
     """The two-term groups of machine.rs:333: (C - [v_i]g1, g2), (-pi_i, [tau - z_i]g2)."""
     return [[(d, g2_gen), (_neg_g1(pi), q)] for d, pi, q in zip(commitment_minus_values, openings, divisors)]
+
+
+def opening_cache_check_encoded(backend: HipBackend, commitments: bytes, wires: int, g1_0: bytes, g2_0: bytes,
+                                g2_1: bytes, points: bytes, n: int, values: bytes, caches: bytes):
+    """Encoded form: returns (results bytes, wires x n; kernel ms)."""
+    res = ctypes.create_string_buffer(max(1, wires * n))
+    ms = ctypes.c_float(0)
+    err = _err()
+    bufs = [ctypes.c_char_p(a or b"\0") for a in (commitments, g1_0, g2_0, g2_1, points, values, caches)]
+    _check(backend.lib.dvgpu_bn254_opening_cache_check(backend.handle, bufs[0], wires, bufs[1], bufs[2], bufs[3],
+                                                       bufs[4], n, bufs[5], bufs[6], res, ctypes.byref(ms), err, 256),
+           err)
+    return res.raw[:wires * n], ms.value
+
+
+def opening_cache_check(backend: HipBackend, commitments: Sequence, g1_0, g2_0, g2_1, points: Sequence[int],
+                        values: Sequence[Sequence[int]], caches: Sequence[Sequence], with_time: bool = False):
+    """validate_initial_caches (machine.rs:357-382) on the device: result[w][i]
+    is whether e(C_w - [v_wi] g1_0, g2_0) == e(pi_wi, g2_1 - [z_i] g2_0)."""
+    from .codec import encode_fr
+    wires, n = len(commitments), len(points)
+    if len(values) != wires or len(caches) != wires or any(len(v) != n for v in values) or \
+            any(len(c) != n for c in caches):
+        raise ValueError("values and caches need one row of n entries per commitment")
+    raw, ms = opening_cache_check_encoded(
+        backend, b"".join(encode_g1(c) for c in commitments), wires, encode_g1(g1_0), encode_g2(g2_0),
+        encode_g2(g2_1), b"".join(encode_fr(z) for z in points), n,
+        b"".join(encode_fr(v) for row in values for v in row), b"".join(encode_g1(p) for row in caches for p in row))
+    out = [[raw[w * n + i] == 1 for i in range(n)] for w in range(wires)]
+    return (out, ms) if with_time else out
+
+
+def first_failure(results: Sequence[Sequence[bool]]):
+    """(wire index, point index) of the first failing check in the order of
+    machine.rs:361-381 (blocks of four points, wires inside a block), or None."""
+    wires, n = len(results), len(results[0]) if results else 0
+    for start in range(0, n, 4):
+        for w in range(wires):
+            for i in range(start, min(start + 4, n)):
+                if not results[w][i]:
+                    return w, i
+    return None
