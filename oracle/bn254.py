@@ -755,3 +755,78 @@ def pairing_product(pairs):
     for p, q in pairs:
         f = f12_mul(f, miller_loop(p, q))
     return final_exponentiation(f)
+
+
+# --------------------------------------------------------------------------
+# Polynomial tools (crates/protocol/src/domain/fast.rs, domain.rs:303-337):
+# the quadratic textbook forms the reference's own tests use as oracles
+# (fast_tests.rs:5-22): products, long division, Horner, Lagrange.
+# --------------------------------------------------------------------------
+
+
+def poly_trim(p):
+    p = [c % R for c in p] or [0]
+    while len(p) > 1 and p[-1] == 0:
+        p.pop()
+    return p
+
+
+def poly_mul(a, b):
+    """multiply_polynomials (domain.rs:326-337): [0] for an empty operand."""
+    if not a or not b:
+        return [0]
+    out = [0] * (len(a) + len(b) - 1)
+    for i, x in enumerate(a):
+        for j, y in enumerate(b):
+            out[i + j] = (out[i + j] + x * y) % R
+    return poly_trim(out)
+
+
+def poly_divide_exact(num, den):
+    """Long division; ValueError for a zero divisor or a nonzero remainder."""
+    num, den = poly_trim(num), poly_trim(den)
+    if den == [0]:
+        raise ValueError("cannot divide by zero polynomial")
+    if len(num) < len(den):
+        if any(num):
+            raise ValueError("nonzero remainder")
+        return [0]
+    rem = list(num)
+    q = [0] * (len(num) - len(den) + 1)
+    li = pow(den[-1], R - 2, R)
+    for k in range(len(q) - 1, -1, -1):
+        c = rem[k + len(den) - 1] * li % R
+        q[k] = c
+        for j, d in enumerate(den):
+            rem[k + j] = (rem[k + j] - c * d) % R
+    if any(rem):
+        raise ValueError("nonzero remainder")
+    return poly_trim(q)
+
+
+def poly_eval(p, x):
+    acc = 0
+    for c in reversed(p):
+        acc = (acc * x + c) % R
+    return acc
+
+
+def lagrange_interpolate(points, values):
+    """naive_interpolate (fast_tests.rs:5-22)."""
+    res = [0] * max(len(points), 1)
+    for i, xi in enumerate(points):
+        num, den = [1], 1
+        for j, xj in enumerate(points):
+            if i != j:
+                num = poly_mul(num, [(-xj) % R, 1])
+                den = den * (xi - xj) % R
+        w = values[i] * pow(den, R - 2, R) % R
+        for k, c in enumerate(num):
+            res[k] = (res[k] + w * c) % R
+    return poly_trim(res)
+
+
+def cauchy_sums_direct(points, weights):
+    """sum_{j != i} w_j / (x_i - x_j) (fast_tests.rs:45-54)."""
+    return [sum(weights[j] * pow((xi - xj) % R, R - 2, R) for j, xj in enumerate(points) if j != i) % R
+            for i, xi in enumerate(points)]

@@ -1,0 +1,111 @@
+"""Polynomial tools with batched device products (dynaverse_poly.h) against
+the quadratic oracles of the reference's own tests
+(crates/protocol/src/domain/fast_tests.rs) on the same shapes."""
+import random
+
+import pytest
+
+from dvgpu import BackendError, HipBackend
+from dvgpu.poly import SubproductTree, fft_multiply, newton_divide_exact
+from oracle import bn254 as o
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def backend():
+    b = HipBackend()
+    yield b
+    b.close()
+
+
+@pytest.mark.parametrize("size", [0, 1, 2, 3, 16, 17, 33, 65])
+def test_tree_product_evaluation_and_interpolation(backend, size):
+    """fast_tests.rs:24-42: points 3i + 2, degrees 0, size, 2 size + 3."""
+    rng = random.Random(15201 + size)
+    points = [3 * i + 2 for i in range(size)]
+    with SubproductTree(backend, points) as tree:
+        van = [1]
+        for x in points:
+            van = o.poly_mul(van, [(-x) % o.R, 1])
+        assert tree.vanishing() == van
+        for degree in (0, size, 2 * size + 3):
+            poly = [rng.randrange(o.R) for _ in range(degree + 1)]
+            values = [o.poly_eval(poly, x) for x in points]
+            assert tree.multipoint_eval(poly) == values
+            assert tree.interpolate(values) == o.lagrange_interpolate(points, values)
+        assert tree.interpolate([0] * size) == [0]
+        with pytest.raises(BackendError):
+            tree.interpolate([0] * (size + 1))
+        der = o.poly_trim([i * c for i, c in enumerate(van)][1:])
+        assert tree.inverse_derivatives() == [pow(o.poly_eval(der, x), o.R - 2, o.R) for x in points]
+
+
+def test_duplicate_points_rejected(backend):
+    with pytest.raises(BackendError) as e:
+        SubproductTree(backend, [1, 1])
+    assert e.value.status == -1
+
+
+@pytest.mark.parametrize("size", [0, 1, 2, 3, 17, 32, 65])
+def test_cauchy_sums(backend, size):
+    """fast_tests.rs:44-62: points 2(size - i) + 1."""
+    rng = random.Random(15202 + size)
+    points = [2 * (size - i) + 1 for i in range(size)]
+    weights = [rng.randrange(o.R) for _ in range(size)]
+    with SubproductTree(backend, points) as tree:
+        assert tree.cauchy_sums(weights) == o.cauchy_sums_direct(points, weights)
+        assert tree.cauchy_sums([0] * size) == [0] * size
+        with pytest.raises(BackendError):
+            tree.cauchy_sums([0] * (size + 1))
+
+
+@pytest.mark.parametrize("degree", [0, 1, 3, 17, 33, 65])
+def test_newton_division(backend, degree):
+    """fast_tests.rs:64-86: dense quotients, every corrupted low coefficient rejected."""
+    rng = random.Random(15203 + degree)
+    den = [rng.randrange(o.R) for _ in range(degree)] + [3]
+    for length in (1, 3, 17, 67):
+        q = [rng.randrange(o.R) for _ in range(length)]
+        num = o.poly_mul(den, q)
+        assert newton_divide_exact(backend, num, den) == o.poly_trim(q) == o.poly_divide_exact(num, den)
+        for index in range(min(degree, 4)):
+            bad = list(num)
+            bad[index] = (bad[index] + 1) % o.R
+            with pytest.raises(BackendError):
+                newton_divide_exact(backend, bad, den)
+    assert newton_divide_exact(backend, [], [1]) == [0]
+    with pytest.raises(BackendError):
+        newton_divide_exact(backend, [], [])
+    with pytest.raises(BackendError):
+        newton_divide_exact(backend, [1], [0, 1])
+
+
+def test_fft_multiply(backend):
+    """fast_tests.rs:88-96 and random shapes."""
+    assert fft_multiply(backend, [1] * 17, [2] * 17) == o.poly_mul([1] * 17, [2] * 17)
+    rng = random.Random(44)
+    for la, lb in ((1, 1), (2, 3), (100, 37), (1000, 1)):
+        a = [rng.randrange(o.R) for _ in range(la)]
+        b = [rng.randrange(o.R) for _ in range(lb)]
+        assert fft_multiply(backend, a, b) == o.poly_mul(a, b)
+    assert fft_multiply(backend, [], [5]) == [0]
+
+
+def test_large_tree_batches_levels(backend):
+    """4096 points: values against Horner at sampled points, interpolation
+    round trip, one device round per tree level for the products."""
+    rng = random.Random(45)
+    n = 4096
+    points = rng.sample(range(1, 1 << 40), n)
+    with SubproductTree(backend, points) as tree:
+        build = tree.stats()
+        assert build["products"] >= n - 1
+        poly = [rng.randrange(o.R) for _ in range(n)]
+        vals = tree.multipoint_eval(poly)
+        for i in (0, 1, 777, n - 1):
+            assert vals[i] == o.poly_eval(poly, points[i])
+        assert tree.interpolate(vals) == o.poly_trim(poly)
+        inter = tree.stats()
+        assert inter["rounds"] == 12                       # one per level
+    print("tree build", build, "interpolate", inter, flush=True)

@@ -48,6 +48,8 @@ void Owner::release_device() noexcept {
     batches.clear();
     for (KzgSrs* z : kzg) kzg_destroy(z);
     kzg.clear();
+    for (PolyTree* t : poly_trees) poly_tree_destroy(t);
+    poly_trees.clear();
     for (auto& kv : ext_allocations) {
         (void)hipFree(kv.first);
         quota.release(kv.second);

@@ -24,6 +24,7 @@ struct ResidentState;  // resident.hip
 struct BatchObj;       // batch.hip
 struct SetupState;     // setup.hip
 struct KzgSrs;         // kzg.hip
+struct PolyTree;       // poly.hip
 
 struct Owner {
     hipStream_t stream = nullptr;
@@ -56,6 +57,8 @@ struct Owner {
     std::vector<BatchObj*> batches;
     // KZG SRS objects created through dynaverse_kzg.h (freed with the owner).
     std::vector<KzgSrs*> kzg;
+    // Subproduct trees created through dynaverse_poly.h (freed with the owner).
+    std::vector<PolyTree*> poly_trees;
     // Measured window width per log2 MSM size (dvgpu_ext_set_tuning); the
     // NTT stages per pass live in `domain`, the task shape in msm_config.
     WindowTable window_table;
@@ -103,5 +106,7 @@ void batch_destroy(BatchObj* b) noexcept;
 void setup_destroy(SetupState* state) noexcept;
 // KZG hooks (kzg.hip).
 void kzg_destroy(KzgSrs* srs) noexcept;
+// Polynomial hooks (poly.hip).
+void poly_tree_destroy(PolyTree* tree) noexcept;
 
 }  // namespace dvgpu

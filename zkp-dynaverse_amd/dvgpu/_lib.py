@@ -153,6 +153,10 @@ class DvgpuKzgTimings(ctypes.Structure):
                 ("domain_transform_ms", ctypes.c_float)]
 
 
+class DvgpuPolyStats(ctypes.Structure):
+    _fields_ = [("rounds", ctypes.c_uint64), ("products", ctypes.c_uint64)]
+
+
 _I = ctypes.c_int
 _E = (ctypes.c_char_p, c_size_t)  # error, capacity
 
@@ -236,6 +240,23 @@ SIGNATURES = {
     # dynaverse_pairing.h
     "dvgpu_bn254_pairing_check": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_void_p, c_void_p,
                                        ctypes.POINTER(ctypes.c_float), *_E]),
+    # dynaverse_poly.h
+    "dvgpu_poly_mul": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t,
+                            ctypes.POINTER(c_size_t), *_E]),
+    "dvgpu_poly_divide_exact": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p,
+                                     c_size_t, ctypes.POINTER(c_size_t), *_E]),
+    "dvgpu_poly_tree_create": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, ctypes.POINTER(c_void_p),
+                                    ctypes.POINTER(DvgpuPolyStats), *_E]),
+    "dvgpu_poly_tree_destroy": (_I, [c_void_p, c_void_p]),
+    "dvgpu_poly_tree_vanishing": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, ctypes.POINTER(c_size_t), *_E]),
+    "dvgpu_poly_tree_inverse_derivatives": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, ctypes.POINTER(c_size_t),
+                                                 *_E]),
+    "dvgpu_poly_tree_eval": (_I, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p,
+                                  ctypes.POINTER(DvgpuPolyStats), *_E]),
+    "dvgpu_poly_tree_interpolate": (_I, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t,
+                                         ctypes.POINTER(c_size_t), ctypes.POINTER(DvgpuPolyStats), *_E]),
+    "dvgpu_poly_tree_cauchy_sums": (_I, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p,
+                                         ctypes.POINTER(DvgpuPolyStats), *_E]),
     "dvgpu_bn254_opening_cache_check": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p,
                                              c_size_t, c_void_p, c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float),
                                              *_E]),
