@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the batched, incremental and served workloads")
     ap.add_argument("--no-kzg", action="store_true",
-                    help="skip the opening-cache precompute and pairing-check workloads")
+                    help="skip the opening-cache, pairing-check and polynomial-tool workloads")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the plain copy kernel (traffic-counter calibration)")
     args = ap.parse_args()
@@ -154,6 +154,8 @@ def main():
         secondary["opening_cache_2p13_x1"] = benchmarks.opening_cache(backend, log_n=13, rows=1, log=log)
         log("# opening-cache validation: 3 x 2^16 pairing-product checks")
         secondary["pairing_cache_check_2p16"] = benchmarks.pairing_cache_check(backend, log=log)
+        log("# polynomial tools: subproduct tree over 2^12 points")
+        secondary["poly_tree_2p12"] = benchmarks.poly_tree(backend, log=log)
 
     if args.calibrate_copy:
         nbytes = 256 << 20

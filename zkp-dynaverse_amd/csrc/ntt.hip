@@ -329,20 +329,39 @@ void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& ent
     require(d.ready, "NTT domain not prepared");
     require(!entries.empty(), "empty NTT batch");
     // Outputs may not overlap any other entry's input or output (the entries
-    // run together); an entry may transform in place.
+    // run together); an entry may transform in place.  Checked in
+    // O(E log E): the output ranges sorted by start must be disjoint, and
+    // every input range may only meet its own entry's output.
+    struct Range {
+        uintptr_t lo, hi;
+        size_t entry;
+    };
+    std::vector<Range> outs;
+    outs.reserve(entries.size());
     for (size_t i = 0; i < entries.size(); ++i) {
         const NttBatchEntry& x = entries[i];
         require(x.log_n >= 0 && x.log_n <= NTT_MAX_LOG && x.rows >= 1 && x.in && x.out, "invalid NTT batch entry");
-        const uintptr_t xo = reinterpret_cast<uintptr_t>(x.out), xb = (x.rows << x.log_n) * 32;
-        for (size_t j = 0; j < entries.size(); ++j) {
-            if (i == j) continue;
-            const NttBatchEntry& y = entries[j];
-            const uintptr_t yb = (y.rows << y.log_n) * 32;
-            for (uintptr_t yp : {reinterpret_cast<uintptr_t>(y.in), reinterpret_cast<uintptr_t>(y.out)})
-                require(xo + xb <= yp || yp + yb <= xo, "NTT batch entries overlap");
+        const uintptr_t o = reinterpret_cast<uintptr_t>(x.out);
+        outs.push_back({o, o + (x.rows << x.log_n) * 32, i});
+    }
+    std::sort(outs.begin(), outs.end(), [](const Range& a, const Range& b) { return a.lo < b.lo; });
+    for (size_t k = 1; k < outs.size(); ++k) require(outs[k - 1].hi <= outs[k].lo, "NTT batch entries overlap");
+    for (size_t j = 0; j < entries.size(); ++j) {
+        const uintptr_t lo = reinterpret_cast<uintptr_t>(entries[j].in),
+                        hi = lo + (entries[j].rows << entries[j].log_n) * 32;
+        // first output range that ends after lo (ends are sorted as the ranges are disjoint)
+        size_t a = 0, b = outs.size();
+        while (a < b) {
+            const size_t m = (a + b) / 2;
+            if (outs[m].hi <= lo) a = m + 1;
+            else b = m;
         }
+        for (size_t k = a; k < outs.size() && outs[k].lo < hi; ++k)
+            require(outs[k].entry == j, "NTT batch entries overlap");
     }
     std::vector<NttBatchDesc> descs(entries.size());
+    Fr scales[2][NTT_MAX_LOG + 1];
+    bool have_scale[2][NTT_MAX_LOG + 1] = {};
     uint64_t work = 0, elems = 0;
     uint32_t blocks[NTT_BATCH_MAX_PASSES] = {0}, tile_max[NTT_BATCH_MAX_PASSES] = {0};
     int max_passes = 0;
@@ -383,8 +402,12 @@ void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& ent
         elems += uint64_t(x.rows) << L;
         require(elems < (uint64_t(1) << 31), "NTT batch too large");
         if (passes) work += uint64_t(x.rows) << L;
-        Fr scale = passes ? output_scale(L, x.inverse, true) : fp_one<FrParams>();  // length 1: identity
-        pack(scale, e.scale);
+        // length 1: identity; otherwise 1 or n^-1, one inversion per distinct length
+        if (passes && !have_scale[x.inverse][L]) {
+            scales[x.inverse][L] = output_scale(L, x.inverse, true);
+            have_scale[x.inverse][L] = true;
+        }
+        pack(passes ? scales[x.inverse][L] : fp_one<FrParams>(), e.scale);
     }
     uint32_t* wp = plan.work.as<uint32_t>(std::max<uint64_t>(work, 1) * 8);
     for (auto& e : descs) e.work = wp + reinterpret_cast<uintptr_t>(e.work) * 8;

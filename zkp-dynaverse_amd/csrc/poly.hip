@@ -298,7 +298,8 @@ struct PolyTree {
     std::vector<W8> points;
     std::vector<std::vector<Poly>> levels;
     std::vector<std::vector<Poly>> recip;
-    std::vector<W8> inverse_derivatives;
+    std::vector<W8> derivatives;          // A'(x_i)
+    std::vector<W8> inverse_derivatives;  // 1 / A'(x_i)
 };
 
 void poly_tree_destroy(PolyTree* t) noexcept { delete t; }
@@ -419,12 +420,22 @@ PolyTree* tree_build(PolyEngine& e, Owner* o, const std::vector<W8>& pts) {
     }
     if (!pts.empty()) {
         const Poly d = differentiate(t->levels.back()[0]);
-        std::vector<W8> der = tree_eval(e, *t, d);
-        for (W8& v : der) {
-            require(!is_zero8(v), "repeated interpolation root");
-            v = from_fr(bn254::inv(to_fr(v)));
+        t->derivatives = tree_eval(e, *t, d);
+        // batch inversion (prefix products, one inversion, back substitution)
+        const size_t k = t->derivatives.size();
+        std::vector<Fr> pre(k);
+        Fr acc = fp_one<FrParams>();
+        for (size_t i = 0; i < k; ++i) {
+            require(!is_zero8(t->derivatives[i]), "repeated interpolation root");
+            pre[i] = acc;
+            acc = mul(acc, to_fr(t->derivatives[i]));
         }
-        t->inverse_derivatives = std::move(der);
+        Fr inv_acc = bn254::inv(acc);
+        t->inverse_derivatives.assign(k, zero8());
+        for (size_t i = k; i-- > 0;) {
+            t->inverse_derivatives[i] = from_fr(mul(inv_acc, pre[i]));
+            inv_acc = mul(inv_acc, to_fr(t->derivatives[i]));
+        }
     }
     return t.release();
 }
@@ -571,8 +582,7 @@ int dvgpu_poly_tree_cauchy_sums(void* backend, const uint8_t* root, void* tree, 
         if (!count) return;
         const Poly w = read_poly(weights, count);
         std::vector<W8> nv;
-        for (size_t i = 0; i < count; ++i)
-            nv.push_back(from_fr(mul(to_fr(w[i]), bn254::inv(to_fr(t->inverse_derivatives[i])))));
+        for (size_t i = 0; i < count; ++i) nv.push_back(from_fr(mul(to_fr(w[i]), to_fr(t->derivatives[i]))));
         const Poly num = tree_interpolate(e, *t, nv);
         const std::vector<W8> first = tree_eval(e, *t, differentiate(num));
         const std::vector<W8> second = tree_eval(e, *t, differentiate(differentiate(t->levels.back()[0])));

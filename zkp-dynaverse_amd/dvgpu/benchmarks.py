@@ -392,13 +392,82 @@ def pairing_cache_check(backend: HipBackend, log_n: int = 16, caches: int = 3, r
     return out
 
 
+# ---- polynomial tools (domain/fast.rs) ---------------------------------------
+
+def poly_tree(backend: HipBackend, log_k: int = 12, reps: int = 3, log=print) -> Dict:
+    """SubproductTree over 2^log_k random points: build, multipoint evaluation
+    and interpolation of a degree 2^log_k - 1 polynomial, Cauchy sums; values
+    checked against Horner at sampled points and the interpolation round trip.
+    Beside it, the per-product path the reference takes (one device product
+    call per tree product, fast.rs:204-234) for the tree build at 2^10."""
+    from .codec import FR_MODULUS
+    from .poly import SubproductTree, fft_multiply
+    r = FR_MODULUS
+    k = 1 << log_k
+    rng = ChaCha20Stream(2030)
+    points = list(dict.fromkeys(rng.uniform_fr(k + 8)))[:k]
+    poly = rng.uniform_fr(k)
+    weights = rng.uniform_fr(k)
+
+    def med(fn):
+        ts = []
+        out = None
+        for _ in range(reps):
+            a = time.perf_counter()
+            out = fn()
+            ts.append((time.perf_counter() - a) * 1e3)
+        return sorted(ts)[len(ts) // 2], out
+
+    t_build, tree = med(lambda: SubproductTree(backend, points))
+    build_stats = tree.stats()
+    t_eval, vals = med(lambda: tree.multipoint_eval(poly))
+    eval_stats = tree.stats()
+    t_interp, back = med(lambda: tree.interpolate(vals))
+    t_cauchy, _ = med(lambda: tree.cauchy_sums(weights))
+
+    def horner(c, x):
+        acc = 0
+        for v in reversed(c):
+            acc = (acc * x + v) % r
+        return acc
+    ok = all(vals[i] == horner(poly, points[i]) for i in (0, 1, k // 2, k - 1)) and back == poly
+    tree.close()
+    # the per-product path: one fft_multiply call per product, level by level
+    small = points[:1024]
+    a = time.perf_counter()
+    level = [[(-x) % r, 1] for x in small]
+    calls = 0
+    while len(level) > 1:
+        nxt = [fft_multiply(backend, level[i], level[i + 1]) for i in range(0, len(level) - 1, 2)]
+        calls += len(nxt)
+        if len(level) % 2:
+            nxt.append(level[-1])
+        level = nxt
+    t_plain = (time.perf_counter() - a) * 1e3
+    a = time.perf_counter()
+    with SubproductTree(backend, small) as t2:
+        batched_products = t2.stats()
+    t_batched = (time.perf_counter() - a) * 1e3
+    out = {"points": k, "build_ms": round(t_build, 2), "build_rounds": build_stats["rounds"],
+           "build_products": build_stats["products"], "eval_ms": round(t_eval, 2),
+           "eval_rounds": eval_stats["rounds"], "interpolate_ms": round(t_interp, 2),
+           "cauchy_sums_ms": round(t_cauchy, 2), "values_and_round_trip_ok": bool(ok),
+           "per_product_path_1024": {"products_only_ms": round(t_plain, 2), "device_calls": calls,
+                                     "batched_full_build_ms": round(t_batched, 2),
+                                     "batched_rounds": batched_products["rounds"]}}
+    log(f"poly tree 2^{log_k}: build {t_build:.1f} ms ({build_stats['rounds']} rounds), eval {t_eval:.1f} ms, "
+        f"interpolate {t_interp:.1f} ms, cauchy {t_cauchy:.1f} ms, ok={ok}; 1024-point products one call each "
+        f"{t_plain:.1f} ms vs batched build {t_batched:.1f} ms")
+    return out
+
+
 def main(argv=None):
     """One workload by name, for profiling: python -m dvgpu.benchmarks opening_cache --log-n 16"""
     import argparse
     import json
     ap = argparse.ArgumentParser(description=main.__doc__)
-    ap.add_argument("workload", choices=["opening_cache", "pairing_cache_check", "batched_launch", "incremental",
-                                         "served_stream"])
+    ap.add_argument("workload", choices=["opening_cache", "pairing_cache_check", "poly_tree", "batched_launch",
+                                         "incremental", "served_stream"])
     ap.add_argument("--log-n", type=int, default=16)
     ap.add_argument("--rows", type=int, default=3)
     ap.add_argument("--reps", type=int, default=3)
@@ -409,6 +478,8 @@ def main(argv=None):
             res = opening_cache(b, a.log_n, a.rows, a.reps)
         elif a.workload == "pairing_cache_check":
             res = pairing_cache_check(b, a.log_n, a.rows, a.reps)
+        elif a.workload == "poly_tree":
+            res = poly_tree(b, a.log_n, a.reps)
         else:
             res = globals()[a.workload](b)
         print(json.dumps(res), flush=True)
