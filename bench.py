@@ -156,6 +156,9 @@ def main():
         secondary["pairing_cache_check_2p16"] = benchmarks.pairing_cache_check(backend, log=log)
         log("# polynomial tools: subproduct tree over 2^12 points")
         secondary["poly_tree_2p12"] = benchmarks.poly_tree(backend, log=log)
+        log("# per-row figures: G2 MSM, G2 validation, resident transform / MSM / readback")
+        with HipBackend() as side:      # its own owner: the resident engine is configured once per owner
+            secondary["row_figures"] = benchmarks.row_figures(side, log=log)
 
     if args.calibrate_copy:
         nbytes = 256 << 20

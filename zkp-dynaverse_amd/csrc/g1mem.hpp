@@ -4,6 +4,7 @@
 #pragma once
 #include "common.hpp"
 #include "curve.hpp"
+#include "glv.hpp"
 
 namespace dvgpu {
 
@@ -42,10 +43,11 @@ __device__ __forceinline__ void stg1(uint32_t* p, const G1& v) {
 // canonical scalar words of a Montgomery Fr value
 __device__ __forceinline__ void fr_bits(const Fr& v, uint32_t (&w)[8]) { from_mont(v, w); }
 
+// [s]P for a per-lane scalar (GLV split, joint double-and-add).
 __device__ inline G1 g1_scale(const G1& p, const Fr& s) {
     uint32_t k[8];
     fr_bits(s, k);
-    return xyzz_mul_words<FqOps, 8>(p, k, 254);
+    return g1_mul_scalar(p, k, false);
 }
 
 // Fr root^e for a 64-bit exponent (Montgomery in, Montgomery out).

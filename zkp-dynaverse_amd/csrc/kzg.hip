@@ -28,6 +28,7 @@
 
 #include "../../include/dynaverse_kzg.h"
 #include "g1mem.hpp"
+#include "gfft.hpp"
 #include "glv.hpp"
 #include "owner.hpp"
 

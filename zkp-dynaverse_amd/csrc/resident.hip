@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "g1mem.hpp"
+#include "gfft.hpp"
 #include "owner.hpp"
 #include "setup_impl.hpp"
 
@@ -162,32 +163,23 @@ __global__ void k_scale(const uint32_t* __restrict__ pts, const uint32_t* __rest
     stg1(out + i * 32, g1_scale(ldg1(pts + i * 32), ldfr(sc + i * 8)));
 }
 
-// One DIF stage of rows x 2^L point transforms: (P, Q) -> (P + Q, (P - Q) w^j).
-__global__ void k_ecntt_stage(uint32_t* __restrict__ data, int L, int s, size_t rows,
-                              const uint32_t* __restrict__ tw) {
-    const size_t half = size_t(1) << (L - 1);
-    const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (b >= half * rows) return;
-    const size_t row = b / half, q = b % half, h = size_t(1) << s;
-    const size_t i = row * (size_t(1) << L) + ((q >> s) << (s + 1)) + (q & (h - 1));
-    const size_t j = q & (h - 1);
-    const G1 P = ldg1(data + i * 32), Q = ldg1(data + (i + h) * 32);
-    stg1(data + i * 32, xyzz_add<FqOps>(P, Q));
-    G1 d = xyzz_add<FqOps>(P, xyzz_neg<FqOps>(Q));
-    if (j) d = g1_scale(d, ldfr(tw + (j << (NTT_MAX_LOG - 1 - s)) * 8));
-    stg1(data + (i + h) * 32, d);
-}
-
-// out[row][k] = x[row][rev(k)] (times n^-1 when scaling)
-__global__ void k_ecntt_bitrev(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int L, size_t total,
-                               int scale, Fr ninv) {
+// out[row][k] = x[row][rev(k)]
+__global__ void k_ecntt_bitrev(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int L, size_t total) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const size_t n = size_t(1) << L, row = i >> L, k = i & (n - 1);
     const uint32_t src = L ? (__brev(uint32_t(k)) >> (32 - L)) : 0;
-    G1 v = ldg1(in + (row * n + src) * 32);
-    if (scale) v = g1_scale(v, ninv);
-    stg1(out + i * 32, v);
+    stg1(out + i * 32, ldg1(in + (row * n + src) * 32));
+}
+
+// out[i] = [c] x[i] for one scalar c shared by every lane.
+__global__ void __launch_bounds__(64) k_g1_mul_const(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                     size_t total, Fr c) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    uint32_t k[8];
+    fr_bits(c, k);
+    stg1(out + i * 32, g1_mul_scalar(ldg1(in + i * 32), k, true));
 }
 
 // Montgomery Fr -> canonical words (MSM scalars)
@@ -437,22 +429,26 @@ void ResidentState::fr_rows(const uint32_t* in, uint32_t* out, int L, size_t row
     ntt_rows_mont(domain, in, out, L, rows, inverse, ntt, stream());
 }
 
+// Natural order in and out: the G1 transform of gfft_rows (wave-shared
+// twiddles, GLV products) with one permutation pass, and for the inverse the
+// 1/n scaling as one shared-scalar product per point.
 void ResidentState::ecntt_rows(const uint32_t* in, uint32_t* out, int L, size_t rows, bool inverse) {
     const size_t n = size_t(1) << L, total = n * rows;
     uint32_t* work = ntt.b.as<uint32_t>(total * 32);
-    HIP_CHECK(hipMemcpyAsync(work, in, total * 128, hipMemcpyDeviceToDevice, stream()));
-    const uint32_t* tw = static_cast<const uint32_t*>(inverse ? domain.inv.ptr : domain.fwd.ptr);
-    for (int s = L - 1; s >= 0; --s) {
-        k_ecntt_stage<<<grid_for((n / 2) * rows, 64), 64, 0, stream()>>>(work, L, s, rows, tw);
+    if (!inverse) {
+        HIP_CHECK(hipMemcpyAsync(work, in, total * 128, hipMemcpyDeviceToDevice, stream()));
+        gfft_rows(domain, work, L, rows, false, stream());
+        k_ecntt_bitrev<<<grid_for(total, 64), 64, 0, stream()>>>(work, out, L, total);
         HIP_CHECK_LAUNCH();
+        return;
     }
-    Fr ninv = fp_one<FrParams>();
-    if (inverse) {
-        uint32_t nw[8] = {0};
-        nw[L / 32] = 1u << (L % 32);
-        ninv = canon(inv(to_mont<FrParams>(nw)));
-    }
-    k_ecntt_bitrev<<<grid_for(total, 64), 64, 0, stream()>>>(work, out, L, total, inverse ? 1 : 0, ninv);
+    k_ecntt_bitrev<<<grid_for(total, 64), 64, 0, stream()>>>(in, work, L, total);
+    HIP_CHECK_LAUNCH();
+    gfft_rows(domain, work, L, rows, true, stream());
+    uint32_t nw[8] = {0};
+    nw[L / 32] = 1u << (L % 32);
+    const Fr ninv = canon(inv(to_mont<FrParams>(nw)));
+    k_g1_mul_const<<<grid_for(total, 64), 64, 0, stream()>>>(work, out, total, ninv);
     HIP_CHECK_LAUNCH();
 }
 

@@ -461,13 +461,89 @@ def poly_tree(backend: HipBackend, log_k: int = 12, reps: int = 3, log=print) ->
     return out
 
 
+# ---- per-row figures of the single-call entry points -----------------------
+
+def row_figures(backend: HipBackend, reps: int = 5, log=print) -> Dict:
+    """One figure per remaining drop-in row, each on device-made synthetic data
+    and checked: the G2 MSM over 2^16 points (against [f(tau)] g2 from the
+    fixed-base kernel), G2 wire validation of 4096 subgroup points (all
+    accepted), and through the resident engine: an ECNTT over 2^12 G1 points
+    (forward then inverse restores the input), a resident G1 MSM over 2^16
+    bases, and a 64-point G1 read normalised on the device."""
+    from .codec import FR_MODULUS, encode_fr
+    from .resident import FR, G1, ResidentBackend
+    from .workload import poly_eval, srs_scalars
+    r = FR_MODULUS
+    out = {}
+
+    def med(fn):
+        ts, res = [], None
+        for _ in range(reps):
+            a = time.perf_counter()
+            res = fn()
+            ts.append((time.perf_counter() - a) * 1e3)
+        return sorted(ts)[len(ts) // 2], res
+
+    # G2 MSM, 2^16 points, the device-resident entry point
+    n = 1 << 16
+    tau, powers = srs_scalars(2031, n)
+    g2pts = backend.to_device(powers)           # scalars for the fixed-base G2 kernel
+    g2_mont = backend.alloc(128 * n)
+    backend.fixed_base_device(g2pts.ptr, n, g2_mont.ptr, g2=True)
+    sc = ChaCha20Stream(2032).uniform_fr_array(n)
+    d_sc = backend.to_device(sc)
+    t, res = med(lambda: backend.msm_device(g2_mont.ptr, d_sc.ptr, n, g2=True))
+    exp, _ = backend.fixed_base_g2_checked_encoded(encode_fr(poly_eval(sc, tau)), 1, 3)
+    out["g2_msm_2p16"] = {"ms": round(t, 3), "mpoints_s": round(n / (t * 1e-3) / 1e6, 2), "ok": res == exp}
+    g2pts.free(); g2_mont.free(); d_sc.free()
+    # G2 wire validation of 4096 subgroup points (fixed-base outputs as wire bytes)
+    m = 4096
+    wire, _ = backend.fixed_base_g2_checked_encoded(powers[:m].tobytes(), m, 3)
+    t, (masks, rep) = med(lambda: backend.inspect_g2_validation(wire, 3))
+    out["g2_validation_4096"] = {"ms": round(t, 3), "points_s": round(m / (t * 1e-3)),
+                                 "ok": rep["invalid_points"] == 0 and rep["checked_points"] == m}
+    # resident engine
+    rb = ResidentBackend(backend)
+    k = 1 << 12
+    pts_raw = backend.fixed_base_encoded(powers[:k].tobytes(), k, False)
+    from .codec import decode_g1
+    pts = [decode_g1(pts_raw[64 * i:64 * i + 64]) for i in range(k)]
+    src, mid, dst = rb.alloc(G1, k), rb.alloc(G1, k), rb.alloc(G1, k)
+    rb.write_points(src, 0, pts)
+
+    def ecntt_round():
+        rb.transform_rows(G1, src, mid, k, 0, 1, False)
+        rb.transform_rows(G1, mid, dst, k, 0, 1, True)
+    t, _ = med(ecntt_round)
+    ok = rb.read_points(dst, 0, 8) == pts[:8] and rb.read_points(dst, k - 8, 8) == pts[k - 8:]
+    out["resident_ecntt_2p12_fwd_inv"] = {"ms": round(t, 3), "ok": bool(ok)}
+    t64, got = med(lambda: rb.read_points(src, 100, 64, device_affine=True))
+    out["resident_read_64_points"] = {"ms": round(t64, 4), "ok": got == pts[100:164]}
+    nb = 1 << 16
+    bases_raw = b"".join(backend.fixed_base_encoded(powers[i:i + 4096].tobytes(), 4096, False)
+                         for i in range(0, nb, 4096))
+    bases = [decode_g1(bases_raw[64 * i:64 * i + 64]) for i in range(nb)]
+    hb = rb.alloc_bases(bases)
+    fr = rb.alloc(FR, nb)
+    scal = ChaCha20Stream(2033).uniform_fr(nb)
+    rb.write_fields(fr, 0, scal)
+    t, res = med(lambda: rb.msm_g1(hb, 0, fr, 0, nb))
+    f_t = 0
+    for c in reversed(scal):
+        f_t = (f_t * tau + c) % r
+    exp = decode_g1(backend.fixed_base_encoded(encode_fr(f_t), 1, False))
+    out["resident_msm_g1_2p16"] = {"ms": round(t, 3), "ok": res == exp}
+    log("row figures: " + ", ".join(f"{key} {v['ms']} ms ok={v['ok']}" for key, v in out.items()))
+    return out
+
+
 def main(argv=None):
     """One workload by name, for profiling: python -m dvgpu.benchmarks opening_cache --log-n 16"""
     import argparse
     import json
     ap = argparse.ArgumentParser(description=main.__doc__)
-    ap.add_argument("workload", choices=["opening_cache", "pairing_cache_check", "poly_tree", "batched_launch",
-                                         "incremental", "served_stream"])
+    ap.add_argument("workload", choices=["opening_cache", "pairing_cache_check", "poly_tree", "row_figures",
+                                         "batched_launch", "incremental", "served_stream"])
     ap.add_argument("--log-n", type=int, default=16)
     ap.add_argument("--rows", type=int, default=3)
     ap.add_argument("--reps", type=int, default=3)
@@ -480,6 +556,8 @@ def main(argv=None):
             res = pairing_cache_check(b, a.log_n, a.rows, a.reps)
         elif a.workload == "poly_tree":
             res = poly_tree(b, a.log_n, a.reps)
+        elif a.workload == "row_figures":
+            res = row_figures(b, a.reps)
         else:
             res = globals()[a.workload](b)
         print(json.dumps(res), flush=True)
