@@ -165,3 +165,25 @@ def test_full_size_caches_sampled_against_dense_openings(backend):
         exp = dense(tau, p, n, indices=idx)
         for i, e in zip(idx, exp):
             assert raw[64 * (r * n + i):64 * (r * n + i + 1)] == o.g1_to_bytes(e), (r, i)
+
+
+def test_largest_convolution(backend):
+    """The largest supported shape: 2^19 coefficients over 2^19 points, a
+    2^20-point convolution (every twiddle of the 2^20 table in use), sampled
+    against the trapdoor; one coefficient more than that is refused."""
+    n = 1 << 19
+    rng = random.Random(77)
+    tau = rng.randrange(2, o.R)
+    powers = [1] * (n + 2)
+    for i in range(1, n + 2):
+        powers[i] = powers[i - 1] * tau % o.R
+    srs = b"".join(backend.fixed_base_encoded(b"".join(o.fr_to_bytes(v) for v in powers[c:c + 4096]),
+                                              len(powers[c:c + 4096]), False) for c in range(0, n + 2, 4096))
+    coeffs = [rng.randrange(o.R) for _ in range(n)]
+    with KzgSrs(backend, encoded=srs) as z:
+        raw = z.precompute_bytes(b"".join(o.fr_to_bytes(v) for v in coeffs), n, 1, n)
+        with pytest.raises(BackendError):
+            z.precompute_bytes(bytes(32 * (n + 2)), n + 2, 1, 1 << 20)   # convolution of 2^21 points
+    idx = [0, 1, 3, n // 2 + 1, n - 1]
+    for i, e in zip(idx, dense(tau, coeffs, n, indices=idx)):
+        assert raw[64 * i:64 * i + 64] == o.g1_to_bytes(e), i
