@@ -26,10 +26,13 @@
 extern "C" {
 #endif
 
-/* Device passes of one call: rounds of batched products and products. */
+/* Work of one call: device rounds of batched products, all products, and
+ * the products that went to the device (a round of small products -- at most
+ * 256 coefficient pairs each, 4096 in total -- is formed on the host). */
 typedef struct DvgpuPolyStats {
     uint64_t rounds;
     uint64_t products;
+    uint64_t device_products;
 } DvgpuPolyStats;
 
 /* fft_multiply_with_backend: left * right (an empty operand gives [0]). */

@@ -107,5 +107,5 @@ def test_large_tree_batches_levels(backend):
             assert vals[i] == o.poly_eval(poly, points[i])
         assert tree.interpolate(vals) == o.poly_trim(poly)
         inter = tree.stats()
-        assert inter["rounds"] == 12                       # one per level
+        assert inter["rounds"] <= 12                       # at most one device round per level
     print("tree build", build, "interpolate", inter, flush=True)

@@ -85,13 +85,44 @@ class PolyEngine {
   public:
     explicit PolyEngine(Owner* o) : o_(o) {}
 
-    // Every job of the round in one device pass.
+    // A round whose products are all small (at most HOST_PRODUCT_MAX
+    // coefficient pairs each, HOST_ROUND_MAX in total) is formed on the host
+    // by schoolbook: a device round costs a fixed few hundred microseconds,
+    // which such a round does not repay.
+    static constexpr size_t HOST_PRODUCT_MAX = 256, HOST_ROUND_MAX = 4096;
+
+    static void host_mul(const Poly& a, const Poly& b, Poly& out) {
+        std::vector<Fr> acc(a.size() + b.size() - 1, fp_zero<FrParams>());
+        std::vector<Fr> bm(b.size());
+        for (size_t j = 0; j < b.size(); ++j) bm[j] = to_fr(b[j]);
+        for (size_t i = 0; i < a.size(); ++i) {
+            const Fr x = to_fr(a[i]);
+            for (size_t j = 0; j < b.size(); ++j) acc[i + j] = bn254::add(acc[i + j], bn254::mul(x, bm[j]));
+        }
+        out.resize(acc.size());
+        for (size_t i = 0; i < acc.size(); ++i) out[i] = from_fr(acc[i]);
+    }
+
+    // Every device-sized job of the round in one device pass.
     void mul(std::vector<MulJob>& jobs) {
         std::vector<size_t> size(jobs.size(), 0), off(jobs.size(), 0);
-        size_t total = 0;
+        size_t total = 0, host_work = 0;
+        bool all_small = true;
+        for (const MulJob& j : jobs) {
+            const size_t w = j.a->size() * j.b->size();
+            all_small = all_small && w <= HOST_PRODUCT_MAX;
+            host_work += w;
+        }
+        const bool on_host = all_small && host_work <= HOST_ROUND_MAX;
         for (size_t j = 0; j < jobs.size(); ++j) {
             const size_t la = jobs[j].a->size(), lb = jobs[j].b->size();
             if (!la || !lb) continue;
+            if (on_host) {
+                host_mul(*jobs[j].a, *jobs[j].b, *jobs[j].out);
+                ++stats_.products;
+                size[j] = SIZE_MAX;  // done
+                continue;
+            }
             size_t s = 1;
             while (s < la + lb - 1) s <<= 1;
             require(s <= DVGPU_MAX_NTT_SIZE, "polynomial product exceeds 2^20 coefficients");
@@ -99,8 +130,10 @@ class PolyEngine {
             off[j] = total;
             total += s;
         }
-        for (size_t j = 0; j < jobs.size(); ++j)
+        for (size_t j = 0; j < jobs.size(); ++j) {
             if (!size[j]) *jobs[j].out = Poly{zero8()};
+            if (size[j] == SIZE_MAX) size[j] = 0;  // host product: skip below
+        }
         if (!total) return;
         require(total < (size_t(1) << 30), "polynomial batch too large");
         // host staging: all padded left operands, then all padded right operands
@@ -140,6 +173,7 @@ class PolyEngine {
         }
         ++stats_.rounds;
         stats_.products += fwd.size() / 2;
+        stats_.device_products += fwd.size() / 2;
     }
 
     // Reciprocals rev(den_j)^-1 mod x^len_j by Newton steps, all divisors in

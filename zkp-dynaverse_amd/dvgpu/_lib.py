@@ -154,7 +154,7 @@ class DvgpuKzgTimings(ctypes.Structure):
 
 
 class DvgpuPolyStats(ctypes.Structure):
-    _fields_ = [("rounds", ctypes.c_uint64), ("products", ctypes.c_uint64)]
+    _fields_ = [("rounds", ctypes.c_uint64), ("products", ctypes.c_uint64), ("device_products", ctypes.c_uint64)]
 
 
 _I = ctypes.c_int

@@ -398,8 +398,9 @@ def poly_tree(backend: HipBackend, log_k: int = 12, reps: int = 3, log=print) ->
     """SubproductTree over 2^log_k random points: build, multipoint evaluation
     and interpolation of a degree 2^log_k - 1 polynomial, Cauchy sums; values
     checked against Horner at sampled points and the interpolation round trip.
-    Beside it, the per-product path the reference takes (one device product
-    call per tree product, fast.rs:204-234) for the tree build at 2^10."""
+    Beside it, the reference's call pattern (one product call per tree
+    product, fast.rs:204-234) for the products of a 2^10 tree, and trees of
+    16 and 256 points (their small rounds are formed on the host)."""
     from .codec import FR_MODULUS
     from .poly import SubproductTree, fft_multiply
     r = FR_MODULUS
@@ -448,11 +449,21 @@ def poly_tree(backend: HipBackend, log_k: int = 12, reps: int = 3, log=print) ->
     with SubproductTree(backend, small) as t2:
         batched_products = t2.stats()
     t_batched = (time.perf_counter() - a) * 1e3
+    small_sizes = {}
+    for ks in (16, 256):
+        sp = points[:ks]
+        t_b, st = med(lambda: SubproductTree(backend, sp))
+        t_e, sv = med(lambda: st.multipoint_eval(poly[:ks]))
+        t_i, sb = med(lambda: st.interpolate(sv))
+        small_sizes[str(ks)] = {"build_ms": round(t_b, 3), "eval_ms": round(t_e, 3), "interpolate_ms": round(t_i, 3),
+                                "round_trip_ok": sb == poly[:ks]}
+        st.close()
     out = {"points": k, "build_ms": round(t_build, 2), "build_rounds": build_stats["rounds"],
            "build_products": build_stats["products"], "eval_ms": round(t_eval, 2),
            "eval_rounds": eval_stats["rounds"], "interpolate_ms": round(t_interp, 2),
            "cauchy_sums_ms": round(t_cauchy, 2), "values_and_round_trip_ok": bool(ok),
-           "per_product_path_1024": {"products_only_ms": round(t_plain, 2), "device_calls": calls,
+           "small_trees": small_sizes,
+           "per_product_path_1024": {"products_only_ms": round(t_plain, 2), "calls": calls,
                                      "batched_full_build_ms": round(t_batched, 2),
                                      "batched_rounds": batched_products["rounds"]}}
     log(f"poly tree 2^{log_k}: build {t_build:.1f} ms ({build_stats['rounds']} rounds), eval {t_eval:.1f} ms, "

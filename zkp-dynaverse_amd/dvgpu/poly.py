@@ -79,7 +79,8 @@ class SubproductTree:
         self.close()
 
     def stats(self) -> dict:
-        return {"rounds": self.last_stats.rounds, "products": self.last_stats.products}
+        return {"rounds": self.last_stats.rounds, "products": self.last_stats.products,
+                "device_products": self.last_stats.device_products}
 
     def _vec(self, fn, cap):
         out = ctypes.create_string_buffer(32 * max(1, cap))
