@@ -90,6 +90,46 @@ HD_CALL Fq12 f12_sqr(const Fq12& a) {
 
 HD Fq12 f12_conj(const Fq12& a) { return {a.c0, f6_neg(a.c1)}; }
 
+// Square of an element of the cyclotomic subgroup (a^(q^6 + 1) = 1)
+// (Granger, Scott, "Faster squaring in the cyclotomic subgroup of sixth
+// degree extensions", PKC 2010, sec. 3.2).  With y = w^3 (y^2 = xi), the
+// element splits into three Fq4 = Fq2[y] parts A = z0 + z1 y, B = z2 + z3 y,
+// C = z4 + z5 y on the coefficients z0 = c0.a0, z4 = c0.a1, z3 = c0.a2,
+// z2 = c1.a0, z1 = c1.a1, z5 = c1.a2; the square needs only A^2, B^2, C^2
+// (each u^2 + xi v^2 and 2uv from one Karatsuba product):
+//   z0' = 3 (A^2)_0 - 2 z0,   z1' = 3 (A^2)_1 + 2 z1,
+//   z2' = 3 xi (C^2)_1 + 2 z2, z3' = 3 (C^2)_0 - 2 z3,
+//   z4' = 3 (B^2)_0 - 2 z4,   z5' = 3 (B^2)_1 + 2 z5.
+HD void fq4_sqr(const Fq2& u, const Fq2& v, Fq2& r0, Fq2& r1) {  // (u + v y)^2
+    const Fq2 uv = f2_mul(u, v);
+    r0 = f2_sub(f2_sub(f2_mul(f2_add(u, v), f2_add(u, f2_mul_xi(v))), uv), f2_mul_xi(uv));
+    r1 = f2_dbl(uv);
+}
+
+HD_CALL Fq12 f12_cyclotomic_sqr(const Fq12& a) {
+    const Fq2 &z0 = a.c0.a0, &z4 = a.c0.a1, &z3 = a.c0.a2, &z2 = a.c1.a0, &z1 = a.c1.a1, &z5 = a.c1.a2;
+    Fq2 a0, a1, b0, b1, c0, c1;
+    fq4_sqr(z0, z1, a0, a1);
+    fq4_sqr(z2, z3, b0, b1);
+    fq4_sqr(z4, z5, c0, c1);
+    const auto three_minus_two = [](const Fq2& t, const Fq2& z) {  // 3t - 2z
+        const Fq2 d = f2_sub(t, z);
+        return f2_add(f2_dbl(d), t);
+    };
+    const auto three_plus_two = [](const Fq2& t, const Fq2& z) {  // 3t + 2z
+        const Fq2 d = f2_add(t, z);
+        return f2_add(f2_dbl(d), t);
+    };
+    Fq12 r;
+    r.c0.a0 = three_minus_two(a0, z0);
+    r.c1.a1 = three_plus_two(a1, z1);
+    r.c1.a0 = three_plus_two(f2_mul_xi(c1), z2);
+    r.c0.a2 = three_minus_two(c0, z3);
+    r.c0.a1 = three_minus_two(b0, z4);
+    r.c1.a2 = three_plus_two(b1, z5);
+    return r;
+}
+
 HD_CALL Fq12 f12_inv(const Fq12& a) {
     const Fq6 den = f6_sub(f6_mul(a.c0, a.c0), f6_mul_v(f6_mul(a.c1, a.c1)));
     const Fq6 di = f6_inv(den);
@@ -224,11 +264,12 @@ HD_CALL Fq12 miller_loop(const Fq& xp, const Fq& yp, const Fq2& x2, const Fq2& y
     return miller_product(&in, 1);
 }
 
-// f^x for the BN parameter x (63 bits), square-and-multiply.
+// f^x for the BN parameter x (63 bits), square-and-multiply, f in the
+// cyclotomic subgroup.
 HD_CALL Fq12 f12_pow_x(const Fq12& f) {
     Fq12 r = f;
     for (int i = 61; i >= 0; --i) {
-        r = f12_sqr(r);
+        r = f12_cyclotomic_sqr(r);
         if ((BN_SEED >> i) & 1u) r = f12_mul(r, f);
     }
     return r;
@@ -246,18 +287,18 @@ HD_CALL Fq12 final_exponentiation(const Fq12& f) {
     Fq12 a = f12_mul(f12_conj(f), f12_inv(f));
     a = f12_mul(f12_frobenius(f12_frobenius(a)), a);
     const Fq12 x1 = f12_pow_x(a), x2 = f12_pow_x(x1), x3 = f12_pow_x(x2);
-    const Fq12 x3_4 = f12_sqr(f12_sqr(x3));
-    const Fq12 x3_36 = f12_mul(f12_sqr(f12_sqr(f12_sqr(x3_4))), x3_4);
-    const Fq12 x2_2 = f12_sqr(x2);
-    const Fq12 x2_6 = f12_mul(f12_sqr(x2_2), x2_2);
-    const Fq12 x2_12 = f12_sqr(x2_6);
+    const Fq12 x3_4 = f12_cyclotomic_sqr(f12_cyclotomic_sqr(x3));
+    const Fq12 x3_36 = f12_mul(f12_cyclotomic_sqr(f12_cyclotomic_sqr(f12_cyclotomic_sqr(x3_4))), x3_4);
+    const Fq12 x2_2 = f12_cyclotomic_sqr(x2);
+    const Fq12 x2_6 = f12_mul(f12_cyclotomic_sqr(x2_2), x2_2);
+    const Fq12 x2_12 = f12_cyclotomic_sqr(x2_6);
     const Fq12 x2_18 = f12_mul(x2_12, x2_6);
     const Fq12 x2_30 = f12_mul(x2_18, x2_12);
-    const Fq12 x1_2 = f12_sqr(x1);
-    const Fq12 x1_6 = f12_mul(f12_sqr(x1_2), x1_2);
-    const Fq12 x1_12 = f12_sqr(x1_6);
+    const Fq12 x1_2 = f12_cyclotomic_sqr(x1);
+    const Fq12 x1_6 = f12_mul(f12_cyclotomic_sqr(x1_2), x1_2);
+    const Fq12 x1_12 = f12_cyclotomic_sqr(x1_6);
     const Fq12 x1_18 = f12_mul(x1_12, x1_6);
-    const Fq12 p0 = f12_conj(f12_mul(f12_mul(x3_36, x2_30), f12_mul(x1_18, f12_sqr(a))));
+    const Fq12 p0 = f12_conj(f12_mul(f12_mul(x3_36, x2_30), f12_mul(x1_18, f12_cyclotomic_sqr(a))));
     const Fq12 p1 = f12_mul(f12_conj(f12_mul(f12_mul(x3_36, x2_18), x1_12)), a);
     const Fq12 p2 = f12_mul(x2_6, a);
     const Fq12 q2 = f12_frobenius(f12_frobenius(p2));
