@@ -187,3 +187,31 @@ def test_largest_convolution(backend):
     idx = [0, 1, 3, n // 2 + 1, n - 1]
     for i, e in zip(idx, dense(tau, coeffs, n, indices=idx)):
         assert raw[64 * i:64 * i + 64] == o.g1_to_bytes(e), i
+
+
+def test_simultaneous_callers_get_independent_correct_openings(backend):
+    """kzg.rs:505-525: two threads on one SRS with different polynomials
+    (the SRS lock serialises the calls; each result is its own)."""
+    import threading
+    pts = o.kzg_srs(7, 16)
+    results, errors = {}, []
+    with KzgSrs(backend, pts) as z:
+        barrier = threading.Barrier(2)
+
+        def run(offset):
+            try:
+                c = coefficients(8, offset)
+                barrier.wait()
+                for _ in range(5):
+                    results[offset] = z.precompute_single_openings(c, 8)
+            except Exception as e:      # surfaced below
+                errors.append(e)
+        threads = [threading.Thread(target=run, args=(off,)) for off in (3, 17)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        assert not errors
+        assert z.retained_basis()[0] == 7
+    for off in (3, 17):
+        assert results[off] == dense(7, coefficients(8, off), 8)
