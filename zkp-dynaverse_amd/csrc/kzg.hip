@@ -293,6 +293,8 @@ void precompute(KzgSrs& z, const uint8_t* coefficients, uint64_t coeff_count, ui
     uint64_t M = 1;
     while (M < 2 * degree - 1) M <<= 1;
     require(M <= DVGPU_MAX_NTT_SIZE, "opening convolution exceeds 2^20 points");
+    require(rows <= (uint64_t(1) << 24) / M && rows <= (uint64_t(1) << 24) / domain_size,
+            "too many polynomials for one call (rows x 2^20 points at most 2^24)");
     const int LM = log2_pow2(M), Ln = log2_pow2(domain_size);
     Owner* o = z.owner;
     hipStream_t st = o->stream;
