@@ -53,6 +53,57 @@ W8 from_fr(const Fr& v) {
     return o;
 }
 
+// Canonical-word addition, subtraction and negation mod r (no Montgomery
+// round trip: these run once per coefficient per round on the host).
+bool geq_r(const W8& a) {
+    for (int i = 7; i >= 0; --i) {
+        if (a[i] != FR_MOD32[i]) return a[i] > FR_MOD32[i];
+    }
+    return true;
+}
+
+W8 sub_r(W8 a) {  // a - r, for r <= a < 2r
+    uint64_t borrow = 0;
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t d = uint64_t(a[i]) - FR_MOD32[i] - borrow;
+        a[i] = uint32_t(d);
+        borrow = (d >> 32) & 1;
+    }
+    return a;
+}
+
+W8 add8(const W8& a, const W8& b) {
+    W8 r;
+    uint64_t carry = 0;
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t t = uint64_t(a[i]) + b[i] + carry;
+        r[i] = uint32_t(t);
+        carry = t >> 32;
+    }
+    return (carry || geq_r(r)) ? sub_r(r) : r;  // a + b < 2r < 2^256: no carry out in practice
+}
+
+W8 sub8(const W8& a, const W8& b) {
+    W8 r;
+    uint64_t borrow = 0;
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t d = uint64_t(a[i]) - b[i] - borrow;
+        r[i] = uint32_t(d);
+        borrow = (d >> 32) & 1;
+    }
+    if (borrow) {  // + r
+        uint64_t carry = 0;
+        for (int i = 0; i < 8; ++i) {
+            const uint64_t t = uint64_t(r[i]) + FR_MOD32[i] + carry;
+            r[i] = uint32_t(t);
+            carry = t >> 32;
+        }
+    }
+    return r;
+}
+
+W8 neg8(const W8& a) { return sub8(W8{}, a); }
+
 void trim(Poly& p) {
     while (p.size() > 1 && is_zero8(p.back())) p.pop_back();
     if (p.empty()) p.push_back(zero8());
@@ -204,8 +255,8 @@ class PolyEngine {
             mul(jobs);
             for (size_t t = 0; t < act.size(); ++t) {  // corr = 2 - rev * inv (mod x^next)
                 corr[t].resize(nlen[t], zero8());
-                for (auto& c : corr[t]) c = from_fr(neg(to_fr(c)));
-                corr[t][0] = from_fr(add(to_fr(corr[t][0]), to_fr(two)));
+                for (auto& c : corr[t]) c = neg8(c);
+                corr[t][0] = add8(corr[t][0], two);
             }
             jobs.clear();
             for (size_t t = 0; t < act.size(); ++t) jobs.push_back({&inv[act[t]], &corr[t], &next[t]});
@@ -273,15 +324,15 @@ class PolyEngine {
             const size_t j = need[t], ld = den[j]->size();
             Poly r(std::max<size_t>(ld - 1, 1), zero8());
             for (size_t i = 0; i < ld - 1; ++i) {
-                const Fr a = i < num[j]->size() ? to_fr((*num[j])[i]) : fp_zero<FrParams>();
-                const Fr b = i < qd[t].size() ? to_fr(qd[t][i]) : fp_zero<FrParams>();
-                r[i] = from_fr(sub(a, b));
+                const W8 a = i < num[j]->size() ? (*num[j])[i] : zero8();
+                const W8 b = i < qd[t].size() ? qd[t][i] : zero8();
+                r[i] = sub8(a, b);
             }
             // the division is exact in the high part by construction; check it
             for (size_t i = ld - 1; i < std::max(num[j]->size(), qd[t].size()); ++i) {
-                const Fr a = i < num[j]->size() ? to_fr((*num[j])[i]) : fp_zero<FrParams>();
-                const Fr b = i < qd[t].size() ? to_fr(qd[t][i]) : fp_zero<FrParams>();
-                require(is_zero(sub(a, b)), "Newton division failed to cancel high coefficients");
+                const W8 a = i < num[j]->size() ? (*num[j])[i] : zero8();
+                const W8 b = i < qd[t].size() ? qd[t][i] : zero8();
+                require(a == b, "Newton division failed to cancel high coefficients");
             }
             rem[j] = std::move(r);
             if (quot) (*quot)[j] = std::move(q[t]);
@@ -401,9 +452,9 @@ Poly tree_interpolate(PolyEngine& e, const PolyTree& t, const std::vector<W8>& v
         for (size_t p = 0; p < pairs; ++p) {
             Poly c(std::max(left[p].size(), right[p].size()), zero8());
             for (size_t j = 0; j < c.size(); ++j) {
-                const Fr a = j < left[p].size() ? to_fr(left[p][j]) : fp_zero<FrParams>();
-                const Fr b = j < right[p].size() ? to_fr(right[p][j]) : fp_zero<FrParams>();
-                c[j] = from_fr(add(a, b));
+                const W8 a = j < left[p].size() ? left[p][j] : zero8();
+                const W8 b = j < right[p].size() ? right[p][j] : zero8();
+                c[j] = add8(a, b);
             }
             trim(c);
             next.push_back(std::move(c));
@@ -426,7 +477,7 @@ PolyTree* tree_build(PolyEngine& e, Owner* o, const std::vector<W8>& pts) {
                 "duplicate interpolation point");
     std::vector<Poly> leaves;
     const W8 one = from_fr(fp_one<FrParams>());
-    for (const W8& p : pts) leaves.push_back(Poly{from_fr(neg(to_fr(p))), one});
+    for (const W8& p : pts) leaves.push_back(Poly{neg8(p), one});
     if (leaves.empty()) leaves.push_back(Poly{one});
     t->levels.push_back(std::move(leaves));
     while (t->levels.back().size() > 1) {
