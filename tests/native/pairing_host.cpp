@@ -65,6 +65,25 @@ int main() {
                 in.push_back({rd(p[0]), rd(p[1]), rd2(p + 2), rd2(p + 4)});
             }
             wr12(final_exponentiation(miller_product(in.data(), k)));
+        } else if (!strcmp(op, "prepared")) {  // pair 0 through prepare_lines: the Miller value
+            const int k = atoi(a[0]);
+            std::vector<PairingInput> in;
+            for (int i = 0; i < k; ++i) {
+                char** p = a + 1 + 6 * i;
+                in.push_back({rd(p[0]), rd(p[1]), rd2(p + 2), rd2(p + 4)});
+            }
+            std::vector<Line> lines(PAIRING_LINES);
+            prepare_lines(in[0].x2, in[0].y2, lines.data());
+            auto src = [&](int i) { return lines[i]; };
+            wr12(miller_product_impl(in.data() + 1, k - 1, src, true, in[0].xp, in[0].yp));
+        } else if (!strcmp(op, "millerprod")) {  // the same product without prepared lines
+            const int k = atoi(a[0]);
+            std::vector<PairingInput> in;
+            for (int i = 0; i < k; ++i) {
+                char** p = a + 1 + 6 * i;
+                in.push_back({rd(p[0]), rd(p[1]), rd2(p + 2), rd2(p + 4)});
+            }
+            wr12(miller_product(in.data(), k));
         } else if (!strcmp(op, "fexp")) {
             wr12(final_exponentiation(rd12(a)));
         } else if (!strcmp(op, "mul")) {

@@ -120,3 +120,13 @@ def test_product_checks(harness):
     assert one == o.F12_ONE
     bad = tower_to_flat(harness("product", 2, *pair_args(diff, o.G2_GEN), *pair_args(o.neg(o.add(opening, o.G1_GEN)), div)))
     assert bad != o.F12_ONE
+
+
+def test_prepared_lines_give_the_same_miller_value(harness):
+    """A pair whose G2 lines are prepared once (prepare_lines) multiplies the
+    same Miller value as the pair computed in the loop."""
+    rng = random.Random(25)
+    pairs = [(o.g1_mul(rng.randrange(1, o.R)), o.mul(o.G2_GEN, rng.randrange(1, o.R), o.FQ2)) for _ in range(3)]
+    args = [a for p, q in pairs for a in pair_args(p, q)]
+    for k in (1, 2, 3):
+        assert harness("prepared", k, *args[:6 * k]) == harness("millerprod", k, *args[:6 * k])

@@ -122,12 +122,34 @@ __global__ void __launch_bounds__(64) k_cache_divisors(const uint32_t* __restric
     }
 }
 
+// The P-independent lines of g2_0's Miller loop (PAIRING_LINES x 3 Fq2,
+// Montgomery words), shared by every check of the call.
+__global__ void k_prepare_lines(Fq2 x, Fq2 y, uint32_t* __restrict__ out) {
+    if (blockIdx.x || threadIdx.x) return;
+    Line lines[PAIRING_LINES];
+    prepare_lines(x, y, lines);
+    for (int i = 0; i < PAIRING_LINES; ++i) {
+        const Fq* c[6] = {&lines[i].l0.c0, &lines[i].l0.c1, &lines[i].l1.c0, &lines[i].l1.c1,
+                          &lines[i].l3.c0, &lines[i].l3.c1};
+        for (int q = 0; q < 6; ++q) stfq(out + i * 48 + q * 8, *c[q]);
+    }
+}
+
+struct GlobalLines {
+    const uint32_t* p;
+    __device__ Line operator()(int i) const {
+        const uint32_t* q = p + i * 48;
+        return {{ldfq(q), ldfq(q + 8)}, {ldfq(q + 16), ldfq(q + 24)}, {ldfq(q + 32), ldfq(q + 40)}};
+    }
+};
+
 // Check (w, i): e(C_w - [v_wi] g1_0, g2_0) == e(pi_wi, D_i), as the product
-// e(C_w - [v_wi] g1_0, g2_0) e(-pi_wi, D_i) == 1.
+// e(C_w - [v_wi] g1_0, g2_0) e(-pi_wi, D_i) == 1; g2_0's lines prepared.
 __global__ void __launch_bounds__(64) k_cache_check(const uint32_t* __restrict__ values, const uint32_t* __restrict__ caches,
                                                     const uint32_t* __restrict__ div, const uint8_t* __restrict__ flags,
                                                     const uint32_t* __restrict__ commitments, uint32_t wires, uint64_t n,
-                                                    Fq g1x, Fq g1y, Fq2 g0x, Fq2 g0y, uint8_t* __restrict__ ok) {
+                                                    Fq g1x, Fq g1y, const uint32_t* __restrict__ lines,
+                                                    uint8_t* __restrict__ ok) {
     const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= uint64_t(wires) * n) return;
     const uint64_t w = t / n, i = t % n;
@@ -157,11 +179,11 @@ __global__ void __launch_bounds__(64) k_cache_check(const uint32_t* __restrict__
     const Affine<FqOps> d = xyzz_to_affine<FqOps>(xyzz_add<FqOps>(c, xyzz_neg<FqOps>(vg)));
     const uint32_t* dv = div + i * 32;
     const bool dinf = all_zero(dv, 32);
-    PairingInput in[2];
+    PairingInput in[1];
     int m = 0;
-    if (!d.inf) in[m++] = {d.x, d.y, g0x, g0y};
     if (!pi0 && !dinf) in[m++] = {px, neg(py), {ldfq(dv), ldfq(dv + 8)}, {ldfq(dv + 16), ldfq(dv + 24)}};
-    ok[t] = f12_is_one(final_exponentiation(m ? miller_product(in, m) : f12_one())) ? 1 : 0;
+    const Fq12 f = (m || !d.inf) ? miller_product_impl(in, m, GlobalLines{lines}, !d.inf, d.x, d.y) : f12_one();
+    ok[t] = f12_is_one(final_exponentiation(f)) ? 1 : 0;
 }
 
 Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
@@ -273,9 +295,10 @@ extern "C" int dvgpu_bn254_opening_cache_check(void* backend, const uint8_t* com
         hipStream_t st = o->stream;
         const size_t total = wires * n;
         const size_t bz = n * 32, bv = total * 32, bc = total * 64, bd = n * 128, bf = (n + 255) & ~size_t(255),
-                     bm = wires * 64, br = (total + 255) & ~size_t(255);
-        auto* base = static_cast<uint8_t*>(o->stage_a.ensure(bz + bv + bc + bd + bf + bm + br));
-        uint8_t *dz = base, *dv = dz + bz, *dc = dv + bv, *dd = dc + bc, *df = dd + bd, *dm = df + bf, *dr = dm + bm;
+                     bm = wires * 64, br = (total + 255) & ~size_t(255), bl = PAIRING_LINES * 48 * 4;
+        auto* base = static_cast<uint8_t*>(o->stage_a.ensure(bz + bv + bc + bd + bf + bm + br + bl));
+        uint8_t *dz = base, *dv = dz + bz, *dc = dv + bv, *dd = dc + bc, *df = dd + bd, *dm = df + bf, *dr = dm + bm,
+                *dl = dr + br;
         HIP_CHECK(hipMemcpyAsync(dz, points, bz, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(dv, values, bv, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(dc, caches, bc, hipMemcpyHostToDevice, st));
@@ -288,10 +311,12 @@ extern "C" int dvgpu_bn254_opening_cache_check(void* backend, const uint8_t* com
         k_cache_divisors<<<grid_for(n, 64), 64, 0, st>>>(reinterpret_cast<const uint32_t*>(dz), n, h0.x, h0.y, h1.x,
                                                           h1.y, reinterpret_cast<uint32_t*>(dd), df);
         HIP_CHECK_LAUNCH();
+        k_prepare_lines<<<1, 64, 0, st>>>(h0.x, h0.y, reinterpret_cast<uint32_t*>(dl));
+        HIP_CHECK_LAUNCH();
         k_cache_check<<<grid_for(total, 64), 64, 0, st>>>(
             reinterpret_cast<const uint32_t*>(dv), reinterpret_cast<const uint32_t*>(dc),
             reinterpret_cast<const uint32_t*>(dd), df, reinterpret_cast<const uint32_t*>(dm), uint32_t(wires), n,
-            g1.x, g1.y, h0.x, h0.y, dr);
+            g1.x, g1.y, reinterpret_cast<const uint32_t*>(dl), dr);
         HIP_CHECK_LAUNCH();
         if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
         HIP_CHECK(hipMemcpyAsync(results, dr, total, hipMemcpyDeviceToHost, st));

@@ -185,13 +185,14 @@ HD_CALL Fq12 f12_mul_line(const Fq12& f, const Line& l) {
     return {f6_add(t0, f6_mul_v(t1)), f6_sub(f6_sub(t2, t0), t1)};
 }
 
-// Tangent line at T (evaluated at (xp, yp)) and T <- 2T.
-HD_CALL Line miller_dbl(G2Proj& t, const Fq& xp, const Fq& yp) {
+// Tangent line at T with its P-dependent factors left out (l0 stands for the
+// coefficient of yp, l1 for that of xp), and T <- 2T.
+HD_CALL Line miller_dbl_coeffs(G2Proj& t) {
     const Fq2 xx = f2_sqr(t.X), yy = f2_sqr(t.Y);
     const Fq2 yz = f2_mul(t.Y, t.Z);
     const Fq2 three_xx = f2_add(f2_dbl(xx), xx);
     // line: (2 Y Z^2) yp - (3 X^2 Z) xp w + (3 X^3 - 2 Y^2 Z) w^3
-    const Line l = {f2_mul_fq(f2_dbl(f2_mul(yz, t.Z)), yp), f2_neg(f2_mul_fq(f2_mul(three_xx, t.Z), xp)),
+    const Line l = {f2_dbl(f2_mul(yz, t.Z)), f2_neg(f2_mul(three_xx, t.Z)),
                     f2_sub(f2_mul(three_xx, t.X), f2_dbl(f2_mul(yy, t.Z)))};
     // dbl-2007-bl, a = 0: w = 3XX, s = 2YZ, R = Y s, B = (X + R)^2 - XX - RR,
     // h = w^2 - 2B; X3 = h s, Y3 = w (B - h) - 2 RR, Z3 = s^3.
@@ -205,13 +206,11 @@ HD_CALL Line miller_dbl(G2Proj& t, const Fq& xp, const Fq& yp) {
     return l;
 }
 
-// Chord through T and the affine Q = (x2, y2) (evaluated at (xp, yp)) and
-// T <- T + Q.
-HD_CALL Line miller_add(G2Proj& t, const Fq2& x2, const Fq2& y2, const Fq& xp, const Fq& yp) {
+// Chord through T and the affine Q = (x2, y2) in the same form, and T <- T + Q.
+HD_CALL Line miller_add_coeffs(G2Proj& t, const Fq2& x2, const Fq2& y2) {
     const Fq2 theta = f2_sub(f2_mul(y2, t.Z), t.Y);  // y2 Z - Y
     const Fq2 lambda = f2_sub(f2_mul(x2, t.Z), t.X); // x2 Z - X
-    const Line l = {f2_mul_fq(lambda, yp), f2_neg(f2_mul_fq(theta, xp)),
-                    f2_sub(f2_mul(theta, x2), f2_mul(lambda, y2))};
+    const Line l = {lambda, f2_neg(theta), f2_sub(f2_mul(theta, x2), f2_mul(lambda, y2))};
     // madd-1998-cmo: uu = u^2, vv = v^2, vvv = v vv, R = vv X,
     // A = uu Z - vvv - 2R; X3 = v A, Y3 = u (R - A) - vvv Y, Z3 = vvv Z.
     const Fq2 uu = f2_sqr(theta), vv = f2_sqr(lambda);
@@ -220,6 +219,14 @@ HD_CALL Line miller_add(G2Proj& t, const Fq2& x2, const Fq2& y2, const Fq& xp, c
     const Fq2 a = f2_sub(f2_sub(f2_mul(uu, t.Z), vvv), f2_dbl(r));
     t = {f2_mul(lambda, a), f2_sub(f2_mul(theta, f2_sub(r, a)), f2_mul(vvv, t.Y)), f2_mul(vvv, t.Z)};
     return l;
+}
+
+// The line at P = (xp, yp).
+HD Line line_at(const Line& c, const Fq& xp, const Fq& yp) { return {f2_mul_fq(c.l0, yp), f2_mul_fq(c.l1, xp), c.l3}; }
+
+HD Line miller_dbl(G2Proj& t, const Fq& xp, const Fq& yp) { return line_at(miller_dbl_coeffs(t), xp, yp); }
+HD Line miller_add(G2Proj& t, const Fq2& x2, const Fq2& y2, const Fq& xp, const Fq& yp) {
+    return line_at(miller_add_coeffs(t, x2, y2), xp, yp);
 }
 
 // pi(Q) on the twist: (conj(x) xi^((q-1)/3), conj(y) xi^((q-1)/2)).
@@ -235,18 +242,58 @@ struct PairingInput {
 
 constexpr int PAIRING_MAX_TERMS = 16;
 
+constexpr int ate_popcount() {
+    int c = 0;
+    for (int i = 0; i < 64; ++i) c += int((PAIRING_ATE_LO >> i) & 1u);
+    for (int i = 0; i < 32; ++i) c += int((PAIRING_ATE_HI >> i) & 1u);
+    return c;
+}
+// Lines of one pair's Miller loop: one tangent per bit below the top, one
+// chord per set bit below the top, the pi(Q) and -pi^2(Q) chords.
+constexpr int PAIRING_LINES = (PAIRING_ATE_BITS - 1) + (ate_popcount() - 1) + 2;
+
+HD uint32_t ate_bit(int i) {
+    return i >= 64 ? (PAIRING_ATE_HI >> (i - 64)) & 1u : uint32_t(PAIRING_ATE_LO >> i) & 1u;
+}
+
+// The P-independent line coefficients of Q's Miller loop, in loop order
+// (PAIRING_LINES entries): for a Q shared by many pairings.
+HD_CALL void prepare_lines(const Fq2& x2, const Fq2& y2, Line* out) {
+    G2Proj t = {x2, y2, f2_one()};
+    int n = 0;
+    for (int i = PAIRING_ATE_BITS - 2; i >= 0; --i) {
+        out[n++] = miller_dbl_coeffs(t);
+        if (ate_bit(i)) out[n++] = miller_add_coeffs(t, x2, y2);
+    }
+    Fq2 q1x, q1y, q2x, q2y;
+    twist_frobenius(x2, y2, q1x, q1y);
+    twist_frobenius(q1x, q1y, q2x, q2y);
+    out[n++] = miller_add_coeffs(t, q1x, q1y);
+    out[n++] = miller_add_coeffs(t, q2x, f2_neg(q2y));
+}
+
 // prod_j f_j for k finite pairs, sharing the squaring of the accumulator
-// (one Miller loop over all pairs).
-HD_CALL Fq12 miller_product(const PairingInput* in, int k) {
+// (one Miller loop over all pairs).  `prepared` (optional): one more pair
+// whose Q's lines come from prepare_lines, evaluated at (pxp, pyp).
+template <class LineSource>
+HD Fq12 miller_product_impl(const PairingInput* in, int k, const LineSource& prepared, bool has_prepared,
+                            const Fq& pxp, const Fq& pyp) {
     G2Proj t[PAIRING_MAX_TERMS];
     for (int j = 0; j < k; ++j) t[j] = {in[j].x2, in[j].y2, f2_one()};
     Fq12 f = f12_one();
+    int n = 0;
     for (int i = PAIRING_ATE_BITS - 2; i >= 0; --i) {
         if (i != PAIRING_ATE_BITS - 2) f = f12_sqr(f);
+        if (has_prepared) f = f12_mul_line(f, line_at(prepared(n++), pxp, pyp));
         for (int j = 0; j < k; ++j) f = f12_mul_line(f, miller_dbl(t[j], in[j].xp, in[j].yp));
-        const uint32_t bit = i >= 64 ? (PAIRING_ATE_HI >> (i - 64)) & 1u : uint32_t(PAIRING_ATE_LO >> i) & 1u;
-        if (bit)
+        if (ate_bit(i)) {
+            if (has_prepared) f = f12_mul_line(f, line_at(prepared(n++), pxp, pyp));
             for (int j = 0; j < k; ++j) f = f12_mul_line(f, miller_add(t[j], in[j].x2, in[j].y2, in[j].xp, in[j].yp));
+        }
+    }
+    if (has_prepared) {
+        f = f12_mul_line(f, line_at(prepared(n++), pxp, pyp));
+        f = f12_mul_line(f, line_at(prepared(n++), pxp, pyp));
     }
     for (int j = 0; j < k; ++j) {
         Fq2 q1x, q1y, q2x, q2y;
@@ -256,6 +303,14 @@ HD_CALL Fq12 miller_product(const PairingInput* in, int k) {
         f = f12_mul_line(f, miller_add(t[j], q2x, f2_neg(q2y), in[j].xp, in[j].yp));
     }
     return f;
+}
+
+struct NoLines {
+    HD Line operator()(int) const { return {}; }
+};
+
+HD_CALL Fq12 miller_product(const PairingInput* in, int k) {
+    return miller_product_impl(in, k, NoLines{}, false, fp_zero<FqParams>(), fp_zero<FqParams>());
 }
 
 // Miller value of one pair.
