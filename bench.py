@@ -54,8 +54,8 @@ def main():
                     help="also run the plain copy kernel (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
-                                   "(every GPU box here has one MI355X)"}))
+        print(json.dumps({"error": "the benchmark measures one MI355X; --gpus N > 1 is not supported "
+                                   "(the multi-GPU sharding is a design, not a measured path)"}))
         sys.exit(2)
 
     import numpy as np
