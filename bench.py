@@ -154,6 +154,8 @@ def main():
         secondary["opening_cache_2p13_x1"] = benchmarks.opening_cache(backend, log_n=13, rows=1, log=log)
         log("# opening-cache validation: 3 x 2^16 pairing-product checks")
         secondary["pairing_cache_check_2p16"] = benchmarks.pairing_cache_check(backend, log=log)
+        log("# verifier pairing equations: 2^16 equations of 2..5 pairs, both modes")
+        secondary["pairing_equations_2p16"] = benchmarks.pairing_equations_mix(backend, log=log)
         log("# polynomial tools: subproduct tree over 2^12 points")
         secondary["poly_tree_2p12"] = benchmarks.poly_tree(backend, log=log)
         log("# per-row figures: G2 MSM, G2 validation, resident transform / MSM / readback")

@@ -41,6 +41,35 @@ int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const uint8_t* g
                               uint8_t* results, uint8_t* gt, float* kernel_ms, char* error,
                               size_t error_capacity);
 
+/* Verification modes of PairingChecks (crates/protocol/src/types.rs:193-196). */
+#define DVGPU_PAIRING_PRODUCT 0
+#define DVGPU_PAIRING_REFERENCE 1
+
+/* A batch of independent pairing equations, each as PairingChecks::equal
+ * states it (crates/protocol/src/verification.rs:37-53):
+ *     e(L.P, L.Q) == prod_j e(R_j.P, R_j.Q),
+ * the pairs of equation i at indices offsets[i] .. offsets[i+1]-1 of g1
+ * (64 B each) and g2 (128 B each), the left pair first.  offsets holds
+ * count + 1 entries, offsets[0] = 0, 1..16 pairs per equation, at most 2^24
+ * pairs.  mode DVGPU_PAIRING_PRODUCT evaluates an equation as product_equal
+ * does (verification.rs:79-103): one Miller product over the left pair and
+ * the right pairs with negated G1 points, one final exponentiation, compared
+ * with one.  DVGPU_PAIRING_REFERENCE evaluates it as the Reference branch of
+ * equal does (verification.rs:46-53): every pair reduced on its own and the
+ * right-hand values multiplied in GT, compared with the left value.
+ * equal_prepared (verification.rs:55-77) is the two-pair Product equation
+ * (left, generator), (right, divisor).  Equations are never merged: results
+ * holds one byte per equation, 1 when it holds.  Replaces the arkworks
+ * pairings of every PairingChecks caller (dynaverse.rs:1003-1113,
+ * dynamo.rs:415-430, consistency.rs:95-106, rebuilding/machine.rs:309-390);
+ * the callers' short-circuit order is the caller's.  Point checks and
+ * identities as for dvgpu_bn254_pairing_check; a bad point fails the call
+ * with -1 and zeroes results.  kernel_ms (optional): event-timed device
+ * duration.  No counterpart in the reference ABI. */
+int dvgpu_bn254_pairing_equations(void* backend, const uint8_t* g1, const uint8_t* g2, const uint32_t* offsets,
+                                  size_t count, int mode, uint8_t* results, float* kernel_ms, char* error,
+                                  size_t error_capacity);
+
 /* The whole opening-cache validation of validate_initial_caches
  * (machine.rs:357-382) and validate_cache_block_prepared (machine.rs:
  * 276-336, Product mode) on the device: for every wire w < wires and point

@@ -40,6 +40,26 @@ __device__ __forceinline__ bool canonical_fq(const uint32_t* p) {
     return is_canonical<FqParams>(w);
 }
 
+// Pair j at p (G1, 16 words) and q (G2, 32 words), canonical LE: false when
+// a coordinate is not canonical or a point is off its curve.  `finite` says
+// whether both points are finite (a pair with an identity contributes one).
+__device__ __forceinline__ bool load_pair(const uint32_t* p, const uint32_t* q, PairingInput& e, bool& finite) {
+    const bool p0 = all_zero(p, 16), q0 = all_zero(q, 32);
+    bool pv = p0, qv = q0;
+    if (!p0 && canonical_fq(p) && canonical_fq(p + 8)) {
+        e.xp = ld_canon_fq(p);
+        e.yp = ld_canon_fq(p + 8);
+        pv = affine_on_curve<FqOps>({e.xp, e.yp, false});
+    }
+    if (!q0 && canonical_fq(q) && canonical_fq(q + 8) && canonical_fq(q + 16) && canonical_fq(q + 24)) {
+        e.x2 = {ld_canon_fq(q), ld_canon_fq(q + 8)};
+        e.y2 = {ld_canon_fq(q + 16), ld_canon_fq(q + 24)};
+        qv = affine_on_curve<Fq2Ops>({e.x2, e.y2, false});
+    }
+    finite = !p0 && !q0;
+    return pv && qv;
+}
+
 __global__ void __launch_bounds__(64) k_pairing_check(const uint32_t* __restrict__ g1, const uint32_t* __restrict__ g2,
                                                       uint32_t terms, uint64_t count, uint8_t* __restrict__ ok,
                                                       uint32_t* __restrict__ gt) {
@@ -49,23 +69,10 @@ __global__ void __launch_bounds__(64) k_pairing_check(const uint32_t* __restrict
     int k = 0;
     bool valid = true;
     for (uint32_t j = 0; j < terms; ++j) {
-        const uint32_t* p = g1 + (g * terms + j) * 16;
-        const uint32_t* q = g2 + (g * terms + j) * 32;
-        const bool p0 = all_zero(p, 16), q0 = all_zero(q, 32);
-        bool pv = p0, qv = q0;
         PairingInput e;
-        if (!p0 && canonical_fq(p) && canonical_fq(p + 8)) {
-            e.xp = ld_canon_fq(p);
-            e.yp = ld_canon_fq(p + 8);
-            pv = affine_on_curve<FqOps>({e.xp, e.yp, false});
-        }
-        if (!q0 && canonical_fq(q) && canonical_fq(q + 8) && canonical_fq(q + 16) && canonical_fq(q + 24)) {
-            e.x2 = {ld_canon_fq(q), ld_canon_fq(q + 8)};
-            e.y2 = {ld_canon_fq(q + 16), ld_canon_fq(q + 24)};
-            qv = affine_on_curve<Fq2Ops>({e.x2, e.y2, false});
-        }
-        valid = valid && pv && qv;
-        if (!p0 && !q0) in[k++] = e;  // e(O, Q) = e(P, O) = 1
+        bool finite;
+        valid = load_pair(g1 + (g * terms + j) * 16, g2 + (g * terms + j) * 32, e, finite) && valid;
+        if (finite) in[k++] = e;  // e(O, Q) = e(P, O) = 1
     }
     if (!valid) {  // reported to the host, which fails the call
         ok[g] = 2;
@@ -84,6 +91,57 @@ __global__ void __launch_bounds__(64) k_pairing_check(const uint32_t* __restrict
             }
         }
     }
+}
+
+// One thread per equation e(L) == prod_j e(R_j) (verification.rs:37-103),
+// pairs [offsets[i], offsets[i + 1]) with the left pair first.
+// Product mode (verification.rs:79-103): one Miller product over L and the
+// right pairs with negated G1 points, one final exponentiation, compared with
+// one.  Reference mode (verification.rs:46-53): every pair reduced on its
+// own, the right-hand values multiplied in GT and compared with the left
+// value (A == B as A * conj(B) == 1: reduced values are unitary).
+// Thread t takes equation order[t] (equations ordered by pair count, so a
+// wave's lanes run Miller products of one length).  ok[i] = 2 flags a bad
+// point for the host.
+__global__ void __launch_bounds__(64) k_pairing_equations(const uint32_t* __restrict__ g1,
+                                                          const uint32_t* __restrict__ g2,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          const uint32_t* __restrict__ order, uint64_t count,
+                                                          int reference_mode, uint8_t* __restrict__ ok) {
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const uint64_t i = order[t];
+    const uint32_t first = offsets[i], terms = offsets[i + 1] - first;
+    PairingInput in[PAIRING_MAX_TERMS];
+    bool live[PAIRING_MAX_TERMS];
+    int k = 0;
+    bool valid = true;
+    for (uint32_t j = 0; j < terms; ++j) {
+        PairingInput e;
+        bool finite;
+        valid = load_pair(g1 + size_t(first + j) * 16, g2 + size_t(first + j) * 32, e, finite) && valid;
+        if (!reference_mode) {
+            if (!finite) continue;
+            if (j) e.yp = neg(e.yp);  // bilinearity moves a right-hand pair to the left
+            in[k++] = e;
+        } else {
+            in[j] = e;
+            live[j] = finite;
+        }
+    }
+    if (!valid) {
+        ok[i] = 2;
+        return;
+    }
+    if (!reference_mode) {
+        ok[i] = f12_is_one(final_exponentiation(k ? miller_product(in, k) : f12_one())) ? 1 : 0;
+        return;
+    }
+    const Fq12 left = live[0] ? final_exponentiation(miller_product(&in[0], 1)) : f12_one();
+    Fq12 right = f12_one();
+    for (uint32_t j = 1; j < terms; ++j)
+        if (live[j]) right = f12_mul(right, final_exponentiation(miller_product(&in[j], 1)));
+    ok[i] = f12_is_one(f12_mul(left, f12_conj(right))) ? 1 : 0;
 }
 
 // ---- opening-cache check (rebuilding/machine.rs:357-382) --------------------
@@ -260,6 +318,68 @@ extern "C" int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const
             std::memset(results, 0, count);
             if (gt) std::memset(gt, 0, count * 384);
             throw InvalidArgument(("pairing check " + std::to_string(i) +
+                                   ": a point is not canonical or not on its curve").c_str());
+        }
+        return 0;
+    });
+}
+
+extern "C" int dvgpu_bn254_pairing_equations(void* backend, const uint8_t* g1, const uint8_t* g2,
+                                             const uint32_t* offsets, size_t count, int mode, uint8_t* results,
+                                             float* kernel_ms, char* error, size_t capacity) {
+    Owner* o = as_owner(backend);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(mode == DVGPU_PAIRING_PRODUCT || mode == DVGPU_PAIRING_REFERENCE, "unknown verification mode");
+        require(count <= (size_t(1) << 24), "too many pairing equations");
+        if (!count) return 0;
+        require(offsets && results, "null pairing-equation buffers");
+        require(offsets[0] == 0, "equation offsets must start at zero");
+        for (size_t i = 0; i < count; ++i) {
+            require(offsets[i + 1] > offsets[i], "every equation needs its left pair");
+            require(offsets[i + 1] - offsets[i] <= uint32_t(PAIRING_MAX_TERMS), "an equation has more than 16 pairs");
+        }
+        const size_t n = offsets[count];
+        require(n <= (size_t(1) << 24), "too many pairs");
+        require(g1 && g2, "null pairing buffers");
+        // equations by pair count (stable counting sort), launched in that order
+        std::vector<uint32_t> order(count);
+        {
+            size_t start[PAIRING_MAX_TERMS + 2] = {0};
+            for (size_t i = 0; i < count; ++i) ++start[offsets[i + 1] - offsets[i] + 1];
+            for (int k = 1; k <= PAIRING_MAX_TERMS + 1; ++k) start[k] += start[k - 1];
+            for (size_t i = 0; i < count; ++i) order[start[offsets[i + 1] - offsets[i]]++] = uint32_t(i);
+        }
+        hipStream_t st = o->stream;
+        // one device allocation: g1 | g2 | offsets | order | results
+        const size_t b1 = n * 64, b2 = n * 128, bo = ((count + 1) * 4 + 255) & ~size_t(255),
+                     bq = (count * 4 + 255) & ~size_t(255);
+        auto* base = static_cast<uint8_t*>(o->stage_a.ensure(b1 + b2 + bo + bq + count));
+        HIP_CHECK(hipMemcpyAsync(base, g1, b1, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(base + b1, g2, b2, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(base + b1 + b2, offsets, (count + 1) * 4, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(base + b1 + b2 + bo, order.data(), count * 4, hipMemcpyHostToDevice, st));
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        if (kernel_ms) {
+            for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+            HIP_CHECK(hipEventRecord(ev[0], st));
+        }
+        k_pairing_equations<<<grid_for(count, 64), 64, 0, st>>>(
+            reinterpret_cast<const uint32_t*>(base), reinterpret_cast<const uint32_t*>(base + b1),
+            reinterpret_cast<const uint32_t*>(base + b1 + b2), reinterpret_cast<const uint32_t*>(base + b1 + b2 + bo),
+            count, mode == DVGPU_PAIRING_REFERENCE, base + b1 + b2 + bo + bq);
+        HIP_CHECK_LAUNCH();
+        if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
+        HIP_CHECK(hipMemcpyAsync(results, base + b1 + b2 + bo + bq, count, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        if (kernel_ms) {
+            HIP_CHECK(hipEventElapsedTime(kernel_ms, ev[0], ev[1]));
+            for (auto& e : ev) (void)hipEventDestroy(e);
+        }
+        const void* bad = std::memchr(results, 2, count);
+        if (bad) {
+            const size_t i = static_cast<const uint8_t*>(bad) - results;
+            std::memset(results, 0, count);
+            throw InvalidArgument(("pairing equation " + std::to_string(i) +
                                    ": a point is not canonical or not on its curve").c_str());
         }
         return 0;

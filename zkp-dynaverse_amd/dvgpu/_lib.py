@@ -240,6 +240,8 @@ SIGNATURES = {
     # dynaverse_pairing.h
     "dvgpu_bn254_pairing_check": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_void_p, c_void_p,
                                        ctypes.POINTER(ctypes.c_float), *_E]),
+    "dvgpu_bn254_pairing_equations": (_I, [c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, ctypes.c_int,
+                                           c_void_p, ctypes.POINTER(ctypes.c_float), *_E]),
     # dynaverse_poly.h
     "dvgpu_poly_mul": (_I, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t,
                             ctypes.POINTER(c_size_t), *_E]),

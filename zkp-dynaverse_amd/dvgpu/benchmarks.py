@@ -392,6 +392,77 @@ def pairing_cache_check(backend: HipBackend, log_n: int = 16, caches: int = 3, r
     return out
 
 
+def pairing_equations_mix(backend: HipBackend, distinct: int = 8, equations: int = 65536, reps: int = 3,
+                          log=print) -> Dict:
+    """The verifiers' pairing equations (verification.rs PairingChecks::equal)
+    through dvgpu_bn254_pairing_equations in both modes: the shapes of
+    dynamo.rs:423-429 (5 and 2 pairs), dynaverse.rs:1056-1058 and
+    consistency.rs:101 (3 pairs), dynaverse.rs:1083 and machine.rs:335
+    (2 pairs); `distinct` instances of each, built to hold from scalars
+    (ab == sum c_j d_j) with G1/G2 points from the fixed-base kernels, each
+    with a failing twin (a + 1), the set tiled to about `equations`
+    equations in one call (one device thread per equation)."""
+    from .codec import FR_MODULUS, encode_fr
+    from .verification import VerificationMode, pairing_equations_encoded
+    r = FR_MODULUS
+    rng = ChaCha20Stream(2031)
+
+    def rnd():
+        return int.from_bytes(rng.uniform_fr_array(1)[0].tobytes(), "little") or 1
+
+    shapes = [(4, False), (1, False), (2, False), (1, True), (2, True), (1, True)]   # (right pairs, b = 1)
+    g1s, g2s, sizes, expect = [], [], [], []
+    for right, unit_b in shapes:
+        for _ in range(distinct):
+            b = 1 if unit_b else rnd()
+            cd = [(rnd(), rnd()) for _ in range(right)]
+            a = sum(c * d for c, d in cd) % r * pow(b, -1, r) % r
+            for holds in (True, False):
+                g1s += [a if holds else (a + 1) % r] + [c for c, _ in cd]
+                g2s += [b] + [d for _, d in cd]
+                sizes.append(1 + right)
+                expect.append(holds)
+    g1 = backend.fixed_base_encoded(b"".join(encode_fr(v) for v in g1s), len(g1s), False)
+    g2 = backend.fixed_base_encoded(b"".join(encode_fr(v) for v in g2s), len(g2s), True)
+    tile = -(-equations // len(sizes))
+    offsets, pairs = [0], 0
+    for _ in range(tile):
+        for k in sizes:
+            pairs += k
+            offsets.append(pairs)
+    g1, g2, expect = g1 * tile, g2 * tile, expect * tile
+    want = bytes(int(e) for e in expect)
+    count = len(expect)
+    out = {"equations": count, "pairs": pairs, "shapes_pairs": [1 + s for s, _ in shapes]}
+    for mode in (VerificationMode.PRODUCT, VerificationMode.REFERENCE):
+        times, walls, ok = [], [], True
+        for _ in range(reps):
+            a = time.perf_counter()
+            res, ms = pairing_equations_encoded(backend, g1, g2, offsets, mode)
+            walls.append((time.perf_counter() - a) * 1e3)
+            times.append(ms)
+            ok = ok and res == want
+        kms = sorted(times)[len(times) // 2]
+        name = mode.name.lower()
+        out[name] = {"kernel_ms": round(kms, 3), "wall_ms": round(sorted(walls)[len(walls) // 2], 3),
+                     "equations_per_s": round(count / (kms * 1e-3)), "decisions_exact": bool(ok)}
+        log(f"pairing equations ({name} mode): {count} equations / {pairs} pairs in {kms:.1f} ms "
+            f"({out[name]['equations_per_s']} eq/s), decisions exact={ok}")
+    # One verification's worth (the first 16 equations: 5-pair shapes and
+    # their twins) in one call: the latency of a lane's Miller products and
+    # final exponentiation, whatever the batch size.
+    small = offsets[16]
+    times = []
+    for _ in range(reps):
+        res, ms = pairing_equations_encoded(backend, g1[:64 * small], g2[:128 * small], offsets[:17],
+                                            VerificationMode.PRODUCT)
+        times.append(ms)
+    out["small_call_16_equations"] = {"pairs": small, "kernel_ms": round(sorted(times)[len(times) // 2], 3),
+                                      "decisions_exact": res == want[:16]}
+    log(f"pairing equations, one call of 16 five-pair equations: {out['small_call_16_equations']['kernel_ms']} ms")
+    return out
+
+
 # ---- polynomial tools (domain/fast.rs) ---------------------------------------
 
 def poly_tree(backend: HipBackend, log_k: int = 12, reps: int = 3, log=print) -> Dict:
@@ -553,8 +624,8 @@ def main(argv=None):
     import argparse
     import json
     ap = argparse.ArgumentParser(description=main.__doc__)
-    ap.add_argument("workload", choices=["opening_cache", "pairing_cache_check", "poly_tree", "row_figures",
-                                         "batched_launch", "incremental", "served_stream"])
+    ap.add_argument("workload", choices=["opening_cache", "pairing_cache_check", "pairing_equations", "poly_tree",
+                                         "row_figures", "batched_launch", "incremental", "served_stream"])
     ap.add_argument("--log-n", type=int, default=16)
     ap.add_argument("--rows", type=int, default=3)
     ap.add_argument("--reps", type=int, default=3)
@@ -565,6 +636,8 @@ def main(argv=None):
             res = opening_cache(b, a.log_n, a.rows, a.reps)
         elif a.workload == "pairing_cache_check":
             res = pairing_cache_check(b, a.log_n, a.rows, a.reps)
+        elif a.workload == "pairing_equations":
+            res = pairing_equations_mix(b, equations=1 << a.log_n, reps=a.reps)
         elif a.workload == "poly_tree":
             res = poly_tree(b, a.log_n, a.reps)
         elif a.workload == "row_figures":
