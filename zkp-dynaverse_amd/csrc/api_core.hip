@@ -112,13 +112,12 @@ int msm_entry(Owner* o, const uint8_t* points, const uint8_t* scalars, size_t le
         MsmConfig cfg = o->msm_config;
         if (!cfg.window_bits) cfg.window_bits = msm_window_for(length, &o->window_table);
         try {
-            uint32_t* window_words = o->msm.windows.as<uint32_t>(
-                size_t(msm_plan(length, cfg).windows) * PointLayout<F>::XYZZ_WORDS);
-            plan = msm_enqueue<F>(mont, dsc, length, o->msm, o->stream, cfg, window_words);
+            const size_t room = msm_plan(length, cfg).outputs;
+            uint32_t* window_words = o->msm.windows.as<uint32_t>(room * PointLayout<F>::XYZZ_WORDS);
+            plan = msm_enqueue<F>(mont, dsc, length, o->msm, o->stream, cfg, window_words, room);
             o->msm.phase_events = nullptr;
-            uint32_t* host = o->msm.host_windows.as<uint32_t>(size_t(plan.windows) * PointLayout<F>::XYZZ_WORDS);
-            HIP_CHECK(hipMemcpyAsync(host, window_words,
-                                     size_t(plan.windows) * PointLayout<F>::XYZZ_WORDS * 4,
+            uint32_t* host = o->msm.host_windows.as<uint32_t>(plan.outputs * PointLayout<F>::XYZZ_WORDS);
+            HIP_CHECK(hipMemcpyAsync(host, window_words, plan.outputs * PointLayout<F>::XYZZ_WORDS * 4,
                                      hipMemcpyDeviceToHost, o->stream));
             HIP_CHECK(hipStreamSynchronize(o->stream));
         } catch (...) {

@@ -35,20 +35,19 @@ int msm_device(Owner* o, const void* points, const void* scalars, size_t n, int 
     cfg.window_bits = window_bits ? window_bits : msm_window_for(n, &o->window_table);
     Events ev(phase_ms != nullptr);
     const MsmPlan p0 = msm_plan(n, cfg);
-    uint32_t* window_words = o->msm.windows.as<uint32_t>(size_t(p0.windows) * XW);
+    uint32_t* window_words = o->msm.windows.as<uint32_t>(p0.outputs * XW);
     o->msm.phase_events = phase_ms ? ev.e : nullptr;
     MsmPlan plan;
     try {
         plan = msm_enqueue<F>(static_cast<const uint32_t*>(points), static_cast<const uint32_t*>(scalars),
-                              n, o->msm, o->stream, cfg, window_words);
+                              n, o->msm, o->stream, cfg, window_words, p0.outputs);
     } catch (...) {
         o->msm.phase_events = nullptr;
         throw;
     }
     o->msm.phase_events = nullptr;
-    uint32_t* host = o->msm.host_windows.as<uint32_t>(size_t(plan.windows) * XW);
-    HIP_CHECK(hipMemcpyAsync(host, window_words, size_t(plan.windows) * XW * 4, hipMemcpyDeviceToHost,
-                             o->stream));
+    uint32_t* host = o->msm.host_windows.as<uint32_t>(plan.outputs * XW);
+    HIP_CHECK(hipMemcpyAsync(host, window_words, plan.outputs * XW * 4, hipMemcpyDeviceToHost, o->stream));
     HIP_CHECK(hipStreamSynchronize(o->stream));
     if (phase_ms)
         for (int k = 0; k + 1 < MSM_PHASE_EVENTS; ++k)

@@ -74,6 +74,7 @@ struct MsmDesc {
     uint32_t hist_base;        // first counter of its histograms (one bucket set per block)
     uint32_t chunk_shift;      // log2 of the points per histogram block
     uint32_t chunk;            // buckets per first-level reduction thread
+    uint32_t plane_base;       // first bit-plane sum of this MSM (plane mode)
 };
 
 // Digits are ranked per block of 2^chunk_shift points and one window in LDS
@@ -94,6 +95,16 @@ struct MsmBatchPlan {
     // error word (total[3]) and read as point 0; only the finalize entry
     // point, which knows the point count, sets it.
     uint32_t point_limit = 0xffffffffu;
+    // Plane mode (single MSMs whose windows are combined on the host): the
+    // bucket reduction leaves, per window and bit k of the bucket weight, the
+    // sum of the buckets whose weight has bit k set (total_planes values, each
+    // summed in plane_slices parts first) instead of one value per window.
+    // Chunk-plane mode (the same callers, larger bucket sets): the chunk
+    // running sums keep their weighted and plain sums and the reduction
+    // leaves, per window, the sum of the weighted ones and the bit-plane sums
+    // of the plain ones over the chunk index (total_planes values).
+    bool planes = false, chunk_planes = false;
+    uint32_t total_planes = 0, plane_slices = 1;
 };
 
 MsmBatchPlan msm_batch_plan(const std::vector<MsmItem>& items, const MsmConfig& cfg,
@@ -128,7 +139,8 @@ void msm_batch_prepare(const MsmBatchPlan& plan, int xyzz_words, MsmWorkspace& w
 
 // points: device Montgomery affine words; point_index: optional indices
 // into `points` (sparse MSMs over resident bases); scalars: canonical words.
-// Leaves plan.total_windows XYZZ values in window_out.  Launches only
+// Leaves plan.total_windows XYZZ values in window_out (plan.total_planes
+// bit-plane sums in plane mode).  Launches only
 // kernels (no memset, no allocation), so it can be captured in a hipGraph.
 template <class F>
 void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, const uint32_t* scalars,
@@ -155,13 +167,18 @@ struct MsmPlan {
     int c, windows;
     uint32_t buckets;
     size_t n;
+    int planes = 0;      // plane sums per window (the single-MSM path's window output)
+    int chunk = 0;       // chunk-plane mode: buckets per chunk (planes = [U, Q_0, Q_1, ...]); 0: bucket planes
+    size_t outputs = 0;  // XYZZ values msm_enqueue leaves in window_out: windows * planes
 };
 int msm_default_window(size_t n);
 MsmPlan msm_plan(size_t n, const MsmConfig& cfg);
 
+// One MSM; window_out holds room for `out_capacity` XYZZ values (at least
+// msm_plan(n, cfg).outputs) and receives the returned plan's outputs.
 template <class F>
 MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, MsmWorkspace& ws,
-                    hipStream_t stream, const MsmConfig& cfg, uint32_t* window_out);
+                    hipStream_t stream, const MsmConfig& cfg, uint32_t* window_out, size_t out_capacity);
 
 template <class F>
 Affine<F> msm_combine_windows(const uint32_t* window_words, const MsmPlan& plan);

@@ -24,7 +24,7 @@ void device_fill_words(uint32_t* p, size_t words, uint32_t value, hipStream_t st
                                        MsmWorkspace&, hipStream_t, uint32_t*);                                \
     template void msm_batch_combine<F>(const uint32_t*, const MsmBatchPlan&, Affine<F>*);                     \
     template MsmPlan msm_enqueue<F>(const uint32_t*, const uint32_t*, size_t, MsmWorkspace&, hipStream_t,    \
-                                    const MsmConfig&, uint32_t*);                                             \
+                                    const MsmConfig&, uint32_t*, size_t);                                     \
     template Affine<F> msm_run<F>(const uint32_t*, const uint32_t*, size_t, MsmWorkspace&, hipStream_t,      \
                                   const MsmConfig&);                                                          \
     template Affine<F> msm_combine_windows<F>(const uint32_t*, const MsmPlan&);                               \

@@ -539,7 +539,10 @@ __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* __restrict_
 // One thread per chunk of L buckets of one window:
 //   out = sum_i (i + 1) S_{jL+i} + (jL) * sum_i S_{jL+i}
 // where S_b is the sum of bucket b's task partials.
-template <class F>
+// SPLIT (chunk-plane mode): write U = sum_i (i + 1) S_{jL+i} and
+// T = sum_i S_{jL+i} as out[2g], out[2g + 1] instead, leaving the offset
+// term jL * T to the chunk planes.
+template <class F, bool SPLIT = false>
 __global__ void __launch_bounds__(256) k_reduce_chunks(const MsmDesc* __restrict__ descs, uint32_t m,
                                                        uint32_t total_chunks, const uint32_t* __restrict__ counts,
                                                        const uint2* __restrict__ offsets,
@@ -558,6 +561,11 @@ __global__ void __launch_bounds__(256) k_reduce_chunks(const MsmDesc* __restrict
         if (cnt)  // the bucket total sits in its first task's partial (k_bucket_sums, k_heavy)
             run = xyzz_add<F>(run, Mem<F>::load_xyzz(partials + size_t(offsets[wb].y) * PointLayout<F>::XYZZ_WORDS));
         acc = xyzz_add<F>(acc, run);
+    }
+    if constexpr (SPLIT) {
+        Mem<F>::store_xyzz(out + size_t(2 * g) * PointLayout<F>::XYZZ_WORDS, acc);
+        Mem<F>::store_xyzz(out + size_t(2 * g + 1) * PointLayout<F>::XYZZ_WORDS, run);
+        return;
     }
     const uint32_t offset = j * uint32_t(chunk);
     if (offset) {
@@ -599,6 +607,164 @@ __global__ void __launch_bounds__(T)
         __syncthreads();
     }
     if (threadIdx.x == 0) Mem<F>::store_xyzz(out + size_t(gw) * XW, acc);
+}
+
+// ---- plane mode: bit-plane sums of the bucket weights ----------------------
+//
+// For one window, sum_i (i + 1) S_i = sum_k 2^k P_k with P_k the sum of the
+// bucket sums S_i whose weight v = i + 1 has bit k set.  Each P_k is a plain
+// parallel sum (depth ~log of the buckets), where the running-sum form is a
+// chain of two adds per bucket plus a scalar multiplication; the host forms
+// sum_k 2^k P_k.  The weights with bit k set are enumerated directly:
+// v_k(j) = ((j >> k) << (k + 1)) | 2^k | (j & (2^k - 1)), increasing in j.
+constexpr int PLANE_THREADS = 256;
+
+__host__ __device__ __forceinline__ uint32_t plane_bits(uint32_t buckets) {
+    uint32_t b = 0;
+    while (b < 32 && (buckets >> b)) ++b;
+    return b;  // bit length of the largest weight
+}
+
+// Weights in [1, buckets] with bit k set.
+__host__ __device__ __forceinline__ uint32_t plane_count(uint32_t buckets, uint32_t k) {
+    const uint64_t period = uint64_t(2) << k, half = uint64_t(1) << k, top = uint64_t(buckets) + 1;
+    const uint64_t rest = top % period;
+    return uint32_t(top / period * half + (rest > half ? rest - half : 0));
+}
+
+// Block g: slice g % slices of plane g / slices (planes numbered per MSM
+// from plane_base, window-major).  out: one XYZZ partial per block.
+template <class F>
+__global__ void __launch_bounds__(PLANE_THREADS)
+    k_plane_sums(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t slices, const uint32_t* __restrict__ counts,
+                 const uint2* __restrict__ offsets, const uint32_t* __restrict__ partials, uint32_t* __restrict__ out) {
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
+    __shared__ uint4 lds[PLANE_THREADS * XW / 4];
+    uint32_t* slot = reinterpret_cast<uint32_t*>(lds) + threadIdx.x * XW;
+    const uint32_t g = blockIdx.x, plane = g / slices, s = g % slices;
+    const MsmDesc d = descs[find_desc<&MsmDesc::plane_base>(descs, m, plane)];
+    const uint32_t pb = plane_bits(d.buckets);
+    const uint32_t local = plane - d.plane_base, w = local / pb, k = local % pb;
+    const uint32_t count = plane_count(d.buckets, k);
+    const uint32_t lo = uint32_t(uint64_t(count) * s / slices), hi = uint32_t(uint64_t(count) * (s + 1) / slices);
+    const size_t wbase = size_t(d.bucket_base) + size_t(w) * d.buckets;
+    Xyzz<F> acc = xyzz_inf<F>();
+    for (uint32_t j = lo + threadIdx.x; j < hi; j += PLANE_THREADS) {
+        const uint32_t v = ((j >> k) << (k + 1)) | (1u << k) | (j & ((1u << k) - 1u));
+        const size_t wb = wbase + (v - 1);
+        if (counts[wb])  // the bucket total sits in its first task's partial (k_bucket_sums, k_heavy)
+            acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(partials + size_t(offsets[wb].y) * XW));
+    }
+    Mem<F>::store_xyzz(slot, acc);
+    __syncthreads();
+    for (uint32_t h = PLANE_THREADS / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(slot + h * XW));
+            Mem<F>::store_xyzz(slot, acc);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Mem<F>::store_xyzz(out + size_t(g) * XW, acc);
+}
+
+// One thread per plane: the sum of its slices.
+template <class F>
+__global__ void __launch_bounds__(64) k_plane_final(const uint32_t* __restrict__ parts, uint32_t planes,
+                                                    uint32_t slices, uint32_t* __restrict__ out) {
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= planes) return;
+    Xyzz<F> acc = Mem<F>::load_xyzz(parts + size_t(p) * slices * XW);
+    for (uint32_t s = 1; s < slices; ++s) acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(parts + (size_t(p) * slices + s) * XW));
+    Mem<F>::store_xyzz(out + size_t(p) * XW, acc);
+}
+
+// Plane mode does about popcount-many adds per bucket where the running sums
+// do two, so it pays only while its launch fits one wave round of the chip
+// (latency-bound chains); larger bucket sets keep the running-sum reduction.
+constexpr uint32_t PLANE_MAX_THREADS = 131072;  // 256 CUs x 4 SIMDs x 2 waves x 64 lanes
+
+// Switch a plan to plane mode (before msm_batch_prepare) when that pays:
+// plane bases, and slices so that a thread adds at most about 8 buckets
+// before the LDS tree.  Returns whether the plan is in plane mode.
+inline bool msm_plan_planes(MsmBatchPlan& plan) {
+    uint32_t total = 0, widest = 1;
+    for (const MsmDesc& d : plan.descs) {
+        total += d.windows * plane_bits(d.buckets);
+        widest = std::max(widest, plane_count(d.buckets, 0));
+    }
+    const uint32_t slices = std::min<uint32_t>(16, std::max<uint32_t>(1, widest / (PLANE_THREADS * 8)));
+    if (uint64_t(total) * slices * PLANE_THREADS > PLANE_MAX_THREADS) return false;
+    total = 0;
+    for (MsmDesc& d : plan.descs) {
+        d.plane_base = total;
+        total += d.windows * plane_bits(d.buckets);
+    }
+    plan.planes = true;
+    plan.total_planes = total;
+    plan.plane_slices = slices;
+    return true;
+}
+
+// ---- chunk-plane mode: bit planes over the chunk index --------------------
+//
+// For larger bucket sets: the chunk running sums (k_reduce_chunks<SPLIT>)
+// give U_j and T_j per chunk j of L buckets, and the window value is
+// sum_j U_j + L * sum_k 2^k Q_k with Q_k the sum of the T_j whose index j has
+// bit k set.  Per window: plane 0 = sum_j U_j, plane 1 + k = Q_k, for
+// k < log2(J), J = buckets / L chunks; the host forms the window value.
+__host__ __device__ __forceinline__ uint32_t chunk_plane_count(uint32_t buckets, uint32_t chunk) {
+    return 1 + plane_bits(buckets / chunk) - 1;  // 1 + log2(J), J a power of two
+}
+
+template <class F>
+__global__ void __launch_bounds__(PLANE_THREADS)
+    k_chunk_planes(const MsmDesc* __restrict__ descs, uint32_t m, const uint32_t* __restrict__ chunk_vals,
+                   uint32_t* __restrict__ out) {
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
+    __shared__ uint4 lds[PLANE_THREADS * XW / 4];
+    uint32_t* slot = reinterpret_cast<uint32_t*>(lds) + threadIdx.x * XW;
+    const uint32_t plane = blockIdx.x;
+    const MsmDesc d = descs[find_desc<&MsmDesc::plane_base>(descs, m, plane)];
+    const uint32_t J = d.buckets / d.chunk, qb = chunk_plane_count(d.buckets, d.chunk);
+    const uint32_t local = plane - d.plane_base, w = local / qb, q = local % qb;
+    const size_t base = size_t(d.chunk_base) + size_t(w) * J;  // chunk index of (window w, j = 0)
+    Xyzz<F> acc = xyzz_inf<F>();
+    if (q == 0) {
+        for (uint32_t j = threadIdx.x; j < J; j += PLANE_THREADS)
+            acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(chunk_vals + (2 * (base + j)) * XW));
+    } else {
+        const uint32_t k = q - 1;
+        for (uint32_t t = threadIdx.x; t < J / 2; t += PLANE_THREADS) {
+            const uint32_t j = ((t >> k) << (k + 1)) | (1u << k) | (t & ((1u << k) - 1u));
+            acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(chunk_vals + (2 * (base + j) + 1) * XW));
+        }
+    }
+    Mem<F>::store_xyzz(slot, acc);
+    __syncthreads();
+    for (uint32_t h = PLANE_THREADS / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(slot + h * XW));
+            Mem<F>::store_xyzz(slot, acc);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Mem<F>::store_xyzz(out + size_t(plane) * XW, acc);
+}
+
+// Switch a plan to chunk-plane mode (before msm_batch_prepare); every
+// descriptor's chunk must divide its bucket count (powers of two).
+inline void msm_plan_chunk_planes(MsmBatchPlan& plan) {
+    uint32_t total = 0;
+    for (MsmDesc& d : plan.descs) {
+        require(d.chunk && d.buckets % d.chunk == 0 && ((d.buckets / d.chunk) & (d.buckets / d.chunk - 1)) == 0,
+                "chunk planes need power-of-two chunks per window");
+        d.plane_base = total;
+        total += d.windows * chunk_plane_count(d.buckets, d.chunk);
+    }
+    plan.chunk_planes = true;
+    plan.total_planes = total;
+    plan.plane_slices = 1;
 }
 
 // Horner over one MSM's window values and conversion to affine, one lane per
@@ -779,7 +945,8 @@ inline void msm_batch_prepare_impl(const MsmBatchPlan& plan, int xw, MsmWorkspac
     ws.tasks.as<uint2>(plan.max_tasks);
     ws.partials.as<uint32_t>(plan.max_tasks * xw);
     ws.heavy.as<uint32_t>(plan.max_heavy);
-    ws.chunks.as<uint32_t>(size_t(plan.total_chunks) * xw);
+    ws.chunks.as<uint32_t>(std::max<size_t>(size_t(plan.total_chunks) * (plan.chunk_planes ? 2 : 1),
+                                            size_t(plan.total_planes) * plan.plane_slices) * xw);
     if (plan.lds_digits) ws.hist.as<uint32_t>(plan.total_hist);
     // Earlier work on this stream may still read the pinned staging area.
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -798,6 +965,8 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
     // The buffers were sized by msm_batch_prepare; check instead of growing.
     require(ws.counts.bytes >= size_t(plan.total_buckets) * 4 && ws.sorted.bytes >= size_t(plan.total_entries) * 4 &&
                 ws.partials.bytes >= plan.max_tasks * XW * 4 && ws.chunks.bytes >= size_t(plan.total_chunks) * XW * 4 &&
+                ws.chunks.bytes >= size_t(plan.total_planes) * plan.plane_slices * XW * 4 &&
+                ws.chunks.bytes >= size_t(plan.total_chunks) * (plan.chunk_planes ? 2 : 1) * XW * 4 &&
                 ws.descs.bytes >= m * sizeof(MsmDesc) && ws.heavy.bytes >= plan.max_heavy * 4 &&
                 ws.block_sums.bytes >= tiles * sizeof(uint2),
             "MSM workspace not prepared for this batch");
@@ -865,6 +1034,27 @@ void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, cons
                                                                                      offsets, tp, partials);
     k_bucket_sums<F><<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(counts, offsets, plan.total_buckets, tp,
                                                                            partials);
+    if (plan.planes) {
+        k_plane_sums<F><<<plan.total_planes * plan.plane_slices, PLANE_THREADS, 0, stream>>>(
+            descs, m, plan.plane_slices, counts, offsets, partials, chunk_vals);
+        HIP_CHECK_LAUNCH();
+        mark(6);
+        k_plane_final<F><<<grid_for(plan.total_planes, 64), 64, 0, stream>>>(chunk_vals, plan.total_planes,
+                                                                               plan.plane_slices, window_out);
+        HIP_CHECK_LAUNCH();
+        mark(7);
+        return;
+    }
+    if (plan.chunk_planes) {
+        k_reduce_chunks<F, true><<<grid_for(plan.total_chunks, 256), 256, 0, stream>>>(
+            descs, m, plan.total_chunks, counts, offsets, partials, chunk_vals);
+        HIP_CHECK_LAUNCH();
+        mark(6);
+        k_chunk_planes<F><<<plan.total_planes, PLANE_THREADS, 0, stream>>>(descs, m, chunk_vals, window_out);
+        HIP_CHECK_LAUNCH();
+        mark(7);
+        return;
+    }
     k_reduce_chunks<F><<<grid_for(plan.total_chunks, 256), 256, 0, stream>>>(
         descs, m, plan.total_chunks, counts, offsets, partials, chunk_vals);
     HIP_CHECK_LAUNCH();
@@ -936,15 +1126,21 @@ inline MsmPlan msm_plan_impl(size_t n, const MsmConfig& cfg) {
     p.windows = (255 + p.c - 1) / p.c;
     p.buckets = 1u << (p.c - 1);
     p.n = n;
+    p.planes = int(plane_bits(p.buckets));
+    p.outputs = size_t(p.windows) * size_t(p.planes);  // room for either reduction's output
     return p;
 }
 
 template <class F>
 MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, MsmWorkspace& ws,
-                    hipStream_t stream, const MsmConfig& cfg, uint32_t* window_out) {
+                    hipStream_t stream, const MsmConfig& cfg, uint32_t* window_out, size_t out_capacity) {
     MsmConfig c = cfg;
     if constexpr (std::is_same<F, Fq2Ops>::value) c.chunk = std::min(c.chunk, 2);  // G2: short reduction chains
-    const MsmBatchPlan plan = msm_batch_plan_impl({MsmItem{n, 0, 0, cfg.window_bits}}, c, nullptr);
+    MsmBatchPlan plan = msm_batch_plan_impl({MsmItem{n, 0, 0, cfg.window_bits}}, c, nullptr);
+    // The windows are combined on the host: bit planes of the bucket weights
+    // for small bucket sets, of the chunk index otherwise.
+    if (!msm_plan_planes(plan)) msm_plan_chunk_planes(plan);
+    require(plan.total_planes <= out_capacity, "MSM window buffer too small");
     msm_batch_prepare_impl(plan, PointLayout<F>::XYZZ_WORDS, ws, stream);
     msm_batch_enqueue<F>(points, nullptr, scalars, plan, ws, stream, window_out);
     MsmPlan p;
@@ -952,16 +1148,49 @@ MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, M
     p.windows = int(plan.descs[0].windows);
     p.buckets = plan.descs[0].buckets;
     p.n = n;
+    p.planes = plan.planes ? int(plane_bits(p.buckets)) : int(chunk_plane_count(p.buckets, plan.descs[0].chunk));
+    p.chunk = plan.chunk_planes ? int(plan.descs[0].chunk) : 0;
+    p.outputs = plan.total_planes;
     return p;
 }
 
+// words: plan.outputs values from msm_enqueue — one per window, or in plane
+// mode (plan.planes > 0) the bit-plane sums, window-major: then each window
+// value sum_k 2^k P_k on its own pool task (Horner over the planes), and
+// Horner over the windows.
 template <class F>
 Affine<F> msm_combine_windows(const uint32_t* words, const MsmPlan& plan) {
-    MsmDesc d{};
-    d.c = uint32_t(plan.c);
-    d.windows = uint32_t(plan.windows);
-    d.window_base = 0;
-    return combine_one<F>(words, d);
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
+    if (!plan.planes) {
+        MsmDesc d{};
+        d.c = uint32_t(plan.c);
+        d.windows = uint32_t(plan.windows);
+        d.window_base = 0;
+        return combine_one<F>(words, d);
+    }
+    require(plan.outputs == size_t(plan.windows) * size_t(plan.planes), "MSM plane count mismatch");
+    std::vector<Xyzz<F>> win(size_t(plan.windows));
+    HostPool::get().parallel_for(win.size(), [&](size_t w) {
+        const uint32_t* base = words + w * size_t(plan.planes) * XW;
+        if (plan.chunk) {  // U + L * sum_k 2^k Q_k, planes [U, Q_0, Q_1, ...]
+            Xyzz<F> acc = xyzz_inf<F>();
+            for (int k = plan.planes - 1; k >= 1; --k)
+                acc = xyzz_add<F>(xyzz_dbl<F>(acc), HostMem<F>::load_xyzz(base + size_t(k) * XW));
+            for (int l = plan.chunk; l > 1; l >>= 1) acc = xyzz_dbl<F>(acc);
+            win[w] = xyzz_add<F>(acc, HostMem<F>::load_xyzz(base));
+            return;
+        }
+        Xyzz<F> acc = HostMem<F>::load_xyzz(base + size_t(plan.planes - 1) * XW);
+        for (int k = plan.planes - 2; k >= 0; --k)
+            acc = xyzz_add<F>(xyzz_dbl<F>(acc), HostMem<F>::load_xyzz(base + size_t(k) * XW));
+        win[w] = acc;
+    });
+    Xyzz<F> acc = win.back();
+    for (int w = plan.windows - 2; w >= 0; --w) {
+        for (int k = 0; k < plan.c; ++k) acc = xyzz_dbl<F>(acc);
+        acc = xyzz_add<F>(acc, win[size_t(w)]);
+    }
+    return xyzz_to_affine<F>(acc);
 }
 
 template <class F>
@@ -969,11 +1198,11 @@ Affine<F> msm_run(const uint32_t* points, const uint32_t* scalars, size_t n, Msm
                   hipStream_t stream, const MsmConfig& cfg) {
     constexpr int XW = PointLayout<F>::XYZZ_WORDS;
     const MsmPlan plan0 = msm_plan_impl(n, cfg);
-    uint32_t* window_words = ws.windows.as<uint32_t>(size_t(plan0.windows) * XW);
-    const MsmPlan plan = msm_enqueue<F>(points, scalars, n, ws, stream, cfg, window_words);
-    uint32_t* host = ws.host_windows.as<uint32_t>(size_t(plan.windows) * XW);
-    HIP_CHECK(hipMemcpyAsync(host, window_words, size_t(plan.windows) * XW * sizeof(uint32_t),
-                             hipMemcpyDeviceToHost, stream));
+    uint32_t* window_words = ws.windows.as<uint32_t>(plan0.outputs * XW);
+    const MsmPlan plan = msm_enqueue<F>(points, scalars, n, ws, stream, cfg, window_words, plan0.outputs);
+    uint32_t* host = ws.host_windows.as<uint32_t>(plan.outputs * XW);
+    HIP_CHECK(hipMemcpyAsync(host, window_words, plan.outputs * XW * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             stream));
     HIP_CHECK(hipStreamSynchronize(stream));
     return msm_combine_windows<F>(host, plan);
 }
