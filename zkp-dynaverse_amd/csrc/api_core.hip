@@ -21,6 +21,7 @@ Owner::Owner() {
     msm.attach(&quota);
     ntt.attach(&quota);
     ntt_batch.attach(&quota);
+    ntt_batch_inv.attach(&quota);
     domain.fwd.quota = domain.inv.quota = &quota;
     stage_a.quota = stage_b.quota = stage_c.quota = &quota;
 }
@@ -39,6 +40,8 @@ void Owner::release_device() noexcept {
     msm.release();
     ntt.release();
     ntt_batch.release();
+    ntt_batch_inv.release();
+    poly_stage.release();
     domain.release();
     stage_a.release();
     stage_b.release();

@@ -409,6 +409,9 @@ void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& ent
         }
         pack(passes ? scales[x.inverse][L] : fp_one<FrParams>(), e.scale);
     }
+    // Growing a device buffer frees one that queued work may still use.
+    if (std::max<uint64_t>(work, 1) * 32 > plan.work.bytes || descs.size() * sizeof(NttBatchDesc) > plan.descs.bytes)
+        HIP_CHECK(hipStreamSynchronize(stream));
     uint32_t* wp = plan.work.as<uint32_t>(std::max<uint64_t>(work, 1) * 8);
     for (auto& e : descs) e.work = wp + reinterpret_cast<uintptr_t>(e.work) * 8;
     plan.count = uint32_t(entries.size());
@@ -418,12 +421,18 @@ void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& ent
         plan.pass_blocks[p] = blocks[p];
         plan.pass_threads[p] = pass_threads(tile_max[p]);
     }
-    // Earlier work on this stream may still read the pinned descriptors.
-    HIP_CHECK(hipStreamSynchronize(stream));
+    // The previous upload from the pinned descriptors may still be pending.
+    if (plan.upload_pending) {
+        HIP_CHECK(hipEventSynchronize(plan.uploaded));
+        plan.upload_pending = false;
+    }
     auto* host = plan.host_descs.as<NttBatchDesc>(descs.size());
     std::memcpy(host, descs.data(), descs.size() * sizeof(NttBatchDesc));
     auto* dev = plan.descs.as<NttBatchDesc>(descs.size());
     HIP_CHECK(hipMemcpyAsync(dev, host, descs.size() * sizeof(NttBatchDesc), hipMemcpyHostToDevice, stream));
+    if (!plan.uploaded) HIP_CHECK(hipEventCreateWithFlags(&plan.uploaded, hipEventDisableTiming));
+    HIP_CHECK(hipEventRecord(plan.uploaded, stream));
+    plan.upload_pending = true;
 }
 
 void ntt_batch_enqueue(const NttDomain& d, const NttBatchPlan& plan, hipStream_t stream) {

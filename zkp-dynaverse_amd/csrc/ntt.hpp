@@ -68,16 +68,24 @@ struct NttBatchPlan {
     uint32_t pass_threads[NTT_BATCH_MAX_PASSES] = {0};
     DeviceBuffer descs, work;
     HostBuffer host_descs;
+    // Recorded after the descriptor upload: the next prepare waits for it
+    // before rewriting host_descs (instead of for the whole stream).
+    hipEvent_t uploaded = nullptr;
+    bool upload_pending = false;
     void attach(Quota* q) { descs.quota = work.quota = q; }
     void release() {
+        if (uploaded && upload_pending) (void)hipEventSynchronize(uploaded);
+        if (uploaded) (void)hipEventDestroy(uploaded);
+        uploaded = nullptr;
+        upload_pending = false;
         descs.release();
         work.release();
         host_descs.release();
     }
 };
 
-// Plan, size the workspace and upload the descriptors (synchronises the
-// stream first).  Must precede ntt_batch_enqueue; capture-safe afterwards.
+// Plan, size the workspace and upload the descriptors (waits for this
+// plan's previous descriptor upload first).  Must precede ntt_batch_enqueue; capture-safe afterwards.
 void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& entries, NttBatchPlan& plan,
                        hipStream_t stream);
 void ntt_batch_enqueue(const NttDomain& d, const NttBatchPlan& plan, hipStream_t stream);

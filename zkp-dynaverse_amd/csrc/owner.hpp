@@ -36,6 +36,8 @@ struct Owner {
     NttWorkspace ntt;
     NttDomain domain;
     NttBatchPlan ntt_batch;                  // fused NTT batches (dynaverse_hip_ext.h)
+    NttBatchPlan ntt_batch_inv;              // inverse half of a polynomial product round (dynaverse_poly.h)
+    HostBuffer poly_stage;                   // pinned operands of a polynomial product round
     DeviceBuffer stage_a, stage_b, stage_c;  // per-call device inputs/outputs
     HostBuffer pinned;                       // per-call host staging
     DvgpuBatchStats batch_stats{};

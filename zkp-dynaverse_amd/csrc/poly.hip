@@ -14,6 +14,7 @@
 // with zero high coefficients); the outputs are trimmed as the reference's.
 #include <algorithm>
 #include <array>
+#include <iterator>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -187,44 +188,56 @@ class PolyEngine {
         }
         if (!total) return;
         require(total < (size_t(1) << 30), "polynomial batch too large");
-        // host staging: all padded left operands, then all padded right operands
-        std::vector<uint32_t> host(2 * total * 8, 0u);
+        // pinned host staging: all padded left operands, then all padded right operands
+        uint32_t* host = o_->poly_stage.as<uint32_t>(2 * total * 8);
+        std::memset(host, 0, 2 * total * 32);
         for (size_t j = 0; j < jobs.size(); ++j) {
             if (!size[j]) continue;
-            std::memcpy(host.data() + off[j] * 8, jobs[j].a->data(), jobs[j].a->size() * 32);
-            std::memcpy(host.data() + (total + off[j]) * 8, jobs[j].b->data(), jobs[j].b->size() * 32);
+            std::memcpy(host + off[j] * 8, jobs[j].a->data(), jobs[j].a->size() * 32);
+            std::memcpy(host + (total + off[j]) * 8, jobs[j].b->data(), jobs[j].b->size() * 32);
         }
         hipStream_t st = o_->stream;
         auto* dev = o_->stage_a.as<uint32_t>(2 * total * 8);
-        HIP_CHECK(hipMemcpyAsync(dev, host.data(), 2 * total * 32, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(dev, host, 2 * total * 32, hipMemcpyHostToDevice, st));
+        // one multi-row entry per run of consecutive products of one padded
+        // length (their rows are adjacent in both operand halves)
         std::vector<NttBatchEntry> fwd, inv;
-        for (size_t j = 0; j < jobs.size(); ++j) {
-            if (!size[j]) continue;
+        size_t device_jobs = 0;
+        for (size_t j = 0; j < jobs.size();) {
+            if (!size[j]) {
+                ++j;
+                continue;
+            }
+            size_t e = j + 1;
+            while (e < jobs.size() && size[e] == size[j] && off[e] == off[e - 1] + size[j]) ++e;
             int L = 0;
             while ((size_t(1) << L) < size[j]) ++L;
             uint32_t* a = dev + off[j] * 8;
             uint32_t* b = dev + (total + off[j]) * 8;
-            fwd.push_back({a, a, L, 1, false});
-            fwd.push_back({b, b, L, 1, false});
-            inv.push_back({a, a, L, 1, true});
+            fwd.push_back({a, a, L, e - j, false});
+            fwd.push_back({b, b, L, e - j, false});
+            inv.push_back({a, a, L, e - j, true});
+            device_jobs += e - j;
+            j = e;
         }
         ntt_batch_prepare(o_->domain, fwd, o_->ntt_batch, st);
         ntt_batch_enqueue(o_->domain, o_->ntt_batch, st);
         k_pointwise_fr<<<grid_for(total, 256), 256, 0, st>>>(dev, dev + total * 8, total);
         HIP_CHECK_LAUNCH();
-        ntt_batch_prepare(o_->domain, inv, o_->ntt_batch, st);
-        ntt_batch_enqueue(o_->domain, o_->ntt_batch, st);
-        HIP_CHECK(hipMemcpyAsync(host.data(), dev, total * 32, hipMemcpyDeviceToHost, st));
+        // its own plan: preparing it does not wait for the forward half
+        ntt_batch_prepare(o_->domain, inv, o_->ntt_batch_inv, st);
+        ntt_batch_enqueue(o_->domain, o_->ntt_batch_inv, st);
+        HIP_CHECK(hipMemcpyAsync(host, dev, total * 32, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
         for (size_t j = 0; j < jobs.size(); ++j) {
             if (!size[j]) continue;
             const size_t n = jobs[j].a->size() + jobs[j].b->size() - 1;
             jobs[j].out->resize(n);
-            std::memcpy(jobs[j].out->data(), host.data() + off[j] * 8, n * 32);
+            std::memcpy(jobs[j].out->data(), host + off[j] * 8, n * 32);
         }
         ++stats_.rounds;
-        stats_.products += fwd.size() / 2;
-        stats_.device_products += fwd.size() / 2;
+        stats_.products += device_jobs;
+        stats_.device_products += device_jobs;
     }
 
     // Reciprocals rev(den_j)^-1 mod x^len_j by Newton steps, all divisors in
@@ -491,17 +504,26 @@ PolyTree* tree_build(PolyEngine& e, Owner* o, const std::vector<W8>& pts) {
         t->levels.push_back(std::move(next));
     }
     // reciprocals for the remainder tree: a node divides its parent's
-    // remainder (degree < deg parent), quotient length deg(parent) - deg(node)
+    // remainder (degree < deg parent), quotient length deg(parent) - deg(node).
+    // They are independent, so every level's Newton steps run in lockstep
+    // (rounds set by the deepest level, not summed over the levels).
     t->recip.resize(t->levels.size());
-    for (size_t l = 0; l + 1 < t->levels.size(); ++l) {
+    {
         std::vector<const Poly*> dens;
-        std::vector<size_t> lens;
-        for (size_t i = 0; i < t->levels[l].size(); ++i) {
-            const size_t dp = t->levels[l + 1][i / 2].size() - 1, dn = t->levels[l][i].size() - 1;
-            dens.push_back(&t->levels[l][i]);
-            lens.push_back(std::max<size_t>(dp > dn ? dp - dn : 0, 1));
+        std::vector<size_t> lens, level_start;
+        for (size_t l = 0; l + 1 < t->levels.size(); ++l) {
+            level_start.push_back(dens.size());
+            for (size_t i = 0; i < t->levels[l].size(); ++i) {
+                const size_t dp = t->levels[l + 1][i / 2].size() - 1, dn = t->levels[l][i].size() - 1;
+                dens.push_back(&t->levels[l][i]);
+                lens.push_back(std::max<size_t>(dp > dn ? dp - dn : 0, 1));
+            }
         }
-        t->recip[l] = e.reciprocals(dens, lens);
+        level_start.push_back(dens.size());
+        std::vector<Poly> all = e.reciprocals(dens, lens);
+        for (size_t l = 0; l + 1 < t->levels.size(); ++l)
+            t->recip[l].assign(std::make_move_iterator(all.begin() + level_start[l]),
+                               std::make_move_iterator(all.begin() + level_start[l + 1]));
     }
     if (!pts.empty()) {
         const Poly d = differentiate(t->levels.back()[0]);

@@ -327,7 +327,9 @@ int dvgpu_ext_update_batch(void* owner, void* points, size_t point_count, int wi
         hipStream_t st = o->stream;
         uint32_t* dev = o->stage_a.as<uint32_t>(total_words);
         HIP_CHECK(hipMemcpyAsync(dev, pin, total_words * 4, hipMemcpyHostToDevice, st));
-        // Plans (each uploads its descriptors after synchronising the stream).
+        // Plans (the MSM plan synchronises the stream before its upload, which
+        // also completes the staged copy above; the NTT plan waits for its own
+        // previous upload only).
         const WindowTable wt = update_window_table(o);
         MsmBatchPlan plan = msm_batch_plan(items, update_msm_config(o), &wt);
         plan.point_limit = uint32_t(point_count);
