@@ -711,12 +711,28 @@ inline bool msm_plan_planes(MsmBatchPlan& plan) {
 // For larger bucket sets: the chunk running sums (k_reduce_chunks<SPLIT>)
 // give U_j and T_j per chunk j of L buckets, and the window value is
 // sum_j U_j + L * sum_k 2^k Q_k with Q_k the sum of the T_j whose index j has
-// bit k set.  Per window: plane 0 = sum_j U_j, plane 1 + k = Q_k, for
-// k < log2(J), J = buckets / L chunks; the host forms the window value.
-__host__ __device__ __forceinline__ uint32_t chunk_plane_count(uint32_t buckets, uint32_t chunk) {
-    return 1 + plane_bits(buckets / chunk) - 1;  // 1 + log2(J), J a power of two
+// bit k set, k < log2(J), J = buckets / L chunks.  Each sum is cut into
+// slices of at most CHUNK_PLANE_SPAN values per thread; per window the
+// outputs are [U slices][Q_0 slices][Q_1 slices]..., summed on the host.
+constexpr uint32_t CHUNK_PLANE_SPAN = 4;
+
+struct ChunkPlaneShape {
+    uint32_t J, qb, su, sq, per_window;  // chunks, Q planes, U / Q slices, outputs per window
+};
+
+__host__ __device__ __forceinline__ ChunkPlaneShape chunk_plane_shape(uint32_t buckets, uint32_t chunk) {
+    ChunkPlaneShape c;
+    c.J = buckets / chunk;
+    c.qb = plane_bits(c.J) - 1;  // log2(J), J a power of two
+    const uint32_t span = PLANE_THREADS * CHUNK_PLANE_SPAN;
+    c.su = (c.J + span - 1) / span;
+    c.sq = (c.J / 2 + span - 1) / span;
+    if (c.sq == 0) c.sq = 1;
+    c.per_window = c.su + c.qb * c.sq;
+    return c;
 }
 
+// Block g: one slice of one plane (plane_base numbers this MSM's outputs).
 template <class F>
 __global__ void __launch_bounds__(PLANE_THREADS)
     k_chunk_planes(const MsmDesc* __restrict__ descs, uint32_t m, const uint32_t* __restrict__ chunk_vals,
@@ -724,18 +740,20 @@ __global__ void __launch_bounds__(PLANE_THREADS)
     constexpr int XW = PointLayout<F>::XYZZ_WORDS;
     __shared__ uint4 lds[PLANE_THREADS * XW / 4];
     uint32_t* slot = reinterpret_cast<uint32_t*>(lds) + threadIdx.x * XW;
-    const uint32_t plane = blockIdx.x;
-    const MsmDesc d = descs[find_desc<&MsmDesc::plane_base>(descs, m, plane)];
-    const uint32_t J = d.buckets / d.chunk, qb = chunk_plane_count(d.buckets, d.chunk);
-    const uint32_t local = plane - d.plane_base, w = local / qb, q = local % qb;
-    const size_t base = size_t(d.chunk_base) + size_t(w) * J;  // chunk index of (window w, j = 0)
+    const uint32_t g = blockIdx.x;
+    const MsmDesc d = descs[find_desc<&MsmDesc::plane_base>(descs, m, g)];
+    const ChunkPlaneShape sh = chunk_plane_shape(d.buckets, d.chunk);
+    const uint32_t local = g - d.plane_base, w = local / sh.per_window, r = local % sh.per_window;
+    const size_t base = size_t(d.chunk_base) + size_t(w) * sh.J;  // chunk index of (window w, j = 0)
     Xyzz<F> acc = xyzz_inf<F>();
-    if (q == 0) {
-        for (uint32_t j = threadIdx.x; j < J; j += PLANE_THREADS)
+    if (r < sh.su) {  // a slice of sum_j U_j
+        const uint32_t lo = uint32_t(uint64_t(sh.J) * r / sh.su), hi = uint32_t(uint64_t(sh.J) * (r + 1) / sh.su);
+        for (uint32_t j = lo + threadIdx.x; j < hi; j += PLANE_THREADS)
             acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(chunk_vals + (2 * (base + j)) * XW));
-    } else {
-        const uint32_t k = q - 1;
-        for (uint32_t t = threadIdx.x; t < J / 2; t += PLANE_THREADS) {
+    } else {  // a slice of Q_k
+        const uint32_t k = (r - sh.su) / sh.sq, s = (r - sh.su) % sh.sq, cnt = sh.J / 2;
+        const uint32_t lo = uint32_t(uint64_t(cnt) * s / sh.sq), hi = uint32_t(uint64_t(cnt) * (s + 1) / sh.sq);
+        for (uint32_t t = lo + threadIdx.x; t < hi; t += PLANE_THREADS) {
             const uint32_t j = ((t >> k) << (k + 1)) | (1u << k) | (t & ((1u << k) - 1u));
             acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(chunk_vals + (2 * (base + j) + 1) * XW));
         }
@@ -749,7 +767,7 @@ __global__ void __launch_bounds__(PLANE_THREADS)
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) Mem<F>::store_xyzz(out + size_t(plane) * XW, acc);
+    if (threadIdx.x == 0) Mem<F>::store_xyzz(out + size_t(g) * XW, acc);
 }
 
 // Switch a plan to chunk-plane mode (before msm_batch_prepare); every
@@ -760,7 +778,7 @@ inline void msm_plan_chunk_planes(MsmBatchPlan& plan) {
         require(d.chunk && d.buckets % d.chunk == 0 && ((d.buckets / d.chunk) & (d.buckets / d.chunk - 1)) == 0,
                 "chunk planes need power-of-two chunks per window");
         d.plane_base = total;
-        total += d.windows * chunk_plane_count(d.buckets, d.chunk);
+        total += d.windows * chunk_plane_shape(d.buckets, d.chunk).per_window;
     }
     plan.chunk_planes = true;
     plan.total_planes = total;
@@ -1127,7 +1145,9 @@ inline MsmPlan msm_plan_impl(size_t n, const MsmConfig& cfg) {
     p.buckets = 1u << (p.c - 1);
     p.n = n;
     p.planes = int(plane_bits(p.buckets));
-    p.outputs = size_t(p.windows) * size_t(p.planes);  // room for either reduction's output
+    // room for either reduction's output (chunk planes need the most with
+    // one bucket per chunk)
+    p.outputs = size_t(p.windows) * std::max<size_t>(size_t(p.planes), chunk_plane_shape(p.buckets, 1).per_window);
     return p;
 }
 
@@ -1148,7 +1168,8 @@ MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, M
     p.windows = int(plan.descs[0].windows);
     p.buckets = plan.descs[0].buckets;
     p.n = n;
-    p.planes = plan.planes ? int(plane_bits(p.buckets)) : int(chunk_plane_count(p.buckets, plan.descs[0].chunk));
+    p.planes = plan.planes ? int(plane_bits(p.buckets))
+                           : int(chunk_plane_shape(p.buckets, plan.descs[0].chunk).per_window);
     p.chunk = plan.chunk_planes ? int(plan.descs[0].chunk) : 0;
     p.outputs = plan.total_planes;
     return p;
@@ -1172,12 +1193,18 @@ Affine<F> msm_combine_windows(const uint32_t* words, const MsmPlan& plan) {
     std::vector<Xyzz<F>> win(size_t(plan.windows));
     HostPool::get().parallel_for(win.size(), [&](size_t w) {
         const uint32_t* base = words + w * size_t(plan.planes) * XW;
-        if (plan.chunk) {  // U + L * sum_k 2^k Q_k, planes [U, Q_0, Q_1, ...]
+        if (plan.chunk) {  // U + L * sum_k 2^k Q_k from [U slices][Q_0 slices][Q_1 slices]...
+            const ChunkPlaneShape sh = chunk_plane_shape(plan.buckets, uint32_t(plan.chunk));
+            auto slices = [&](size_t first, uint32_t count) {
+                Xyzz<F> v = HostMem<F>::load_xyzz(base + first * XW);
+                for (uint32_t q = 1; q < count; ++q) v = xyzz_add<F>(v, HostMem<F>::load_xyzz(base + (first + q) * XW));
+                return v;
+            };
             Xyzz<F> acc = xyzz_inf<F>();
-            for (int k = plan.planes - 1; k >= 1; --k)
-                acc = xyzz_add<F>(xyzz_dbl<F>(acc), HostMem<F>::load_xyzz(base + size_t(k) * XW));
+            for (int k = int(sh.qb) - 1; k >= 0; --k)
+                acc = xyzz_add<F>(xyzz_dbl<F>(acc), slices(sh.su + size_t(k) * sh.sq, sh.sq));
             for (int l = plan.chunk; l > 1; l >>= 1) acc = xyzz_dbl<F>(acc);
-            win[w] = xyzz_add<F>(acc, HostMem<F>::load_xyzz(base));
+            win[w] = xyzz_add<F>(acc, slices(0, sh.su));
             return;
         }
         Xyzz<F> acc = HostMem<F>::load_xyzz(base + size_t(plan.planes - 1) * XW);
