@@ -615,6 +615,17 @@ def row_figures(backend: HipBackend, reps: int = 5, log=print) -> Dict:
         f_t = (f_t * tau + c) % r
     exp = decode_g1(backend.fixed_base_encoded(encode_fr(f_t), 1, False))
     out["resident_msm_g1_2p16"] = {"ms": round(t, 3), "ok": res == exp}
+    # G1 rows at the rebuild's size (n = 65536, program.rs:278-315): one
+    # 2^16-point transform forward and back, the round trip checked
+    s16, m16, d16 = rb.alloc(G1, nb), rb.alloc(G1, nb), rb.alloc(G1, nb)
+    rb.write_points(s16, 0, bases)
+
+    def ecntt16():
+        rb.transform_rows(G1, s16, m16, nb, 0, 1, False)
+        rb.transform_rows(G1, m16, d16, nb, 0, 1, True)
+    t, _ = med(ecntt16)
+    ok16 = rb.read_points(d16, 0, 8) == bases[:8] and rb.read_points(d16, nb - 8, 8) == bases[nb - 8:]
+    out["resident_ecntt_2p16_fwd_inv"] = {"ms": round(t, 3), "ok": bool(ok16)}
     log("row figures: " + ", ".join(f"{key} {v['ms']} ms ok={v['ok']}" for key, v in out.items()))
     return out
 
