@@ -203,7 +203,9 @@ class PairingChecks:
 
     def run(self) -> List[bool]:
         """Every queued equation in one device call, each decided on its own;
-        the trace counts each as equal() would.  The queue is emptied."""
+        the trace counts each as equal() would.  The queue is emptied, also
+        when the call fails (the error, e.g. a point off its curve, is raised
+        and nothing is counted)."""
         queued, self._queued = self._queued, []
         if not queued:
             return []
