@@ -420,29 +420,41 @@ PolyTree* find_tree(Owner* o, void* t) {
     return *it;
 }
 
-// Remainders of `poly` modulo every leaf (fast.rs:73-84): its values.
-std::vector<W8> tree_eval(PolyEngine& e, const PolyTree& t, const Poly& poly) {
-    if (t.points.empty()) return {};
+// Remainders of each polynomial modulo every leaf (fast.rs:73-84): its
+// values.  Several polynomials over one tree run their remainder trees in
+// lockstep (one device pass per level for all of them).
+std::vector<std::vector<W8>> tree_eval_many(PolyEngine& e, const PolyTree& t, const std::vector<const Poly*>& polys) {
+    const size_t P = polys.size();
+    std::vector<std::vector<W8>> out(P);
+    if (t.points.empty() || !P) return out;
     const size_t top = t.levels.size() - 1;
-    std::vector<Poly> rem;
+    std::vector<Poly> rem;  // P blocks of `width` remainders (one per node of the current level)
     {
-        std::vector<const Poly*> num{&poly}, den{&t.levels[top][0]};
+        std::vector<const Poly*> num(polys.begin(), polys.end()), den(P, &t.levels[top][0]);
         e.div_rem(num, den, nullptr, nullptr, rem);
     }
+    size_t width = 1;
     for (size_t l = top; l-- > 0;) {
+        const size_t nodes = t.levels[l].size();
         std::vector<const Poly*> num, den, rc;
-        for (size_t i = 0; i < t.levels[l].size(); ++i) {
-            num.push_back(&rem[i / 2]);
-            den.push_back(&t.levels[l][i]);
-            rc.push_back(&t.recip[l][i]);
-        }
+        for (size_t q = 0; q < P; ++q)
+            for (size_t i = 0; i < nodes; ++i) {
+                num.push_back(&rem[q * width + i / 2]);
+                den.push_back(&t.levels[l][i]);
+                rc.push_back(&t.recip[l][i]);
+            }
         std::vector<Poly> next;
         e.div_rem(num, den, &rc, nullptr, next);
         rem = std::move(next);
+        width = nodes;
     }
-    std::vector<W8> out;
-    for (const Poly& r : rem) out.push_back(r[0]);
+    for (size_t q = 0; q < P; ++q)
+        for (size_t i = 0; i < width; ++i) out[q].push_back(rem[q * width + i][0]);
     return out;
+}
+
+std::vector<W8> tree_eval(PolyEngine& e, const PolyTree& t, const Poly& poly) {
+    return std::move(tree_eval_many(e, t, {&poly})[0]);
 }
 
 // fast.rs:90-116
@@ -691,8 +703,10 @@ int dvgpu_poly_tree_cauchy_sums(void* backend, const uint8_t* root, void* tree, 
         std::vector<W8> nv;
         for (size_t i = 0; i < count; ++i) nv.push_back(from_fr(mul(to_fr(w[i]), to_fr(t->derivatives[i]))));
         const Poly num = tree_interpolate(e, *t, nv);
-        const std::vector<W8> first = tree_eval(e, *t, differentiate(num));
-        const std::vector<W8> second = tree_eval(e, *t, differentiate(differentiate(t->levels.back()[0])));
+        const Poly d1 = differentiate(num), d2 = differentiate(differentiate(t->levels.back()[0]));
+        const std::vector<std::vector<W8>> ev = tree_eval_many(e, *t, {&d1, &d2});  // both in lockstep
+        const std::vector<W8>& first = ev[0];
+        const std::vector<W8>& second = ev[1];
         const Fr half = bn254::inv(add(fp_one<FrParams>(), fp_one<FrParams>()));
         for (size_t i = 0; i < count; ++i) {
             const Fr s = mul(sub(to_fr(first[i]), mul(mul(to_fr(w[i]), to_fr(second[i])), half)),
