@@ -206,7 +206,7 @@ class HipBackend:
         if len(points) != len(scalars):
             raise ValueError("MSM length mismatch")
         if len(points) > MAX_MSM_SIZE:
-            raise ValueError("MSM exceeds baseline size bound (2^20)")
+            raise ValueError("MSM exceeds the ABI bound of 2^20 points")
         if not points:
             return None
         for p in points:
@@ -257,7 +257,7 @@ class HipBackend:
         bins = {}
         for i, ln in enumerate(lengths):
             if ln > MAX_MSM_SIZE:
-                raise ValueError("MSM exceeds baseline size bound (2^20)")
+                raise ValueError("MSM exceeds the ABI bound of 2^20 points")
             if ln > 0:
                 bins.setdefault((ln - 1).bit_length(), []).append(i)
         for b in sorted(bins):
@@ -289,7 +289,7 @@ class HipBackend:
             if ln > len(request.point_sets[i]):
                 raise ValueError("point range out of bounds")
             if ln > MAX_MSM_SIZE:
-                raise ValueError("MSM exceeds baseline size bound (2^20)")
+                raise ValueError("MSM exceeds the ABI bound of 2^20 points")
             for p in request.point_sets[i][:ln]:
                 if not g1_on_curve(p):
                     raise ValueError("invalid G1 point")
@@ -305,7 +305,7 @@ class HipBackend:
         lengths = []
         for r in requests:
             if len(r.entries) > MAX_MSM_SIZE:
-                raise ValueError("MSM exceeds baseline size bound (2^20)")
+                raise ValueError("MSM exceeds the ABI bound of 2^20 points")
             r.validate()
             for idx, _ in r.entries:
                 if not g1_on_curve(r.bases[idx]):
@@ -324,7 +324,7 @@ class HipBackend:
         if len(points) != len(scalars):
             raise ValueError("G2 MSM length mismatch")
         if len(points) > MAX_MSM_SIZE:
-            raise ValueError("G2 MSM exceeds baseline size bound (2^20)")
+            raise ValueError("G2 MSM exceeds the ABI bound of 2^20 points")
         for p in points:
             if not g2_on_curve(p):
                 raise ValueError("invalid G2 point")
@@ -340,7 +340,7 @@ class HipBackend:
         if n == 0 or n & (n - 1):
             raise ValueError("NTT size is not a power of two")
         if n > MAX_NTT_SIZE:
-            raise ValueError("NTT exceeds baseline size bound (2^20)")
+            raise ValueError("NTT exceeds the ABI bound of 2^20 points")
         enc = b"".join(encode_fr(v) for v in values)
         out = self.ntt_encoded(enc, n, inverse)
         decoded = [decode_fr(out[32 * i:32 * i + 32]) for i in range(n)]
