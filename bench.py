@@ -5,9 +5,11 @@ Workload (the single-call drop-in configuration): a BN254 G1 MSM of 2^20
 points over the seeded SRS [tau^i] g1 with uniform 254-bit Fr scalars, plus a
 2^16 MSM and forward+inverse Fr NTTs of 2^16 and 2^20 reported alongside.
 A step is one 2^20 G1 MSM with points and scalars already resident in HBM;
-`value` is its throughput in Mpoints/s.  The result is checked every run
-against an independent device computation: sum s_i [tau^i] g1 must equal
-[f(tau)] g1 with f the scalar polynomial (fixed-base setup kernel).
+`value` is its throughput in Mpoints/s.  The result is checked every run:
+in the CPU-baseline leg against the oracle's closed form [f(tau)] g1 (f the
+scalar polynomial, Horner on the host and one oracle scalar multiplication)
+and the C Pippenger over the same points; with --no-cpu-baseline only
+against the device's fixed-base kernel.
 
 roofline: the dominant kernel is the bucket accumulation (XYZZ mixed adds).
 Its algorithmic work is 10 Montgomery products x 136 v_mad_u64_u32-equivalent
@@ -75,9 +77,11 @@ def main():
     d_sc = backend.to_device(scal.reshape(-1))
     print(f"# inputs ready in {time.time() - t_setup:.1f}s", file=sys.stderr, flush=True)
 
-    # Independent check value [f(tau)] g1 through the fixed-base kernel.
+    # Check value [f(tau)] g1 through the fixed-base kernel (the oracle's
+    # closed form is checked in the CPU-baseline leg).
     f_tau = poly_eval(scal, tau)
     expect = backend.fixed_base_encoded(encode_fr(f_tau), 1, False)
+    verified_by = "device fixed-base kernel [f(tau)]g1"
 
     for _ in range(args.warmup):
         backend.msm_device(d_srs.ptr, d_sc.ptr, n, window_bits=args.window)
@@ -136,9 +140,40 @@ def main():
         d_in.free()
         d_out.free()
 
+    # The plain drop-in (dvgpu_bn254_msm): canonical host points and scalars,
+    # host-to-device copies, conversion and the same MSM, per call.
+    canon_pts = b"".join(backend.fixed_base_encoded(powers[i:i + 4096].tobytes(), min(4096, n - i), False)
+                         for i in range(0, n, 4096))
+    sc_bytes = scal.tobytes()
+    for log_p in sorted({16, args.log_n}):
+        npl = 1 << log_p
+        if npl > n:
+            continue
+        backend.msm_encoded(canon_pts[:64 * npl], sc_bytes[:32 * npl], npl)
+        ts = []
+        for _ in range(5):
+            a = time.perf_counter()
+            plain_res = backend.msm_encoded(canon_pts[:64 * npl], sc_bytes[:32 * npl], npl)
+            ts.append(time.perf_counter() - a)
+        tp = sorted(ts)[len(ts) // 2]
+        secondary[f"plain_msm_2p{log_p}_ms"] = round(tp * 1e3, 3)
+        secondary[f"plain_msm_2p{log_p}_mpoints_s"] = round(npl / tp / 1e6, 1)
+        if npl == n:
+            secondary["plain_msm_matches_device_entry"] = plain_res == res
+
+    from dvgpu import benchmarks
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    log("# n=256 update sequences (plain per-call drop-in)")
+    small = {}
+    for wl in ("hotspot", "spreading"):
+        run = benchmarks.small_sequence_gpu(backend, wl, timed=True)
+        recs = run["records"]
+        small[wl] = {"updates": len(recs), "gpu_ms_per_update": round(sum(run["ms"]) / len(recs), 3),
+                     "gpu_p50_ms": round(sorted(run["ms"])[len(recs) // 2], 3),
+                     "all_checks_ok": all(r["incremental_ok"] and r["roundtrip_ok"] and r["verified"] for r in recs)}
+    secondary["small_sequence_n256"] = small
+
     if not args.no_serving:
-        from dvgpu import benchmarks
-        log = lambda m: print(m, file=sys.stderr, flush=True)
         log("# batched launch")
         secondary["batched_config"] = benchmarks.batched_launch(backend, log=log)
         log("# incremental n=2^16")
@@ -147,8 +182,6 @@ def main():
         secondary["served_stream"] = benchmarks.served_stream(backend, log=log)
 
     if not args.no_kzg:
-        from dvgpu import benchmarks
-        log = lambda m: print(m, file=sys.stderr, flush=True)
         log("# opening caches n=2^16")
         secondary["opening_cache_2p16"] = benchmarks.opening_cache(backend, log=log)
         secondary["opening_cache_2p13_x1"] = benchmarks.opening_cache(backend, log_n=13, rows=1, log=log)
@@ -181,22 +214,52 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import cref
-        threads = min(os.cpu_count() or 1, 16)
+        # the GPU host's CPU share for one GPU is 16 cores (whatever nproc says)
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        from oracle import bn254 as ob
         ns = n  # the full workload: about 10 core-seconds of C Pippenger
-        # canonical points of the sample via the fixed-base ABI (chunks of 4096)
-        pts = b"".join(backend.fixed_base_encoded(powers[i:i + 4096].tobytes(),
-                                                  min(4096, ns - i), False)
-                       for i in range(0, ns, 4096))
-        sc_bytes = scal[:ns].tobytes()
         a = time.perf_counter()
-        cres = cref.msm_g1(pts, sc_bytes, ns, threads)
+        cres = cref.msm_g1(canon_pts, sc_bytes, ns, threads)
         dt = time.perf_counter() - a
-        f_s = poly_eval(scal[:ns], tau)
-        cpu_ok = cres == backend.fixed_base_encoded(encode_fr(f_s), 1, False)
+        # the oracle's closed form: Horner over Fr, one scalar multiplication
+        oracle_val = ob.g1_to_bytes(ob.g1_mul(f_tau))
+        verified_by = "oracle closed form [f(tau)]g1 and the C Pippenger"
+        ok = ok and res == oracle_val and cres == oracle_val
         cpu = {"value": ns / dt / 1e6, "unit": "Mpoints/s", "cores": threads, "kind": "port",
                "sample": f"one G1 MSM over all 2^{ns.bit_length() - 1} SRS points with the same "
                          f"scalars, C Pippenger (oracle/c, not arkworks), {dt:.1f}s, "
-                         f"result {'matches' if cpu_ok else 'DIFFERS FROM'} [f(tau)]g1"}
+                         f"result {'matches' if cres == oracle_val else 'DIFFERS FROM'} the oracle's [f(tau)]g1"}
+        # the n = 256 sequences on the host: the same MSMs and transforms per
+        # update through the C oracle (the pairing check is not included: the
+        # C oracle has none)
+        from dvgpu.workload import small_sequence
+        from dvgpu.codec import root_of_unity_2_20
+        rt = encode_fr(root_of_unity_2_20())
+        for wl in ("hotspot", "spreading"):
+            tau_s, coeffs_s, ups_s = small_sequence(1, 256, 8, 4, wl)
+            from dvgpu.workload import tau_powers
+            pts_s = backend.fixed_base_encoded(tau_powers(tau_s, 256).tobytes(), 256, False)
+            f_s = [int.from_bytes(coeffs_s[i].tobytes(), "little") for i in range(256)]
+            enc = lambda vals: b"".join(v.to_bytes(32, "little") for v in vals)
+            c_prev = cref.msm_g1(pts_s, enc(f_s), 256, 1)
+            a = time.perf_counter()
+            for up in ups_s:
+                d = {}
+                for i, v in up:
+                    d[i] = (d.get(i, 0) + v - f_s[i]) % FR_MODULUS
+                    f_s[i] = v
+                c_new = cref.msm_g1(pts_s, enc(f_s), 256, 1)
+                ix = sorted(d)
+                cref.msm_g1(c_prev + b"".join(pts_s[64 * i:64 * i + 64] for i in ix),
+                            encode_fr(1) + enc(d[i] for i in ix), len(ix) + 1, 1)
+                ev = cref.ntt(enc(f_s), 256, False, rt, 1)
+                cref.ntt(ev, 256, True, rt, 1)
+                cref.msm_g1(pts_s[:64 * 255], enc(f_s[1:]), 255, 1)   # opening-sized MSM
+                c_prev = c_new
+            dt_s = time.perf_counter() - a
+            secondary["small_sequence_n256"][wl]["cpu_ms_per_update"] = round(dt_s * 1e3 / len(ups_s), 3)
+            secondary["small_sequence_n256"][wl]["cpu_cores"] = 1
+            secondary["small_sequence_n256"][wl]["cpu_kind"] = "port (C oracle, not arkworks; pairing excluded)"
 
     if cpu is not None and not args.no_kzg:
         # Opening-cache sample: one cache of 2^13 coefficients over 2^13 points
@@ -211,7 +274,6 @@ def main():
         gpu_ms = secondary.get("opening_cache_2p13_x1", {}).get("cold_ms")
         # Pairing-product sample: the oracle's optimal ate pairing (Python
         # big integers, one core) on four two-term checks.
-        from oracle import bn254 as ob
         a = time.perf_counter()
         for k in range(4):
             p1 = ob.g1_mul(k + 2)
@@ -240,7 +302,7 @@ def main():
         "data": "synthetic (ChaCha20-seeded SRS [tau^i]g1 and uniform Fr scalars)",
         "config": {"workload": f"single-call G1 MSM over 2^{args.log_n} SRS points, device-resident inputs",
                    "msm_log_n": args.log_n, "window_bits": c, "windows": windows,
-                   "verified_against_fixed_base": bool(ok)},
+                   "verified": bool(ok), "verified_by": verified_by},
         "roofline": {"bound": "valu", "kernel": "k_accumulate (XYZZ mixed-add bucket accumulation)",
                      "achieved": achieved / 1e12, "peak": PEAK_MAC_PER_S / 1e12, "unit": "TMAC/s",
                      "frac": achieved / PEAK_MAC_PER_S, "traffic": traffic,

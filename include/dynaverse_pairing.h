@@ -90,6 +90,25 @@ int dvgpu_bn254_opening_cache_check(void* backend, const uint8_t* commitments, s
                                     const uint8_t* values, const uint8_t* caches, uint8_t* results,
                                     float* kernel_ms, char* error, size_t error_capacity);
 
+/* KZG single-point opening checks, one proof per item: item t is accepted
+ * (results[t] = 1) when
+ *     e(C_t - [y_t] g1_0, g2_0) == e(W_t, D_t),
+ * with the commitment C_t and opening W_t (count x 64 B G1), the value
+ * y_t = f(z_t) (count x 32 B Fr) and the divisor D_t = [tau]g2 - [z_t]g2
+ * (count x 128 B G2, the verifier's frozen G2 term of its session), g1_0 and
+ * g2_0 the generators.  This is the pairing verification of a KZG update
+ * proof (crates/protocol/src/dynaverse.rs:447-451, the cache equation of
+ * rebuilding/machine.rs:321-335) for many proofs of many SRSs at once,
+ * evaluated as one two-term Miller product (g2_0's lines prepared once) and
+ * one final exponentiation per item.  All inputs are checked on the device
+ * (canonical, on E / E'); a bad one fails the call with -1 and zeroes
+ * results.  D_t must be in the r-torsion, as for every G2 input here.
+ * count up to 2^24.  No counterpart in the reference ABI. */
+int dvgpu_bn254_kzg_verify_batch(void* backend, const uint8_t* g1_0, const uint8_t* g2_0,
+                                 const uint8_t* commitments, const uint8_t* values, const uint8_t* openings,
+                                 const uint8_t* divisors, size_t count, uint8_t* results, float* kernel_ms,
+                                 char* error, size_t error_capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,7 +87,8 @@ from dvgpu.serving import Server, request_cost
 
 class FakeStore:
     def __init__(self, sizes):
-        self.sessions = {sid: SimpleNamespace(n=n, busy=False) for sid, n in enumerate(sizes)}
+        self.sessions = {sid: SimpleNamespace(n=n, busy=False, divisor=sid.to_bytes(4, "little"))
+                         for sid, n in enumerate(sizes)}
 
 
 class FakeLane:
@@ -116,9 +117,27 @@ class FakeLane:
         pass
 
 
+class FakeVerifier:
+    """Rejects proofs whose session id is in `bad`; records call sizes and
+    checks that each proof comes with its own session's divisor."""
+
+    def __init__(self, bad=(), delay=0.003):
+        self.bad, self.delay, self.calls = set(bad), delay, []
+
+    def run(self, proofs, divisors):
+        time.sleep(self.delay)
+        self.calls.append(len(proofs))
+        return [int.from_bytes(d, "little") not in self.bad for d in divisors]
+
+    def close(self):
+        pass
+
+
 def _serve(store, reqs, **kw):
     FakeLane.log, FakeLane.batches = {}, []
     lane_kw = kw.pop("lane", {})
+    if "verifier_factory" not in kw:
+        kw.setdefault("verify", False)
     srv = Server(store, lane_factory=lambda: FakeLane(**lane_kw), **kw)
     try:
         handles = [srv.submit(sid, np.array([seq], dtype=np.uint32), b"") for sid, seq in reqs]
@@ -178,3 +197,39 @@ def test_update_record_layout_matches_struct():
     assert UPDATE_DTYPE.itemsize == ctypes.sizeof(_lib.DvgpuUpdate)
     for name, _ in _lib.DvgpuUpdate._fields_:
         assert UPDATE_DTYPE.fields[name][1] == getattr(_lib.DvgpuUpdate, name).offset, name
+
+
+def test_server_verifies_every_proof_before_resolving():
+    store = FakeStore([256] * 12)
+    ver = FakeVerifier(bad={5})
+    reqs = [(i % 12, i) for i in range(240)]
+    srv, handles, results = _serve(store, reqs, lanes=2, max_batch=8, verifier_factory=lambda: ver)
+    from dvgpu.serving import VerificationError
+    for (sid, _), h, r in zip(reqs, handles, results):
+        if sid == 5:
+            assert isinstance(r, VerificationError)
+        else:
+            assert isinstance(r, bytes)
+        assert h.verified >= h.done > 0 and h.finished == h.verified
+    assert srv.verified == 220 and srv.rejected == 20
+    assert sum(ver.calls) == 240 and len(ver.calls) == srv.verify_calls
+    assert max(ver.calls) > 8            # one call verifies several finished batches
+
+
+def test_server_close_fails_queued_requests():
+    """Requests still queued when the server closes get an exception instead
+    of staying pending for ever (one lane, one request per batch)."""
+    FakeLane.log, FakeLane.batches = {}, []
+    store = FakeStore([256])
+    srv = Server(store, lanes=1, max_batch=1, verify=False, lane_factory=lambda: FakeLane(delay=0.05))
+    handles = [srv.submit(0, np.array([i], dtype=np.uint32), b"") for i in range(4)]
+    time.sleep(0.01)
+    srv.close()
+    outcomes = []
+    for h in handles:
+        try:
+            outcomes.append(h.future.result(timeout=5))
+        except RuntimeError as e:
+            outcomes.append(str(e))
+    assert isinstance(outcomes[0], bytes)               # the batch on the lane finished
+    assert outcomes[1:] == ["server closed"] * 3

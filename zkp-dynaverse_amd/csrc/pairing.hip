@@ -244,6 +244,67 @@ __global__ void __launch_bounds__(64) k_cache_check(const uint32_t* __restrict__
     ok[t] = f12_is_one(final_exponentiation(f)) ? 1 : 0;
 }
 
+// KZG opening check of item t (one proof each):
+//     e(C_t - [y_t] g1_0, g2_0) == e(W_t, D_t),
+// as the product e(C_t - [y_t] g1_0, g2_0) e(-W_t, D_t) == 1, g2_0's lines
+// prepared.  Every input is canonical on the wire (C_t, W_t 64 B; y_t 32 B;
+// D_t 128 B) and checked here: ok[t] = 2 flags a bad one for the host.
+__global__ void __launch_bounds__(64) k_kzg_verify(const uint32_t* __restrict__ commitments,
+                                                   const uint32_t* __restrict__ values,
+                                                   const uint32_t* __restrict__ openings,
+                                                   const uint32_t* __restrict__ divisors, uint64_t count, Fq g1x,
+                                                   Fq g1y, const uint32_t* __restrict__ lines,
+                                                   uint8_t* __restrict__ ok) {
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const uint32_t* cw = commitments + t * 16;
+    const uint32_t* v = values + t * 8;
+    const uint32_t* pi = openings + t * 16;
+    const uint32_t* dv = divisors + t * 32;
+    bool valid = canonical_fr(v);
+    const bool c0 = all_zero(cw, 16), pi0 = all_zero(pi, 16), dinf = all_zero(dv, 32);
+    G1 c = xyzz_inf<FqOps>();
+    if (!c0) {
+        valid = valid && canonical_fq(cw) && canonical_fq(cw + 8);
+        if (valid) {
+            const Fq x = ld_canon_fq(cw), y = ld_canon_fq(cw + 8);
+            valid = affine_on_curve<FqOps>({x, y, false});
+            c = G1{x, y, fp_one<FqParams>(), fp_one<FqParams>()};
+        }
+    }
+    Fq px = fp_zero<FqParams>(), py = fp_zero<FqParams>();
+    if (!pi0) {
+        valid = valid && canonical_fq(pi) && canonical_fq(pi + 8);
+        if (valid) {
+            px = ld_canon_fq(pi);
+            py = ld_canon_fq(pi + 8);
+            valid = affine_on_curve<FqOps>({px, py, false});
+        }
+    }
+    Fq2 dx = f2_zero(), dy = f2_zero();
+    if (!dinf) {
+        for (int q = 0; q < 4; ++q) valid = valid && canonical_fq(dv + 8 * q);
+        if (valid) {
+            dx = {ld_canon_fq(dv), ld_canon_fq(dv + 8)};
+            dy = {ld_canon_fq(dv + 16), ld_canon_fq(dv + 24)};
+            valid = affine_on_curve<Fq2Ops>({dx, dy, false});
+        }
+    }
+    if (!valid) {
+        ok[t] = 2;
+        return;
+    }
+    uint32_t k[8];
+    for (int q = 0; q < 8; ++q) k[q] = v[q];
+    const G1 vg = g1_mul_scalar({g1x, g1y, fp_one<FqParams>(), fp_one<FqParams>()}, k, false);
+    const Affine<FqOps> d = xyzz_to_affine<FqOps>(xyzz_add<FqOps>(c, xyzz_neg<FqOps>(vg)));
+    PairingInput in[1];
+    int m = 0;
+    if (!pi0 && !dinf) in[m++] = {px, neg(py), dx, dy};
+    const Fq12 f = (m || !d.inf) ? miller_product_impl(in, m, GlobalLines{lines}, !d.inf, d.x, d.y) : f12_one();
+    ok[t] = f12_is_one(final_exponentiation(f)) ? 1 : 0;
+}
+
 Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
 
 Affine<FqOps> host_g1(const uint8_t* b, const char* what) {
@@ -451,6 +512,59 @@ extern "C" int dvgpu_bn254_opening_cache_check(void* backend, const uint8_t* com
             std::memset(results, 0, total);
             throw InvalidArgument(("cache check wire " + std::to_string(t / n) + " index " + std::to_string(t % n) +
                                    ": a point, value or cache point is not canonical or not on its curve").c_str());
+        }
+        return 0;
+    });
+}
+
+extern "C" int dvgpu_bn254_kzg_verify_batch(void* backend, const uint8_t* g1_0, const uint8_t* g2_0,
+                                            const uint8_t* commitments, const uint8_t* values,
+                                            const uint8_t* openings, const uint8_t* divisors, size_t count,
+                                            uint8_t* results, float* kernel_ms, char* error, size_t capacity) {
+    Owner* o = as_owner(backend);
+    return guarded(o, error, capacity, [&]() -> int {
+        require(count <= (size_t(1) << 24), "too many KZG openings");
+        if (!count) return 0;
+        require(g1_0 && g2_0 && commitments && values && openings && divisors && results,
+                "null KZG verification buffers");
+        const Affine<FqOps> g1 = host_g1(g1_0, "g1 generator is not a canonical curve point");
+        require(!g1.inf, "g1 generator is the identity");
+        const Affine<Fq2Ops> h0 = host_g2(g2_0, "g2 generator is not a canonical twist point");
+        require(!h0.inf, "g2 generator is the identity");
+        hipStream_t st = o->stream;
+        const size_t bc = count * 64, bv = count * 32, bw = count * 64, bd = count * 128,
+                     br = (count + 255) & ~size_t(255), bl = PAIRING_LINES * 48 * 4;
+        auto* base = static_cast<uint8_t*>(o->stage_a.ensure(bc + bv + bw + bd + br + bl));
+        uint8_t *dc = base, *dv = dc + bc, *dw = dv + bv, *dd = dw + bw, *dr = dd + bd, *dl = dr + br;
+        HIP_CHECK(hipMemcpyAsync(dc, commitments, bc, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(dv, values, bv, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(dw, openings, bw, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(dd, divisors, bd, hipMemcpyHostToDevice, st));
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        if (kernel_ms) {
+            for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+            HIP_CHECK(hipEventRecord(ev[0], st));
+        }
+        k_prepare_lines<<<1, 64, 0, st>>>(h0.x, h0.y, reinterpret_cast<uint32_t*>(dl));
+        HIP_CHECK_LAUNCH();
+        k_kzg_verify<<<grid_for(count, 64), 64, 0, st>>>(
+            reinterpret_cast<const uint32_t*>(dc), reinterpret_cast<const uint32_t*>(dv),
+            reinterpret_cast<const uint32_t*>(dw), reinterpret_cast<const uint32_t*>(dd), count, g1.x, g1.y,
+            reinterpret_cast<const uint32_t*>(dl), dr);
+        HIP_CHECK_LAUNCH();
+        if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
+        HIP_CHECK(hipMemcpyAsync(results, dr, count, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        if (kernel_ms) {
+            HIP_CHECK(hipEventElapsedTime(kernel_ms, ev[0], ev[1]));
+            for (auto& e : ev) (void)hipEventDestroy(e);
+        }
+        const void* bad = std::memchr(results, 2, count);
+        if (bad) {
+            const size_t t = static_cast<const uint8_t*>(bad) - results;
+            std::memset(results, 0, count);
+            throw InvalidArgument(("KZG opening " + std::to_string(t) +
+                                   ": a point or value is not canonical or not on its curve").c_str());
         }
         return 0;
     });

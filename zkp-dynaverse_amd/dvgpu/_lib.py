@@ -262,6 +262,8 @@ SIGNATURES = {
     "dvgpu_bn254_opening_cache_check": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p,
                                              c_size_t, c_void_p, c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float),
                                              *_E]),
+    "dvgpu_bn254_kzg_verify_batch": (_I, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_size_t, c_void_p, ctypes.POINTER(ctypes.c_float), *_E]),
     "dvgpu_ext_batch_create": (_I, [c_void_p, ctypes.POINTER(DvgpuBatchSpec), _I, ctypes.POINTER(c_void_p), *_E]),
     "dvgpu_ext_batch_run": (_I, [c_void_p, c_void_p, c_void_p, c_void_p,
                                  ctypes.POINTER(DvgpuG2ValidationReport), ctypes.POINTER(ctypes.c_float), *_E]),

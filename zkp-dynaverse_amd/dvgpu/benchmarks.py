@@ -12,6 +12,7 @@ import numpy as np
 
 from .backend import HipBackend
 from .batching import DeviceBatch, MsmItem, NttRows, msm_batch_device
+from .codec import FR_MODULUS
 from .rng import ChaCha20Stream
 from .serving import (PlainSession, Server, SessionSpec, SessionStore, merge_update, percentile, request_stream,
                       session_pool, update_batch)
@@ -120,44 +121,132 @@ def _coset_wire(backend: HipBackend, g2pts, count: int) -> bytes:
     return b"".join(out)
 
 
+def small_sequence_gpu(backend: HipBackend, workload: str = "hotspot", seed: int = 1, n: int = 256,
+                       updates: int = 8, k: int = 4, t: int = 3, timed: bool = False) -> Dict:
+    """The plain n = 256 update sequence through the per-call drop-in.
+
+    SRS [tau^i] g1 (i < n), g2 and [tau] g2 from the seeded tau; f uniform.
+    Every update rewrites k coefficients (workload.small_sequence), then
+    recomputes the commitment C = MSM(P, f) (dvgpu_bn254_msm), the
+    evaluations NTT(f) and the coefficients back by INTT (dvgpu_bn254_ntt),
+    and verifies: C against the incremental form C_prev + sum d_j P_i_j (a
+    sparse MSM with the cached point, recompute-and-compare), INTT(NTT(f)) ==
+    f, and the KZG opening at z = w_n^t, W = [q(tau)] g1 with
+    q = (f - f(z)) / (X - z), y = NTT(f)[t], by its pairing check
+    e(C - [y] g1, g2) == e(W, [tau - z] g2) (dvgpu_bn254_kzg_verify_batch).
+    Returns per-update records (C, W, y bytes, the check flags) and, with
+    `timed`, the per-update wall-clock ms."""
+    from .codec import encode_fr, root_of_unity_2_20
+    from .pairing import verify_update_proofs
+    from .workload import small_sequence
+    R = FR_MODULUS
+    tau, coeffs, ups = small_sequence(seed, n, updates, k, workload)
+    P = backend.fixed_base_encoded(tau_powers(tau, n).tobytes(), n, False)
+    z = pow(pow(root_of_unity_2_20(), (1 << 20) // n, R), t, R)
+    divisor = backend.fixed_base_encoded(encode_fr((tau - z) % R), 1, True)
+    f = [int.from_bytes(coeffs[i].tobytes(), "little") for i in range(n)]
+    enc = lambda vals: b"".join(v.to_bytes(32, "little") for v in vals)
+    C = backend.msm_encoded(P, enc(f), n)
+    records, ms = [], []
+    for up in ups:
+        a = time.perf_counter()
+        deltas = {}
+        for i, v in up:
+            deltas[i] = (deltas.get(i, 0) + v - f[i]) % R
+            f[i] = v
+        idx = sorted(deltas)
+        C_new = backend.msm_encoded(P, enc(f), n)
+        # incremental form: the cached point with scalar 1 plus the deltas
+        inc = backend.msm_encoded(C + b"".join(P[64 * i:64 * i + 64] for i in idx),
+                                  encode_fr(1) + enc(deltas[i] for i in idx), len(idx) + 1)
+        evals = backend.ntt_encoded(enc(f), n, False)
+        back = backend.ntt_encoded(evals, n, True)
+        y = int.from_bytes(evals[32 * t:32 * t + 32], "little")
+        q, acc = [0] * (n - 1), 0          # synthetic division by (X - z)
+        for j in range(n - 1, 0, -1):
+            acc = (acc * z + f[j]) % R
+            q[j - 1] = acc
+        W = backend.msm_encoded(P[:64 * (n - 1)], enc(q), n - 1)
+        ok = verify_update_proofs(backend, [C_new + W + y.to_bytes(32, "little")], [divisor])[0]
+        ms.append((time.perf_counter() - a) * 1e3)
+        records.append({"C": C_new, "W": W, "y": y, "evals": evals, "incremental_ok": inc == C_new,
+                        "roundtrip_ok": back == enc(f), "verified": bool(ok)})
+        C = C_new
+    out = {"tau": tau, "z": z, "f": f, "records": records}
+    if timed:
+        out["ms"] = ms
+    return out
+
+
+def incremental_updates(n: int = 1 << 16, count: int = 16, seed: int = 50):
+    """The incremental configuration's updates: k cycling 1, 16, 256, 4096;
+    even updates hotspot (uniform over a fixed seed-chosen subset of 1% of
+    the indices, 655 at n = 2^16), odd ones spreading (uniform over n); deltas
+    uniform Fr.  Each as merge_update's (indices, deltas bytes, deltas)."""
+    rng = ChaCha20Stream(seed)
+    hot = [rng.below(n) for _ in range(max(1, n // 100))]
+    ups = []
+    for j in range(count):
+        k = (1, 16, 256, 4096)[j % 4]
+        idx = [hot[rng.below(len(hot))] if j % 2 == 0 else rng.below(n) for _ in range(k)]
+        ups.append(merge_update(idx, rng.uniform_fr(k)))
+    return ups
+
+
 def incremental(backend: HipBackend, log=print) -> Dict:
     """n = 2^16 resident session, 16 updates with k cycling {1, 16, 256, 4096},
     hotspot (a fixed 1% index subset) and spreading alternating; served path
-    (one update per call, the state on the device) against the plain path."""
+    (one update per call, the state on the device) against the plain path.
+    Every update's proof is pairing-verified on the device as the reference
+    verifies each update (dynaverse.rs:447-451): separately timed per update
+    (`*_verified_*` figures: update + its own verification call) and once for
+    all 16 proofs together."""
+    from .pairing import verify_update_proofs
     spec = SessionSpec(0, 1 << 16, 5)
     store = SessionStore(backend, 1 << 30)
     s = store.admit(spec)
     plain = PlainSession(backend, spec)
-    rng = ChaCha20Stream(50)
-    hot = [rng.below(spec.n) for _ in range(655)]
-    ups = []
-    for j in range(16):
-        k = (1, 16, 256, 4096)[j % 4]
-        idx = [hot[rng.below(len(hot))] if j % 2 == 0 else rng.below(spec.n) for _ in range(k)]
-        ups.append(merge_update(idx, rng.uniform_fr(k)))
-    served_ms, plain_ms, same = [], [], True
+    ups = incremental_updates(spec.n)
+    served_ms, plain_ms, verify_ms, same, proofs = [], [], [], True, []
     for idx, raw, dv in ups:
         a = time.perf_counter()
         p1 = update_batch(backend, store, [s.update_struct(idx, raw)])[0]
         b = time.perf_counter()
-        p2 = plain.apply(backend, idx, raw, dv)
+        ok1 = verify_update_proofs(backend, [p1], [s.divisor])[0]
         c = time.perf_counter()
+        p2 = plain.apply(backend, idx, raw, dv)
+        d = time.perf_counter()
         served_ms.append((b - a) * 1e3)
-        plain_ms.append((c - b) * 1e3)
-        same &= p1 == p2
+        verify_ms.append((c - b) * 1e3)
+        plain_ms.append((d - c) * 1e3)
+        same &= p1 == p2 and ok1
+        proofs.append(p1)
     same &= store.evaluations(s) == plain.evals
+    a = time.perf_counter()
+    all_ok = all(verify_update_proofs(backend, proofs, [s.divisor] * len(proofs)))
+    batch_verify_ms = (time.perf_counter() - a) * 1e3
     store.close()
+    sv = [x + y for x, y in zip(served_ms, verify_ms)]
+    pv = [x + y for x, y in zip(plain_ms, verify_ms)]
     return {"served_updates_per_s": round(16 / (sum(served_ms) * 1e-3), 1),
             "served_p50_ms": round(percentile(served_ms, 50), 3), "served_p99_ms": round(percentile(served_ms, 99), 3),
             "plain_updates_per_s": round(16 / (sum(plain_ms) * 1e-3), 1),
             "plain_p50_ms": round(percentile(plain_ms, 50), 3), "plain_p99_ms": round(percentile(plain_ms, 99), 3),
-            "identical": bool(same)}
+            "verify_call_p50_ms": round(percentile(verify_ms, 50), 3),
+            "served_verified_updates_per_s": round(16 / (sum(sv) * 1e-3), 1),
+            "served_verified_p50_ms": round(percentile(sv, 50), 3),
+            "served_verified_p99_ms": round(percentile(sv, 99), 3),
+            "plain_verified_updates_per_s": round(16 / (sum(pv) * 1e-3), 1),
+            "all_16_verified_one_call_ms": round(batch_verify_ms, 3),
+            "verified_all": bool(all_ok), "identical": bool(same)}
 
 
 def _drive(srv: Server, stream, requests_prepared):
     """Open-loop driver: submit each request at its arrival offset.  The
     cyclic garbage collector is paused meanwhile (its pauses would land in
-    the tail latency of whichever requests are in flight)."""
+    the tail latency of whichever requests are in flight).  Latency runs to
+    the request's verified result (its proof, when the server does not
+    verify)."""
     gc.collect()
     gc.disable()
     try:
@@ -172,78 +261,132 @@ def _drive(srv: Server, stream, requests_prepared):
             r.future.result(timeout=300)
     finally:
         gc.enable()
-    lat = [(r.done - r.arrival) * 1e3 for r in reqs]
-    span = max(r.done for r in reqs) - t0
+    lat = [(r.finished - r.arrival) * 1e3 for r in reqs]
+    span = max(r.finished for r in reqs) - t0
     return reqs, lat, span
 
 
 def _parts(reqs) -> Dict:
-    """p99 of the latency components: submission lag behind the scheduled
-    arrival, queueing until a batch starts, and the batch's service time."""
+    """p50/p99 of the latency components: submission lag behind the
+    scheduled arrival, queueing until a batch starts, the batch's service
+    time, and the wait for the verification call after the proof."""
     lag = [(r.submitted - r.arrival) * 1e3 for r in reqs]
     wait = [(r.started - r.submitted) * 1e3 for r in reqs]
     svc = [(r.done - r.started) * 1e3 for r in reqs]
+    ver = [(r.finished - r.done) * 1e3 for r in reqs]
+    proof = [(r.done - r.arrival) * 1e3 for r in reqs]
     return {"p99_submit_lag_ms": round(percentile(lag, 99), 3), "p99_queue_ms": round(percentile(wait, 99), 3),
-            "p99_batch_ms": round(percentile(svc, 99), 3), "p50_batch_ms": round(percentile(svc, 50), 3)}
+            "p99_batch_ms": round(percentile(svc, 99), 3), "p50_batch_ms": round(percentile(svc, 50), 3),
+            "p50_verify_wait_ms": round(percentile(ver, 50), 3), "p99_verify_wait_ms": round(percentile(ver, 99), 3),
+            "p50_proof_ms": round(percentile(proof, 50), 3), "p99_proof_ms": round(percentile(proof, 99), 3)}
+
+
+def _plain_replay(plain, stream, prepared, backend) -> List[bytes]:
+    """The plain path over a stream in arrival order (the order the server
+    applies each session's requests in)."""
+    return [plain[sid].apply(backend, mi, raw, dv) for (_, sid, _, _), (mi, raw, dv) in zip(stream, prepared)]
 
 
 def served_stream(backend: HipBackend, counts=(96, 72, 48, 24), count: int = 4000, qps_points=(),
-                  plain_count: int = 600, lanes: int = 2, log=print) -> Dict:
+                  plain_verified_count: int = 24, lanes: int = 2, log=print) -> Dict:
     """Poisson mixed-n update stream: served (continuous batching, resident
-    state) at several arrival rates and saturated, against the plain path
-    run back to back (its maximum rate).  Every proof is compared.  Sessions
-    per size follow the request mix (a session's updates apply in order, so
-    the pool, not the batch limit, bounds how many requests can share a
-    batch)."""
+    state, every proof pairing-verified on the device before its request
+    resolves) at several arrival rates and saturated, against the plain
+    path run back to back (its maximum rate), with and without its own
+    per-update verification.  Every served proof of every run is compared
+    with the plain path's bytes.  Sessions per size follow the request mix
+    (a session's updates apply in order, so the pool, not the batch limit,
+    bounds how many requests can share a batch)."""
+    from .pairing import verify_update_proofs
     specs = session_pool(counts=counts, seed=6)
     store = SessionStore(backend, 32 << 30)
     t_setup = time.perf_counter()
     for sp in specs:
         store.admit(sp)
     log(f"# served: {len(specs)} sessions resident in {time.perf_counter() - t_setup:.1f}s")
-    # Plain reference run over the first plain_count requests (closed loop).
     base_stream = request_stream(specs, count, 1.0, seed=6)
     prepared = [merge_update(idx, ds) for _, _, idx, ds in base_stream]
     plain = {sp.sid: PlainSession(backend, sp) for sp in specs}
-    p_lat, p_out = [], []
+    # Plain path, closed loop, over the whole stream (the reference's per-call
+    # operator path: msm_batch + ntt), unverified.
+    p_lat = []
+    p_out = []
     t0 = time.perf_counter()
-    for (t, sid, _, _), (mi, raw, dv) in zip(base_stream[:plain_count], prepared[:plain_count]):
+    for (t, sid, _, _), (mi, raw, dv) in zip(base_stream, prepared):
         a = time.perf_counter()
         p_out.append(plain[sid].apply(backend, mi, raw, dv))
         p_lat.append((time.perf_counter() - a) * 1e3)
-    plain_rate = plain_count / (time.perf_counter() - t0)
-    log(f"# plain path: {plain_rate:.0f} updates/s")
-    out = {"sessions": len(specs), "plain_updates_per_s": round(plain_rate, 1),
-           "plain_p50_ms": round(percentile(p_lat, 50), 3), "plain_p99_ms": round(percentile(p_lat, 99), 3)}
-    # Served, saturated: every request queued at t = 0 (throughput ceiling).
-    # The served sessions start from the same anchors, so the first
-    # plain_count proofs must match the plain run exactly.
+    plain_rate = count / (time.perf_counter() - t0)
+    # Plain path with the reference's per-update verification (one pairing
+    # call per update), on fresh copies of the first requests' sessions.
+    fresh: Dict[int, PlainSession] = {}
+    pv_lat = []
+    pv_ok = True
+    for (t, sid, _, _), (mi, raw, dv) in zip(base_stream[:plain_verified_count], prepared[:plain_verified_count]):
+        if sid not in fresh:
+            fresh[sid] = PlainSession(backend, specs[sid])
+        a = time.perf_counter()
+        proof = fresh[sid].apply(backend, mi, raw, dv)
+        pv_ok &= verify_update_proofs(backend, [proof], [fresh[sid].divisor])[0]
+        pv_lat.append((time.perf_counter() - a) * 1e3)
+    plain_verified_rate = len(pv_lat) / (sum(pv_lat) * 1e-3)
+    log(f"# plain path: {plain_rate:.0f} updates/s, {plain_verified_rate:.1f}/s with per-update verification")
+    out = {"sessions": len(specs), "requests_per_run": count,
+           "plain_updates_per_s": round(plain_rate, 1),
+           "plain_p50_ms": round(percentile(p_lat, 50), 3), "plain_p99_ms": round(percentile(p_lat, 99), 3),
+           "plain_verified_updates_per_s": round(plain_verified_rate, 1),
+           "plain_verified_p50_ms": round(percentile(pv_lat, 50), 3),
+           "plain_verified_p99_ms": round(percentile(pv_lat, 99), 3),
+           "plain_verified_sample": plain_verified_count}
+    identical, verified, rejected = True, 0, 0
+    # Served, saturated: every request queued at t = 0 (throughput ceiling),
+    # verification on.  Sessions start from the same anchors as the plain
+    # ones, so every proof must equal the plain run's.
     srv = Server(store, lanes=lanes, max_batch=256)
     sat_stream = [(0.0, sid, idx, ds) for _, sid, idx, ds in base_stream]
     reqs, lat, span = _drive(srv, sat_stream, [(m, r) for m, r, _ in prepared])
-    same = [r.future.result() for r in reqs[:plain_count]] == p_out
+    identical &= [r.future.result() for r in reqs] == p_out
+    verified += srv.verified
+    rejected += srv.rejected
     sat_rate = count / span
     out.update({"served_saturated_updates_per_s": round(sat_rate, 1),
                 "served_saturated_p50_ms": round(percentile(lat, 50), 3),
                 "served_saturated_p99_ms": round(percentile(lat, 99), 3),
-                "mean_batch": round(float(np.mean(srv.batch_sizes)), 1), "identical_to_plain": bool(same),
-                "speedup_vs_plain": round(sat_rate / plain_rate, 2)})
-    log(f"# served saturated: {sat_rate:.0f} updates/s (x{sat_rate / plain_rate:.1f}), mean batch "
-        f"{np.mean(srv.batch_sizes):.1f}")
+                "mean_batch": round(float(np.mean(srv.batch_sizes)), 1),
+                "verify_calls": srv.verify_calls, "mean_verify_call": round(float(np.mean(srv.verify_sizes)), 1),
+                "p50_verify_call_ms": round(percentile(srv.verify_ms, 50), 3),
+                "speedup_vs_plain_verified": round(sat_rate / plain_verified_rate, 1),
+                "speedup_vs_plain_unverified": round(sat_rate / plain_rate, 2)})
+    log(f"# served saturated (verified): {sat_rate:.0f} updates/s, mean batch {np.mean(srv.batch_sizes):.1f}, "
+        f"{srv.verify_calls} verification calls")
+    srv.close()
+    # The same requests' throughput with the verification stage off, for
+    # comparison with the unverified plain rate (state continues; the
+    # plain sessions replay the same requests).
+    srv = Server(store, lanes=lanes, max_batch=256, verify=False)
+    reqs, lat, span = _drive(srv, sat_stream, [(m, r) for m, r, _ in prepared])
+    identical &= [r.future.result() for r in reqs] == _plain_replay(plain, base_stream, prepared, backend)
+    out["served_saturated_unverified_updates_per_s"] = round(count / span, 1)
     srv.close()
     # Open loop at set arrival rates (continuing the same sessions' states).
     pts = []
-    for q in qps_points or (0.5 * sat_rate, 0.8 * sat_rate, plain_rate):
+    for q in qps_points or (0.25 * sat_rate, 0.5 * sat_rate, plain_rate):
         stream = request_stream(specs, count, q, seed=int(q) + 7)
-        prep = [merge_update(idx, ds)[:2] for _, _, idx, ds in stream]
+        prep = [merge_update(idx, ds) for _, _, idx, ds in stream]
         srv = Server(store, lanes=lanes, max_batch=256)
-        reqs, lat, span = _drive(srv, stream, prep)
+        reqs, lat, span = _drive(srv, stream, [(m, r) for m, r, _ in prep])
+        identical &= [r.future.result() for r in reqs] == _plain_replay(plain, stream, prep, backend)
+        verified += srv.verified
+        rejected += srv.rejected
         pts.append({"qps": round(q, 1), "updates_per_s": round(count / span, 1),
                     "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
                     "mean_batch": round(float(np.mean(srv.batch_sizes)), 1), **_parts(reqs)})
-        log(f"# served at {q:.0f} QPS: p50 {pts[-1]['p50_ms']} ms, p99 {pts[-1]['p99_ms']} ms")
+        log(f"# served at {q:.0f} QPS (verified): p50 {pts[-1]['p50_ms']} ms, p99 {pts[-1]['p99_ms']} ms")
         srv.close()
     out["open_loop"] = pts
+    out.update({"verified_all": bool(rejected == 0 and verified == count * (1 + len(pts)) and pv_ok),
+                "proofs_verified": verified, "proofs_rejected": rejected,
+                "identical_to_plain": bool(identical), "proofs_compared": count * (2 + len(pts))})
     store.close()
     return out
 

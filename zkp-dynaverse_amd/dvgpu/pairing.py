@@ -104,3 +104,45 @@ def first_failure(results: Sequence[Sequence[bool]]):
                 if not results[w][i]:
                     return w, i
     return None
+
+
+def generators(backend: HipBackend) -> Tuple[bytes, bytes]:
+    """(g1, g2) canonical generator encodings, [1]G through the setup entry."""
+    from .codec import encode_fr
+    gens = getattr(backend, "_generators", None)
+    if gens is None:
+        one = encode_fr(1)
+        gens = (backend.fixed_base_encoded(one, 1, False), backend.fixed_base_encoded(one, 1, True))
+        backend._generators = gens
+    return gens
+
+
+def kzg_verify_encoded(backend: HipBackend, commitments: bytes, values: bytes, openings: bytes, divisors: bytes,
+                       count: int) -> Tuple[bytes, float]:
+    """dvgpu_bn254_kzg_verify_batch: result byte t is 1 when
+    e(C_t - [y_t] g1, g2) == e(W_t, D_t) (dynaverse.rs:447-451).  Inputs are
+    count x 64 / 32 / 64 / 128 canonical bytes; returns (results, kernel ms)."""
+    if not count:
+        return b"", 0.0
+    if len(commitments) != 64 * count or len(values) != 32 * count or len(openings) != 64 * count or \
+            len(divisors) != 128 * count:
+        raise ValueError("KZG verification inputs need count x 64 / 32 / 64 / 128 bytes")
+    g1, g2 = generators(backend)
+    res = ctypes.create_string_buffer(count)
+    ms = ctypes.c_float(0)
+    err = _err()
+    bufs = [ctypes.c_char_p(a) for a in (g1, g2, commitments, values, openings, divisors)]
+    _check(backend.lib.dvgpu_bn254_kzg_verify_batch(backend.handle, *bufs, count, res, ctypes.byref(ms), err, 256),
+           err)
+    return res.raw[:count], ms.value
+
+
+def verify_update_proofs(backend: HipBackend, proofs: Sequence[bytes], divisors: Sequence[bytes]) -> List[bool]:
+    """Pairing verification of update proofs C' (64) || W' (64) || y' (32):
+    proof t holds when e(C' - [y'] g1, g2) == e(W', D_t), D_t = [tau - z]g2 of
+    its session."""
+    if not proofs:
+        return []
+    raw, _ = kzg_verify_encoded(backend, b"".join(p[:64] for p in proofs), b"".join(p[128:160] for p in proofs),
+                                b"".join(p[64:128] for p in proofs), b"".join(divisors), len(proofs))
+    return [b == 1 for b in raw]

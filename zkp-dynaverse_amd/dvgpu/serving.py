@@ -14,13 +14,19 @@ HBM, inside two arenas carved out of a fixed memory budget:
   cached value y = f(z).
 
 An update request changes k coefficients; its proof is (C', W', y'), which
-satisfies the KZG opening relation f'(tau) - y' = q'(tau) (tau - z).  One
+satisfies the KZG opening relation f'(tau) - y' = q'(tau) (tau - z), checked
+as the reference verifies an update (crates/protocol/src/dynaverse.rs:
+447-451): e(C' - [y'] g1, g2) == e(W', D) with the session's frozen G2 term
+D = [tau - z] g2, on the device (dvgpu_bn254_kzg_verify_batch).  One
 ``dvgpu_ext_update_batch`` call serves a whole batch of requests (two sparse
 window-table MSMs per request, scatter-add, linear term, a fused
 multi-length NTT).  The ``Server`` keeps ``lanes`` such calls in flight, each
 on its own owner and stream; a request that arrives while others run joins
 the next batch that a free lane takes (continuous batching).  Requests of one
-session are applied in arrival order, one at a time.
+session are applied in arrival order, one at a time.  With verification on
+(the default), a verification stage with its own owner and stream pairing-
+checks every proof, many batches' proofs per call, while the lanes go on
+with later batches; a request's future resolves only after its proof passed.
 
 ``PlainSession`` is the one-call-at-a-time baseline through the reference's
 operator API (``msm_batch`` over host-gathered bases, ``ntt`` on host data),
@@ -130,6 +136,7 @@ class Session:
     fr_base: int          # fr: [f (n)] [evals (n)] [l (n)] [y]
     busy: bool = False
     record: tuple = ()
+    divisor: bytes = b""  # [tau - z] g2, the verifier's frozen G2 term (128 B)
 
     @property
     def n(self):
@@ -148,6 +155,15 @@ class Session:
         u.deltas = ctypes.cast(ctypes.c_char_p(deltas), ctypes.c_void_p).value if len(idx) else None
         u._keep = (idx, deltas)       # the host arrays live as long as the struct
         return u
+
+
+def session_divisor(backend: HipBackend, tau: int, z: int) -> bytes:
+    """[tau - z] g2 (canonical, 128 B) through the setup entry point."""
+    return backend.fixed_base_encoded(encode_fr((tau - z) % R), 1, True)
+
+
+class VerificationError(RuntimeError):
+    """A served proof failed its pairing check."""
 
 
 def session_rows(n: int) -> int:
@@ -209,6 +225,7 @@ class SessionStore:
         tau, t, z, coeffs = spec.derive()
         s = Session(spec, n.bit_length() - 1, tau, t, z, row, fb)
         s.record = session_record(s)
+        s.divisor = session_divisor(self.b, tau, z)
         a, u, l = session_scalars(tau, z, n)
         tmp = self.b.to_device(np.concatenate([a.reshape(-1), u.reshape(-1)]))
         rb = 64 * TABLE_ROW
@@ -313,7 +330,12 @@ class Request:
     future: Future = field(default_factory=Future)
     submitted: float = 0.0
     started: float = 0.0
-    done: float = 0.0
+    done: float = 0.0        # proof computed
+    verified: float = 0.0    # proof verified (0 when the server does not verify)
+
+    @property
+    def finished(self) -> float:
+        return self.verified or self.done
 
 
 def request_cost(n: int, k: int) -> int:
@@ -344,14 +366,34 @@ class HipLane:
         self.b.close()
 
 
+class VerifierLane:
+    """The verification stage's worker: its own owner and stream, so a
+    verification call overlaps the lanes' update batches on the device."""
+
+    def __init__(self):
+        self.b = HipBackend()
+
+    def run(self, proofs: Sequence[bytes], divisors: Sequence[bytes]) -> List[bool]:
+        from .pairing import verify_update_proofs
+        return verify_update_proofs(self.b, proofs, divisors)
+
+    def close(self):
+        self.b.close()
+
+
 class Server:
     """Continuous-batching request loop over a SessionStore.
 
     `lane_factory()` makes one worker lane (an object with run(store, batch)
-    and close()); the default is a HipLane."""
+    and close()); the default is a HipLane.  `verify` adds the verification
+    stage (`verifier_factory()`, default VerifierLane: run(proofs, divisors)
+    -> list of bools); each request then resolves after its proof passed, or
+    with a VerificationError.  close() fails every request still queued with
+    RuntimeError("server closed")."""
 
     def __init__(self, store: SessionStore, lanes: int = 2, max_batch: int = 256,
-                 scratch_budget: int = 1 << 30, max_k: int = 64, lane_factory=None):
+                 scratch_budget: int = 1 << 30, max_k: int = 64, lane_factory=None, verify: bool = True,
+                 verifier_factory=None, max_verify: int = 1 << 16):
         self.store = store
         self.lanes = lanes
         self.max_batch = max_batch
@@ -369,11 +411,25 @@ class Server:
         self.workers = [factory() for _ in range(lanes)]
         self.work: List[collections.deque] = [collections.deque() for _ in range(lanes)]
         self.idle = list(range(lanes))
+        # verification stage: finished batches wait here for one verify call
+        self.vcv = threading.Condition()
+        self.vqueue: collections.deque = collections.deque()
+        self.vstop = False
+        self.max_verify = max_verify
+        self.verify_calls = 0
+        self.verify_sizes: List[int] = []
+        self.verify_ms: List[float] = []
+        self.verified = 0
+        self.rejected = 0
+        self.verifier = (verifier_factory or VerifierLane)() if verify else None
         self.threads = [threading.Thread(target=self._lane, args=(i,), daemon=True) for i in range(lanes)]
         self.sched = threading.Thread(target=self._schedule, daemon=True)
+        self.vthread = threading.Thread(target=self._verify_loop, daemon=True) if verify else None
         for t in self.threads:
             t.start()
         self.sched.start()
+        if self.vthread:
+            self.vthread.start()
 
     def submit(self, sid: int, indices: np.ndarray, deltas: bytes, arrival: Optional[float] = None) -> Request:
         if sid not in self.store.sessions:
@@ -450,21 +506,76 @@ class Server:
                 self.device_ms.append(ms)
                 self.idle.append(i)
                 self.cv.notify_all()
+            if self.verifier is not None and err is None:
+                with self.vcv:
+                    self.vqueue.append((batch, proofs))
+                    self.vcv.notify_all()
+                continue
             for k, r in enumerate(batch):
                 if err is not None:
                     r.future.set_exception(err)
                 else:
                     r.future.set_result(proofs[k])
 
+    def _verify_loop(self):
+        """Takes every finished batch waiting (up to max_verify proofs) into
+        one verification call; resolves each request by its own result."""
+        while True:
+            with self.vcv:
+                while not self.vstop and not self.vqueue:
+                    self.vcv.wait()
+                if not self.vqueue:
+                    return
+                reqs, proofs = [], []
+                while self.vqueue and (not reqs or len(reqs) + len(self.vqueue[0][0]) <= self.max_verify):
+                    batch, pr = self.vqueue.popleft()
+                    reqs.extend(batch)
+                    proofs.extend(pr)
+            t0 = time.perf_counter()
+            try:
+                ok = self.verifier.run(proofs, [self.store.sessions[r.sid].divisor for r in reqs])
+            except Exception as e:          # delivered to every request of the call
+                for r in reqs:
+                    r.future.set_exception(e)
+                continue
+            t1 = time.perf_counter()
+            self.verify_calls += 1
+            self.verify_sizes.append(len(reqs))
+            self.verify_ms.append((t1 - t0) * 1e3)
+            for r, p, good in zip(reqs, proofs, ok):
+                r.verified = t1
+                if good:
+                    self.verified += 1
+                    r.future.set_result(p)
+                else:
+                    self.rejected += 1
+                    r.future.set_exception(VerificationError(f"session {r.sid}: update proof failed its pairing check"))
+
     def close(self):
+        """Stops taking batches; batches already on a lane finish (and are
+        verified); every request still queued fails with "server closed"."""
         with self.cv:
             self.stop_flag = True
             self.cv.notify_all()
         self.sched.join()
         for t in self.threads:
             t.join()
+        if self.vthread is not None:
+            with self.vcv:
+                self.vstop = True
+                self.vcv.notify_all()
+            self.vthread.join()
+            self.verifier.close()
         for w in self.workers:
             w.close()
+        with self.cv:
+            left = [r for q in self.pending.values() for r in q]
+            for q in self.pending.values():
+                q.clear()
+            self.ready.clear()
+        for r in left:
+            if not r.future.done():
+                r.future.set_exception(RuntimeError("server closed"))
 
 
 # ---- one-call-at-a-time baseline ----------------------------------------------
@@ -486,6 +597,7 @@ class PlainSession:
         n = spec.n
         tau, t, z, coeffs = spec.derive()
         self.spec, self.n, self.t, self.z, self.tau = spec, n, t, z, tau
+        self.divisor = session_divisor(backend, tau, z)
         a, u, l = session_scalars(tau, z, n)
         self.P = _fixed_base_host(backend, a)
         self.U = _fixed_base_host(backend, u)
@@ -514,6 +626,15 @@ class PlainSession:
             self.y = (self.y + d * self.l[i]) % R
         self.evals = backend.ntt_encoded(self.f.tobytes(), self.n, False)
         return self.C + self.W + self.y.to_bytes(32, "little")
+
+    def apply_verified(self, backend: HipBackend, idx: np.ndarray, deltas: bytes, dvals: List[int]) -> bytes:
+        """apply, then the reference's per-update verification (dynaverse.rs:
+        447-451) as one pairing-check call for this one proof."""
+        from .pairing import verify_update_proofs
+        proof = self.apply(backend, idx, deltas, dvals)
+        if not verify_update_proofs(backend, [proof], [self.divisor])[0]:
+            raise VerificationError(f"session {self.spec.sid}: update proof failed its pairing check")
+        return proof
 
 
 # ---- seeded request streams ------------------------------------------------------

@@ -37,3 +37,29 @@ def poly_eval(coeff_bytes: np.ndarray, x: int) -> int:
     for i in range(arr.shape[0] - 1, -1, -1):
         acc = (acc * x + int.from_bytes(arr[i].tobytes(), "little")) % r
     return acc
+
+
+def small_sequence(seed: int = 1, n: int = 256, updates: int = 8, k: int = 4, workload: str = "hotspot",
+                   hot: int = 16):
+    """The plain n = 256 update sequence: (tau, coefficients (n, 32), updates).
+
+    tau uniform nonzero Fr, coefficients uniform Fr; each of `updates`
+    updates rewrites k indices (repeats allowed; a repeat rewrites twice, the
+    last value wins) drawn from a fixed seed-chosen set of `hot` indices
+    ("hotspot") or uniformly over n ("spreading"), each with a fresh uniform
+    Fr value.  An update is a list of (index, new value)."""
+    if workload not in ("hotspot", "spreading"):
+        raise ValueError("workload is hotspot or spreading")
+    rng = ChaCha20Stream(seed)
+    tau = rng.nonzero_fr()
+    coeffs = rng.uniform_fr_array(n)
+    hot_set = []
+    while len(hot_set) < hot:
+        i = rng.below(n)
+        if i not in hot_set:
+            hot_set.append(i)
+    ups = []
+    for _ in range(updates):
+        idx = [hot_set[rng.below(hot)] if workload == "hotspot" else rng.below(n) for _ in range(k)]
+        ups.append(list(zip(idx, rng.uniform_fr(k))))
+    return tau, coeffs, ups
