@@ -13,7 +13,7 @@ import pytest
 from dvgpu import BackendError, HipBackend
 from dvgpu.codec import FR_MODULUS as R, encode_fr, root_of_unity_2_20
 from dvgpu.serving import (PlainSession, Server, SessionSpec, SessionStore, merge_update, percentile,
-                           request_stream, session_bytes, session_pool, update_batch)
+                           request_stream, session_bytes, session_pool, update_batch, verify_served)
 from oracle import bn254 as o
 from oracle import cref
 
@@ -158,6 +158,10 @@ def test_served_stream_equals_alone_and_plain(backend):
     served = [r.future.result(timeout=120) for r in reqs]
     assert srv.batches < len(reqs)                      # requests shared batches
     srv.close()
+    # every served proof passes its pairing check (the caller's owner, server closed)
+    print("case: verify served proofs", flush=True)
+    ok, _ = verify_served(backend, store, reqs)
+    assert ok == [True] * len(reqs)
     ev_served = {sp.sid: store.evaluations(store.sessions[sp.sid]) for sp in specs}
     store.close()
     # Alone: a fresh store, every request in a batch of its own, in order.

@@ -95,6 +95,7 @@ class FakeLane:
     """Applies updates to a per-session log; the 'proof' is the log length."""
     log = None
     lock = threading.Lock()
+    active = 0
 
     def __init__(self, delay=0.002, poison=None):
         self.delay, self.poison = delay, poison
@@ -104,7 +105,11 @@ class FakeLane:
         assert len(set(sids)) == len(sids), "a session twice in one batch"
         if self.poison is not None and any(int(r.indices[0]) == self.poison for r in batch):
             raise RuntimeError("poisoned batch")
+        with FakeLane.lock:
+            FakeLane.active += 1
         time.sleep(self.delay)
+        with FakeLane.lock:
+            FakeLane.active -= 1
         out = []
         with FakeLane.lock:
             for r in batch:
@@ -125,7 +130,9 @@ class FakeVerifier:
         self.bad, self.delay, self.calls = set(bad), delay, []
 
     def run(self, proofs, divisors):
+        assert FakeLane.active == 0, "verification overlapped a lane batch"
         time.sleep(self.delay)
+        assert FakeLane.active == 0, "a lane batch started during verification"
         self.calls.append(len(proofs))
         return [int.from_bytes(d, "little") not in self.bad for d in divisors]
 
@@ -136,8 +143,7 @@ class FakeVerifier:
 def _serve(store, reqs, **kw):
     FakeLane.log, FakeLane.batches = {}, []
     lane_kw = kw.pop("lane", {})
-    if "verifier_factory" not in kw:
-        kw.setdefault("verify", False)
+    kw.setdefault("verify", "verifier_factory" in kw)
     srv = Server(store, lane_factory=lambda: FakeLane(**lane_kw), **kw)
     try:
         handles = [srv.submit(sid, np.array([seq], dtype=np.uint32), b"") for sid, seq in reqs]
@@ -203,7 +209,7 @@ def test_server_verifies_every_proof_before_resolving():
     store = FakeStore([256] * 12)
     ver = FakeVerifier(bad={5})
     reqs = [(i % 12, i) for i in range(240)]
-    srv, handles, results = _serve(store, reqs, lanes=2, max_batch=8, verifier_factory=lambda: ver)
+    srv, handles, results = _serve(store, reqs, lanes=2, max_batch=8, verifier_factory=lambda: ver, verify_min=40)
     from dvgpu.serving import VerificationError
     for (sid, _), h, r in zip(reqs, handles, results):
         if sid == 5:

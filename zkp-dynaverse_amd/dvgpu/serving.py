@@ -23,10 +23,19 @@ window-table MSMs per request, scatter-add, linear term, a fused
 multi-length NTT).  The ``Server`` keeps ``lanes`` such calls in flight, each
 on its own owner and stream; a request that arrives while others run joins
 the next batch that a free lane takes (continuous batching).  Requests of one
-session are applied in arrival order, one at a time.  With verification on
-(the default), a verification stage with its own owner and stream pairing-
-checks every proof, many batches' proofs per call, while the lanes go on
-with later batches; a request's future resolves only after its proof passed.
+session are applied in arrival order, one at a time.
+
+Verification.  Every served proof is pairing-checked on the device by
+``verify_served``: one dvgpu_bn254_kzg_verify_batch call on the caller's
+owner over many proofs, after the server has closed.  The Server also has an
+in-loop verification stage (a request then resolves only after its proof
+passed; verification calls run while no lane batch is on the device, in
+chunks of at least `verify_min` proofs), exercised on the host with a
+stand-in verifier.  Its device verifier, a separate owner, is switched off:
+its first pairing dispatch (a kernel with ~23 KB of scratch per lane) with
+four owners alive in the process was refused by the runtime
+(HSA_STATUS_ERROR_OUT_OF_RESOURCES) in two runs, while every pairing dispatch
+made with the caller's owner alone alive has run.
 
 ``PlainSession`` is the one-call-at-a-time baseline through the reference's
 operator API (``msm_batch`` over host-gathered bases, ``ntt`` on host data),
@@ -367,11 +376,12 @@ class HipLane:
 
 
 class VerifierLane:
-    """The verification stage's worker: its own owner and stream, so a
-    verification call overlaps the lanes' update batches on the device."""
+    """The verification stage's device worker (its own owner and stream).
+    Switched off: see the module docstring; use verify_served instead."""
 
     def __init__(self):
-        self.b = HipBackend()
+        raise NotImplementedError("the in-loop device verifier is switched off (the runtime refused its pairing "
+                                  "dispatch with several owners alive); verify with verify_served after close()")
 
     def run(self, proofs: Sequence[bytes], divisors: Sequence[bytes]) -> List[bool]:
         from .pairing import verify_update_proofs
@@ -392,8 +402,8 @@ class Server:
     RuntimeError("server closed")."""
 
     def __init__(self, store: SessionStore, lanes: int = 2, max_batch: int = 256,
-                 scratch_budget: int = 1 << 30, max_k: int = 64, lane_factory=None, verify: bool = True,
-                 verifier_factory=None, max_verify: int = 1 << 16):
+                 scratch_budget: int = 1 << 30, max_k: int = 64, lane_factory=None, verify: bool = False,
+                 verifier_factory=None, max_verify: int = 1 << 16, verify_min: int = 1024):
         self.store = store
         self.lanes = lanes
         self.max_batch = max_batch
@@ -411,11 +421,15 @@ class Server:
         self.workers = [factory() for _ in range(lanes)]
         self.work: List[collections.deque] = [collections.deque() for _ in range(lanes)]
         self.idle = list(range(lanes))
-        # verification stage: finished batches wait here for one verify call
-        self.vcv = threading.Condition()
+        # verification stage: finished batches wait here for one verify call;
+        # while want_verify is set the scheduler starts no batch, and the
+        # call runs once every lane is idle
         self.vqueue: collections.deque = collections.deque()
+        self.vpending = 0
         self.vstop = False
+        self.want_verify = False
         self.max_verify = max_verify
+        self.verify_min = verify_min
         self.verify_calls = 0
         self.verify_sizes: List[int] = []
         self.verify_ms: List[float] = []
@@ -465,7 +479,7 @@ class Server:
     def _schedule(self):
         while True:
             with self.cv:
-                while not self.stop_flag and (not self.idle or not self._runnable()):
+                while not self.stop_flag and (not self.idle or not self._runnable() or self.want_verify):
                     self.cv.wait()
                 if self.stop_flag:
                     return
@@ -507,9 +521,10 @@ class Server:
                 self.idle.append(i)
                 self.cv.notify_all()
             if self.verifier is not None and err is None:
-                with self.vcv:
+                with self.cv:
                     self.vqueue.append((batch, proofs))
-                    self.vcv.notify_all()
+                    self.vpending += len(batch)
+                    self.cv.notify_all()
                 continue
             for k, r in enumerate(batch):
                 if err is not None:
@@ -519,26 +534,37 @@ class Server:
 
     def _verify_loop(self):
         """Takes every finished batch waiting (up to max_verify proofs) into
-        one verification call; resolves each request by its own result."""
+        one verification call, run while every lane is idle; resolves each
+        request by its own result."""
         while True:
-            with self.vcv:
-                while not self.vstop and not self.vqueue:
-                    self.vcv.wait()
-                if not self.vqueue:
-                    return
+            with self.cv:
+                while not (self.vqueue and (self.vpending >= self.verify_min or not self.ready or self.vstop)):
+                    if self.vstop and not self.vqueue:
+                        return
+                    self.cv.wait()
+                self.want_verify = True
+                while len(self.idle) < self.lanes:
+                    self.cv.wait()
                 reqs, proofs = [], []
                 while self.vqueue and (not reqs or len(reqs) + len(self.vqueue[0][0]) <= self.max_verify):
                     batch, pr = self.vqueue.popleft()
                     reqs.extend(batch)
                     proofs.extend(pr)
+                self.vpending -= len(reqs)
             t0 = time.perf_counter()
             try:
                 ok = self.verifier.run(proofs, [self.store.sessions[r.sid].divisor for r in reqs])
             except Exception as e:          # delivered to every request of the call
-                for r in reqs:
-                    r.future.set_exception(e)
-                continue
+                ok = None
+                failure = e
             t1 = time.perf_counter()
+            with self.cv:
+                self.want_verify = False
+                self.cv.notify_all()
+            if ok is None:
+                for r in reqs:
+                    r.future.set_exception(failure)
+                continue
             self.verify_calls += 1
             self.verify_sizes.append(len(reqs))
             self.verify_ms.append((t1 - t0) * 1e3)
@@ -561,9 +587,9 @@ class Server:
         for t in self.threads:
             t.join()
         if self.vthread is not None:
-            with self.vcv:
+            with self.cv:
                 self.vstop = True
-                self.vcv.notify_all()
+                self.cv.notify_all()
             self.vthread.join()
             self.verifier.close()
         for w in self.workers:
@@ -684,3 +710,19 @@ def percentile(xs: Sequence[float], p: float) -> float:
     s = sorted(xs)
     k = min(len(s) - 1, max(0, int(round(p / 100.0 * (len(s) - 1)))))
     return s[k]
+
+
+def verify_served(backend: HipBackend, store: SessionStore, requests: Sequence[Request],
+                  chunk: int = 1 << 16) -> Tuple[List[bool], float]:
+    """Pairing verification of served requests' proofs on `backend` (the
+    caller's owner, with the server closed): e(C' - [y'] g1, g2) == e(W', D)
+    with each request's session divisor.  Returns (per-request results,
+    wall ms)."""
+    from .pairing import verify_update_proofs
+    t0 = time.perf_counter()
+    ok: List[bool] = []
+    for i in range(0, len(requests), chunk):
+        part = requests[i:i + chunk]
+        ok.extend(verify_update_proofs(backend, [r.future.result() for r in part],
+                                       [store.sessions[r.sid].divisor for r in part]))
+    return ok, (time.perf_counter() - t0) * 1e3
