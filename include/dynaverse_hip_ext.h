@@ -69,7 +69,8 @@ int dvgpu_ext_fixed_base_device(void* owner, const void* scalars, size_t n, int 
 typedef struct DvgpuTuning {
     int32_t msm_window_bits[25];  /* by log2 MSM size: window width c, 4..20 */
     int32_t ntt_pass_stages[21];  /* by log2 NTT length: butterfly stages per
-                                     LDS pass (radix 2^s), 1..11 */
+                                     LDS round trip, 1 (radix-2) or 2..11
+                                     (radix-4) */
     int32_t msm_task_points;      /* max points one accumulation task adds, 1..1024 */
     int32_t msm_chunk;            /* buckets per first-level reduction thread: 1, 2, 4, 8 */
     int32_t msm_device_finalize_min; /* batches of at least this many MSMs sum their

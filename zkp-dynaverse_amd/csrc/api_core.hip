@@ -22,7 +22,7 @@ Owner::Owner() {
     ntt.attach(&quota);
     ntt_batch.attach(&quota);
     ntt_batch_inv.attach(&quota);
-    domain.fwd.quota = domain.inv.quota = &quota;
+    domain.set_quota(&quota);
     stage_a.quota = stage_b.quota = stage_c.quota = &quota;
 }
 

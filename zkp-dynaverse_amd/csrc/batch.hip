@@ -45,7 +45,7 @@ struct BatchObj {
     void attach(Quota* q) {
         w1.attach(q);
         w2.attach(q);
-        domain.fwd.quota = domain.inv.quota = q;
+        domain.set_quota(q);
         win1.quota = win2.quota = fin1.quota = fin2.quota = report_dev.quota = q;
     }
 
@@ -452,7 +452,7 @@ int dvgpu_ext_batch_create(void* owner, const DvgpuBatchSpec* spec, int use_grap
             std::memcpy(rw, spec->root, 32);
             require(ntt_root_has_order_2_20(rw), "NTT root must have order 2^20");
             std::memcpy(b->domain.pass_stages, o->domain.pass_stages, sizeof(b->domain.pass_stages));
-            b->domain.fwd.quota = b->domain.inv.quota = &o->quota;
+            b->domain.set_quota(&o->quota);
             ntt_domain_prepare(b->domain, rw, st);
             // Entries run in order; consecutive entries with no overlap fuse.
             std::vector<std::vector<NttBatchEntry>> groups;

@@ -47,7 +47,7 @@ struct KzgSrs {
     DeviceBuffer coeffs, scal, conv, h, out;
     void attach(Quota* q) {
         powers.quota = basis.quota = coeffs.quota = scal.quota = conv.quota = h.quota = out.quota = q;
-        domain.fwd.quota = domain.inv.quota = q;
+        domain.set_quota(q);
         ntt.attach(q);
     }
 };
@@ -212,7 +212,7 @@ int gfft_split(int L, uint64_t rows) {
 void gfft_rows(const NttDomain& d, uint32_t* data, int L, uint64_t rows, bool inverse, hipStream_t st) {
     if (!L || !rows) return;
     const int k = gfft_split(L, rows), m = L - k;
-    const auto* tw = static_cast<const uint32_t*>(inverse ? d.inv.ptr : d.fwd.ptr);
+    const auto* tw = static_cast<const uint32_t*>(inverse ? d.inv.pow.ptr : d.fwd.pow.ptr);
     const unsigned stage_grid = grid_for(rows << (L - 1), GFFT_THREADS), diag_grid = grid_for(rows << L, GFFT_THREADS);
     if (!inverse) {
         for (int s = L - 1; s >= m; --s) {

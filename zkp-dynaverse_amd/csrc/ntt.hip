@@ -1,14 +1,33 @@
 // Natural-order radix-2 NTT over BN254 Fr (decimation in frequency).
 //
-// A transform of length 2^L runs as P = ceil(L / 11) LDS passes plus one
-// permutation pass.  A pass owns a bit range [a, a + S) of the element index
-// and runs those S butterfly stages on tiles of 2^S x C elements held in LDS
-// (C consecutive low indices keep the global accesses contiguous).  The
-// butterfly is (x, y) -> (x + y, (x - y) * w): x - y is formed limb-wise
-// without a carry pass and fed straight into the Montgomery product.  The last
-// pass is followed by a bit-reversal gather that also leaves Montgomery form
-// (and applies 1/n on the inverse) with one product per element.
+// A transform of length n = 2^L runs as one LDS pass (L <= 11) or as two
+// (the four-step split n = 2^S1 x 2^a, S1 = L/2, a = L - S1):
+//   pass 1: for every column x2 < 2^a, the S1 DIF stages over the high index
+//           bits with the full twiddles omega_(2^(a+s+1))^(j 2^a + x2); the
+//           column's outputs leave in natural order, row k1 = its frequency;
+//   pass 2: for every row k1, the a DIF stages over the row with the
+//           sub-transform twiddles; output k2 of row k1 is X[k1 + 2^S1 k2].
+// Inside a pass every sub-transform sits in LDS (nine 29-bit limb planes), its
+// DIF leaves frequency k at position bitrev(k), and the store reads position
+// bitrev(k) for output k: no separate permutation pass exists.  A tile holds
+// U sub-transforms side by side (U columns, or U rows), so pass 1 reads and
+// writes runs of U elements and pass 2 writes runs of U elements.
+//
+// The transform is linear and its twiddles are Montgomery values, so it maps
+// the raw words of its input to the raw words of its output whatever form
+// they are in: canonical in gives canonical out, Montgomery form in gives
+// Montgomery form out, with no conversion product.  The inverse applies 1/n
+// with one product per element as the last pass stores.
+//
+// Butterflies: (x, y) -> (x + y, (x - y) w), x - y formed limb-wise without a
+// carry pass and fed straight into the Montgomery product.  Launches with
+// enough work run two stages per LDS round trip (radix-4 groups of four
+// elements in registers, two lazy sums reduced once); small launches keep
+// one stage per round trip, which doubles the threads at work.  The stage
+// tables hold twiddles pre-split into limbs; the last round of a
+// sub-transform with twiddles of order 2 and 4 skips its unit products.
 #include "ntt.hpp"
+#include "ntt_tile.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -20,119 +39,48 @@ using namespace bn254;
 
 namespace {
 
-constexpr int TILE = 2048;  // elements per LDS tile (72 KiB as nine limb planes)
-constexpr int THREADS = 256;        // element-wise kernels
-constexpr int PASS_THREADS = 1024;  // DIF passes: one butterfly per thread and stage on a full tile
+using namespace ntt;
 
-// Threads for a DIF pass over tiles of `tile` elements (one per butterfly).
-inline unsigned pass_threads(uint32_t tile) {
-    return unsigned(std::min<uint32_t>(PASS_THREADS, std::max<uint32_t>(64, tile / 2)));
-}
-constexpr int MAX_STAGES = 11;
+constexpr int THREADS = 256;  // element-wise kernels
 
-__device__ __forceinline__ Fr ld8(const uint32_t* p) {
-    const uint4 a = reinterpret_cast<const uint4*>(p)[0];
-    const uint4 b = reinterpret_cast<const uint4*>(p)[1];
-    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    return unpack<FrParams>(w);
-}
-
-__device__ __forceinline__ void st8(uint32_t* p, const Fr& v) {
-    uint32_t w[8];
-    pack(v, w);
-    reinterpret_cast<uint4*>(p)[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    reinterpret_cast<uint4*>(p)[1] = make_uint4(w[4], w[5], w[6], w[7]);
-}
-
-// One DIF pass over index bits [a, a + S) of `rows` transforms of length 2^L;
-// `g` is the tile of this block.  CANON_IN: the input holds canonical
-// integers (converted on load).
-// The tile lives in LDS as nine limb planes (limb k of element e at
-// lds[k * TILE + e]): butterflies read and write limbs directly, with no
-// packing, and consecutive lanes touch consecutive words.
-__device__ __forceinline__ Fr lds_ld(const uint32_t* __restrict__ lds, uint32_t e) {
-    Fr x;
-#pragma unroll
-    for (int k = 0; k < LIMBS; ++k) x.l[k] = lds[k * TILE + e];
-    return x;
-}
-
-__device__ __forceinline__ void lds_st(uint32_t* __restrict__ lds, uint32_t e, const Fr& x) {
-#pragma unroll
-    for (int k = 0; k < LIMBS; ++k) lds[k * TILE + e] = x.l[k];
-}
-
-template <bool CANON_IN>
-__device__ __forceinline__ void dif_tile(uint32_t* __restrict__ lds, const uint32_t* __restrict__ in,
-                                         uint32_t* __restrict__ out, int L, int a, int S, int C,
-                                         const uint32_t* __restrict__ tw, size_t g) {
-    const size_t n = size_t(1) << L;
-    const uint32_t tile_elems = (1u << S) * C;
-    const size_t tiles_per_row = n / tile_elems;
-    const size_t row = g / tiles_per_row;
-    const size_t t = g % tiles_per_row;
-    const size_t lo_groups = (size_t(1) << a) / C;
-    const size_t hi = t / lo_groups;
-    const size_t lo0 = (t % lo_groups) * C;
-    const size_t base = row * n + (hi << (a + S)) + lo0;
-
-    for (uint32_t e = threadIdx.x; e < tile_elems; e += blockDim.x) {
-        const uint32_t mid = e / C, c = e % C;
-        const uint32_t* src = in + (base + (size_t(mid) << a) + c) * 8;
-        const uint4 v0 = reinterpret_cast<const uint4*>(src)[0];
-        const uint4 v1 = reinterpret_cast<const uint4*>(src)[1];
-        const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-        lds_st(lds, e, CANON_IN ? to_mont<FrParams>(w) : unpack<FrParams>(w));
-    }
-    __syncthreads();
-
-    const uint32_t half = tile_elems / 2;
-    for (int sl = S - 1; sl >= 0; --sl) {
-        const int s = a + sl;
-        const uint32_t h = 1u << sl;
-        for (uint32_t b = threadIdx.x; b < half; b += blockDim.x) {
-            const uint32_t c = b % C, q = b / C;
-            const uint32_t m1 = ((q >> sl) << (sl + 1)) | (q & (h - 1));
-            const uint32_t m2 = m1 + h;
-            const size_t j = (size_t(q & (h - 1)) << a) + lo0 + c;
-            const Fr w = ld8(tw + (j << (NTT_MAX_LOG - 1 - s)) * 8);
-            const uint32_t e1 = m1 * C + c, e2 = m2 * C + c;
-            const Fr x = lds_ld(lds, e1), y = lds_ld(lds, e2);
-            lds_st(lds, e1, reduce(norm(add_w(x, y))));
-            lds_st(lds, e2, mul(sub_w(x, y), w));
-        }
-        __syncthreads();
-    }
-
-    for (uint32_t e = threadIdx.x; e < tile_elems; e += blockDim.x) {
-        const uint32_t mid = e / C, c = e % C;
-        st8(out + (base + (size_t(mid) << a) + c) * 8, lds_ld(lds, e));
-    }
-}
-
-template <bool CANON_IN>
-__global__ void __launch_bounds__(PASS_THREADS) k_dif_pass(const uint32_t* __restrict__ in,
-                                                      uint32_t* __restrict__ out, int L, int a, int S,
-                                                      int C, const uint32_t* __restrict__ tw) {
+template <bool R4>
+__global__ void __launch_bounds__(R4 ? 512 : 1024) k_ntt_pass(NttPass p) {
     __shared__ uint32_t lds[LIMBS * TILE];
-    dif_tile<CANON_IN>(lds, in, out, L, a, S, C, tw, blockIdx.x);
+    ntt_tile<R4>(lds, p, xcd_order(blockIdx.x, gridDim.x));
 }
 
-// out[row][k] = x[row][rev_L(k)] * scale, scale given as raw limbs: a plain
-// integer leaves Montgomery form, a Montgomery value keeps it.
-template <bool CANON_OUT>
-__global__ void __launch_bounds__(THREADS) k_bitrev_scale(const uint32_t* __restrict__ in,
-                                                          uint32_t* __restrict__ out, int L,
-                                                          size_t total, Fr scale) {
-    const size_t i = size_t(blockIdx.x) * THREADS + threadIdx.x;
-    if (i >= total) return;
-    const size_t n = size_t(1) << L;
-    const size_t row = i >> L, k = i & (n - 1);
-    const uint32_t src = L ? (__brev(uint32_t(k)) >> (32 - L)) : 0;
-    Fr x = mul(ld8(in + (row * n + src) * 8), scale);
-    if (CANON_OUT) x = canon(x);
-    st8(out + i * 8, x);
+// ---- batches of transforms of several lengths -------------------------------
+
+struct NttBatchDesc {
+    NttPass pass[MAX_PASSES];
+    uint32_t block_base[MAX_PASSES];  // first block of this entry in pass p
+};
+
+// Last entry whose first block of pass p is at or below `key` (an entry with
+// no block in pass p shares its base with the next one, which wins).
+__device__ __forceinline__ uint32_t find_entry(const NttBatchDesc* __restrict__ d, uint32_t m, int p, uint32_t key) {
+    uint32_t lo = 0, hi = m - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (d[mid].block_base[p] <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
+
+template <bool R4>
+__global__ void __launch_bounds__(R4 ? 512 : 1024) k_ntt_pass_batch(const NttBatchDesc* __restrict__ descs,
+                                                                    uint32_t m, int p) {
+    __shared__ uint32_t lds[LIMBS * TILE];
+    __shared__ uint32_t entry;
+    const uint32_t lb = xcd_order(blockIdx.x, gridDim.x);
+    if (threadIdx.x == 0) entry = find_entry(descs, m, p, lb);
+    __syncthreads();
+    const NttBatchDesc& d = descs[entry];
+    ntt_tile<R4>(lds, d.pass[p], lb - d.block_base[p]);
+}
+
+// ---- twiddle tables ------------------------------------------------------------
 
 __global__ void k_twiddles(const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
                            uint32_t* __restrict__ tw, uint32_t count) {
@@ -141,58 +89,15 @@ __global__ void k_twiddles(const uint32_t* __restrict__ lo, const uint32_t* __re
     st8(tw + size_t(k) * 8, canon(mul(ld8(lo + (k & 1023) * 8), ld8(hi + (k >> 10) * 8))));
 }
 
-// ---- batches of transforms of several lengths -------------------------------
-
-// Entry `e` of a fused batch, with its pass schedule (the same schedule as a
-// single call of that length: pass p covers bits [a_p, a_p + S_p)).
-struct NttBatchDesc {
-    const uint32_t* in;
-    uint32_t* out;
-    uint32_t* work;
-    uint32_t L, rows, passes, inverse;
-    uint32_t a[NTT_BATCH_MAX_PASSES], S[NTT_BATCH_MAX_PASSES], C[NTT_BATCH_MAX_PASSES];
-    uint32_t block_base[NTT_BATCH_MAX_PASSES];  // first block of this entry in pass p
-    uint32_t elem_base;                          // first element in the bit-reversal launch
-    uint32_t scale[8];                           // raw limbs of the output scale
-};
-
-__device__ __forceinline__ uint32_t find_entry(const NttBatchDesc* __restrict__ d, uint32_t m, int p, uint32_t key) {
-    uint32_t lo = 0, hi = m - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        const uint32_t b = p < 0 ? d[mid].elem_base : d[mid].block_base[p];
-        if (b <= key) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+__global__ void k_stage_table(const uint32_t* __restrict__ pw, uint32_t* __restrict__ out, uint32_t total) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < total) stage_table_entry(pw, out, e);
 }
 
-template <bool CANON_IN>
-__global__ void __launch_bounds__(PASS_THREADS) k_dif_pass_batch(const NttBatchDesc* __restrict__ descs, uint32_t m,
-                                                            int p, const uint32_t* __restrict__ tw_fwd,
-                                                            const uint32_t* __restrict__ tw_inv) {
-    __shared__ uint32_t lds[LIMBS * TILE];
-    __shared__ uint32_t entry;
-    if (threadIdx.x == 0) entry = find_entry(descs, m, p, blockIdx.x);
-    __syncthreads();
-    const NttBatchDesc& d = descs[entry];
-    dif_tile<CANON_IN>(lds, CANON_IN ? d.in : d.work, d.work, int(d.L), int(d.a[p]), int(d.S[p]), int(d.C[p]),
-                       d.inverse ? tw_inv : tw_fwd, blockIdx.x - d.block_base[p]);
-}
-
-__global__ void __launch_bounds__(THREADS) k_bitrev_batch(const NttBatchDesc* __restrict__ descs, uint32_t m,
-                                                          uint32_t total) {
-    const uint32_t i = blockIdx.x * THREADS + threadIdx.x;
-    if (i >= total) return;
-    const NttBatchDesc& d = descs[find_entry(descs, m, -1, i)];
-    const uint32_t local = i - d.elem_base;
-    const uint32_t L = d.L;
-    const size_t n = size_t(1) << L;
-    const size_t row = local >> L, k = local & (n - 1);
-    const uint32_t src = L ? (__brev(uint32_t(k)) >> (32 - L)) : 0;
-    const uint32_t* from = d.passes ? d.work : d.in;
-    const Fr scale = unpack<FrParams>(d.scale);
-    st8(d.out + size_t(local) * 8, canon(mul(ld8(from + (row * n + src) * 8), scale)));
+__global__ void k_pass1_table(const uint32_t* __restrict__ pw, uint32_t* __restrict__ out, uint32_t L, uint32_t S,
+                              uint32_t logCT, uint32_t total) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < total) pass1_table_entry(pw, out, L, S, logCT, e);
 }
 
 Fr fr_from_words(const uint32_t (&w)[8]) { return to_mont<FrParams>(w); }
@@ -202,6 +107,57 @@ bool fr_is_one(const Fr& x) {
     for (int i = 0; i < LIMBS; ++i)
         if (c.l[i] != one.l[i]) return false;
     return true;
+}
+
+// The Montgomery limbs of 1/2^L.
+Fr inverse_scale(int L) {
+    uint32_t nw[8] = {0};
+    nw[L / 32] = 1u << (L % 32);
+    return canon(inv(to_mont<FrParams>(nw)));
+}
+
+// The pass-1 table of length 2^L for one direction, built on first use.
+const uint32_t* pass1_table(NttTables& t, int L, hipStream_t stream) {
+    DeviceBuffer& b = t.pass1[L];
+    if (!b.ptr) {
+        const uint32_t total = pass1_table_entries(L);
+        uint32_t* out = b.as<uint32_t>(size_t(total) * LIMBS);
+        k_pass1_table<<<grid_for(total, THREADS), THREADS, 0, stream>>>(static_cast<const uint32_t*>(t.pow.ptr), out,
+                                                                         uint32_t(L), uint32_t(L / 2),
+                                                                         pass1_log_ct(L), total);
+        HIP_CHECK_LAUNCH();
+    }
+    return static_cast<const uint32_t*>(b.ptr);
+}
+
+// The passes of one entry, tables built as needed.
+LengthPlan plan_entry(NttDomain& d, const uint32_t* in, uint32_t* out, uint32_t* work, int L, uint64_t rows,
+                      bool inverse, hipStream_t stream) {
+    require(L >= 0 && L <= NTT_MAX_LOG, "NTT length must be a power of two <= 2^20");
+    require(rows >= 1 && rows < (uint64_t(1) << 31), "NTT row count out of range");
+    NttTables& t = inverse ? d.inv : d.fwd;
+    const Fr scale = inverse && L ? inverse_scale(L) : fp_one<FrParams>();
+    const uint32_t* p1 = L > SINGLE_MAX_LOG ? pass1_table(t, L, stream) : nullptr;
+    const LengthPlan lp = plan_length(in, out, work, L, rows, static_cast<const uint32_t*>(t.stage.ptr), p1,
+                                      inverse && L ? &scale : nullptr);
+    for (int p = 0; p < lp.passes; ++p)
+        require(lp.pass[p].mode == PASS_ROWS || rows * lp.pass[p].tiles < (uint64_t(1) << 31), "NTT batch too large");
+    return lp;
+}
+
+// Stages per LDS round trip for a launch: the tuning table's choice for the
+// length (1: radix-2 rounds, 2 or more: radix-4 rounds), else by size.
+bool radix4_for(const NttDomain& d, int L, uint64_t blocks, uint32_t tile) {
+    const int forced = d.pass_stages[L];
+    return forced ? forced >= 2 : use_radix4(blocks, tile);
+}
+
+void launch_pass(const NttPass& p, bool r4, uint32_t blocks, uint32_t tile, hipStream_t stream) {
+    if (r4)
+        k_ntt_pass<true><<<blocks, threads_for(tile, true), 0, stream>>>(p);
+    else
+        k_ntt_pass<false><<<blocks, threads_for(tile, false), 0, stream>>>(p);
+    HIP_CHECK_LAUNCH();
 }
 
 }  // namespace
@@ -218,10 +174,12 @@ void ntt_domain_prepare(NttDomain& d, const uint32_t (&root)[8], hipStream_t str
     if (d.ready && std::memcmp(d.root_words, root, 32) == 0) return;
     require(ntt_root_has_order_2_20(root), "NTT root must have order 2^20");
     d.ready = false;
+    for (auto* t : {&d.fwd, &d.inv})
+        for (auto& b : t->pass1) b.release();  // tables of another root
     const uint32_t count = 1u << (NTT_MAX_LOG - 1);
     const Fr w = fr_from_words(root);
     const Fr wi = inv(w);
-    DeviceBuffer stage;
+    DeviceBuffer staging;
     std::vector<uint32_t> host(size_t(1024 + 512) * 8 * 2);
     for (int dir = 0; dir < 2; ++dir) {
         const Fr base = dir ? wi : w;
@@ -243,88 +201,46 @@ void ntt_domain_prepare(NttDomain& d, const uint32_t (&root)[8], hipStream_t str
             acc = mul(acc, step);
         }
     }
-    uint32_t* dev = stage.as<uint32_t>(host.size());
+    uint32_t* dev = staging.as<uint32_t>(host.size());
     HIP_CHECK(hipMemcpyAsync(dev, host.data(), host.size() * 4, hipMemcpyHostToDevice, stream));
-    uint32_t* f = d.fwd.as<uint32_t>(size_t(count) * 8);
-    uint32_t* b = d.inv.as<uint32_t>(size_t(count) * 8);
-    k_twiddles<<<grid_for(count, 256), 256, 0, stream>>>(dev, dev + 1024 * 8, f, count);
-    HIP_CHECK_LAUNCH();
-    k_twiddles<<<grid_for(count, 256), 256, 0, stream>>>(dev + 1536 * 8, dev + 1536 * 8 + 1024 * 8, b, count);
-    HIP_CHECK_LAUNCH();
-    HIP_CHECK(hipStreamSynchronize(stream));  // `stage` and `host` die here
+    const uint32_t stage_total = (1u << NTT_STAGE_TABLE_LOG) - 1;
+    for (int dir = 0; dir < 2; ++dir) {
+        NttTables& t = dir ? d.inv : d.fwd;
+        uint32_t* pw = t.pow.as<uint32_t>(size_t(count) * 8);
+        const uint32_t* src = dev + size_t(dir) * 1536 * 8;
+        k_twiddles<<<grid_for(count, THREADS), THREADS, 0, stream>>>(src, src + 1024 * 8, pw, count);
+        HIP_CHECK_LAUNCH();
+        uint32_t* st = t.stage.as<uint32_t>(size_t(stage_total) * LIMBS);
+        k_stage_table<<<grid_for(stage_total, THREADS), THREADS, 0, stream>>>(pw, st, stage_total);
+        HIP_CHECK_LAUNCH();
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));  // `staging` and `host` die here
     std::memcpy(d.root_words, root, 32);
     d.ready = true;
 }
 
-// Scale applied by the bit-reversal pass to Montgomery work values:
-// canonical output wants a plain integer (1 or n^-1); Montgomery output wants
-// the Montgomery form of the same value.
-static Fr output_scale(int L, bool inverse, bool canonical) {
-    Fr scale = fp_zero<FrParams>();
-    if (canonical) {
-        if (!inverse) {
-            scale.l[0] = 1;
-        } else {
-            uint32_t nw[8] = {0};
-            nw[L / 32] = 1u << (L % 32);
-            const Fr ninv_m = inv(to_mont<FrParams>(nw));
-            uint32_t plain[8];
-            from_mont(ninv_m, plain);
-            scale = unpack<FrParams>(plain);
-        }
-    } else {
-        if (!inverse) {
-            scale = fp_one<FrParams>();
-        } else {
-            uint32_t nw[8] = {0};
-            nw[L / 32] = 1u << (L % 32);
-            scale = canon(inv(to_mont<FrParams>(nw)));
-        }
-    }
-    return scale;
-}
-
-static void run_rows(const NttDomain& d, const uint32_t* in, uint32_t* out, int L, size_t rows,
-                     bool inverse, bool canonical, NttWorkspace& ws, hipStream_t stream) {
+static void run_rows(NttDomain& d, const uint32_t* in, uint32_t* out, int L, size_t rows, bool inverse,
+                     NttWorkspace& ws, hipStream_t stream) {
     require(d.ready, "NTT domain not prepared");
     require(L >= 0 && L <= NTT_MAX_LOG, "NTT length must be a power of two <= 2^20");
-    const size_t n = size_t(1) << L;
-    const size_t total = n * rows;
-    if (!total) return;
-    uint32_t* work = ws.a.as<uint32_t>(total * 8);
-    const uint32_t* tw = static_cast<const uint32_t*>(inverse ? d.inv.ptr : d.fwd.ptr);
-    if (L == 0) {
-        HIP_CHECK(hipMemcpyAsync(out, in, total * 32, hipMemcpyDeviceToDevice, stream));
-        return;
-    }
-    const int max_stages = d.pass_stages[L] >= 1 && d.pass_stages[L] <= MAX_STAGES ? d.pass_stages[L] : MAX_STAGES;
-    const int passes = (L + max_stages - 1) / max_stages;
-    int top = L;
-    bool first = true;
-    for (int p = 0; p < passes; ++p) {
-        const int remaining_passes = passes - p;
-        const int S = (top + remaining_passes - 1) / remaining_passes;
-        const int a = top - S;
-        const int C = int(std::min<size_t>(size_t(1) << a, size_t(TILE >> S)));
-        const size_t blocks = rows * (n / (size_t(1u << S) * C));
-        if (first && canonical)
-            k_dif_pass<true><<<unsigned(blocks), pass_threads((1u << S) * C), 0, stream>>>(in, work, L, a, S, C, tw);
-        else
-            k_dif_pass<false><<<unsigned(blocks), pass_threads((1u << S) * C), 0, stream>>>(first ? in : work, work, L,
-                                                                                           a, S, C, tw);
-        HIP_CHECK_LAUNCH();
-        first = false;
-        top = a;
-    }
-    const Fr scale = output_scale(L, inverse, canonical);
-    if (canonical)
-        k_bitrev_scale<true><<<grid_for(total, THREADS), THREADS, 0, stream>>>(work, out, L, total, scale);
-    else
-        k_bitrev_scale<false><<<grid_for(total, THREADS), THREADS, 0, stream>>>(work, out, L, total, scale);
-    HIP_CHECK_LAUNCH();
+    if (!rows) return;
+    uint32_t* work = L > SINGLE_MAX_LOG ? ws.a.as<uint32_t>((rows << L) * 8) : nullptr;
+    const LengthPlan lp = plan_entry(d, in, out, work, L, rows, inverse, stream);
+    for (int p = 0; p < lp.passes; ++p)
+        launch_pass(lp.pass[p], radix4_for(d, L, lp.blocks[p], lp.tile[p]), lp.blocks[p], lp.tile[p], stream);
 }
 
-void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& entries, NttBatchPlan& plan,
+void ntt_rows(NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows, bool inverse,
+              NttWorkspace& ws, hipStream_t stream) {
+    run_rows(d, in, out, log_n, rows, inverse, ws, stream);
+}
+
+void ntt_rows_mont(NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows, bool inverse,
+                   NttWorkspace& ws, hipStream_t stream) {
+    run_rows(d, in, out, log_n, rows, inverse, ws, stream);
+}
+
+void ntt_batch_prepare(NttDomain& d, const std::vector<NttBatchEntry>& entries, NttBatchPlan& plan,
                        hipStream_t stream) {
     require(d.ready, "NTT domain not prepared");
     require(!entries.empty(), "empty NTT batch");
@@ -359,67 +275,51 @@ void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& ent
         for (size_t k = a; k < outs.size() && outs[k].lo < hi; ++k)
             require(outs[k].entry == j, "NTT batch entries overlap");
     }
+    // Work area: every two-pass entry gets its own slice (offsets first,
+    // pointers once the buffer is sized).
+    uint64_t work = 0;
+    std::vector<uint64_t> work_at(entries.size(), 0);
+    for (size_t i = 0; i < entries.size(); ++i)
+        if (entries[i].log_n > SINGLE_MAX_LOG) {
+            work_at[i] = work;
+            work += uint64_t(entries[i].rows) << entries[i].log_n;
+        }
+    // Growing a device buffer frees one that queued work may still use.
+    const size_t desc_bytes = entries.size() * sizeof(NttBatchDesc);
+    if (std::max<uint64_t>(work, 1) * 32 > plan.work.bytes || desc_bytes > plan.descs.bytes)
+        HIP_CHECK(hipStreamSynchronize(stream));
+    uint32_t* wp = plan.work.as<uint32_t>(std::max<uint64_t>(work, 1) * 8);
     std::vector<NttBatchDesc> descs(entries.size());
-    Fr scales[2][NTT_MAX_LOG + 1];
-    bool have_scale[2][NTT_MAX_LOG + 1] = {};
-    uint64_t work = 0, elems = 0;
-    uint32_t blocks[NTT_BATCH_MAX_PASSES] = {0}, tile_max[NTT_BATCH_MAX_PASSES] = {0};
+    uint64_t blocks[NTT_BATCH_MAX_PASSES] = {0, 0}, groups[NTT_BATCH_MAX_PASSES] = {0, 0};
+    uint32_t tile_max[NTT_BATCH_MAX_PASSES] = {0, 0};
+    int forced[NTT_BATCH_MAX_PASSES] = {-1, -1};  // the entries' common tuned choice, -2 when they differ
     int max_passes = 0;
     for (size_t i = 0; i < entries.size(); ++i) {
         const NttBatchEntry& x = entries[i];
+        const LengthPlan lp = plan_entry(d, x.in, x.out, wp + work_at[i] * 8, x.log_n, x.rows, x.inverse, stream);
         NttBatchDesc& e = descs[i];
-        const int L = x.log_n;
-        const size_t n = size_t(1) << L;
-        int stages = d.pass_stages[L] >= 1 && d.pass_stages[L] <= MAX_STAGES ? d.pass_stages[L] : MAX_STAGES;
-        stages = std::max(stages, (L + NTT_BATCH_MAX_PASSES - 1) / NTT_BATCH_MAX_PASSES);
-        const int passes = L ? (L + stages - 1) / stages : 0;
-        e.in = x.in;
-        e.out = x.out;
-        e.L = uint32_t(L);
-        e.rows = uint32_t(x.rows);
-        e.passes = uint32_t(passes);
-        e.inverse = x.inverse ? 1u : 0u;
-        e.work = reinterpret_cast<uint32_t*>(work);  // offset until the workspace is known
-        int top = L;
         for (int p = 0; p < NTT_BATCH_MAX_PASSES; ++p) {
-            e.block_base[p] = blocks[p];
-            if (p >= passes) continue;
-            const int remaining = passes - p;
-            const int S = (top + remaining - 1) / remaining;
-            const int a = top - S;
-            const int C = int(std::min<size_t>(size_t(1) << a, size_t(TILE >> S)));
-            e.a[p] = uint32_t(a);
-            e.S[p] = uint32_t(S);
-            e.C[p] = uint32_t(C);
-            const uint64_t nb = uint64_t(x.rows) * (n / (size_t(1u << S) * C));
-            tile_max[p] = std::max<uint32_t>(tile_max[p], (1u << S) * uint32_t(C));
-            require(uint64_t(blocks[p]) + nb < (uint64_t(1) << 31), "NTT batch too large");
-            blocks[p] += uint32_t(nb);
-            top = a;
+            e.block_base[p] = uint32_t(blocks[p]);
+            if (p >= lp.passes) {
+                e.pass[p] = lp.pass[0];  // never read: this entry has no block in pass p
+                continue;
+            }
+            e.pass[p] = lp.pass[p];
+            const int f = d.pass_stages[x.log_n] ? (d.pass_stages[x.log_n] >= 2 ? 2 : 1) : 0;
+            forced[p] = forced[p] == -1 || forced[p] == f ? f : -2;
+            blocks[p] += lp.blocks[p];
+            groups[p] += uint64_t(lp.blocks[p]) * (lp.tile[p] >> 2);
+            tile_max[p] = std::max(tile_max[p], lp.tile[p]);
+            require(blocks[p] < (uint64_t(1) << 31), "NTT batch too large");
         }
-        max_passes = std::max(max_passes, passes);
-        e.elem_base = uint32_t(elems);
-        elems += uint64_t(x.rows) << L;
-        require(elems < (uint64_t(1) << 31), "NTT batch too large");
-        if (passes) work += uint64_t(x.rows) << L;
-        // length 1: identity; otherwise 1 or n^-1, one inversion per distinct length
-        if (passes && !have_scale[x.inverse][L]) {
-            scales[x.inverse][L] = output_scale(L, x.inverse, true);
-            have_scale[x.inverse][L] = true;
-        }
-        pack(passes ? scales[x.inverse][L] : fp_one<FrParams>(), e.scale);
+        max_passes = std::max(max_passes, lp.passes);
     }
-    // Growing a device buffer frees one that queued work may still use.
-    if (std::max<uint64_t>(work, 1) * 32 > plan.work.bytes || descs.size() * sizeof(NttBatchDesc) > plan.descs.bytes)
-        HIP_CHECK(hipStreamSynchronize(stream));
-    uint32_t* wp = plan.work.as<uint32_t>(std::max<uint64_t>(work, 1) * 8);
-    for (auto& e : descs) e.work = wp + reinterpret_cast<uintptr_t>(e.work) * 8;
     plan.count = uint32_t(entries.size());
     plan.max_passes = max_passes;
-    plan.total_elems = uint32_t(elems);
     for (int p = 0; p < NTT_BATCH_MAX_PASSES; ++p) {
-        plan.pass_blocks[p] = blocks[p];
-        plan.pass_threads[p] = pass_threads(tile_max[p]);
+        plan.pass_blocks[p] = uint32_t(blocks[p]);
+        plan.pass_radix4[p] = forced[p] > 0 ? forced[p] == 2 : groups[p] >= R4_MIN_GROUPS;
+        plan.pass_threads[p] = threads_for(std::max<uint32_t>(tile_max[p], 2), plan.pass_radix4[p]);
     }
     // The previous upload from the pinned descriptors may still be pending.
     if (plan.upload_pending) {
@@ -427,9 +327,9 @@ void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& ent
         plan.upload_pending = false;
     }
     auto* host = plan.host_descs.as<NttBatchDesc>(descs.size());
-    std::memcpy(host, descs.data(), descs.size() * sizeof(NttBatchDesc));
+    std::memcpy(host, descs.data(), desc_bytes);
     auto* dev = plan.descs.as<NttBatchDesc>(descs.size());
-    HIP_CHECK(hipMemcpyAsync(dev, host, descs.size() * sizeof(NttBatchDesc), hipMemcpyHostToDevice, stream));
+    HIP_CHECK(hipMemcpyAsync(dev, host, desc_bytes, hipMemcpyHostToDevice, stream));
     if (!plan.uploaded) HIP_CHECK(hipEventCreateWithFlags(&plan.uploaded, hipEventDisableTiming));
     HIP_CHECK(hipEventRecord(plan.uploaded, stream));
     plan.upload_pending = true;
@@ -438,30 +338,14 @@ void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& ent
 void ntt_batch_enqueue(const NttDomain& d, const NttBatchPlan& plan, hipStream_t stream) {
     require(d.ready && plan.count, "NTT batch not prepared");
     const auto* descs = static_cast<const NttBatchDesc*>(plan.descs.ptr);
-    const auto* fwd = static_cast<const uint32_t*>(d.fwd.ptr);
-    const auto* inv_tw = static_cast<const uint32_t*>(d.inv.ptr);
     for (int p = 0; p < plan.max_passes; ++p) {
         if (!plan.pass_blocks[p]) continue;
-        if (p == 0)
-            k_dif_pass_batch<true><<<plan.pass_blocks[p], plan.pass_threads[p], 0, stream>>>(descs, plan.count, p, fwd,
-                                                                                              inv_tw);
+        if (plan.pass_radix4[p])
+            k_ntt_pass_batch<true><<<plan.pass_blocks[p], plan.pass_threads[p], 0, stream>>>(descs, plan.count, p);
         else
-            k_dif_pass_batch<false><<<plan.pass_blocks[p], plan.pass_threads[p], 0, stream>>>(descs, plan.count, p,
-                                                                                               fwd, inv_tw);
+            k_ntt_pass_batch<false><<<plan.pass_blocks[p], plan.pass_threads[p], 0, stream>>>(descs, plan.count, p);
         HIP_CHECK_LAUNCH();
     }
-    k_bitrev_batch<<<grid_for(plan.total_elems, THREADS), THREADS, 0, stream>>>(descs, plan.count, plan.total_elems);
-    HIP_CHECK_LAUNCH();
-}
-
-void ntt_rows(const NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows,
-              bool inverse, NttWorkspace& ws, hipStream_t stream) {
-    run_rows(d, in, out, log_n, rows, inverse, true, ws, stream);
-}
-
-void ntt_rows_mont(const NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows,
-                   bool inverse, NttWorkspace& ws, hipStream_t stream) {
-    run_rows(d, in, out, log_n, rows, inverse, false, ws, stream);
 }
 
 }  // namespace dvgpu

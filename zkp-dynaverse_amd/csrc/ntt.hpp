@@ -7,16 +7,48 @@
 
 namespace dvgpu {
 
-constexpr int NTT_MAX_LOG = 20;  // domain built from a root of order 2^20
+constexpr int NTT_MAX_LOG = 20;        // domain built from a root of order 2^20
+constexpr int NTT_SINGLE_MAX_LOG = 11; // lengths up to 2^11 run as one LDS pass
+constexpr int NTT_STAGE_TABLE_LOG = 11;
 
-// Twiddles omega^k (k < 2^19) for one direction, in Montgomery memory form,
-// built from the 2^20-th root the caller supplies.
+// Twiddle tables of one direction, built from the 2^20-th root the caller
+// supplies (its inverse for the inverse direction):
+//  * pow: omega^k for k < 2^19, Montgomery memory form (8 words each); the
+//    group transforms of kzg.hip read it, and the tables below derive from it;
+//  * stage: the sub-transform twiddles omega_(2^(s+1))^j, j < 2^s, for local
+//    stages s < 11 (shared by every single pass and every second pass), as
+//    nine 29-bit limb planes per stage;
+//  * pass1[L] (2^12 <= 2^L <= 2^20, built on first use): the first pass of a
+//    two-pass transform of length 2^L, per column x2 < 2^a (a = L - L/2),
+//    stage s < L/2 and j < 2^s, omega_(2^(a+s+1))^(j 2^a + x2), as limb planes
+//    grouped by tiles of columns so that a tile reads them in runs.
+struct NttTables {
+    DeviceBuffer pow, stage;
+    DeviceBuffer pass1[NTT_MAX_LOG + 1];
+    void set_quota(Quota* q) {
+        pow.quota = stage.quota = q;
+        for (auto& b : pass1) b.quota = q;
+    }
+    void release() {
+        pow.release();
+        stage.release();
+        for (auto& b : pass1) b.release();
+    }
+};
+
 struct NttDomain {
     uint32_t root_words[8] = {0};  // canonical root this domain was built from
     bool ready = false;
-    DeviceBuffer fwd, inv;         // 2^19 x 8 words each
-    // Butterfly stages per LDS pass by log2 length (0 = the maximum, 11).
+    NttTables fwd, inv;
+    // Butterfly stages per LDS round trip by log2 length (the tuning table):
+    // 1 = radix-2 rounds, 2 or more = radix-4 rounds, 0 = by launch size.
+    // The pass split itself is fixed: one pass up to 2^11, two passes of
+    // L/2 and L - L/2 stages above.
     int pass_stages[NTT_MAX_LOG + 1] = {0};
+    void set_quota(Quota* q) {
+        fwd.set_quota(q);
+        inv.set_quota(q);
+    }
     void release() {
         fwd.release();
         inv.release();
@@ -38,20 +70,20 @@ struct NttWorkspace {
 };
 
 // rows independent transforms of length n = 2^log_n, each row contiguous.
-// in: canonical LE words (8 per element); out: canonical LE words.
-// in and out may alias.  `scale_inverse` applies 1/n on inverse.
-void ntt_rows(const NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows,
+// The transform is linear and its twiddles are Montgomery values, so words
+// in give words out in the same form: canonical in, canonical out
+// (ntt_rows); Montgomery memory form in and out (ntt_rows_mont).  Outputs are
+// fully reduced.  in and out may alias.  The inverse applies 1/n.
+void ntt_rows(NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows,
               bool inverse, NttWorkspace& ws, hipStream_t stream);
-
-// Same, but input and output in Montgomery memory form (resident data).
-void ntt_rows_mont(const NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows,
+void ntt_rows_mont(NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows,
                    bool inverse, NttWorkspace& ws, hipStream_t stream);
 
 // ---- fused batches of transforms of several lengths --------------------------
 // Every entry (rows x 2^log_n, canonical in and out) runs in the same launch
-// sequence: one launch per pass for all entries, then one bit-reversal
-// launch.  Entries must not overlap each other (each may run in place).
-constexpr int NTT_BATCH_MAX_PASSES = 6;
+// sequence: one launch per pass for all entries (two at most).  Entries must
+// not overlap each other (each may run in place).
+constexpr int NTT_BATCH_MAX_PASSES = 2;
 
 struct NttBatchEntry {
     const uint32_t* in;
@@ -62,10 +94,11 @@ struct NttBatchEntry {
 };
 
 struct NttBatchPlan {
-    uint32_t count = 0, total_elems = 0;
+    uint32_t count = 0;
     int max_passes = 0;
     uint32_t pass_blocks[NTT_BATCH_MAX_PASSES] = {0};
     uint32_t pass_threads[NTT_BATCH_MAX_PASSES] = {0};
+    bool pass_radix4[NTT_BATCH_MAX_PASSES] = {false, false};
     DeviceBuffer descs, work;
     HostBuffer host_descs;
     // Recorded after the descriptor upload: the next prepare waits for it
@@ -84,9 +117,10 @@ struct NttBatchPlan {
     }
 };
 
-// Plan, size the workspace and upload the descriptors (waits for this
-// plan's previous descriptor upload first).  Must precede ntt_batch_enqueue; capture-safe afterwards.
-void ntt_batch_prepare(const NttDomain& d, const std::vector<NttBatchEntry>& entries, NttBatchPlan& plan,
+// Plan, size the workspace, build any missing twiddle table and upload the
+// descriptors (waits for this plan's previous descriptor upload first).
+// Must precede ntt_batch_enqueue; capture-safe afterwards.
+void ntt_batch_prepare(NttDomain& d, const std::vector<NttBatchEntry>& entries, NttBatchPlan& plan,
                        hipStream_t stream);
 void ntt_batch_enqueue(const NttDomain& d, const NttBatchPlan& plan, hipStream_t stream);
 

@@ -819,7 +819,7 @@ int dvgpu_resident_configure(void* backend, uint64_t limit, uint64_t journal, co
         const uint64_t previous_limit = o->quota.limit;
         o->quota.limit = limit;
         try {
-            s->domain.fwd.quota = s->domain.inv.quota = &o->quota;
+            s->domain.set_quota(&o->quota);
             s->ntt.attach(&o->quota);
             for (DeviceBuffer* b : {&s->affine_ws, &s->scalar_ws, &s->g2_ws, &s->host_stage_dev}) b->quota = &o->quota;
             s->quota(journal, 0);

@@ -1,6 +1,6 @@
 """Autotuner for the launch shapes: MSM window width c per log2 size, the
 bucket-task shape (points per accumulation task, buckets per reduction
-thread) and the NTT radix (butterfly stages per LDS pass) per log2 length.
+thread) and the NTT radix (butterfly stages per LDS round trip) per log2 length.
 
 The sweep runs on the device it tunes, over device-resident inputs shaped like
 the batched workloads: at every log2 size L it times a batch of
@@ -164,6 +164,9 @@ def tune_shape(backend, inp: _Inputs, windows: Dict[int, int], logs, reps: int, 
 
 
 def tune_ntt(backend, logs, reps: int, log=print) -> Dict[int, dict]:
+    """Per length 2^L (in rows of 2^20 elements in total), butterfly stages
+    per LDS round trip: 1 (radix-2 rounds) against 2 (radix-4 rounds); both
+    must give identical results."""
     out = {}
     base = get(backend)
     vals = ChaCha20Stream(79).uniform_fr_array(1 << 20).reshape(-1)
@@ -172,14 +175,7 @@ def tune_ntt(backend, logs, reps: int, log=print) -> Dict[int, dict]:
     for L in logs:
         rows = 1 << (20 - L)
         times, ref = {}, None
-        passes_seen = set()
-        for s in range(11, 2, -1):
-            passes = -(-L // s)
-            if passes in passes_seen:
-                continue                       # same pass count -> same split
-            passes_seen.add(passes)
-            if passes > 4:
-                break
+        for s in (1, 2):
             apply(backend, dict(base, ntt_pass_stages={L: s}))
             backend.ntt_device(d_in.ptr, d_out.ptr, 1 << L, rows, False)
             got = d_out.download().tobytes()
@@ -193,7 +189,7 @@ def tune_ntt(backend, logs, reps: int, log=print) -> Dict[int, dict]:
             times[s] = statistics.median(ms)
         best = min(times, key=times.get)
         out[L] = {"stages": best, "rows": rows, "ms": {str(s): round(t, 4) for s, t in times.items()}}
-        log(f"ntt 2^{L} x {rows}: best stages/pass={best} {times[best]:.4f} ms; "
+        log(f"ntt 2^{L} x {rows}: best stages/round={best} {times[best]:.4f} ms; "
             + " ".join(f"{s}:{t:.4f}" for s, t in times.items()))
     apply(backend, base)
     d_in.free()
