@@ -59,8 +59,11 @@ def test_srs_points_match_oracle(env):
         assert got == o.g1_to_bytes(o.g1_mul(pow(tau, i, o.R))), i
 
 
-@pytest.mark.parametrize("window", [0, 16])
+@pytest.mark.parametrize("window", [0, 12, 13, 14, 16])
 def test_msm_2p20_device_entry(env, window):
+    """The default (c = 15, chunk-plane reduction), narrower windows (bucket-
+    or chunk-plane reductions, more windows) and c = 16 (2^15 buckets: the
+    global-memory digit path)."""
     b, tau, _, scal, d_srs, d_sc = env
     print(f"2^20 G1 MSM, device entry, window {window or 'default'}", flush=True)
     assert b.msm_device(d_srs.ptr, d_sc.ptr, N, window_bits=window) == oracle_commitment(scal, tau)
@@ -97,3 +100,13 @@ def test_msm_2p20_linearity(env):
     assert o.add(a, c) == s
     d_t.free()
     d_st.free()
+
+
+@pytest.mark.parametrize("log_n", [17, 18, 19])
+def test_msm_prefix_sizes(env, log_n):
+    """2^17, 2^18 and 2^19 (their default window widths) over the SRS prefix,
+    against the oracle's closed form."""
+    b, tau, _, scal, d_srs, d_sc = env
+    n = 1 << log_n
+    print(f"G1 MSM 2^{log_n}, device entry", flush=True)
+    assert b.msm_device(d_srs.ptr, d_sc.ptr, n) == oracle_commitment(scal[:n], tau)

@@ -1,0 +1,200 @@
+// BN254 G1 arithmetic for the host epilogues (the window combination of an
+// MSM: a few hundred dependent doublings and additions on one core).
+//
+// Field elements are four 64-bit limbs in Montgomery form with R = 2^256,
+// fully reduced; products are CIOS on unsigned __int128 (about four times
+// fewer multiply instructions than the device's 29-bit limbs compiled for the
+// host).  Device words (eight 32-bit words, Montgomery with R = 2^261, below
+// 2^256) enter by one product with 2^251: x 2^261 2^251 2^-256 = x 2^256.
+// Points are XYZZ (x = X/ZZ, y = Y/ZZZ, ZZ^3 = ZZZ^2), identity ZZ = 0, with
+// the same formulas as the device (dbl-2008-s-1, add-2008-s, a = 0).
+#pragma once
+#include <stdint.h>
+
+#include "bn254_constants.hpp"
+
+namespace dvgpu {
+namespace hostg1 {
+
+using u128 = unsigned __int128;
+
+struct Fe {
+    uint64_t l[4];
+};
+
+inline constexpr uint64_t word64(const uint32_t* w, int i) { return uint64_t(w[2 * i]) | (uint64_t(w[2 * i + 1]) << 32); }
+
+constexpr uint64_t P[4] = {word64(bn254::FqParams::MOD32, 0), word64(bn254::FqParams::MOD32, 1),
+                           word64(bn254::FqParams::MOD32, 2), word64(bn254::FqParams::MOD32, 3)};
+
+// -p^-1 mod 2^64 by Newton iteration (each step doubles the correct bits).
+constexpr uint64_t neg_inv64(uint64_t p0) {
+    uint64_t x = 1;
+    for (int i = 0; i < 6; ++i) x *= 2 - p0 * x;
+    return ~x + 1;
+}
+constexpr uint64_t PINV = neg_inv64(P[0]);
+
+inline bool geq_p(const uint64_t (&a)[4]) {
+    for (int i = 3; i >= 0; --i) {
+        if (a[i] != P[i]) return a[i] > P[i];
+    }
+    return true;
+}
+
+inline void sub_p(uint64_t (&a)[4]) {
+    u128 borrow = 0;
+    for (int i = 0; i < 4; ++i) {
+        const u128 d = u128(a[i]) - P[i] - borrow;
+        a[i] = uint64_t(d);
+        borrow = (d >> 64) & 1;
+    }
+}
+
+inline Fe mul(const Fe& a, const Fe& b) {
+    uint64_t t[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 4; ++i) {
+        u128 c = 0;
+        for (int j = 0; j < 4; ++j) {
+            c += u128(a.l[j]) * b.l[i] + t[j];
+            t[j] = uint64_t(c);
+            c >>= 64;
+        }
+        c += t[4];
+        t[4] = uint64_t(c);
+        t[5] = uint64_t(c >> 64);
+        const uint64_t m = t[0] * PINV;
+        c = u128(m) * P[0] + t[0];
+        c >>= 64;
+        for (int j = 1; j < 4; ++j) {
+            c += u128(m) * P[j] + t[j];
+            t[j - 1] = uint64_t(c);
+            c >>= 64;
+        }
+        c += t[4];
+        t[3] = uint64_t(c);
+        t[4] = t[5] + uint64_t(c >> 64);
+    }
+    uint64_t r[4] = {t[0], t[1], t[2], t[3]};
+    if (t[4] || geq_p(r)) sub_p(r);  // the sum is below 2p
+    return {{r[0], r[1], r[2], r[3]}};
+}
+
+inline Fe add(const Fe& a, const Fe& b) {
+    uint64_t r[4];
+    u128 c = 0;
+    for (int i = 0; i < 4; ++i) {
+        c += u128(a.l[i]) + b.l[i];
+        r[i] = uint64_t(c);
+        c >>= 64;
+    }
+    if (c || geq_p(r)) sub_p(r);
+    return {{r[0], r[1], r[2], r[3]}};
+}
+
+inline Fe sub(const Fe& a, const Fe& b) {
+    uint64_t r[4];
+    u128 borrow = 0;
+    for (int i = 0; i < 4; ++i) {
+        const u128 d = u128(a.l[i]) - b.l[i] - borrow;
+        r[i] = uint64_t(d);
+        borrow = (d >> 64) & 1;
+    }
+    if (borrow) {
+        u128 c = 0;
+        for (int i = 0; i < 4; ++i) {
+            c += u128(r[i]) + P[i];
+            r[i] = uint64_t(c);
+            c >>= 64;
+        }
+    }
+    return {{r[0], r[1], r[2], r[3]}};
+}
+
+inline bool is_zero(const Fe& a) { return !(a.l[0] | a.l[1] | a.l[2] | a.l[3]); }
+
+// Device Montgomery words (R = 2^261, any value below 2^256) -> host form.
+inline Fe from_device(const uint32_t* w) {
+    const Fe x = {{word64(w, 0), word64(w, 1), word64(w, 2), word64(w, 3)}};
+    const Fe c = {{0, 0, 0, uint64_t(1) << (251 - 192)}};  // 2^251 < p
+    return mul(x, c);
+}
+
+// Host form -> canonical little-endian 32-bit words.
+inline void to_words(const Fe& a, uint32_t (&w)[8]) {
+    const Fe x = mul(a, Fe{{1, 0, 0, 0}});
+    for (int i = 0; i < 4; ++i) {
+        w[2 * i] = uint32_t(x.l[i]);
+        w[2 * i + 1] = uint32_t(x.l[i] >> 32);
+    }
+}
+
+// a^(p - 2) by left-to-right square and multiply (a nonzero).
+inline Fe inv(const Fe& a) {
+    uint64_t e[4] = {P[0] - 2, P[1], P[2], P[3]};  // p is odd and > 2: no borrow
+    Fe r = a;
+    bool started = false;
+    for (int i = 3; i >= 0; --i)
+        for (int b = 63; b >= 0; --b) {
+            if (started) r = mul(r, r);
+            if ((e[i] >> b) & 1) {
+                r = started ? mul(r, a) : a;
+                started = true;
+            }
+        }
+    return r;
+}
+
+struct Xyzz {
+    Fe X, Y, ZZ, ZZZ;
+};
+
+inline Xyzz infinity() { return {{{0, 0, 0, 0}}, {{0, 0, 0, 0}}, {{0, 0, 0, 0}}, {{0, 0, 0, 0}}}; }
+inline bool is_inf(const Xyzz& p) { return is_zero(p.ZZ); }
+
+// Device XYZZ words (X || Y || ZZ || ZZZ, eight words each) -> host point.
+inline Xyzz from_device_xyzz(const uint32_t* w) {
+    Xyzz p = {from_device(w), from_device(w + 8), from_device(w + 16), from_device(w + 24)};
+    if (is_zero(p.ZZ)) p = infinity();
+    return p;
+}
+
+inline Xyzz dbl(const Xyzz& p) {
+    if (is_inf(p) || is_zero(p.Y)) return infinity();
+    const Fe U = add(p.Y, p.Y);
+    const Fe V = mul(U, U);
+    const Fe W = mul(U, V);
+    const Fe S = mul(p.X, V);
+    const Fe xx = mul(p.X, p.X);
+    const Fe M = add(add(xx, xx), xx);
+    const Fe X3 = sub(sub(mul(M, M), S), S);
+    const Fe Y3 = sub(mul(M, sub(S, X3)), mul(W, p.Y));
+    return {X3, Y3, mul(V, p.ZZ), mul(W, p.ZZZ)};
+}
+
+inline Xyzz add(const Xyzz& p, const Xyzz& q) {
+    if (is_inf(p)) return q;
+    if (is_inf(q)) return p;
+    const Fe U1 = mul(p.X, q.ZZ), U2 = mul(q.X, p.ZZ);
+    const Fe S1 = mul(p.Y, q.ZZZ), S2 = mul(q.Y, p.ZZZ);
+    const Fe Pd = sub(U2, U1), R = sub(S2, S1);
+    if (is_zero(Pd)) return is_zero(R) ? dbl(p) : infinity();
+    const Fe PP = mul(Pd, Pd), PPP = mul(Pd, PP), Q = mul(U1, PP);
+    const Fe X3 = sub(sub(sub(mul(R, R), PPP), Q), Q);
+    const Fe Y3 = sub(mul(R, sub(Q, X3)), mul(S1, PPP));
+    return {X3, Y3, mul(mul(p.ZZ, q.ZZ), PP), mul(mul(p.ZZZ, q.ZZZ), PPP)};
+}
+
+// Canonical affine words (x || y, zero words for the identity).
+inline void to_affine_words(const Xyzz& p, uint32_t (&x)[8], uint32_t (&y)[8]) {
+    if (is_inf(p)) {
+        for (int i = 0; i < 8; ++i) x[i] = y[i] = 0;
+        return;
+    }
+    const Fe t = inv(mul(p.ZZ, p.ZZZ));
+    to_words(mul(p.X, mul(t, p.ZZZ)), x);  // X / ZZ
+    to_words(mul(p.Y, mul(t, p.ZZ)), y);   // Y / ZZZ
+}
+
+}  // namespace hostg1
+}  // namespace dvgpu

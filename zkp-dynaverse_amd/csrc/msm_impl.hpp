@@ -23,6 +23,7 @@
 #include <cstring>
 #include <type_traits>
 
+#include "host_g1.hpp"
 #include "host_pool.hpp"
 
 namespace dvgpu {
@@ -974,118 +975,155 @@ inline void msm_batch_prepare_impl(const MsmBatchPlan& plan, int xw, MsmWorkspac
     HIP_CHECK(hipMemcpyAsync(dev, host, m * sizeof(MsmDesc), hipMemcpyHostToDevice, stream));
 }
 
-template <class F>
-void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, const uint32_t* scalars,
-                       const MsmBatchPlan& plan, MsmWorkspace& ws, hipStream_t stream, uint32_t* window_out) {
-    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
-    const uint32_t m = uint32_t(plan.descs.size());
-    const uint32_t tiles = uint32_t((plan.total_buckets + SCAN_TILE - 1) / SCAN_TILE);
+// Buffers of a prepared plan, checked once.
+struct MsmBuffers {
+    const MsmDesc* descs;
+    uint32_t *counts, *digits, *ranks, *sorted, *total, *partials, *heavy, *chunk_vals, *hist;
+    uint2 *offsets, *tile_sums, *tasks;
+    uint32_t m, tiles;
+};
+
+inline MsmBuffers msm_buffers(const MsmBatchPlan& plan, MsmWorkspace& ws, int xw) {
+    MsmBuffers b;
+    b.m = uint32_t(plan.descs.size());
+    b.tiles = uint32_t((plan.total_buckets + SCAN_TILE - 1) / SCAN_TILE);
     // The buffers were sized by msm_batch_prepare; check instead of growing.
     require(ws.counts.bytes >= size_t(plan.total_buckets) * 4 && ws.sorted.bytes >= size_t(plan.total_entries) * 4 &&
-                ws.partials.bytes >= plan.max_tasks * XW * 4 && ws.chunks.bytes >= size_t(plan.total_chunks) * XW * 4 &&
-                ws.chunks.bytes >= size_t(plan.total_planes) * plan.plane_slices * XW * 4 &&
-                ws.chunks.bytes >= size_t(plan.total_chunks) * (plan.chunk_planes ? 2 : 1) * XW * 4 &&
-                ws.descs.bytes >= m * sizeof(MsmDesc) && ws.heavy.bytes >= plan.max_heavy * 4 &&
-                ws.block_sums.bytes >= tiles * sizeof(uint2),
+                ws.partials.bytes >= plan.max_tasks * xw * 4 && ws.chunks.bytes >= size_t(plan.total_chunks) * xw * 4 &&
+                ws.chunks.bytes >= size_t(plan.total_planes) * plan.plane_slices * xw * 4 &&
+                ws.chunks.bytes >= size_t(plan.total_chunks) * (plan.chunk_planes ? 2 : 1) * xw * 4 &&
+                ws.descs.bytes >= b.m * sizeof(MsmDesc) && ws.heavy.bytes >= plan.max_heavy * 4 &&
+                ws.block_sums.bytes >= b.tiles * sizeof(uint2) &&
+                (!plan.lds_digits || ws.hist.bytes >= size_t(plan.total_hist) * 4),
             "MSM workspace not prepared for this batch");
-    const MsmDesc* descs = static_cast<const MsmDesc*>(ws.descs.ptr);
-    uint32_t* counts = static_cast<uint32_t*>(ws.counts.ptr);
-    uint2* offsets = static_cast<uint2*>(ws.offsets.ptr);
-    uint32_t* digits = static_cast<uint32_t*>(ws.digits.ptr);
-    uint32_t* ranks = static_cast<uint32_t*>(ws.ranks.ptr);
-    uint32_t* sorted = static_cast<uint32_t*>(ws.sorted.ptr);
-    uint2* tile_sums = static_cast<uint2*>(ws.block_sums.ptr);
-    uint32_t* total = static_cast<uint32_t*>(ws.total.ptr);  // [0] entries, [1] tasks, [2] heavy buckets
-    uint2* tasks = static_cast<uint2*>(ws.tasks.ptr);
-    uint32_t* partials = static_cast<uint32_t*>(ws.partials.ptr);
-    uint32_t* heavy = static_cast<uint32_t*>(ws.heavy.ptr);
-    uint32_t* chunk_vals = static_cast<uint32_t*>(ws.chunks.ptr);
-    const uint32_t tp = plan.task_points;
+    b.descs = static_cast<const MsmDesc*>(ws.descs.ptr);
+    b.counts = static_cast<uint32_t*>(ws.counts.ptr);
+    b.offsets = static_cast<uint2*>(ws.offsets.ptr);
+    b.digits = static_cast<uint32_t*>(ws.digits.ptr);
+    b.ranks = static_cast<uint32_t*>(ws.ranks.ptr);
+    b.sorted = static_cast<uint32_t*>(ws.sorted.ptr);
+    b.tile_sums = static_cast<uint2*>(ws.block_sums.ptr);
+    b.total = static_cast<uint32_t*>(ws.total.ptr);  // [0] entries, [1] tasks, [2] heavy buckets, [3] error
+    b.tasks = static_cast<uint2*>(ws.tasks.ptr);
+    b.partials = static_cast<uint32_t*>(ws.partials.ptr);
+    b.heavy = static_cast<uint32_t*>(ws.heavy.ptr);
+    b.chunk_vals = static_cast<uint32_t*>(ws.chunks.ptr);
+    b.hist = plan.lds_digits ? static_cast<uint32_t*>(ws.hist.ptr) : nullptr;
+    return b;
+}
 
-    hipEvent_t* ev = ws.phase_events;
-    auto mark = [&](int k) {
-        if (ev) HIP_CHECK(hipEventRecord(ev[k], stream));
-    };
-    uint32_t* hist = plan.lds_digits ? static_cast<uint32_t*>(ws.hist.ptr) : nullptr;
-    require(!plan.lds_digits || ws.hist.bytes >= size_t(plan.total_hist) * 4, "MSM workspace not prepared");
-    mark(0);
-    k_fill<<<1, 64, 0, stream>>>(total, 4, 0u);
+inline void msm_mark(hipEvent_t* ev, int k, hipStream_t stream) {
+    if (ev) HIP_CHECK(hipEventRecord(ev[k], stream));
+}
+
+// Digits, bucket counts, the (count, task) scan, placement and the task list.
+// Marks ev[k0 .. k0 + 4]: start, counted, scanned, placed, tasks.
+inline void msm_sort_stage(const uint32_t* point_index, const uint32_t* scalars, const MsmBatchPlan& plan,
+                           const MsmBuffers& b, hipStream_t stream, hipEvent_t* ev, int k0) {
+    const uint32_t m = b.m, tp = plan.task_points;
+    msm_mark(ev, k0, stream);
+    k_fill<<<1, 64, 0, stream>>>(b.total, 4, 0u);
     if (plan.lds_digits) {
         k_digits_lds<false><<<plan.total_hist_blocks, plan.digit_threads, 0, stream>>>(
-            descs, m, plan.total_hist_blocks, scalars, hist, point_index, plan.point_limit, total + 3, sorted);
-        k_hist_prefix<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(descs, m, plan.total_buckets, hist,
-                                                                           counts);
+            b.descs, m, plan.total_hist_blocks, scalars, b.hist, point_index, plan.point_limit, b.total + 3, b.sorted);
+        k_hist_prefix<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(b.descs, m, plan.total_buckets, b.hist,
+                                                                           b.counts);
     } else {
         k_fill<<<unsigned(std::min<size_t>(grid_for(plan.total_buckets, 256), 4096)), 256, 0, stream>>>(
-            counts, plan.total_buckets, 0u);
-        k_digits<<<grid_for(plan.total_threads, 256), 256, 0, stream>>>(descs, m, plan.total_threads, scalars,
-                                                                       counts, digits, ranks);
+            b.counts, plan.total_buckets, 0u);
+        k_digits<<<grid_for(plan.total_threads, 256), 256, 0, stream>>>(b.descs, m, plan.total_threads, scalars,
+                                                                       b.counts, b.digits, b.ranks);
     }
     HIP_CHECK_LAUNCH();
-    mark(1);
-    k_scan_tiles<<<tiles, SCAN_THREADS, 0, stream>>>(counts, plan.total_buckets, tp, tile_sums);
-    k_scan_sums<<<1, SCAN_THREADS, 0, stream>>>(tile_sums, tiles, total);
-    k_scan_apply<<<tiles, SCAN_THREADS, 0, stream>>>(counts, plan.total_buckets, tp, tile_sums, offsets);
+    msm_mark(ev, k0 + 1, stream);
+    k_scan_tiles<<<b.tiles, SCAN_THREADS, 0, stream>>>(b.counts, plan.total_buckets, tp, b.tile_sums);
+    k_scan_sums<<<1, SCAN_THREADS, 0, stream>>>(b.tile_sums, b.tiles, b.total);
+    k_scan_apply<<<b.tiles, SCAN_THREADS, 0, stream>>>(b.counts, plan.total_buckets, tp, b.tile_sums, b.offsets);
     HIP_CHECK_LAUNCH();
     if (plan.lds_digits) {
-        k_hist_absolute<<<grid_for(plan.total_hist, 256), 256, 0, stream>>>(descs, m, plan.total_hist, offsets, hist);
+        k_hist_absolute<<<grid_for(plan.total_hist, 256), 256, 0, stream>>>(b.descs, m, plan.total_hist, b.offsets,
+                                                                          b.hist);
         HIP_CHECK_LAUNCH();
     }
-    mark(2);
+    msm_mark(ev, k0 + 2, stream);
     if (plan.lds_digits)
         k_digits_lds<true><<<plan.total_hist_blocks, plan.digit_threads, 0, stream>>>(
-            descs, m, plan.total_hist_blocks, scalars, hist, point_index, plan.point_limit, total + 3, sorted);
+            b.descs, m, plan.total_hist_blocks, scalars, b.hist, point_index, plan.point_limit, b.total + 3, b.sorted);
     else
-        k_scatter<<<grid_for(plan.total_entries, 256), 256, 0, stream>>>(descs, m, plan.total_entries, digits, ranks,
-                                                                        offsets, point_index, plan.point_limit,
-                                                                        total + 3, sorted);
+        k_scatter<<<grid_for(plan.total_entries, 256), 256, 0, stream>>>(b.descs, m, plan.total_entries, b.digits,
+                                                                        b.ranks, b.offsets, point_index,
+                                                                        plan.point_limit, b.total + 3, b.sorted);
     HIP_CHECK_LAUNCH();
-    mark(3);
-    k_tasks<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(counts, offsets, plan.total_buckets, tp, tasks,
-                                                                  heavy, total + 2);
+    msm_mark(ev, k0 + 3, stream);
+    k_tasks<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(b.counts, b.offsets, plan.total_buckets, tp,
+                                                                  b.tasks, b.heavy, b.total + 2);
     HIP_CHECK_LAUNCH();
-    mark(4);
-    k_accumulate<F><<<grid_for(plan.max_tasks, 256), 256, 0, stream>>>(points, sorted, tasks, total, partials);
+    msm_mark(ev, k0 + 4, stream);
+}
+
+template <class F>
+void msm_accumulate_stage(const uint32_t* points, const MsmBatchPlan& plan, const MsmBuffers& b,
+                          hipStream_t stream) {
+    k_accumulate<F><<<grid_for(plan.max_tasks, 256), 256, 0, stream>>>(points, b.sorted, b.tasks, b.total,
+                                                                        b.partials);
     HIP_CHECK_LAUNCH();
-    mark(5);
-    k_heavy<F><<<unsigned(std::min<size_t>(plan.max_heavy, 1024)), 256, 0, stream>>>(heavy, total + 2, counts,
-                                                                                     offsets, tp, partials);
-    k_bucket_sums<F><<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(counts, offsets, plan.total_buckets, tp,
-                                                                           partials);
+}
+
+// Heavy buckets, bucket sums and the window reduction.  Marks ev[k0] after
+// the bucket sums and ev[k0 + 1] after the window values.
+template <class F>
+void msm_reduce_stage(const MsmBatchPlan& plan, const MsmBuffers& b, hipStream_t stream, uint32_t* window_out,
+                      hipEvent_t* ev, int k0) {
+    const uint32_t m = b.m, tp = plan.task_points;
+    k_heavy<F><<<unsigned(std::min<size_t>(plan.max_heavy, 1024)), 256, 0, stream>>>(b.heavy, b.total + 2, b.counts,
+                                                                                     b.offsets, tp, b.partials);
+    k_bucket_sums<F><<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(b.counts, b.offsets, plan.total_buckets,
+                                                                           tp, b.partials);
     if (plan.planes) {
         k_plane_sums<F><<<plan.total_planes * plan.plane_slices, PLANE_THREADS, 0, stream>>>(
-            descs, m, plan.plane_slices, counts, offsets, partials, chunk_vals);
+            b.descs, m, plan.plane_slices, b.counts, b.offsets, b.partials, b.chunk_vals);
         HIP_CHECK_LAUNCH();
-        mark(6);
-        k_plane_final<F><<<grid_for(plan.total_planes, 64), 64, 0, stream>>>(chunk_vals, plan.total_planes,
+        msm_mark(ev, k0, stream);
+        k_plane_final<F><<<grid_for(plan.total_planes, 64), 64, 0, stream>>>(b.chunk_vals, plan.total_planes,
                                                                                plan.plane_slices, window_out);
         HIP_CHECK_LAUNCH();
-        mark(7);
+        msm_mark(ev, k0 + 1, stream);
         return;
     }
     if (plan.chunk_planes) {
         k_reduce_chunks<F, true><<<grid_for(plan.total_chunks, 256), 256, 0, stream>>>(
-            descs, m, plan.total_chunks, counts, offsets, partials, chunk_vals);
+            b.descs, m, plan.total_chunks, b.counts, b.offsets, b.partials, b.chunk_vals);
         HIP_CHECK_LAUNCH();
-        mark(6);
-        k_chunk_planes<F><<<plan.total_planes, PLANE_THREADS, 0, stream>>>(descs, m, chunk_vals, window_out);
+        msm_mark(ev, k0, stream);
+        k_chunk_planes<F><<<plan.total_planes, PLANE_THREADS, 0, stream>>>(b.descs, m, b.chunk_vals, window_out);
         HIP_CHECK_LAUNCH();
-        mark(7);
+        msm_mark(ev, k0 + 1, stream);
         return;
     }
     k_reduce_chunks<F><<<grid_for(plan.total_chunks, 256), 256, 0, stream>>>(
-        descs, m, plan.total_chunks, counts, offsets, partials, chunk_vals);
+        b.descs, m, plan.total_chunks, b.counts, b.offsets, b.partials, b.chunk_vals);
     HIP_CHECK_LAUNCH();
-    mark(6);
+    msm_mark(ev, k0, stream);
     if (plan.max_window_chunks <= 32)  // small bucket sets: one value per thread, fewest levels
-        k_window_tree<F, 32><<<plan.total_windows, 32, 0, stream>>>(descs, m, chunk_vals, window_out);
+        k_window_tree<F, 32><<<plan.total_windows, 32, 0, stream>>>(b.descs, m, b.chunk_vals, window_out);
     else if (plan.max_window_chunks <= 128)
-        k_window_tree<F, 128><<<plan.total_windows, 128, 0, stream>>>(descs, m, chunk_vals, window_out);
+        k_window_tree<F, 128><<<plan.total_windows, 128, 0, stream>>>(b.descs, m, b.chunk_vals, window_out);
     else
         k_window_tree<F, TreeThreads<F>::value><<<plan.total_windows, TreeThreads<F>::value, 0, stream>>>(
-            descs, m, chunk_vals, window_out);
+            b.descs, m, b.chunk_vals, window_out);
     HIP_CHECK_LAUNCH();
-    mark(7);
+    msm_mark(ev, k0 + 1, stream);
+}
+
+template <class F>
+void msm_batch_enqueue(const uint32_t* points, const uint32_t* point_index, const uint32_t* scalars,
+                       const MsmBatchPlan& plan, MsmWorkspace& ws, hipStream_t stream, uint32_t* window_out) {
+    const MsmBuffers b = msm_buffers(plan, ws, PointLayout<F>::XYZZ_WORDS);
+    hipEvent_t* ev = ws.phase_events;  // start, digits, scan, scatter, tasks, accumulate, reduce, tree
+    msm_sort_stage(point_index, scalars, plan, b, stream, ev, 0);
+    msm_accumulate_stage<F>(points, plan, b, stream);
+    msm_mark(ev, 5, stream);
+    msm_reduce_stage<F>(plan, b, stream, window_out, ev, 6);
 }
 
 template <class F>
@@ -1116,16 +1154,45 @@ struct HostMem<Fq2Ops> {
 };
 
 
+// Point arithmetic of the host epilogues: G1 in the host's own 64-bit-limb
+// field (host_g1.hpp), G2 in the device code compiled for the host.
+template <class F>
+struct HostArith;
+template <>
+struct HostArith<FqOps> {
+    using P = hostg1::Xyzz;
+    static P load(const uint32_t* w) { return hostg1::from_device_xyzz(w); }
+    static P inf() { return hostg1::infinity(); }
+    static P add(const P& a, const P& b) { return hostg1::add(a, b); }
+    static P dbl(const P& a) { return hostg1::dbl(a); }
+    static Affine<FqOps> affine(const P& p) {
+        if (hostg1::is_inf(p)) return {fp_zero<FqParams>(), fp_zero<FqParams>(), true};
+        uint32_t x[8], y[8];
+        hostg1::to_affine_words(p, x, y);
+        return {to_mont<FqParams>(x), to_mont<FqParams>(y), false};
+    }
+};
+template <>
+struct HostArith<Fq2Ops> {
+    using P = Xyzz<Fq2Ops>;
+    static P load(const uint32_t* w) { return HostMem<Fq2Ops>::load_xyzz(w); }
+    static P inf() { return xyzz_inf<Fq2Ops>(); }
+    static P add(const P& a, const P& b) { return xyzz_add<Fq2Ops>(a, b); }
+    static P dbl(const P& a) { return xyzz_dbl<Fq2Ops>(a); }
+    static Affine<Fq2Ops> affine(const P& p) { return xyzz_to_affine<Fq2Ops>(p); }
+};
+
 template <class F>
 Affine<F> combine_one(const uint32_t* words, const MsmDesc& d) {
+    using H = HostArith<F>;
     constexpr int XW = PointLayout<F>::XYZZ_WORDS;
     const uint32_t* base = words + size_t(d.window_base) * XW;
-    Xyzz<F> acc = HostMem<F>::load_xyzz(base + size_t(d.windows - 1) * XW);
+    typename H::P acc = H::load(base + size_t(d.windows - 1) * XW);
     for (int w = int(d.windows) - 2; w >= 0; --w) {
-        for (uint32_t k = 0; k < d.c; ++k) acc = xyzz_dbl<F>(acc);
-        acc = xyzz_add<F>(acc, HostMem<F>::load_xyzz(base + size_t(w) * XW));
+        for (uint32_t k = 0; k < d.c; ++k) acc = H::dbl(acc);
+        acc = H::add(acc, H::load(base + size_t(w) * XW));
     }
-    return xyzz_to_affine<F>(acc);
+    return H::affine(acc);
 }
 
 // Window combination on the host, one MSM per pool task (a CPU core runs
@@ -1151,23 +1218,23 @@ inline MsmPlan msm_plan_impl(size_t n, const MsmConfig& cfg) {
     return p;
 }
 
-template <class F>
-MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, MsmWorkspace& ws,
-                    hipStream_t stream, const MsmConfig& cfg, uint32_t* window_out, size_t out_capacity) {
-    MsmConfig c = cfg;
-    if constexpr (std::is_same<F, Fq2Ops>::value) c.chunk = std::min(c.chunk, 2);  // G2: short reduction chains
-    MsmBatchPlan plan = msm_batch_plan_impl({MsmItem{n, 0, 0, cfg.window_bits}}, c, nullptr);
-    // The windows are combined on the host: bit planes of the bucket weights
-    // for small bucket sets, of the chunk index otherwise.
+// The plan of one MSM (or one window range of it) whose windows are
+// combined on the host: bit planes of the bucket weights for small bucket
+// sets, of the chunk index otherwise.
+inline MsmBatchPlan msm_single_plan_impl(const MsmItem& item, MsmConfig c, bool g2) {
+    if (g2) c.chunk = std::min(c.chunk, 2);  // G2: short reduction chains
+    MsmBatchPlan plan = msm_batch_plan_impl({item}, c, nullptr);
     if (!msm_plan_planes(plan)) msm_plan_chunk_planes(plan);
-    require(plan.total_planes <= out_capacity, "MSM window buffer too small");
-    msm_batch_prepare_impl(plan, PointLayout<F>::XYZZ_WORDS, ws, stream);
-    msm_batch_enqueue<F>(points, nullptr, scalars, plan, ws, stream, window_out);
+    return plan;
+}
+
+// What a single-MSM plan leaves per window, for the host combination.
+inline MsmPlan msm_outputs_of(const MsmBatchPlan& plan) {
     MsmPlan p;
     p.c = int(plan.descs[0].c);
     p.windows = int(plan.descs[0].windows);
     p.buckets = plan.descs[0].buckets;
-    p.n = n;
+    p.n = plan.descs[0].n;
     p.planes = plan.planes ? int(plane_bits(p.buckets))
                            : int(chunk_plane_shape(p.buckets, plan.descs[0].chunk).per_window);
     p.chunk = plan.chunk_planes ? int(plan.descs[0].chunk) : 0;
@@ -1175,13 +1242,52 @@ MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, M
     return p;
 }
 
+template <class F>
+MsmPlan msm_enqueue(const uint32_t* points, const uint32_t* scalars, size_t n, MsmWorkspace& ws,
+                    hipStream_t stream, const MsmConfig& cfg, uint32_t* window_out, size_t out_capacity) {
+    MsmItem item{n, 0, 0, cfg.window_bits};
+    const MsmBatchPlan plan = msm_single_plan_impl(item, cfg, std::is_same<F, Fq2Ops>::value);
+    require(plan.total_planes <= out_capacity, "MSM window buffer too small");
+    msm_batch_prepare_impl(plan, PointLayout<F>::XYZZ_WORDS, ws, stream);
+    msm_batch_enqueue<F>(points, nullptr, scalars, plan, ws, stream, window_out);
+    return msm_outputs_of(plan);
+}
+
 // words: plan.outputs values from msm_enqueue — one per window, or in plane
 // mode (plan.planes > 0) the bit-plane sums, window-major: then each window
 // value sum_k 2^k P_k on its own pool task (Horner over the planes), and
 // Horner over the windows.
+// Plane mode: the value of every window, each on its own pool task.
+template <class F>
+std::vector<typename HostArith<F>::P> msm_window_values(const uint32_t* words, const MsmPlan& plan) {
+    using H = HostArith<F>;
+    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
+    require(plan.planes && plan.outputs == size_t(plan.windows) * size_t(plan.planes), "MSM plane count mismatch");
+    std::vector<typename H::P> win(size_t(plan.windows));
+    HostPool::get().parallel_for(win.size(), [&](size_t w) {
+        const uint32_t* base = words + w * size_t(plan.planes) * XW;
+        if (plan.chunk) {  // U + L * sum_k 2^k Q_k from [U slices][Q_0 slices][Q_1 slices]...
+            const ChunkPlaneShape sh = chunk_plane_shape(plan.buckets, uint32_t(plan.chunk));
+            auto slices = [&](size_t first, uint32_t count) {
+                typename H::P v = H::load(base + first * XW);
+                for (uint32_t q = 1; q < count; ++q) v = H::add(v, H::load(base + (first + q) * XW));
+                return v;
+            };
+            typename H::P acc = H::inf();
+            for (int k = int(sh.qb) - 1; k >= 0; --k) acc = H::add(H::dbl(acc), slices(sh.su + size_t(k) * sh.sq, sh.sq));
+            for (int l = plan.chunk; l > 1; l >>= 1) acc = H::dbl(acc);
+            win[w] = H::add(acc, slices(0, sh.su));
+            return;
+        }
+        typename H::P acc = H::load(base + size_t(plan.planes - 1) * XW);
+        for (int k = plan.planes - 2; k >= 0; --k) acc = H::add(H::dbl(acc), H::load(base + size_t(k) * XW));
+        win[w] = acc;
+    });
+    return win;
+}
+
 template <class F>
 Affine<F> msm_combine_windows(const uint32_t* words, const MsmPlan& plan) {
-    constexpr int XW = PointLayout<F>::XYZZ_WORDS;
     if (!plan.planes) {
         MsmDesc d{};
         d.c = uint32_t(plan.c);
@@ -1189,35 +1295,14 @@ Affine<F> msm_combine_windows(const uint32_t* words, const MsmPlan& plan) {
         d.window_base = 0;
         return combine_one<F>(words, d);
     }
-    require(plan.outputs == size_t(plan.windows) * size_t(plan.planes), "MSM plane count mismatch");
-    std::vector<Xyzz<F>> win(size_t(plan.windows));
-    HostPool::get().parallel_for(win.size(), [&](size_t w) {
-        const uint32_t* base = words + w * size_t(plan.planes) * XW;
-        if (plan.chunk) {  // U + L * sum_k 2^k Q_k from [U slices][Q_0 slices][Q_1 slices]...
-            const ChunkPlaneShape sh = chunk_plane_shape(plan.buckets, uint32_t(plan.chunk));
-            auto slices = [&](size_t first, uint32_t count) {
-                Xyzz<F> v = HostMem<F>::load_xyzz(base + first * XW);
-                for (uint32_t q = 1; q < count; ++q) v = xyzz_add<F>(v, HostMem<F>::load_xyzz(base + (first + q) * XW));
-                return v;
-            };
-            Xyzz<F> acc = xyzz_inf<F>();
-            for (int k = int(sh.qb) - 1; k >= 0; --k)
-                acc = xyzz_add<F>(xyzz_dbl<F>(acc), slices(sh.su + size_t(k) * sh.sq, sh.sq));
-            for (int l = plan.chunk; l > 1; l >>= 1) acc = xyzz_dbl<F>(acc);
-            win[w] = xyzz_add<F>(acc, slices(0, sh.su));
-            return;
-        }
-        Xyzz<F> acc = HostMem<F>::load_xyzz(base + size_t(plan.planes - 1) * XW);
-        for (int k = plan.planes - 2; k >= 0; --k)
-            acc = xyzz_add<F>(xyzz_dbl<F>(acc), HostMem<F>::load_xyzz(base + size_t(k) * XW));
-        win[w] = acc;
-    });
-    Xyzz<F> acc = win.back();
+    using H = HostArith<F>;
+    const std::vector<typename H::P> win = msm_window_values<F>(words, plan);
+    typename H::P acc = win.back();
     for (int w = plan.windows - 2; w >= 0; --w) {
-        for (int k = 0; k < plan.c; ++k) acc = xyzz_dbl<F>(acc);
-        acc = xyzz_add<F>(acc, win[size_t(w)]);
+        for (int k = 0; k < plan.c; ++k) acc = H::dbl(acc);
+        acc = H::add(acc, win[size_t(w)]);
     }
-    return xyzz_to_affine<F>(acc);
+    return H::affine(acc);
 }
 
 template <class F>
