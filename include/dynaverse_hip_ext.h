@@ -122,6 +122,29 @@ int dvgpu_ext_msm_batch_finalize(void* owner, const void* points, size_t point_c
                                  const uint32_t* result_slots, void* slot_points, size_t slot_count,
                                  char* error, size_t error_capacity);
 
+/* Bytes of the device array behind dvgpu_ext_msm_batch_finalize_noalloc
+ * (one per device the library runs on, held until the process ends). */
+#define DVGPU_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The dvgpu_ext_msm_batch_finalize computation (no result
+ * slots) with every intermediate buffer taken from a fixed device array of
+ * DVGPU_NOALLOC_SCRATCH_BYTES: the call makes no hipMalloc, hipFree or other
+ * allocation.  results_device (device, count x 64 / 128 bytes) is required;
+ * results_host (optional, host) receives a copy.  A batch whose buffers do not
+ * fit the array fails with -1 and leaves the owner usable: at the default
+ * window widths one G1 MSM of up to 2^17 points or one G2 MSM of 2^16 fits,
+ * a G1 MSM of 2^18 does not.
+ * Every call on one device shares the array: calls that overlap on one device
+ * (two threads or two owners) race, and callers serialise them. */
+int dvgpu_ext_msm_batch_finalize_noalloc(void* owner, const void* points, size_t point_count,
+                                         const void* point_index, size_t index_count,
+                                         const void* scalars, size_t scalar_count,
+                                         const DvgpuMsmItem* items, size_t count, int g2,
+                                         void* results_device, uint8_t* results_host,
+                                         char* error, size_t error_capacity);
+
 typedef struct DvgpuNttRows {
     const void* in;          /* device canonical words */
     void* out;

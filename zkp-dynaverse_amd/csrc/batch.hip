@@ -119,15 +119,11 @@ void BatchObj::enqueue_msms(const uint32_t* pts, const uint32_t* idx, const uint
     }
 }
 
-namespace {
-
-Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
-
 // Plan the non-empty items; map[i] is the caller's index of planned MSM i.
 // G2 MSMs: every group operation costs about three G1 ones, so the serial
 // bucket-reduction chains are cut short (two buckets per reduction thread,
 // whatever the G1-tuned per-size table says).
-MsmConfig g2_config(const Owner* o, WindowTable& table) {
+static MsmConfig g2_config(const Owner* o, WindowTable& table) {
     MsmConfig cfg = o->msm_config;
     cfg.chunk = std::min(cfg.chunk, 2);
     table = o->window_table;
@@ -155,6 +151,10 @@ MsmBatchPlan plan_items(const DvgpuMsmItem* items, size_t count, const Owner* o,
     for (size_t k = 0; k < plan.descs.size(); ++k) plan.descs[k].out_index = uint32_t(map[k]);
     return plan;
 }
+
+namespace {
+
+Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
 
 // Batches of at least this many MSMs finish their window sums on the device.
 size_t finalize_threshold(const Owner* o) { return o->msm_finalize_min > 0 ? size_t(o->msm_finalize_min) : 4096; }

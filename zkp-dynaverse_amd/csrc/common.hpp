@@ -47,24 +47,35 @@ struct Quota {
 };
 
 // A named, grow-only device buffer.  Contents are not preserved on growth.
+// A borrowed buffer views memory it does not own (a slice of a fixed scratch
+// area): it never allocates or frees, and refuses to grow.
 struct DeviceBuffer {
     void* ptr = nullptr;
     size_t bytes = 0;
     Quota* quota = nullptr;
+    bool borrowed = false;
     DeviceBuffer() = default;
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
     ~DeviceBuffer() { release(); }
     void release() noexcept {
-        if (ptr) {
+        if (ptr && !borrowed) {
             (void)hipFree(ptr);
             if (quota) quota->release(bytes);
         }
         ptr = nullptr;
         bytes = 0;
+        borrowed = false;
+    }
+    void borrow(void* p, size_t n) {
+        release();
+        ptr = p;
+        bytes = n;
+        borrowed = true;
     }
     void* ensure(size_t need) {
         if (need <= bytes) return ptr;
+        if (borrowed) throw InvalidArgument("borrowed device buffer too small");
         size_t grow = need + need / 4;
         grow = (grow + 255) & ~size_t(255);
         if (quota) quota->charge(grow);

@@ -158,6 +158,13 @@ template <class F>
 void msm_batch_finalize_enqueue(const uint32_t* window_words, const MsmBatchPlan& plan, const MsmWorkspace& ws,
                                 uint8_t* out, uint32_t* slot_points, hipStream_t stream);
 
+// Point the workspace's buffers at slices of `capacity` bytes of device
+// memory at `base` (nothing is allocated; the window values go to
+// ws.windows); returns the bytes used, or throws InvalidArgument when the
+// plan does not fit.  Descriptors are not copied.
+size_t msm_workspace_borrow(const MsmBatchPlan& plan, int xyzz_words, uint8_t* base, size_t capacity,
+                            MsmWorkspace& ws);
+
 // Zero `words` 32-bit words of device memory with a kernel (capturable).
 void device_fill_words(uint32_t* p, size_t words, uint32_t value, hipStream_t stream);
 

@@ -13,6 +13,11 @@ void msm_batch_prepare(const MsmBatchPlan& plan, int xyzz_words, MsmWorkspace& w
     msm_batch_prepare_impl(plan, xyzz_words, ws, stream);
 }
 
+size_t msm_workspace_borrow(const MsmBatchPlan& plan, int xyzz_words, uint8_t* base, size_t capacity,
+                            MsmWorkspace& ws) {
+    return msm_workspace_borrow_impl(plan, xyzz_words, base, capacity, ws);
+}
+
 void device_fill_words(uint32_t* p, size_t words, uint32_t value, hipStream_t stream) {
     if (!words) return;
     k_fill<<<unsigned(std::min<size_t>(grid_for(words, 256), 4096)), 256, 0, stream>>>(p, words, value);

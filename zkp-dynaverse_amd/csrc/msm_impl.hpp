@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstring>
 #include <type_traits>
+#include <utility>
 
 #include "host_g1.hpp"
 #include "host_pool.hpp"
@@ -948,25 +949,75 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
     return p;
 }
 
+// Bytes of every device buffer a plan's kernels use (descriptors and the
+// window values included), for the growing workspace and for a fixed
+// scratch area alike.
+struct MsmBufferBytes {
+    size_t counts, offsets, digits, ranks, sorted, block_sums, total, tasks, partials, heavy, chunks, hist, descs,
+        windows;
+};
+
+inline MsmBufferBytes msm_buffer_bytes(const MsmBatchPlan& plan, int xw) {
+    const size_t tiles = (plan.total_buckets + SCAN_TILE - 1) / SCAN_TILE;
+    MsmBufferBytes b;
+    b.counts = size_t(plan.total_buckets) * 4;
+    b.offsets = size_t(plan.total_buckets) * sizeof(uint2);
+    b.digits = plan.lds_digits ? 0 : size_t(plan.total_entries) * 4;
+    b.ranks = b.digits;
+    b.sorted = size_t(plan.total_entries) * 4;
+    b.block_sums = tiles * sizeof(uint2);
+    b.total = 4 * 4;
+    b.tasks = plan.max_tasks * sizeof(uint2);
+    b.partials = plan.max_tasks * size_t(xw) * 4;
+    b.heavy = plan.max_heavy * 4;
+    b.chunks = std::max<size_t>(size_t(plan.total_chunks) * (plan.chunk_planes ? 2 : 1),
+                                size_t(plan.total_planes) * plan.plane_slices) * size_t(xw) * 4;
+    b.hist = plan.lds_digits ? size_t(plan.total_hist) * 4 : 0;
+    b.descs = plan.descs.size() * sizeof(MsmDesc);
+    b.windows = size_t(plan.total_windows) * size_t(xw) * 4;
+    return b;
+}
+
+// The plan's buffers as slices of `capacity` bytes at `base` (256-byte
+// aligned); returns the bytes used, or throws when they do not fit.
+inline size_t msm_workspace_borrow_impl(const MsmBatchPlan& plan, int xw, uint8_t* base, size_t capacity,
+                                        MsmWorkspace& ws) {
+    const MsmBufferBytes b = msm_buffer_bytes(plan, xw);
+    const std::pair<DeviceBuffer*, size_t> parts[] = {
+        {&ws.counts, b.counts},   {&ws.offsets, b.offsets}, {&ws.digits, b.digits},   {&ws.ranks, b.ranks},
+        {&ws.sorted, b.sorted},   {&ws.block_sums, b.block_sums}, {&ws.total, b.total}, {&ws.tasks, b.tasks},
+        {&ws.partials, b.partials}, {&ws.heavy, b.heavy},   {&ws.chunks, b.chunks},   {&ws.hist, b.hist},
+        {&ws.descs, b.descs},     {&ws.windows, b.windows}};
+    size_t used = 0;
+    for (const auto& p : parts) used += (std::max<size_t>(p.second, 1) + 255) & ~size_t(255);
+    require(used <= capacity, "MSM batch needs more scratch than the allocation-free entry point holds");
+    size_t at = 0;
+    for (const auto& p : parts) {
+        const size_t n = (std::max<size_t>(p.second, 1) + 255) & ~size_t(255);
+        p.first->borrow(base + at, n);
+        at += n;
+    }
+    return used;
+}
+
 inline void msm_batch_prepare_impl(const MsmBatchPlan& plan, int xw, MsmWorkspace& ws, hipStream_t stream) {
     const size_t m = plan.descs.size();
     require(m > 0, "empty MSM batch");
-    const uint32_t tiles = uint32_t((plan.total_buckets + SCAN_TILE - 1) / SCAN_TILE);
-    ws.counts.as<uint32_t>(plan.total_buckets);
-    ws.offsets.as<uint2>(plan.total_buckets);
+    const MsmBufferBytes b = msm_buffer_bytes(plan, xw);
+    ws.counts.ensure(b.counts);
+    ws.offsets.ensure(b.offsets);
     if (!plan.lds_digits) {
-        ws.digits.as<uint32_t>(plan.total_entries);
-        ws.ranks.as<uint32_t>(plan.total_entries);
+        ws.digits.ensure(b.digits);
+        ws.ranks.ensure(b.ranks);
     }
-    ws.sorted.as<uint32_t>(plan.total_entries);
-    ws.block_sums.as<uint2>(tiles);
-    ws.total.as<uint32_t>(4);
-    ws.tasks.as<uint2>(plan.max_tasks);
-    ws.partials.as<uint32_t>(plan.max_tasks * xw);
-    ws.heavy.as<uint32_t>(plan.max_heavy);
-    ws.chunks.as<uint32_t>(std::max<size_t>(size_t(plan.total_chunks) * (plan.chunk_planes ? 2 : 1),
-                                            size_t(plan.total_planes) * plan.plane_slices) * xw);
-    if (plan.lds_digits) ws.hist.as<uint32_t>(plan.total_hist);
+    ws.sorted.ensure(b.sorted);
+    ws.block_sums.ensure(b.block_sums);
+    ws.total.ensure(b.total);
+    ws.tasks.ensure(b.tasks);
+    ws.partials.ensure(b.partials);
+    ws.heavy.ensure(b.heavy);
+    ws.chunks.ensure(b.chunks);
+    if (plan.lds_digits) ws.hist.ensure(b.hist);
     // Earlier work on this stream may still read the pinned staging area.
     HIP_CHECK(hipStreamSynchronize(stream));
     MsmDesc* host = ws.host_descs.as<MsmDesc>(m);

@@ -14,6 +14,7 @@
 #include <unordered_map>
 
 #include "../../include/dynaverse_gpu.h"
+#include "../../include/dynaverse_hip_ext.h"
 #include "common.hpp"
 #include "msm.hpp"
 #include "ntt.hpp"
@@ -104,6 +105,9 @@ DvgpuResidentStats resident_stats(const ResidentState* state);
 DvgpuResidentDiagnostics resident_diagnostics(const ResidentState* state);
 // Batch hooks (batch.hip).
 void batch_destroy(BatchObj* b) noexcept;
+// Plan the non-empty items of a batched MSM call; map[i] is the caller's
+// index of planned MSM i (descriptors carry it as out_index).
+MsmBatchPlan plan_items(const DvgpuMsmItem* items, size_t count, const Owner* o, std::vector<size_t>& map, bool g2);
 // Setup hooks (setup.hip).
 void setup_destroy(SetupState* state) noexcept;
 // KZG hooks (kzg.hip).

@@ -222,6 +222,9 @@ SIGNATURES = {
     "dvgpu_ext_msm_batch_finalize": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_size_t,
                                           ctypes.POINTER(DvgpuMsmItem), c_size_t, _I, c_void_p, c_void_p,
                                           ctypes.POINTER(ctypes.c_uint32), c_void_p, c_size_t, *_E]),
+    "dvgpu_ext_msm_batch_finalize_noalloc": (_I, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p,
+                                                  c_size_t, ctypes.POINTER(DvgpuMsmItem), c_size_t, _I, c_void_p,
+                                                  c_void_p, *_E]),
     "dvgpu_ext_ntt_batch_device": (_I, [c_void_p, ctypes.POINTER(DvgpuNttRows), c_size_t, c_void_p,
                                         ctypes.POINTER(ctypes.c_float), *_E]),
     "dvgpu_ext_update_batch": (_I, [c_void_p, c_void_p, c_size_t, _I, c_void_p, c_size_t,

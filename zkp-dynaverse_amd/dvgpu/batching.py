@@ -69,6 +69,25 @@ def msm_batch_finalize(backend: HipBackend, points: DeviceArray, scalars: Device
     return [out.raw[width * i:width * (i + 1)] for i in range(len(items))] if host else None
 
 
+def msm_batch_finalize_noalloc(backend: HipBackend, points: DeviceArray, scalars: DeviceArray,
+                               items: Sequence[MsmItem], results: DeviceArray, g2: bool = False,
+                               index: Optional[DeviceArray] = None, host: bool = True):
+    """msm_batch_finalize without any device allocation: intermediates live in
+    the library's fixed scratch array (NOALLOC_SCRATCH_BYTES per device), so
+    batches that do not fit fail with BackendError (status -1) and calls
+    overlapping on one device race.  `results` (device) is required."""
+    arr = _items(items)
+    width = 128 if g2 else 64
+    out = ctypes.create_string_buffer(max(1, width * len(items))) if host else None
+    err = _err()
+    _check(backend.lib.dvgpu_ext_msm_batch_finalize_noalloc(
+        backend.handle, points.ptr, points.nbytes // width, index.ptr if index else None,
+        index.nbytes // 4 if index else 0, scalars.ptr, scalars.nbytes // 32, arr, len(items), int(g2),
+        results.ptr, out, err, 256), err)
+    return [out.raw[width * i:width * (i + 1)] for i in range(len(items))] if host else None
+
+
+NOALLOC_SCRATCH_BYTES = 64 << 20
 NO_SLOT = 0xFFFFFFFF
 
 
