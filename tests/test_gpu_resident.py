@@ -244,3 +244,90 @@ def test_profiled_call(rb):
     h = rb.alloc(FR, 8)
     _, prof = rb.call(6, kind=FR, a=h, length=0, profile=True)
     assert prof["native_wall_ns"] > 0
+
+
+def test_transaction_refusals_are_whole_operations():
+    """A transpose whose destination runs need more journal than is left, and
+    a packet whose scratch would fit but whose output would not, are refused
+    before anything is saved or written (reference: transform_packet.rs:131,
+    whole-packet preflight)."""
+    b = HipBackend()
+    rb = ResidentBackend(b, limit=1 << 30, journal=32 * 24)      # 24 Fr elements
+    vals = rand_fr(21, 32)
+    src, dst = rb.alloc(FR, 32), rb.alloc(FR, 32)
+    rb.write_fields(src, 0, vals)
+    rb.begin_transaction()
+    d0 = rb.diagnostics()
+    print("case: transpose refused by the journal", flush=True)
+    with pytest.raises(BackendError) as e:
+        rb.transpose(FR, src, dst, 4, 8, 0, 32)                   # 32 destinations > 24
+    assert e.value.status == -1 and "journal" in e.value.message
+    d1 = rb.diagnostics()
+    assert (d1["journal_saved_bytes"], d1["journal_saved_ranges"]) == (d0["journal_saved_bytes"],
+                                                                     d0["journal_saved_ranges"])
+    assert rb.read_fields(dst, 0, 32) == [0] * 32
+    rb.transpose(FR, src, dst, 4, 8, 3, 20)                       # 20 destinations fit
+    d2 = rb.diagnostics()
+    assert d2["journal_saved_bytes"] - d1["journal_saved_bytes"] == 20 * 32
+    assert d2["journal_saved_ranges"] - d1["journal_saved_ranges"] == 8   # one run per column
+    got = rb.read_fields(dst, 0, 32)
+    for k, v in cpu_transpose(vals, 4, 8, 3, 20).items():
+        assert got[k] == v
+    rb.rollback()
+    assert rb.read_fields(dst, 0, 32) == [0] * 32
+    a, s, c = rb.alloc(FR, 16), rb.alloc(FR, 16), rb.alloc(FR, 16)
+    rb.write_fields(a, 0, vals[:16])
+    rb.begin_transaction()
+    d3 = rb.diagnostics()
+    print("case: packet refused by the journal", flush=True)
+    with pytest.raises(BackendError) as e:
+        rb.transform_four_step(FR, a, s, c, 16, False)           # 16 + 16 elements > 24
+    assert e.value.status == -1
+    assert rb.diagnostics()["journal_saved_bytes"] == d3["journal_saved_bytes"]
+    assert rb.read_fields(s, 0, 16) == [0] * 16 and rb.read_fields(c, 0, 16) == [0] * 16
+    rb.rollback()
+    rb.transform_four_step(FR, a, s, c, 16, False)               # no transaction: no journal
+    assert rb.read_fields(c, 0, 16) == o.ntt(vals[:16], False)
+    b.close()
+
+
+def test_quota_admission_keeps_owner_healthy():
+    """Configuration charges the twiddle tables and the journal arena; an
+    allocation whose bytes plus the workspace of operations over it exceed
+    the quota is refused with -1 and a message naming the quota, and the
+    owner keeps working (reference: resident_configuration.rs:118,
+    setup_tables.rs:58)."""
+    b = HipBackend()
+    rb = ResidentBackend(b, limit=256 << 20, journal=1 << 20)
+    assert rb.stats()["bytes_allocated"] >= (32 << 20) + (1 << 20)
+    h = rb.alloc(FR, 1 << 16)
+    print("case: resident allocation over the quota", flush=True)
+    with pytest.raises(BackendError) as e:
+        rb.alloc(G1, 1 << 20)
+    assert e.value.status == -1 and "quota" in e.value.message
+    assert rb.buffer_len(h) == 1 << 16
+    rb.write_fields(h, 7, [5])
+    assert rb.read_fields(h, 7, 1) == [5]
+    b.close()
+
+
+def test_profiled_write_splits_journal_and_work():
+    import ctypes
+    from dvgpu.codec import encode_fr
+    from dvgpu.resident import WRITE, ZIP
+    b = HipBackend()
+    rb = ResidentBackend(b, limit=1 << 30, journal=1 << 16)
+    h = rb.alloc(FR, 64)
+    buf = ctypes.create_string_buffer(b"".join(encode_fr(v) for v in rand_fr(22, 64)))
+    rb.begin_transaction()
+    _, prof = rb.call(WRITE, kind=FR, a=h, length=64, input=ctypes.addressof(buf), profile=True)
+    assert prof["event_mask"] == 7                                # journal, work and whole-call events
+    assert prof["h2d_bytes"] == 64 * 32 and prof["d2d_bytes"] == 64 * 32
+    assert prof["event_total_ns"] > 0 and prof["native_wall_ns"] >= prof["wait_host_ns"]
+    _, prof = rb.call(WRITE, kind=FR, a=h, length=64, input=ctypes.addressof(buf), profile=True)
+    assert prof["event_mask"] == 6 and prof["d2d_bytes"] == 0    # already saved: no journal work
+    rb.rollback()
+    assert rb.read_fields(h, 0, 64) == [0] * 64
+    with pytest.raises(BackendError):
+        rb.call(ZIP, kind=FR, a=h, b=h, c=h, length=0, profile=True)
+    b.close()

@@ -8,17 +8,9 @@
 #include <string>
 #include <vector>
 
+#include "errors.hpp"
+
 namespace dvgpu {
-
-// Invalid caller input: maps to status -1 and leaves the owner healthy.
-struct InvalidArgument : std::invalid_argument {
-    using std::invalid_argument::invalid_argument;
-};
-
-// Device/runtime failure: maps to status -2 and retires the owner.
-struct DeviceError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
 
 inline void hip_check(hipError_t status, const char* what) {
     if (status != hipSuccess)
@@ -27,9 +19,6 @@ inline void hip_check(hipError_t status, const char* what) {
 #define HIP_CHECK(expr) ::dvgpu::hip_check((expr), #expr)
 #define HIP_CHECK_LAUNCH() ::dvgpu::hip_check(hipGetLastError(), "kernel launch")
 
-inline void require(bool ok, const char* message) {
-    if (!ok) throw InvalidArgument(message);
-}
 
 // Memory accounting shared by every device allocation an owner makes.  The
 // limit models the reference's per-owner quota; 0 means unlimited.
@@ -76,8 +65,7 @@ struct DeviceBuffer {
     void* ensure(size_t need) {
         if (need <= bytes) return ptr;
         if (borrowed) throw InvalidArgument("borrowed device buffer too small");
-        size_t grow = need + need / 4;
-        grow = (grow + 255) & ~size_t(255);
+        const size_t grow = grown(need);
         if (quota) quota->charge(grow);
         void* fresh = nullptr;
         const hipError_t st = hipMalloc(&fresh, grow);
@@ -92,6 +80,9 @@ struct DeviceBuffer {
     }
     template <class T>
     T* as(size_t count) { return static_cast<T*>(ensure(count * sizeof(T))); }
+    // Bytes an ensure(need) on an empty buffer allocates (25% headroom,
+    // 256-byte granules).
+    static size_t grown(size_t need) { return ((need + need / 4) + 255) & ~size_t(255); }
 };
 
 // Pinned host staging buffer (hipHostMalloc), grow-only.

@@ -1,0 +1,26 @@
+// Exception types that map onto the C ABI's status codes (host-only, no HIP
+// dependency, so host-side logic can be built and tested without a GPU).
+#pragma once
+#include <stdexcept>
+#include <string>
+
+namespace dvgpu {
+
+// Invalid caller input: maps to status -1 and leaves the owner healthy.
+struct InvalidArgument : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+
+// Device/runtime failure: maps to status -2 and retires the owner.
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void require(bool ok, const char* message) {
+    if (!ok) throw InvalidArgument(message);
+}
+inline void require(bool ok, const std::string& message) {
+    if (!ok) throw InvalidArgument(message);
+}
+
+}  // namespace dvgpu

@@ -162,6 +162,10 @@ void msm_batch_finalize_enqueue(const uint32_t* window_words, const MsmBatchPlan
 // memory at `base` (nothing is allocated; the window values go to
 // ws.windows); returns the bytes used, or throws InvalidArgument when the
 // plan does not fit.  Descriptors are not copied.
+// Device bytes msm_run holds in its workspace after an MSM of n points (the
+// single-MSM plan's buffers, each with its growth headroom).
+uint64_t msm_run_workspace_bytes(size_t n, const MsmConfig& cfg, bool g2);
+
 size_t msm_workspace_borrow(const MsmBatchPlan& plan, int xyzz_words, uint8_t* base, size_t capacity,
                             MsmWorkspace& ws);
 

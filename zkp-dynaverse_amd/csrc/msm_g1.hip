@@ -13,6 +13,18 @@ void msm_batch_prepare(const MsmBatchPlan& plan, int xyzz_words, MsmWorkspace& w
     msm_batch_prepare_impl(plan, xyzz_words, ws, stream);
 }
 
+uint64_t msm_run_workspace_bytes(size_t n, const MsmConfig& cfg, bool g2) {
+    if (!n) return 0;
+    const MsmBatchPlan plan = msm_single_plan_impl(MsmItem{n, 0, 0, cfg.window_bits}, cfg, g2);
+    const int xw = g2 ? PointLayout<Fq2Ops>::XYZZ_WORDS : PointLayout<FqOps>::XYZZ_WORDS;
+    const MsmBufferBytes b = msm_buffer_bytes(plan, xw);
+    uint64_t total = 0;
+    for (size_t part : {b.counts, b.offsets, b.digits, b.ranks, b.sorted, b.block_sums, b.total, b.tasks, b.partials,
+                        b.heavy, b.chunks, b.hist, b.descs, b.windows})
+        if (part) total += DeviceBuffer::grown(part);
+    return total;
+}
+
 size_t msm_workspace_borrow(const MsmBatchPlan& plan, int xyzz_words, uint8_t* base, size_t capacity,
                             MsmWorkspace& ws) {
     return msm_workspace_borrow_impl(plan, xyzz_words, base, capacity, ws);

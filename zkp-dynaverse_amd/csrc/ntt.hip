@@ -170,6 +170,14 @@ bool ntt_root_has_order_2_20(const uint32_t (&root)[8]) {
     return fr_is_one(sqr(p));
 }
 
+uint64_t ntt_domain_bytes(int max_log) {
+    uint64_t one = DeviceBuffer::grown(size_t(1u << (NTT_MAX_LOG - 1)) * 32) +
+                   DeviceBuffer::grown(size_t((1u << NTT_STAGE_TABLE_LOG) - 1) * LIMBS * 4);
+    for (int L = SINGLE_MAX_LOG + 1; L <= std::min(max_log, NTT_MAX_LOG); ++L)
+        one += DeviceBuffer::grown(size_t(pass1_table_entries(L)) * LIMBS * 4);
+    return 2 * one;
+}
+
 void ntt_domain_prepare(NttDomain& d, const uint32_t (&root)[8], hipStream_t stream) {
     if (d.ready && std::memcmp(d.root_words, root, 32) == 0) return;
     require(ntt_root_has_order_2_20(root), "NTT root must have order 2^20");

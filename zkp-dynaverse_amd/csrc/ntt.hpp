@@ -63,6 +63,11 @@ bool ntt_root_has_order_2_20(const uint32_t (&root)[8]);
 // Build (or reuse) the twiddle tables for `root` on `stream`.
 void ntt_domain_prepare(NttDomain& d, const uint32_t (&root)[8], hipStream_t stream);
 
+// Device bytes of a domain's tables, both directions: the power and stage
+// tables ntt_domain_prepare builds, plus (max_log >= 12) the first-pass
+// tables of every length 2^12..2^max_log, built on first use.
+uint64_t ntt_domain_bytes(int max_log);
+
 struct NttWorkspace {
     DeviceBuffer a, b;
     void attach(Quota* q) { a.quota = b.quota = q; }

@@ -1,14 +1,26 @@
-// Resident operator engine (reference: native/src/icicle_resident.cu,
-// semantics of crates/protocol/src/incremental_rebuild/operators.rs:387-590).
+// Resident operator engine: owner-tagged device buffers and the operators
+// the incremental rebuilder runs on them (semantics of
+// crates/protocol/src/incremental_rebuild/operators.rs:387-590 over the ABI
+// of native/src/icicle_resident.cu / dynaverse_gpu.h:99-158).
 //
-// Owner-tagged device buffers of three kinds:
-//   0 Fr      -- Montgomery words (32 B), converted at WRITE / READ;
-//   1 G1      -- XYZZ Montgomery words (128 B); boundary form affine 64 B;
-//   2 bases   -- immutable affine Montgomery words (64 B), MSM layout.
-// All-zero words are Fr 0 and the G1 point at infinity, so ALLOC zero-fills
-// with one memset.  Transactions save each element on its first write only
-// (host bitmap per buffer, device journal arena), ROLLBACK restores in
-// reverse order, COMMIT clears only the touched marks.
+// Storage.  Fr buffers hold Montgomery words (32 B), G1 buffers XYZZ
+// Montgomery words (128 B), packed bases affine Montgomery words (64 B, the
+// MSM layout, immutable).  All-zero words are Fr 0 and the point at
+// infinity, so a new buffer is one fill.  Values cross the boundary in
+// canonical form (Fr 32 B, affine G1 64 B) through pinned host staging and
+// one conversion kernel.
+//
+// Operations.  Each call resolves and checks every buffer range first, then
+// plans its journal saves (resident_journal.hpp: all or nothing), and only
+// then issues stream work: the saves, the operation's kernels, and one
+// synchronisation that books the device time.  Refusals (-1) therefore
+// leave buffers, journal and owner untouched.
+//
+// Admission.  Every allocation is checked before it is made against the
+// owner's quota and the device's free memory, together with the transient
+// workspace an operation over the longest buffer can leave allocated (MSM
+// plan buffers, staging, NTT tables), computed from the same plans the
+// operations use.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -19,15 +31,13 @@
 #include "g1mem.hpp"
 #include "gfft.hpp"
 #include "owner.hpp"
+#include "resident_journal.hpp"
 #include "setup_impl.hpp"
 
 namespace dvgpu {
 
 namespace {
 
-constexpr uint64_t DOMAIN_BYTES = (uint64_t(DVGPU_MAX_NTT_SIZE) + 1 + 6 + 64) * 32;
-constexpr uint64_t WORKSPACE_MARGIN = 64ull << 20;
-std::atomic<uint32_t> next_tag{1};
 
 __global__ void k_fr_in(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -182,605 +192,697 @@ __global__ void __launch_bounds__(64) k_g1_mul_const(const uint32_t* __restrict_
     stg1(out + i * 32, g1_mul_scalar(ldg1(in + i * 32), k, true));
 }
 
-// Montgomery Fr -> canonical words (MSM scalars)
-__global__ void k_fr_canon(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, size_t n) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t w[8];
-    from_mont(ldfr(in + i * 8), w);
-    for (int k = 0; k < 8; ++k) out[i * 8 + k] = w[k];
+using namespace resident;
+
+std::atomic<uint32_t> owner_tags{1};
+constexpr size_t MAX_BUFFERS = 1024;
+constexpr uint32_t OPCODES = DVGPU_RESIDENT_TRANSFORM_PACKET + 1;
+
+uint64_t since_ns(std::chrono::steady_clock::time_point t0) {
+    return uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count());
 }
 
-uint64_t checked_product(uint64_t a, uint64_t b) {
-    require(!a || b <= UINT64_MAX / a, "resident size overflow");
-    return a * b;
+int log2_ceil(uint64_t n) {
+    int l = 0;
+    while (l < 63 && (uint64_t(1) << l) < n) ++l;
+    return l;
 }
 
-bool overlaps(uint64_t a, uint64_t b, uint64_t n) { return n && (a < b ? b - a < n : a - b < n); }
+bool power_of_two(uint64_t n) { return n && !(n & (n - 1)); }
 
-void alias(uint64_t src, uint64_t so, uint64_t dst, uint64_t doff, uint64_t n, bool exact_ok) {
-    require(src != dst || !overlaps(so, doff, n) || (exact_ok && so == doff), "overlapping resident ranges");
-}
+const char* kind_name(uint32_t kind) { return kind == KIND_FR ? "Fr" : (kind == KIND_G1 ? "G1" : "packed bases"); }
 
-struct RBuffer {
+// One resident buffer.
+struct Slot {
     void* ptr = nullptr;
-    uint64_t length = 0;
-    uint32_t kind = 0;
-    bool writable = false;
-    std::vector<uint8_t> marks;
-    size_t width() const { return kind == 0 ? 32 : (kind == 1 ? 128 : 64); }
-    size_t boundary_width() const { return kind == 0 ? 32 : 64; }
-    bool marked(size_t i) const { return (marks[i / 8] >> (i % 8)) & 1; }
-    void mark(size_t i, bool on) {
-        const uint8_t bit = uint8_t(1u << (i % 8));
-        if (on) marks[i / 8] |= bit;
-        else marks[i / 8] &= uint8_t(~bit);
-    }
-    uint8_t* at(uint64_t off) const { return static_cast<uint8_t*>(ptr) + off * width(); }
+    uint64_t length = 0;  // elements
+    uint32_t kind = KIND_FR;
+    bool writable_sealed = false;  // stays writable after seal
+    uint32_t width() const { return stored_width(kind); }
+    uint8_t* at(uint64_t i) const { return static_cast<uint8_t*>(ptr) + i * width(); }
 };
 
-struct JournalRange {
-    uint64_t buffer, offset, length, arena;
+// A checked element range of one buffer.
+struct View {
+    Slot* slot;
+    uint32_t index;
+    uint64_t off, n;
+    uint32_t* words() const { return reinterpret_cast<uint32_t*>(slot->at(off)); }
+    WriteRange write() const { return {index, off, off + n, slot->width()}; }
+};
+
+// Two ranges of one buffer may not overlap, except (when `same_ok`) exactly.
+void check_alias(const View& x, const View& y, bool same_ok) {
+    if (x.index != y.index || !x.n || !y.n) return;
+    const bool overlap = x.off < y.off + y.n && y.off < x.off + x.n;
+    require(!overlap || (same_ok && x.off == y.off && x.n == y.n), "resident operation reads and writes overlapping ranges");
+}
+
+// Stream work between two events plus the host time spent issuing it.
+struct Span {
+    hipEvent_t begin = nullptr, end = nullptr;
+    bool recorded = false;
+    uint64_t host_ns = 0;
 };
 
 }  // namespace
 
 struct ResidentState {
-    Owner* owner = nullptr;
-    uint32_t tag = 0;
-    uint64_t limit = 0, journal_capacity = 0, journal_used = 0, max_length = 0;
-    bool sealed = false, transaction = false;
-    uint8_t* journal = nullptr;
-    std::vector<RBuffer> buffers;
-    std::vector<JournalRange> undo;
+    Owner* owner;
+    uint32_t tag;
+    uint64_t limit;
+    std::vector<Slot> slots;
+    Journal journal;
+    uint8_t* arena = nullptr;  // journal arena, journal.capacity() bytes
+    bool sealed = false;
+    uint64_t longest = 0;         // longest buffer allocated (elements)
+    uint64_t transient_seen = 0;  // transient bytes after the previous operation
     DvgpuResidentStats stats{};
     DvgpuResidentDiagnostics diag{};
-    hipEvent_t started = nullptr, finished = nullptr, prof[4] = {};
-    DvgpuResidentProfile* profile = nullptr;
-    bool op_active = false;
-    std::chrono::steady_clock::time_point op_started{};
     NttDomain domain;
     NttWorkspace ntt;
-    DeviceBuffer affine_ws, scalar_ws, g2_ws, host_stage_dev;
+    DeviceBuffer affine, canon_scalars, g2_points, boundary;
     HostBuffer pinned;
+    Span whole, saves_span, work_span;
+    DvgpuResidentProfile* profile = nullptr;
+
+    ResidentState(Owner* o, uint64_t quota_limit, uint64_t journal_bytes)
+        : owner(o), tag(owner_tags.fetch_add(1)), limit(quota_limit), journal(journal_bytes) {
+        require(tag != 0, "resident owner tags exhausted");
+        domain.set_quota(&o->quota);
+        ntt.attach(&o->quota);
+        for (DeviceBuffer* b : {&affine, &canon_scalars, &g2_points, &boundary}) b->quota = &o->quota;
+        for (Span* s : {&whole, &saves_span, &work_span}) {
+            HIP_CHECK(hipEventCreate(&s->begin));
+            HIP_CHECK(hipEventCreate(&s->end));
+        }
+        slots.reserve(MAX_BUFFERS);
+    }
+
+    ~ResidentState() {
+        if (owner->stream) (void)hipStreamSynchronize(owner->stream);
+        for (Slot& s : slots)
+            if (s.ptr) {
+                (void)hipFree(s.ptr);
+                owner->quota.release(s.length * s.width());
+            }
+        if (arena) {
+            (void)hipFree(arena);
+            owner->quota.release(journal.capacity());
+        }
+        for (Span* s : {&whole, &saves_span, &work_span})
+            for (hipEvent_t e : {s->begin, s->end})
+                if (e) (void)hipEventDestroy(e);
+    }
 
     hipStream_t stream() const { return owner->stream; }
 
-    RBuffer& buffer(uint64_t id) {
-        require((id >> 32) == tag, "resident handle belongs to another owner");
-        const uint64_t slot = id & 0xffffffffu;
-        require(slot && slot <= buffers.size(), "unknown resident handle");
-        return buffers[slot - 1];
+    // ---- admission -----------------------------------------------------------
+    // Transient bytes an operation over `len` elements may leave allocated:
+    // the owner's MSM workspace (a G2 MSM is the largest), canonical scalars,
+    // affine points, G2 staging (wire and Montgomery), boundary staging, the
+    // NTT work buffers (G1 rows, Fr rows above one pass) and the first-pass
+    // NTT tables of every length up to len.
+    uint64_t transient_need(uint64_t len) const {
+        const uint64_t m = std::min<uint64_t>(len, DVGPU_MAX_MSM_SIZE);
+        const uint64_t t = std::min<uint64_t>(len, DVGPU_MAX_TRANSFORM_MATRIX_SIZE);
+        uint64_t need = msm_run_workspace_bytes(size_t(m), owner->msm_config, true);
+        for (uint64_t bytes : {m * 32, m * 64, m * 256, mul_or_throw(len, 64), t * 128, t * 32})
+            if (bytes) need += DeviceBuffer::grown(bytes);
+        const int L = std::min(log2_ceil(len), NTT_MAX_LOG);
+        need += ntt_domain_bytes(L) - ntt_domain_bytes(0);
+        return need;
     }
-    RBuffer& range(uint64_t id, uint64_t off, uint64_t n, int kind = -1, bool write = false) {
-        RBuffer& b = buffer(id);
-        require(kind < 0 || b.kind == unsigned(kind), "resident buffer kind mismatch");
-        require(off <= b.length && n <= b.length - off, "resident range out of bounds");
-        require(!write || !sealed || b.writable, "resident buffer is immutable after seal");
+
+    uint64_t transient_now() const {
+        uint64_t b = affine.bytes + canon_scalars.bytes + g2_points.bytes + boundary.bytes + ntt.a.bytes + ntt.b.bytes;
+        for (const DeviceBuffer* x : owner->msm.all) b += x->bytes;
+        for (const NttTables* t : {&domain.fwd, &domain.inv})
+            for (const DeviceBuffer& p : t->pass1) b += p.bytes;
         return b;
     }
 
-    void save(uint64_t id, uint64_t off, uint64_t n) {
-        if (!transaction || !n) return;
-        const auto entered = std::chrono::steady_clock::now();
-        if (profile && !(profile->event_mask & 1)) {
-            HIP_CHECK(hipEventRecord(prof[0], stream()));
-            profile->event_mask |= 1;
-        }
-        RBuffer& b = buffer(id);
-        const uint64_t end = off + n;
-        uint64_t entries = 0, ranges = 0;
-        bool prior = false;
-        for (uint64_t i = off; i < end; ++i) {
-            const bool fresh = !b.marked(i);
-            if (fresh) {
-                ++entries;
-                if (!prior) ++ranges;
-            }
-            prior = fresh;
-        }
-        const uint64_t bytes = checked_product(entries, b.width());
-        require(bytes <= journal_capacity - journal_used, "resident transaction journal exhausted");
-        require(ranges <= undo.capacity() - undo.size(), "resident transaction range limit exceeded");
-        uint64_t i = off;
-        while (i < end) {
-            if (b.marked(i)) {
-                ++i;
-                continue;
-            }
-            const uint64_t first = i;
-            while (i < end && !b.marked(i)) ++i;
-            const uint64_t count = i - first, size = count * b.width();
-            HIP_CHECK(hipMemcpyAsync(journal + journal_used, b.at(first), size, hipMemcpyDeviceToDevice, stream()));
-            undo.push_back({id, first, count, journal_used});
-            for (uint64_t j = first; j < i; ++j) b.mark(j, true);
-            journal_used += size;
-            diag.journal_saved_bytes += size;
-            diag.explicit_d2d_bytes += size;
-            ++diag.journal_saved_ranges;
-        }
-        diag.journal_peak_bytes = std::max(diag.journal_peak_bytes, journal_used);
-        if (profile) {
-            HIP_CHECK(hipEventRecord(prof[1], stream()));
-            profile->journal_host_ns += uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(
-                                                     std::chrono::steady_clock::now() - entered)
-                                                     .count());
-        }
-    }
-
-    // Journal space for first writes to two whole buffers, checked before
-    // either is saved (icicle_resident.cu:142-160).
-    void preflight_pair(uint64_t first, uint64_t second, uint64_t n) {
-        if (!transaction) return;
-        uint64_t bytes = 0, ranges = 0;
-        for (const uint64_t id : {first, second}) {
-            RBuffer& b = buffer(id);
-            uint64_t entries = 0;
-            bool prior = false;
-            for (uint64_t i = 0; i < n; ++i) {
-                const bool fresh = !b.marked(i);
-                if (fresh) {
-                    ++entries;
-                    if (!prior) ++ranges;
-                }
-                prior = fresh;
-            }
-            bytes += checked_product(entries, b.width());
-        }
-        require(bytes <= journal_capacity - journal_used, "resident packet journal exhausted");
-        require(ranges <= undo.capacity() - undo.size(), "resident packet range limit exceeded");
-    }
-
-    void op_begin() {
-        if (!profile) return;
-        op_started = std::chrono::steady_clock::now();
-        op_active = true;
-        HIP_CHECK(hipEventRecord(prof[2], stream()));
-        profile->event_mask |= 2;
-    }
-    void op_end() {
-        if (!profile || !op_active) return;
-        HIP_CHECK(hipEventRecord(prof[3], stream()));
-        profile->operation_host_ns += uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(
-                                                   std::chrono::steady_clock::now() - op_started)
-                                                   .count());
-        op_active = false;
-    }
-    void wait() {
-        const auto entered = std::chrono::steady_clock::now();
-        const hipError_t r = hipStreamSynchronize(stream());
-        if (profile)
-            profile->wait_host_ns += uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(
-                                                  std::chrono::steady_clock::now() - entered)
-                                                  .count());
-        hip_check(r, "resident stream synchronize");
-    }
-    // Peak-memory admission before any allocation (icicle_resident.cu:219-229).
-    void quota(uint64_t extra, uint64_t next_max) {
-        const uint64_t workspace = WORKSPACE_MARGIN + checked_product(next_max, 1024);
-        const uint64_t allocated = owner->quota.used;
-        require(allocated <= limit && extra <= limit - allocated && workspace <= limit - allocated - extra,
-                "resident peak memory quota exceeded before allocation");
+    // Refuse (status -1) unless `bytes` more plus the transient workspace of
+    // operations over `len` elements fit the quota and the free device memory.
+    void admit(uint64_t bytes, uint64_t len) {
+        const uint64_t reserve = transient_need(len), used = owner->quota.used;
+        const bool fits = used <= limit && bytes <= limit - used && reserve <= limit - used - bytes;
+        require(fits, "resident memory quota exceeded: " + std::to_string(used) + " bytes in use, " +
+                          std::to_string(bytes) + " requested, " + std::to_string(reserve) +
+                          " kept for operation workspace, limit " + std::to_string(limit));
         size_t free_bytes = 0, total_bytes = 0;
         HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
-        require(extra <= free_bytes && workspace <= free_bytes - extra,
-                "insufficient device memory for resident buffers and bounded workspace");
-    }
-    void complete() {
-        op_end();
-        HIP_CHECK_LAUNCH();
-        HIP_CHECK(hipEventRecord(finished, stream()));
-        wait();
-        float ms = 0;
-        HIP_CHECK(hipEventElapsedTime(&ms, started, finished));
-        diag.device_operation_ns += uint64_t(double(ms) * 1e6);
-        ++diag.device_operations;
-        if (profile) {
-            profile->event_total_ns = uint64_t(double(ms) * 1e6);
-            profile->event_mask |= 4;
-            if (profile->event_mask & 1) {
-                HIP_CHECK(hipEventElapsedTime(&ms, prof[0], prof[1]));
-                profile->event_journal_ns = uint64_t(double(ms) * 1e6);
-            }
-            if (profile->event_mask & 2) {
-                HIP_CHECK(hipEventElapsedTime(&ms, prof[2], prof[3]));
-                profile->event_operation_ns = uint64_t(double(ms) * 1e6);
-            }
-        }
-    }
-    void publish(bool seal) {
-        if (seal) {
-            sealed = true;
-            return;
-        }
-        for (const auto& r : undo) {
-            RBuffer& b = buffer(r.buffer);
-            for (uint64_t i = r.offset; i < r.offset + r.length; ++i) b.mark(i, false);
-        }
-        undo.clear();
-        journal_used = 0;
-        transaction = false;
-    }
-    ~ResidentState() {
-        if (owner && owner->stream) (void)hipStreamSynchronize(owner->stream);
-        for (auto& b : buffers)
-            if (b.ptr) {
-                (void)hipFree(b.ptr);
-                owner->quota.release(b.length * b.width());
-            }
-        if (journal) {
-            (void)hipFree(journal);
-            owner->quota.release(journal_capacity);
-        }
-        for (hipEvent_t e : {started, finished, prof[0], prof[1], prof[2], prof[3]})
-            if (e) (void)hipEventDestroy(e);
+        require(bytes <= free_bytes && reserve <= free_bytes - bytes,
+                "device memory too small for the resident allocation and its operation workspace (quota " +
+                    std::to_string(limit) + ", free " + std::to_string(free_bytes) + ")");
     }
 
-    uint64_t execute(const DvgpuResidentOp& op);
-    void ecntt_rows(const uint32_t* in, uint32_t* out, int L, size_t rows, bool inverse);
-    void fr_rows(const uint32_t* in, uint32_t* out, int L, size_t rows, bool inverse);
+    void* device_alloc(uint64_t bytes) {
+        owner->quota.charge(bytes);
+        void* p = nullptr;
+        const hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            owner->quota.release(bytes);
+            throw DeviceError(std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        ++diag.resident_allocations;
+        stats.bytes_allocated += bytes;
+        return p;
+    }
+
+    // The NTT tables and the journal arena, after the configuration is admitted.
+    void build(const uint32_t (&root)[8]) {
+        std::memcpy(domain.pass_stages, owner->domain.pass_stages, sizeof(domain.pass_stages));
+        ntt_domain_prepare(domain, root, stream());
+        diag.resident_allocations += 4;  // power and stage tables, both directions
+        if (journal.capacity()) arena = static_cast<uint8_t*>(device_alloc(journal.capacity()));
+        // Everything the owner holds counts, retained inputs of earlier calls too.
+        stats.bytes_allocated = owner->quota.used;
+        diag.journal_metadata_capacity_bytes = journal.record_bound() * sizeof(SavedRun);
+    }
+
+    // ---- buffers ---------------------------------------------------------------
+    View view(uint64_t handle, uint64_t off, uint64_t n) {
+        const uint32_t i = slot_of(handle, tag, slots.size());
+        Slot& s = slots[i];
+        require(off <= s.length && n <= s.length - off, "resident range [" + std::to_string(off) + ", +" +
+                                                             std::to_string(n) + ") outside a buffer of " +
+                                                             std::to_string(s.length) + " elements");
+        return {&s, i, off, n};
+    }
+    View output(uint64_t handle, uint64_t off, uint64_t n) {
+        const View v = view(handle, off, n);
+        require(v.slot->kind != KIND_BASES, "packed MSM bases are immutable");
+        require(!sealed || v.slot->writable_sealed, "resident buffer is immutable after seal");
+        return v;
+    }
+    static void expect(const View& v, uint32_t kind) {
+        if (v.slot->kind != kind)
+            throw InvalidArgument(std::string("resident buffer kind mismatch: holds ") + kind_name(v.slot->kind) +
+                                  ", operation needs " + kind_name(kind));
+    }
+
+    // ---- stream work of one operation ---------------------------------------
+    void open_span(Span& s) {
+        HIP_CHECK(hipEventRecord(s.begin, stream()));
+        s.recorded = true;
+    }
+    void close_span(Span& s) { HIP_CHECK(hipEventRecord(s.end, stream())); }
+    static uint64_t span_ns(const Span& s) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, s.begin, s.end));
+        return uint64_t(double(ms) * 1e6);
+    }
+
+    void start() {
+        saves_span.recorded = work_span.recorded = false;
+        saves_span.host_ns = work_span.host_ns = 0;
+        open_span(whole);
+    }
+
+    // Copy the old values of planned saves into the arena, then record them.
+    void save(const std::vector<SavedRun>& saves) {
+        if (saves.empty()) return;
+        const auto t0 = std::chrono::steady_clock::now();
+        if (profile) open_span(saves_span);
+        for (const SavedRun& s : saves) {
+            HIP_CHECK(hipMemcpyAsync(arena + s.arena, slots[s.slot].at(s.lo), s.bytes(), hipMemcpyDeviceToDevice,
+                                     stream()));
+            diag.journal_saved_bytes += s.bytes();
+            diag.explicit_d2d_bytes += s.bytes();
+        }
+        diag.journal_saved_ranges += saves.size();
+        journal.record(saves);
+        diag.journal_peak_bytes = std::max(diag.journal_peak_bytes, journal.used());
+        if (profile) close_span(saves_span);
+        saves_span.host_ns += since_ns(t0);
+    }
+
+    // The operation's own kernels and copies.
+    template <class F>
+    void issue(F&& f) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (profile) open_span(work_span);
+        f();
+        HIP_CHECK_LAUNCH();
+        if (profile) close_span(work_span);
+        work_span.host_ns += since_ns(t0);
+    }
+
+    // Wait for the operation and book its device time.
+    void finish() {
+        close_span(whole);
+        const auto t0 = std::chrono::steady_clock::now();
+        const hipError_t waited = hipStreamSynchronize(stream());
+        const uint64_t wait_ns = since_ns(t0);
+        hip_check(waited, "resident operation");
+        const uint64_t total = span_ns(whole);
+        diag.device_operation_ns += total;
+        ++diag.device_operations;
+        const uint64_t transient = transient_now();
+        if (transient > transient_seen) {
+            ++diag.temporary_allocations;
+            transient_seen = transient;
+        }
+        diag.temporary_peak_bytes = std::max(diag.temporary_peak_bytes, transient);
+        if (!profile) return;
+        profile->wait_host_ns += wait_ns;
+        profile->journal_host_ns += saves_span.host_ns;
+        profile->operation_host_ns += work_span.host_ns;
+        profile->event_total_ns = total;
+        profile->event_mask |= 4;
+        if (saves_span.recorded) {
+            profile->event_journal_ns = span_ns(saves_span);
+            profile->event_mask |= 1;
+        }
+        if (work_span.recorded) {
+            profile->event_operation_ns = span_ns(work_span);
+            profile->event_mask |= 2;
+        }
+    }
+
+    void host_copy(void* dst, const void* src, size_t bytes) {
+        const auto t0 = std::chrono::steady_clock::now();
+        std::memcpy(dst, src, bytes);
+        if (profile) profile->conversion_host_ns += since_ns(t0);
+    }
+
+    // ---- transforms ------------------------------------------------------------
+    void fr_rows(const uint32_t* in, uint32_t* out, int L, size_t rows, bool inverse) {
+        ntt_rows_mont(domain, in, out, L, rows, inverse, ntt, stream());
+    }
+
+    // Natural order in and out: the G1 transform of gfft_rows (wave-shared
+    // twiddles, GLV products) with one permutation pass, and for the inverse
+    // the 1/n scaling as one shared-scalar product per point.
+    void g1_rows(const uint32_t* in, uint32_t* out, int L, size_t rows, bool inverse) {
+        const size_t n = size_t(1) << L, total = n * rows;
+        uint32_t* work = ntt.b.as<uint32_t>(total * 32);
+        if (!inverse) {
+            HIP_CHECK(hipMemcpyAsync(work, in, total * 128, hipMemcpyDeviceToDevice, stream()));
+            gfft_rows(domain, work, L, rows, false, stream());
+            k_ecntt_bitrev<<<grid_for(total, 64), 64, 0, stream()>>>(work, out, L, total);
+            HIP_CHECK_LAUNCH();
+            return;
+        }
+        k_ecntt_bitrev<<<grid_for(total, 64), 64, 0, stream()>>>(in, work, L, total);
+        HIP_CHECK_LAUNCH();
+        gfft_rows(domain, work, L, rows, true, stream());
+        uint32_t nw[8] = {0};
+        nw[L / 32] = 1u << (L % 32);
+        const Fr ninv = canon(inv(to_mont<FrParams>(nw)));
+        k_g1_mul_const<<<grid_for(total, 64), 64, 0, stream()>>>(work, out, total, ninv);
+    }
+
+    void rows_of(uint32_t kind, const uint32_t* in, uint32_t* out, int L, size_t rows, bool inverse) {
+        if (kind == KIND_FR) fr_rows(in, out, L, rows, inverse);
+        else g1_rows(in, out, L, rows, inverse);
+    }
+
+    void transpose_of(uint32_t kind, const uint32_t* in, uint32_t* out, size_t rows, size_t cols, size_t start,
+                      size_t n, bool twiddle, const Fr& root) {
+        if (kind == KIND_FR)
+            k_transpose_fr<<<grid_for(n, 128), 128, 0, stream()>>>(in, out, rows, cols, start, n, twiddle, root);
+        else
+            k_transpose_g1<<<grid_for(n, 64), 64, 0, stream()>>>(in, out, rows, cols, start, n, twiddle, root);
+        HIP_CHECK_LAUNCH();
+    }
+
+    // ---- operations ------------------------------------------------------------
+    uint64_t op_alloc(const DvgpuResidentOp& op);
+    uint64_t op_length(const DvgpuResidentOp& op) { return view(op.a, 0, 0).slot->length; }
+    uint64_t op_kind(const DvgpuResidentOp& op) { return view(op.a, 0, 0).slot->kind; }
+    uint64_t op_begin(const DvgpuResidentOp&) {
+        journal.begin();
+        return 0;
+    }
+    uint64_t op_rollback(const DvgpuResidentOp& op);
+    uint64_t op_transfer(const DvgpuResidentOp& op);
+    uint64_t op_copy(const DvgpuResidentOp& op);
+    uint64_t op_transpose(const DvgpuResidentOp& op);
+    uint64_t op_transform(const DvgpuResidentOp& op);
+    uint64_t op_packet(const DvgpuResidentOp& op);
+    uint64_t op_elementwise(const DvgpuResidentOp& op);
+    uint64_t op_msm_g1(const DvgpuResidentOp& op);
+    uint64_t op_msm_g2(const DvgpuResidentOp& op);
+
+    uint64_t call(const DvgpuResidentOp& op);
+
+    void seal() { sealed = true; }
+    void commit() { journal.close(); }
 };
+
+
+uint64_t ResidentState::call(const DvgpuResidentOp& op) {
+    using Handler = uint64_t (ResidentState::*)(const DvgpuResidentOp&);
+    // One handler per opcode (dynaverse_gpu.h numbering).
+    static const Handler handlers[OPCODES] = {
+        nullptr,
+        &ResidentState::op_alloc,        // ALLOC
+        &ResidentState::op_length,       // LENGTH
+        &ResidentState::op_kind,         // KIND
+        &ResidentState::op_begin,        // BEGIN
+        &ResidentState::op_rollback,     // ROLLBACK
+        &ResidentState::op_transfer,     // WRITE
+        &ResidentState::op_transfer,     // READ
+        &ResidentState::op_copy,         // COPY
+        &ResidentState::op_copy,         // REVERSE
+        &ResidentState::op_transpose,    // TRANSPOSE
+        &ResidentState::op_transform,    // TRANSFORM
+        &ResidentState::op_elementwise,  // ZIP
+        &ResidentState::op_elementwise,  // SCALE
+        &ResidentState::op_msm_g1,       // MSM_G1
+        &ResidentState::op_msm_g2,       // MSM_G2
+        &ResidentState::op_alloc,        // ALLOC_BASES
+        &ResidentState::op_packet,       // TRANSFORM_PACKET
+    };
+    ++stats.calls;
+    require(op.opcode < OPCODES && handlers[op.opcode], "unknown resident opcode " + std::to_string(op.opcode));
+    return (this->*handlers[op.opcode])(op);
+}
 
 void resident_destroy(ResidentState* s) noexcept { delete s; }
 DvgpuResidentStats resident_stats(const ResidentState* s) { return s ? s->stats : DvgpuResidentStats{}; }
 DvgpuResidentDiagnostics resident_diagnostics(const ResidentState* s) { return s ? s->diag : DvgpuResidentDiagnostics{}; }
 
-void ResidentState::fr_rows(const uint32_t* in, uint32_t* out, int L, size_t rows, bool inverse) {
-    ntt_rows_mont(domain, in, out, L, rows, inverse, ntt, stream());
+// ALLOC (kind Fr or G1, flags = writable after seal) and ALLOC_BASES (input:
+// canonical affine points, packed and immutable).  Returns the handle.
+uint64_t ResidentState::op_alloc(const DvgpuResidentOp& op) {
+    const bool bases = op.opcode == DVGPU_RESIDENT_ALLOC_BASES;
+    const uint64_t n = op.length;
+    require(!sealed, "resident buffers cannot be allocated after seal");
+    require(!journal.open(), "resident buffers cannot be allocated inside a transaction");
+    require(bases || op.kind <= KIND_G1, "resident buffer kind must be 0 (Fr) or 1 (G1)");
+    require(slots.size() < MAX_BUFFERS, "resident buffer table is full");
+    require(!bases || !n || op.input, "resident bases need their points");
+    Slot s;
+    s.kind = bases ? uint32_t(KIND_BASES) : op.kind;
+    s.length = n;
+    s.writable_sealed = !bases && op.flags != 0;
+    const uint64_t bytes = mul_or_throw(n, s.width());
+    admit(bytes, std::max(longest, n));
+    if (bytes) s.ptr = device_alloc(bytes);
+    slots.push_back(s);
+    longest = std::max(longest, n);
+    const Slot& b = slots.back();
+    start();
+    if (bases && n) {
+        uint8_t* pin = pinned.as<uint8_t>(n * 64);
+        host_copy(pin, op.input, n * 64);
+        uint32_t* stage = boundary.as<uint32_t>(n * 16);
+        issue([&] {
+            HIP_CHECK(hipMemcpyAsync(stage, pin, n * 64, hipMemcpyHostToDevice, stream()));
+            k_bases_in<<<grid_for(n, 128), 128, 0, stream()>>>(stage, static_cast<uint32_t*>(b.ptr), n);
+        });
+        stats.h2d_bytes += n * 64;
+        diag.physical_h2d_bytes += n * 64;
+    } else if (bytes) {
+        issue([&] { device_fill_words(static_cast<uint32_t*>(b.ptr), bytes / 4, 0u, stream()); });
+    }
+    finish();
+    return handle_of(tag, uint32_t(slots.size() - 1));
 }
 
-// Natural order in and out: the G1 transform of gfft_rows (wave-shared
-// twiddles, GLV products) with one permutation pass, and for the inverse the
-// 1/n scaling as one shared-scalar product per point.
-void ResidentState::ecntt_rows(const uint32_t* in, uint32_t* out, int L, size_t rows, bool inverse) {
-    const size_t n = size_t(1) << L, total = n * rows;
-    uint32_t* work = ntt.b.as<uint32_t>(total * 32);
-    if (!inverse) {
-        HIP_CHECK(hipMemcpyAsync(work, in, total * 128, hipMemcpyDeviceToDevice, stream()));
-        gfft_rows(domain, work, L, rows, false, stream());
-        k_ecntt_bitrev<<<grid_for(total, 64), 64, 0, stream()>>>(work, out, L, total);
-        HIP_CHECK_LAUNCH();
-        return;
-    }
-    k_ecntt_bitrev<<<grid_for(total, 64), 64, 0, stream()>>>(in, work, L, total);
-    HIP_CHECK_LAUNCH();
-    gfft_rows(domain, work, L, rows, true, stream());
-    uint32_t nw[8] = {0};
-    nw[L / 32] = 1u << (L % 32);
-    const Fr ninv = canon(inv(to_mont<FrParams>(nw)));
-    k_g1_mul_const<<<grid_for(total, 64), 64, 0, stream()>>>(work, out, total, ninv);
-    HIP_CHECK_LAUNCH();
+// Restore every saved run (newest first) and close the transaction.
+uint64_t ResidentState::op_rollback(const DvgpuResidentOp&) {
+    require(journal.open(), "no resident transaction is open");
+    start();
+    issue([&] {
+        const auto& log = journal.log();
+        for (auto r = log.rbegin(); r != log.rend(); ++r) {
+            HIP_CHECK(hipMemcpyAsync(slots[r->slot].at(r->lo), arena + r->arena, r->bytes(), hipMemcpyDeviceToDevice,
+                                     stream()));
+            diag.explicit_d2d_bytes += r->bytes();
+        }
+    });
+    finish();
+    journal.close();
+    return 0;
 }
 
-static int log2_exact(uint64_t n) {
-    int l = 0;
-    while ((uint64_t(1) << l) < n) ++l;
-    return l;
-}
-
-uint64_t ResidentState::execute(const DvgpuResidentOp& op) {
-    ++stats.calls;
-    const size_t n = op.length;
-    if (op.opcode == DVGPU_RESIDENT_LENGTH) return buffer(op.a).length;
-    if (op.opcode == DVGPU_RESIDENT_KIND) return buffer(op.a).kind;
-    if (op.opcode == DVGPU_RESIDENT_BEGIN) {
-        require(!transaction, "resident transaction already open");
-        transaction = true;
-        journal_used = 0;
-        undo.clear();
-        return 0;
-    }
-    HIP_CHECK(hipEventRecord(started, stream()));
-    hipStream_t st = stream();
-    if (op.opcode == DVGPU_RESIDENT_ALLOC || op.opcode == DVGPU_RESIDENT_ALLOC_BASES) {
-        const bool packed = op.opcode == DVGPU_RESIDENT_ALLOC_BASES;
-        require(!sealed && !transaction, "resident allocation after seal or transaction start");
-        require(op.kind <= 1 && buffers.size() < 1024, "resident kind or buffer limit invalid");
-        require(!packed || !n || op.input, "null affine basis input");
-        RBuffer b;
-        b.length = op.length;
-        b.kind = packed ? 2u : op.kind;
-        b.writable = !packed && op.flags != 0;
-        const uint64_t bytes = checked_product(n, b.width());
-        quota(bytes, std::max<uint64_t>(max_length, op.length));
-        b.marks.assign(n / 8 + (n % 8 != 0), 0);
-        if (bytes) {
-            owner->quota.charge(bytes);
-            const hipError_t e = hipMalloc(&b.ptr, bytes);
-            if (e != hipSuccess) {
-                owner->quota.release(bytes);
-                throw DeviceError(std::string("hipMalloc: ") + hipGetErrorString(e));
-            }
-            ++diag.resident_allocations;
-        }
-        buffers.push_back(std::move(b));
-        RBuffer& nb = buffers.back();
-        stats.bytes_allocated += bytes;
-        max_length = std::max<uint64_t>(max_length, op.length);
-        if (packed && n) {
-            uint32_t* stage = host_stage_dev.as<uint32_t>(n * 16);
-            HIP_CHECK(hipMemcpyAsync(stage, op.input, n * 64, hipMemcpyHostToDevice, st));
-            k_bases_in<<<grid_for(n, 128), 128, 0, st>>>(stage, static_cast<uint32_t*>(nb.ptr), n);
-            stats.h2d_bytes += n * 64;
-            diag.physical_h2d_bytes += n * 64;
-        } else if (bytes) {
-            HIP_CHECK(hipMemsetAsync(nb.ptr, 0, bytes, st));
-        }
-        complete();
-        return (uint64_t(tag) << 32) | buffers.size();
-    }
-    if (op.opcode == DVGPU_RESIDENT_ROLLBACK) {
-        require(transaction, "no resident transaction open");
-        for (auto it = undo.rbegin(); it != undo.rend(); ++it) {
-            RBuffer& b = buffer(it->buffer);
-            HIP_CHECK(hipMemcpyAsync(b.at(it->offset), journal + it->arena, it->length * b.width(),
-                                     hipMemcpyDeviceToDevice, st));
-            diag.explicit_d2d_bytes += it->length * b.width();
-        }
-        complete();
-        publish(false);
-        return 0;
-    }
-    if (op.opcode == DVGPU_RESIDENT_WRITE || op.opcode == DVGPU_RESIDENT_READ) {
-        const bool write = op.opcode == DVGPU_RESIDENT_WRITE;
-        RBuffer& b = range(op.a, op.a_offset, n, -1, write);
-        require(b.kind == op.kind || (b.kind == 2 && op.kind == 1), "resident buffer kind mismatch");
-        require(!write || b.kind != 2, "packed MSM bases are immutable");
-        require(write || (op.flags <= DVGPU_READ_DEVICE_AFFINE && (!op.flags || op.kind == 1)),
-                "invalid resident read flags");
-        require(!n || (write ? op.input != nullptr : op.output != nullptr), "null resident host buffer");
-        if (!n) {
-            complete();
-            return 0;
-        }
-        if (write) save(op.a, op.a_offset, n);
-        const size_t bw = b.boundary_width();
-        uint32_t* stage = host_stage_dev.as<uint32_t>(n * bw / 4);
-        uint8_t* pin = pinned.as<uint8_t>(n * bw);
-        op_begin();
+// WRITE / READ: canonical host values to or from a range (kind Fr, or G1 for
+// G1 buffers and, read-only, packed bases; flags DVGPU_READ_DEVICE_AFFINE
+// accepted for G1 reads, whose points are always normalised on the device).
+uint64_t ResidentState::op_transfer(const DvgpuResidentOp& op) {
+    const bool write = op.opcode == DVGPU_RESIDENT_WRITE;
+    const uint64_t n = op.length;
+    const View v = write ? output(op.a, op.a_offset, n) : view(op.a, op.a_offset, n);
+    const uint32_t kind = v.slot->kind;
+    require(kind == op.kind || (kind == KIND_BASES && op.kind == KIND_G1), "resident buffer kind mismatch");
+    require(write || op.flags == 0 || (op.flags == DVGPU_READ_DEVICE_AFFINE && op.kind == KIND_G1),
+            "invalid resident read flags");
+    require(!n || (write ? op.input != nullptr : op.output != nullptr), "null resident host buffer");
+    const std::vector<SavedRun> saves = write ? journal.plan({v.write()}) : std::vector<SavedRun>{};
+    const uint32_t bw = boundary_width(kind);
+    start();
+    uint8_t* pin = nullptr;
+    if (n) {
+        save(saves);
+        pin = pinned.as<uint8_t>(n * bw);
+        uint32_t* stage = boundary.as<uint32_t>(n * bw / 4);
         if (write) {
-            HIP_CHECK(hipMemcpyAsync(stage, op.input, n * bw, hipMemcpyHostToDevice, st));
-            if (b.kind == 0)
-                k_fr_in<<<grid_for(n, 128), 128, 0, st>>>(stage, reinterpret_cast<uint32_t*>(b.at(op.a_offset)), n);
-            else
-                k_g1_in<<<grid_for(n, 128), 128, 0, st>>>(stage, reinterpret_cast<uint32_t*>(b.at(op.a_offset)), n);
-            HIP_CHECK_LAUNCH();
+            host_copy(pin, op.input, n * bw);
+            issue([&] {
+                HIP_CHECK(hipMemcpyAsync(stage, pin, n * bw, hipMemcpyHostToDevice, stream()));
+                if (kind == KIND_FR) k_fr_in<<<grid_for(n, 128), 128, 0, stream()>>>(stage, v.words(), n);
+                else k_g1_in<<<grid_for(n, 128), 128, 0, stream()>>>(stage, v.words(), n);
+            });
         } else {
-            const uint32_t* src = reinterpret_cast<const uint32_t*>(b.at(op.a_offset));
-            if (b.kind == 0) k_fr_out<<<grid_for(n, 128), 128, 0, st>>>(src, stage, n);
-            else if (b.kind == 1) k_g1_out<true><<<grid_for(n, 128), 128, 0, st>>>(src, stage, n);
-            else k_bases_out<<<grid_for(n, 128), 128, 0, st>>>(src, stage, n);
-            HIP_CHECK_LAUNCH();
-            HIP_CHECK(hipMemcpyAsync(pin, stage, n * bw, hipMemcpyDeviceToHost, st));
-        }
-        op_end();
-        complete();
-        if (!write) std::memcpy(op.output, pin, n * bw);
-        if (write) {
-            stats.h2d_bytes += n * bw;
-            diag.physical_h2d_bytes += n * bw;
-        } else {
-            stats.d2h_bytes += n * bw;
-            diag.physical_d2h_bytes += n * bw;
-        }
-        return 0;
-    }
-    if (op.opcode == DVGPU_RESIDENT_COPY || op.opcode == DVGPU_RESIDENT_REVERSE) {
-        RBuffer& src = range(op.a, op.a_offset, n);
-        require(src.kind <= 1, "packed MSM bases cannot be copied or reversed");
-        RBuffer& dst = range(op.c, op.c_offset, n, int(src.kind), true);
-        if (op.opcode == DVGPU_RESIDENT_REVERSE) require(src.kind == op.kind, "reverse kind mismatch");
-        alias(op.a, op.a_offset, op.c, op.c_offset, n, op.opcode == DVGPU_RESIDENT_COPY || n <= 1);
-        if (!n || (op.opcode == DVGPU_RESIDENT_COPY && op.a == op.c && op.a_offset == op.c_offset)) {
-            complete();
-            return 0;
-        }
-        save(op.c, op.c_offset, n);
-        op_begin();
-        if (op.opcode == DVGPU_RESIDENT_COPY) {
-            HIP_CHECK(hipMemcpyAsync(dst.at(op.c_offset), src.at(op.a_offset), n * src.width(),
-                                     hipMemcpyDeviceToDevice, st));
-            diag.explicit_d2d_bytes += n * src.width();
-        } else {
-            k_reverse<<<grid_for(n, 128), 128, 0, st>>>(reinterpret_cast<const uint32_t*>(src.at(op.a_offset)),
-                                                       reinterpret_cast<uint32_t*>(dst.at(op.c_offset)), n,
-                                                       int(src.width() / 4));
-        }
-        complete();
-        return 0;
-    }
-    if (op.opcode == DVGPU_RESIDENT_TRANSFORM_PACKET) {
-        require(op.kind <= 1 && op.flags <= 1 && n && !(n & (n - 1)) && n <= DVGPU_MAX_TRANSFORM_PACKET_SIZE &&
-                    op.a != op.b && op.a != op.c && op.b != op.c,
-                "invalid resident packet shape, kind or alias");
-        RBuffer& src = range(op.a, 0, n, int(op.kind));
-        RBuffer& scr = range(op.b, 0, n, int(op.kind), true);
-        RBuffer& dst = range(op.c, 0, n, int(op.kind), true);
-        // w_n (or its inverse) from the configured 2^20-th root.
-        const int L = log2_exact(n);
-        Fr root = to_mont<FrParams>(domain.root_words);
-        for (int k = L; k < 20; ++k) root = sqr(root);
-        if (op.flags) root = inv(root);
-        root = canon(root);
-        preflight_pair(op.b, op.c, n);
-        save(op.b, 0, n);
-        save(op.c, 0, n);
-        const size_t a = size_t(1) << (L / 2), bdim = n / a;
-        const bool inv_dir = op.flags != 0;
-        op_begin();
-        auto* in = reinterpret_cast<const uint32_t*>(src.ptr);
-        auto* tmp = reinterpret_cast<uint32_t*>(scr.ptr);
-        auto* out = reinterpret_cast<uint32_t*>(dst.ptr);
-        const unsigned blocks = grid_for(n, 128);
-        if (op.kind == 0) {
-            k_transpose_fr<<<blocks, 128, 0, st>>>(in, out, a, bdim, 0, n, 0, root);
-            fr_rows(out, tmp, log2_exact(a), bdim, inv_dir);
-            k_transpose_fr<<<blocks, 128, 0, st>>>(tmp, out, bdim, a, 0, n, 1, root);
-            fr_rows(out, tmp, log2_exact(bdim), a, inv_dir);
-            k_transpose_fr<<<blocks, 128, 0, st>>>(tmp, out, a, bdim, 0, n, 0, root);
-            stats.fr_transform_elements += 2 * n;
-        } else {
-            k_transpose_g1<<<blocks, 128, 0, st>>>(in, out, a, bdim, 0, n, 0, root);
-            ecntt_rows(out, tmp, log2_exact(a), bdim, inv_dir);
-            k_transpose_g1<<<blocks, 128, 0, st>>>(tmp, out, bdim, a, 0, n, 1, root);
-            ecntt_rows(out, tmp, log2_exact(bdim), a, inv_dir);
-            k_transpose_g1<<<blocks, 128, 0, st>>>(tmp, out, a, bdim, 0, n, 0, root);
-            stats.g1_transform_elements += 2 * n;
-        }
-        HIP_CHECK_LAUNCH();
-        diag.explicit_d2d_bytes += ((a == 1) + (bdim == 1)) * n * src.width();
-        complete();
-        return 0;
-    }
-    if (op.opcode == DVGPU_RESIDENT_TRANSPOSE) {
-        require(op.kind <= 1, "packed MSM bases cannot be transposed");
-        const size_t rows = op.rows, cols = op.cols, start = op.a_offset;
-        const uint64_t total = checked_product(rows, cols);
-        require(rows && cols && total <= DVGPU_MAX_TRANSFORM_MATRIX_SIZE && start <= total && n <= total - start &&
-                    op.a != op.c,
-                "invalid resident transpose shape or alias");
-        RBuffer& src = range(op.a, 0, total, int(op.kind));
-        RBuffer& dst = range(op.c, 0, total, int(op.kind), true);
-        if (!n) {
-            complete();
-            return 0;
-        }
-        Fr root = fp_one<FrParams>();
-        if (op.flags) {
-            require(op.input != nullptr, "null transpose root");
-            uint32_t w[8];
-            std::memcpy(w, op.input, 32);
-            root = canon(to_mont<FrParams>(w));
-        }
-        // Transposed targets are contiguous within each column: journal only those.
-        const size_t end = start + n, first = start / cols, last = (end - 1) / cols;
-        const size_t count_cols = std::min(cols, n);
-        for (size_t step = 0; step < count_cols; ++step) {
-            const size_t col = (start % cols + step) % cols;
-            const size_t lo = first + (col < start % cols);
-            const size_t hi = last - (col > (end - 1) % cols);
-            if (lo <= hi && hi < rows) save(op.c, col * rows + lo, hi - lo + 1);
-        }
-        op_begin();
-        if (op.kind == 0)
-            k_transpose_fr<<<grid_for(n, 128), 128, 0, st>>>(static_cast<const uint32_t*>(src.ptr),
-                                                            static_cast<uint32_t*>(dst.ptr), rows, cols, start, n,
-                                                            op.flags ? 1 : 0, root);
-        else
-            k_transpose_g1<<<grid_for(n, 64), 64, 0, st>>>(static_cast<const uint32_t*>(src.ptr),
-                                                          static_cast<uint32_t*>(dst.ptr), rows, cols, start, n,
-                                                          op.flags ? 1 : 0, root);
-        complete();
-        return 0;
-    }
-    if (op.opcode == DVGPU_RESIDENT_TRANSFORM) {
-        require(op.kind <= 1, "packed MSM bases cannot be transformed");
-        const size_t row_len = op.cols, count = op.rows;
-        const uint64_t offset = checked_product(op.a_offset, row_len), total = checked_product(row_len, count);
-        require(row_len && !(row_len & (row_len - 1)) && row_len <= DVGPU_MAX_NTT_SIZE && count <= INT_MAX &&
-                    total <= DVGPU_MAX_TRANSFORM_MATRIX_SIZE && op.a != op.c,
-                "invalid resident transform shape or alias");
-        RBuffer& src = range(op.a, offset, total, int(op.kind));
-        RBuffer& dst = range(op.c, offset, total, int(op.kind), true);
-        if (!total) {
-            complete();
-            return 0;
-        }
-        save(op.c, offset, total);
-        op_begin();
-        const int L = log2_exact(row_len);
-        if (op.kind == 0) {
-            fr_rows(reinterpret_cast<const uint32_t*>(src.at(offset)), reinterpret_cast<uint32_t*>(dst.at(offset)), L,
-                    count, op.flags != 0);
-            stats.fr_transform_elements += total;
-        } else {
-            ecntt_rows(reinterpret_cast<const uint32_t*>(src.at(offset)), reinterpret_cast<uint32_t*>(dst.at(offset)), L,
-                       count, op.flags != 0);
-            stats.g1_transform_elements += total;
-        }
-        complete();
-        return 0;
-    }
-    if (op.opcode == DVGPU_RESIDENT_ZIP || op.opcode == DVGPU_RESIDENT_SCALE) {
-        const bool scale = op.opcode == DVGPU_RESIDENT_SCALE;
-        RBuffer& left = range(op.a, op.a_offset, n, scale ? 1 : 0);
-        RBuffer& right = range(op.b, op.b_offset, n, 0);
-        RBuffer& dst = range(op.c, op.c_offset, n, scale ? 1 : 0, true);
-        require(scale || op.flags <= 2, "invalid resident field binary operation");
-        alias(op.a, op.a_offset, op.c, op.c_offset, n, true);
-        alias(op.b, op.b_offset, op.c, op.c_offset, n, true);
-        if (!n) {
-            complete();
-            return 0;
-        }
-        save(op.c, op.c_offset, n);
-        op_begin();
-        if (scale)
-            k_scale<<<grid_for(n, 64), 64, 0, st>>>(reinterpret_cast<const uint32_t*>(left.at(op.a_offset)),
-                                                   reinterpret_cast<const uint32_t*>(right.at(op.b_offset)),
-                                                   reinterpret_cast<uint32_t*>(dst.at(op.c_offset)), n);
-        else
-            k_zip<<<grid_for(n, 128), 128, 0, st>>>(reinterpret_cast<const uint32_t*>(left.at(op.a_offset)),
-                                                   reinterpret_cast<const uint32_t*>(right.at(op.b_offset)),
-                                                   reinterpret_cast<uint32_t*>(dst.at(op.c_offset)), n, op.flags);
-        complete();
-        return 0;
-    }
-    if (op.opcode == DVGPU_RESIDENT_MSM_G1 || op.opcode == DVGPU_RESIDENT_MSM_G2) {
-        RBuffer& sc = range(op.b, op.b_offset, n, 0);
-        require(n <= DVGPU_MAX_MSM_SIZE && op.output, "invalid resident MSM length or output");
-        uint32_t* canon_sc = scalar_ws.as<uint32_t>(std::max<size_t>(n, 1) * 8);
-        if (op.opcode == DVGPU_RESIDENT_MSM_G1) {
-            RBuffer& pts = range(op.a, op.a_offset, n);
-            require(pts.kind == 1 || pts.kind == 2, "resident MSM point kind mismatch");
-            uint8_t result[64] = {0};
-            if (n) {
-                op_begin();
-                const uint32_t* bases = reinterpret_cast<const uint32_t*>(pts.at(op.a_offset));
-                if (pts.kind == 1) {
-                    uint32_t* aff = affine_ws.as<uint32_t>(n * 16);
-                    k_g1_out<false><<<grid_for(n, 128), 128, 0, st>>>(bases, aff, n);
-                    bases = aff;
-                }
-                k_fr_canon<<<grid_for(n, 128), 128, 0, st>>>(reinterpret_cast<const uint32_t*>(sc.at(op.b_offset)),
-                                                            canon_sc, n);
+            issue([&] {
+                if (kind == KIND_FR) k_fr_out<<<grid_for(n, 128), 128, 0, stream()>>>(v.words(), stage, n);
+                else if (kind == KIND_G1) k_g1_out<true><<<grid_for(n, 128), 128, 0, stream()>>>(v.words(), stage, n);
+                else k_bases_out<<<grid_for(n, 128), 128, 0, stream()>>>(v.words(), stage, n);
                 HIP_CHECK_LAUNCH();
-                const Affine<FqOps> r = msm_run<FqOps>(bases, canon_sc, n, owner->msm, st, owner->msm_config);
-                affine_to_bytes<FqOps>(r, result);
-                op_end();
-                diag.physical_d2h_bytes += 64;
-            }
-            complete();
-            std::memcpy(op.output, result, 64);
-            stats.msm_g1_points += n;
-            stats.d2h_bytes += 64;
-        } else {
-            require(!n || op.input, "null resident G2 bases");
-            uint8_t result[128] = {0};
-            if (n) {
-                op_begin();
-                uint8_t* stage = g2_ws.as<uint8_t>(n * 128 * 2);
-                HIP_CHECK(hipMemcpyAsync(stage, op.input, n * 128, hipMemcpyHostToDevice, st));
-                uint32_t* mont = reinterpret_cast<uint32_t*>(stage + n * 128);
-                points_to_mont<Fq2Ops>(stage, mont, n, st);
-                k_fr_canon<<<grid_for(n, 128), 128, 0, st>>>(reinterpret_cast<const uint32_t*>(sc.at(op.b_offset)),
-                                                            canon_sc, n);
-                HIP_CHECK_LAUNCH();
-                const Affine<Fq2Ops> r = msm_run<Fq2Ops>(mont, canon_sc, n, owner->msm, st, owner->msm_config);
-                affine_to_bytes<Fq2Ops>(r, result);
-                op_end();
-                diag.physical_h2d_bytes += n * 128;
-                diag.physical_d2h_bytes += 128;
-            }
-            complete();
-            std::memcpy(op.output, result, 128);
-            stats.msm_g2_points += n;
-            stats.h2d_bytes += n * 128;
-            stats.d2h_bytes += 128;
+                HIP_CHECK(hipMemcpyAsync(pin, stage, n * bw, hipMemcpyDeviceToHost, stream()));
+            });
         }
-        return 0;
     }
-    throw InvalidArgument("unknown resident opcode");
+    finish();
+    if (write) {
+        stats.h2d_bytes += n * bw;
+        diag.physical_h2d_bytes += n * bw;
+    } else {
+        if (n) host_copy(op.output, pin, n * bw);
+        stats.d2h_bytes += n * bw;
+        diag.physical_d2h_bytes += n * bw;
+    }
+    return 0;
+}
+
+// COPY a -> c, or REVERSE (c[i] = a[n - 1 - i]); Fr or G1 buffers.
+uint64_t ResidentState::op_copy(const DvgpuResidentOp& op) {
+    const bool reverse = op.opcode == DVGPU_RESIDENT_REVERSE;
+    const uint64_t n = op.length;
+    const View src = view(op.a, op.a_offset, n);
+    require(src.slot->kind != KIND_BASES, "packed MSM bases cannot be copied or reversed");
+    if (reverse) require(src.slot->kind == op.kind, "resident reverse kind mismatch");
+    const View dst = output(op.c, op.c_offset, n);
+    expect(dst, src.slot->kind);
+    check_alias(src, dst, !reverse || n <= 1);
+    const bool nothing = !n || (!reverse && src.index == dst.index && src.off == dst.off);
+    const std::vector<SavedRun> saves = nothing ? std::vector<SavedRun>{} : journal.plan({dst.write()});
+    start();
+    if (!nothing) {
+        save(saves);
+        const uint32_t w = src.slot->width();
+        issue([&] {
+            if (reverse) {
+                k_reverse<<<grid_for(n, 128), 128, 0, stream()>>>(src.words(), dst.words(), n, int(w / 4));
+            } else {
+                HIP_CHECK(hipMemcpyAsync(dst.words(), src.words(), n * w, hipMemcpyDeviceToDevice, stream()));
+                diag.explicit_d2d_bytes += n * w;
+            }
+        });
+    }
+    finish();
+    return 0;
+}
+
+// TRANSPOSE elements [a_offset, a_offset + length) of the rows x cols
+// matrix a into c (element r * cols + col -> col * rows + r), each value
+// optionally times root^(r * col) (flags != 0, root canonical at input).
+uint64_t ResidentState::op_transpose(const DvgpuResidentOp& op) {
+    require(op.kind <= KIND_G1, "resident transpose kind must be 0 (Fr) or 1 (G1)");
+    const uint64_t rows = op.rows, cols = op.cols, start_at = op.a_offset, n = op.length;
+    const uint64_t total = mul_or_throw(rows, cols);
+    require(rows && cols && total <= DVGPU_MAX_TRANSFORM_MATRIX_SIZE && start_at <= total && n <= total - start_at &&
+                op.a != op.c,
+            "invalid resident transpose shape or alias");
+    const View src = view(op.a, 0, total), dst = output(op.c, 0, total);
+    expect(src, op.kind);
+    expect(dst, op.kind);
+    Fr root = fp_one<FrParams>();
+    if (op.flags) {
+        require(op.input != nullptr, "null resident transpose root");
+        uint32_t w[8];
+        std::memcpy(w, op.input, 32);
+        require(is_canonical<FrParams>(w), "resident transpose root is not a canonical Fr element");
+        root = canon(to_mont<FrParams>(w));
+    }
+    std::vector<Run> runs;
+    transpose_destinations(rows, cols, start_at, n, runs);
+    std::vector<WriteRange> writes;
+    writes.reserve(runs.size());
+    for (const Run& r : runs) writes.push_back({dst.index, r.lo, r.hi, dst.slot->width()});
+    const std::vector<SavedRun> saves = journal.plan(writes);
+    start();
+    if (n) {
+        save(saves);
+        issue([&] { transpose_of(op.kind, src.words(), dst.words(), rows, cols, start_at, n, op.flags != 0, root); });
+    }
+    finish();
+    return 0;
+}
+
+// TRANSFORM rows [a_offset, a_offset + rows) of length cols (a power of two)
+// from a into the same rows of c; flags != 0 for the inverse.
+uint64_t ResidentState::op_transform(const DvgpuResidentOp& op) {
+    require(op.kind <= KIND_G1, "resident transform kind must be 0 (Fr) or 1 (G1)");
+    const uint64_t len = op.cols, count = op.rows;
+    const uint64_t first = mul_or_throw(op.a_offset, len), total = mul_or_throw(len, count);
+    require(power_of_two(len) && len <= DVGPU_MAX_NTT_SIZE && count <= INT_MAX &&
+                total <= DVGPU_MAX_TRANSFORM_MATRIX_SIZE && op.a != op.c,
+            "invalid resident transform shape or alias");
+    const View src = view(op.a, first, total), dst = output(op.c, first, total);
+    expect(src, op.kind);
+    expect(dst, op.kind);
+    const std::vector<SavedRun> saves = journal.plan({dst.write()});
+    start();
+    if (total) {
+        save(saves);
+        issue([&] { rows_of(op.kind, src.words(), dst.words(), log2_ceil(len), size_t(count), op.flags != 0); });
+        (op.kind == KIND_FR ? stats.fr_transform_elements : stats.g1_transform_elements) += total;
+    }
+    finish();
+    return 0;
+}
+
+// TRANSFORM_PACKET: a length-n (n <= 4096) natural-order transform of a into
+// c as a four-step over an a1 x a2 grid (a1 = 2^(L/2)), with b as scratch;
+// b and c are journaled as a whole before either is touched.
+uint64_t ResidentState::op_packet(const DvgpuResidentOp& op) {
+    const uint64_t n = op.length;
+    require(op.kind <= KIND_G1 && op.flags <= 1 && power_of_two(n) && n <= DVGPU_MAX_TRANSFORM_PACKET_SIZE &&
+                op.a != op.b && op.a != op.c && op.b != op.c,
+            "invalid resident packet shape, kind or alias");
+    const View src = view(op.a, 0, n), scratch = output(op.b, 0, n), dst = output(op.c, 0, n);
+    for (const View* v : {&src, &scratch, &dst}) expect(*v, op.kind);
+    const std::vector<SavedRun> saves = journal.plan({scratch.write(), dst.write()});
+    const int L = log2_ceil(n);
+    const bool inverse = op.flags != 0;
+    // omega_n (or its inverse) from the domain's 2^20-th root.
+    Fr w = to_mont<FrParams>(domain.root_words);
+    for (int k = L; k < NTT_MAX_LOG; ++k) w = sqr(w);
+    if (inverse) w = inv(w);
+    w = canon(w);
+    const size_t a1 = size_t(1) << (L / 2), a2 = size_t(n) / a1;
+    start();
+    save(saves);
+    issue([&] {
+        uint32_t* tmp = scratch.words();
+        uint32_t* out = dst.words();
+        transpose_of(op.kind, src.words(), out, a1, a2, 0, n, false, w);
+        rows_of(op.kind, out, tmp, log2_ceil(a1), a2, inverse);
+        transpose_of(op.kind, tmp, out, a2, a1, 0, n, true, w);
+        rows_of(op.kind, out, tmp, log2_ceil(a2), a1, inverse);
+        transpose_of(op.kind, tmp, out, a1, a2, 0, n, false, w);
+    });
+    (op.kind == KIND_FR ? stats.fr_transform_elements : stats.g1_transform_elements) += 2 * n;
+    finish();
+    return 0;
+}
+
+// ZIP: c = a (+, -, *) b over Fr (flags 0, 1, 2); SCALE: c = [b] a over G1.
+uint64_t ResidentState::op_elementwise(const DvgpuResidentOp& op) {
+    const bool scale = op.opcode == DVGPU_RESIDENT_SCALE;
+    const uint32_t kind = scale ? uint32_t(KIND_G1) : uint32_t(KIND_FR);
+    const uint64_t n = op.length;
+    const View left = view(op.a, op.a_offset, n), right = view(op.b, op.b_offset, n);
+    const View dst = output(op.c, op.c_offset, n);
+    expect(left, kind);
+    expect(right, KIND_FR);
+    expect(dst, kind);
+    require(scale || op.flags <= 2, "invalid resident field operation (0 add, 1 subtract, 2 multiply)");
+    check_alias(left, dst, true);
+    check_alias(right, dst, true);
+    const std::vector<SavedRun> saves = journal.plan({dst.write()});
+    start();
+    if (n) {
+        save(saves);
+        issue([&] {
+            if (scale) k_scale<<<grid_for(n, 64), 64, 0, stream()>>>(left.words(), right.words(), dst.words(), n);
+            else k_zip<<<grid_for(n, 128), 128, 0, stream()>>>(left.words(), right.words(), dst.words(), n, op.flags);
+        });
+    }
+    finish();
+    return 0;
+}
+
+// MSM_G1: sum b[i] a[i] over G1 or packed-bases points; 64-byte canonical
+// affine result at output.
+uint64_t ResidentState::op_msm_g1(const DvgpuResidentOp& op) {
+    const uint64_t n = op.length;
+    const View sc = view(op.b, op.b_offset, n);
+    expect(sc, KIND_FR);
+    require(n <= DVGPU_MAX_MSM_SIZE && op.output, "invalid resident MSM length or output");
+    const View pts = view(op.a, op.a_offset, n);
+    require(pts.slot->kind != KIND_FR, "resident MSM points must be G1 or packed bases");
+    uint8_t result[64] = {0};
+    start();
+    if (n) {
+        issue([&] {
+            const uint32_t* bases = pts.words();
+            if (pts.slot->kind == KIND_G1) {
+                uint32_t* aff = affine.as<uint32_t>(n * 16);
+                k_g1_out<false><<<grid_for(n, 128), 128, 0, stream()>>>(bases, aff, n);
+                bases = aff;
+            }
+            uint32_t* s = canon_scalars.as<uint32_t>(n * 8);
+            k_fr_out<<<grid_for(n, 128), 128, 0, stream()>>>(sc.words(), s, n);
+            HIP_CHECK_LAUNCH();
+            affine_to_bytes<FqOps>(msm_run<FqOps>(bases, s, n, owner->msm, stream(), owner->msm_config), result);
+        });
+        diag.physical_d2h_bytes += 64;
+    }
+    finish();
+    std::memcpy(op.output, result, 64);
+    stats.msm_g1_points += n;
+    stats.d2h_bytes += 64;
+    return 0;
+}
+
+// MSM_G2: sum b[i] Q_i over n canonical G2 points Q_i given at input (128 B
+// each); 128-byte canonical affine result at output.
+uint64_t ResidentState::op_msm_g2(const DvgpuResidentOp& op) {
+    const uint64_t n = op.length;
+    const View sc = view(op.b, op.b_offset, n);
+    expect(sc, KIND_FR);
+    require(n <= DVGPU_MAX_MSM_SIZE && op.output, "invalid resident MSM length or output");
+    require(!n || op.input, "null resident G2 points");
+    uint8_t result[128] = {0};
+    start();
+    if (n) {
+        uint8_t* pin = pinned.as<uint8_t>(n * 128);
+        host_copy(pin, op.input, n * 128);
+        issue([&] {
+            uint8_t* wire = g2_points.as<uint8_t>(n * 256);
+            uint32_t* mont = reinterpret_cast<uint32_t*>(wire + n * 128);
+            HIP_CHECK(hipMemcpyAsync(wire, pin, n * 128, hipMemcpyHostToDevice, stream()));
+            points_to_mont<Fq2Ops>(wire, mont, n, stream());
+            uint32_t* s = canon_scalars.as<uint32_t>(n * 8);
+            k_fr_out<<<grid_for(n, 128), 128, 0, stream()>>>(sc.words(), s, n);
+            HIP_CHECK_LAUNCH();
+            affine_to_bytes<Fq2Ops>(msm_run<Fq2Ops>(mont, s, n, owner->msm, stream(), owner->msm_config), result);
+        });
+        diag.physical_h2d_bytes += n * 128;
+        diag.physical_d2h_bytes += 128;
+    }
+    finish();
+    std::memcpy(op.output, result, 128);
+    stats.msm_g2_points += n;
+    stats.h2d_bytes += n * 128;
+    stats.d2h_bytes += 128;
+    return 0;
 }
 
 }  // namespace dvgpu
@@ -793,53 +895,32 @@ Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
 
 extern "C" {
 
-// icicle_msm.cu:267-293
+// Reference: dynaverse_gpu.h:99-158 and native/src/icicle_resident.cu (the
+// configure / call / seal / commit / stats ABI).  The quota covers every
+// device byte the owner holds from here on, including inputs retained by
+// earlier calls; the configuration is refused (-1, owner healthy, retry
+// allowed) unless the NTT tables, the journal arena and the transient
+// workspace of empty operations fit.
 int dvgpu_resident_configure(void* backend, uint64_t limit, uint64_t journal, const uint8_t* root, char* error,
                              size_t capacity) {
     Owner* o = as_owner(backend);
     return guarded(o, error, capacity, [&]() -> int {
-        require(root && !o->resident, "resident configuration already set or root absent");
+        require(root != nullptr, "resident configuration needs the 2^20-th root of unity");
+        require(!o->resident, "resident engine already configured for this owner");
         uint32_t rw[8];
         std::memcpy(rw, root, 32);
         require(ntt_root_has_order_2_20(rw), "resident NTT root must have order 2^20");
-        require(limit >= DOMAIN_BYTES + WORKSPACE_MARGIN && journal <= limit - DOMAIN_BYTES - WORKSPACE_MARGIN,
-                "resident quota cannot contain the fixed domain, journal and workspace reserve");
-        auto s = std::make_unique<ResidentState>();
-        s->owner = o;
-        s->tag = next_tag.fetch_add(1);
-        require(s->tag != 0, "resident owner tag exhausted");
-        s->limit = limit;
-        s->journal_capacity = journal;
-        s->buffers.reserve(1024);
-        s->undo.reserve(std::max<uint64_t>(1, journal / 32));
-        s->diag.journal_metadata_capacity_bytes = s->undo.capacity() * sizeof(JournalRange);
-        for (hipEvent_t* e : {&s->started, &s->finished, &s->prof[0], &s->prof[1], &s->prof[2], &s->prof[3]})
-            HIP_CHECK(hipEventCreate(e));
-        // From here on every owner allocation counts against the limit.
-        const uint64_t previous_limit = o->quota.limit;
+        auto s = std::make_unique<ResidentState>(o, limit, journal);
+        const uint64_t fixed = ntt_domain_bytes(0);
+        require(fixed <= UINT64_MAX - journal, "resident journal size overflows 64 bits");
+        s->admit(fixed + journal, 0);
+        const uint64_t previous = o->quota.limit;
         o->quota.limit = limit;
         try {
-            s->domain.set_quota(&o->quota);
-            s->ntt.attach(&o->quota);
-            for (DeviceBuffer* b : {&s->affine_ws, &s->scalar_ws, &s->g2_ws, &s->host_stage_dev}) b->quota = &o->quota;
-            s->quota(journal, 0);
-            std::memcpy(s->domain.pass_stages, o->domain.pass_stages, sizeof(s->domain.pass_stages));
-            ntt_domain_prepare(s->domain, rw, o->stream);
-            s->stats.bytes_allocated = DOMAIN_BYTES;
-            s->diag.resident_allocations = 2;
-            if (journal) {
-                o->quota.charge(journal);
-                const hipError_t e = hipMalloc(&s->journal, journal);
-                if (e != hipSuccess) {
-                    o->quota.release(journal);
-                    s->journal = nullptr;
-                    throw DeviceError(std::string("hipMalloc: ") + hipGetErrorString(e));
-                }
-                s->stats.bytes_allocated += journal;
-                ++s->diag.resident_allocations;
-            }
-        } catch (const InvalidArgument&) {
-            o->quota.limit = previous_limit;
+            s->build(rw);
+        } catch (...) {
+            s.reset();
+            o->quota.limit = previous;
             throw;
         }
         o->resident = s.release();
@@ -850,50 +931,53 @@ int dvgpu_resident_configure(void* backend, uint64_t limit, uint64_t journal, co
 int dvgpu_resident_call(void* backend, const DvgpuResidentOp* op, uint64_t* result, char* error, size_t capacity) {
     Owner* o = as_owner(backend);
     return guarded(o, error, capacity, [&]() -> int {
-        require(op && result && o->resident, "invalid resident call");
-        *result = o->resident->execute(*op);
+        require(op && result, "null resident operation or result");
+        require(o->resident != nullptr, "resident engine not configured");
+        *result = o->resident->call(*op);
         return 0;
     });
 }
 
+// Reference: crates/gpu/src/resident_profile.rs (host and event split of one
+// call).  Profiles the operations that move or transform data.
 int dvgpu_resident_call_profiled(void* backend, const DvgpuResidentOp* op, uint64_t* result,
                                  DvgpuResidentProfile* profile, char* error, size_t capacity) {
     Owner* o = as_owner(backend);
     return guarded(o, error, capacity, [&]() -> int {
-        require(op && result && profile && o->resident, "invalid profiled resident call");
-        require(op->opcode == DVGPU_RESIDENT_WRITE || op->opcode == DVGPU_RESIDENT_READ ||
-                    op->opcode == DVGPU_RESIDENT_TRANSPOSE || op->opcode == DVGPU_RESIDENT_TRANSFORM ||
-                    op->opcode == DVGPU_RESIDENT_TRANSFORM_PACKET,
-                "unsupported profiled resident operation");
+        require(op && result && profile, "null resident operation, result or profile");
+        require(o->resident != nullptr, "resident engine not configured");
+        const uint32_t c = op->opcode;
+        require(c == DVGPU_RESIDENT_WRITE || c == DVGPU_RESIDENT_READ || c == DVGPU_RESIDENT_TRANSPOSE ||
+                    c == DVGPU_RESIDENT_TRANSFORM || c == DVGPU_RESIDENT_TRANSFORM_PACKET,
+                "operation cannot be profiled (write, read, transpose, transform or packet only)");
         ResidentState* s = o->resident;
         *profile = {};
-        struct Guard {
-            ResidentState* s;
-            ~Guard() {
-                s->profile = nullptr;
-                s->op_active = false;
-            }
-        } guard{s};
-        const uint64_t started = now_ns();
-        s->profile = profile;
+        const uint64_t t0 = now_ns();
         const DvgpuResidentDiagnostics before = s->diag;
-        *result = s->execute(*op);
+        s->profile = profile;
+        try {
+            *result = s->call(*op);
+        } catch (...) {
+            s->profile = nullptr;
+            throw;
+        }
+        s->profile = nullptr;
         profile->h2d_bytes = s->diag.physical_h2d_bytes - before.physical_h2d_bytes;
         profile->d2h_bytes = s->diag.physical_d2h_bytes - before.physical_d2h_bytes;
         profile->d2d_bytes = s->diag.explicit_d2d_bytes - before.explicit_d2d_bytes;
-        profile->native_wall_ns = now_ns() - started;
+        profile->native_wall_ns = now_ns() - t0;
         return 0;
     });
 }
 
 void dvgpu_resident_seal(void* backend) {
     Owner* o = as_owner(backend);
-    if (o && o->resident) o->resident->publish(true);
+    if (o && o->resident) o->resident->seal();
 }
 
 void dvgpu_resident_commit(void* backend) {
     Owner* o = as_owner(backend);
-    if (o && o->resident) o->resident->publish(false);
+    if (o && o->resident) o->resident->commit();
 }
 
 int dvgpu_resident_stats(void* backend, DvgpuResidentStats* out) {
