@@ -98,17 +98,14 @@ int msm_entry(Owner* o, const uint8_t* points, const uint8_t* scalars, size_t le
         uint32_t* dsc = o->stage_b.as<uint32_t>(length * 8);
         uint32_t* mont = o->msm.points.as<uint32_t>(length * PointLayout<F>::AFFINE_WORDS);
         const uint64_t t1 = now_ns();
-        hipEvent_t ev[MSM_PHASE_EVENTS + 2] = {};
-        if (timeline) {
-            for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-            HIP_CHECK(hipEventRecord(ev[MSM_PHASE_EVENTS], o->stream));
-        }
+        ScopedEvents<MSM_PHASE_EVENTS + 2> ev(timeline != nullptr);
+        if (timeline) HIP_CHECK(hipEventRecord(ev[MSM_PHASE_EVENTS], o->stream));
         HIP_CHECK(hipMemcpyAsync(dpts, points, length * PB, hipMemcpyHostToDevice, o->stream));
         HIP_CHECK(hipMemcpyAsync(dsc, scalars, length * 32, hipMemcpyHostToDevice, o->stream));
         points_to_mont<F>(dpts, mont, length, o->stream);
         if (timeline) {
             HIP_CHECK(hipEventRecord(ev[MSM_PHASE_EVENTS + 1], o->stream));
-            o->msm.phase_events = ev;
+            o->msm.phase_events = ev.e;
         }
         Affine<F> r;
         MsmPlan plan;
@@ -125,8 +122,6 @@ int msm_entry(Owner* o, const uint8_t* points, const uint8_t* scalars, size_t le
             HIP_CHECK(hipStreamSynchronize(o->stream));
         } catch (...) {
             o->msm.phase_events = nullptr;
-            if (timeline)
-                for (auto& e : ev) (void)hipEventDestroy(e);
             throw;
         }
         const uint64_t t2 = now_ns();
@@ -144,11 +139,7 @@ int msm_entry(Owner* o, const uint8_t* points, const uint8_t* scalars, size_t le
             // Phases: 0 upload+convert, 1 digits, 2 scan, 3 kernels (scatter,
             // tasks, accumulate, reduce, tree).  Kernel slots follow the
             // device phases in order.
-            float ms = 0;
-            auto span = [&](int a, int b) {
-                HIP_CHECK(hipEventElapsedTime(&ms, ev[a], ev[b]));
-                return uint64_t(double(ms) * 1e6);
-            };
+            auto span = [&](int a, int b) { return ev.ns(a, b); };
             timeline->c = uint64_t(plan.c);
             timeline->bitsize = 254;
             timeline->windows = uint64_t(plan.windows);
@@ -171,7 +162,6 @@ int msm_entry(Owner* o, const uint8_t* points, const uint8_t* scalars, size_t le
             timeline->bytes[0] = uint64_t(length) * (PB + 32);
             timeline->instrumentation_inclusive_ns = 0;
             timeline->instrumentation_total_ns = 0;
-            for (auto& e : ev) (void)hipEventDestroy(e);
         }
         return 0;
     });
@@ -201,7 +191,7 @@ int dvgpu_backend_create_with_large_bucket_factor(uint32_t factor, void** output
         auto owner = std::make_unique<Owner>();
         owner->large_bucket_factor = factor;
         owner->identity = std::string(props.name[0] ? props.name : "AMD Instinct") + "/" + props.gcnArchName;
-        HIP_CHECK(hipStreamCreateWithFlags(&owner->stream, hipStreamNonBlocking));
+        create_stream(owner->stream);
         *output = owner.release();
         return 0;
     } catch (const std::exception& e) {

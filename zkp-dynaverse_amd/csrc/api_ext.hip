@@ -15,17 +15,7 @@ __global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst,
         dst[i] = src[i];
 }
 
-struct Events {
-    hipEvent_t e[MSM_PHASE_EVENTS] = {};
-    explicit Events(bool on) {
-        if (on)
-            for (auto& x : e) HIP_CHECK(hipEventCreate(&x));
-    }
-    ~Events() {
-        for (auto& x : e)
-            if (x) (void)hipEventDestroy(x);
-    }
-};
+using Events = ScopedEvents<MSM_PHASE_EVENTS>;
 
 template <class F>
 int msm_device(Owner* o, const void* points, const void* scalars, size_t n, int window_bits,

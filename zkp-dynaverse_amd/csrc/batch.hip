@@ -263,11 +263,8 @@ int dvgpu_ext_msm_batch_device(void* owner, const void* points, const void* poin
         }
         msm_batch_prepare(plan, xw, o->msm, o->stream);
         uint32_t* win = o->msm.windows.as<uint32_t>(size_t(plan.total_windows) * xw);
-        hipEvent_t ev[MSM_PHASE_EVENTS] = {};
-        if (phase_ms) {
-            for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-            o->msm.phase_events = ev;
-        }
+        ScopedEvents<MSM_PHASE_EVENTS> ev(phase_ms != nullptr);
+        if (phase_ms) o->msm.phase_events = ev.e;
         try {
             if (g2)
                 msm_batch_enqueue<Fq2Ops>(static_cast<const uint32_t*>(points), static_cast<const uint32_t*>(point_index),
@@ -277,8 +274,6 @@ int dvgpu_ext_msm_batch_device(void* owner, const void* points, const void* poin
                                          static_cast<const uint32_t*>(scalars), plan, o->msm, o->stream, win);
         } catch (...) {
             o->msm.phase_events = nullptr;
-            for (auto& e : ev)
-                if (e) (void)hipEventDestroy(e);
             throw;
         }
         o->msm.phase_events = nullptr;
@@ -297,10 +292,8 @@ int dvgpu_ext_msm_batch_device(void* owner, const void* points, const void* poin
                                      o->stream));
         }
         HIP_CHECK(hipStreamSynchronize(o->stream));
-        if (phase_ms) {
-            for (int k = 0; k + 1 < MSM_PHASE_EVENTS; ++k) HIP_CHECK(hipEventElapsedTime(&phase_ms[k], ev[k], ev[k + 1]));
-            for (auto& e : ev) (void)hipEventDestroy(e);
-        }
+        if (phase_ms)
+            for (int k = 0; k + 1 < MSM_PHASE_EVENTS; ++k) phase_ms[k] = ev.ms(k, k + 1);
         if (on_device) std::memcpy(results, host, count * pb);
         else if (g2) publish<Fq2Ops>(host, plan, map, count, results);
         else publish<FqOps>(host, plan, map, count, results);
@@ -380,20 +373,12 @@ int dvgpu_ext_ntt_batch_device(void* owner, const DvgpuNttRows* rows, size_t cou
         for (size_t i = 0; i < count; ++i) entries.push_back(ntt_entry(rows[i]));
         ntt_domain_prepare(o->domain, rw, o->stream);
         ntt_batch_prepare(o->domain, entries, o->ntt_batch, o->stream);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (kernel_ms) {
-            HIP_CHECK(hipEventCreate(&e0));
-            HIP_CHECK(hipEventCreate(&e1));
-            HIP_CHECK(hipEventRecord(e0, o->stream));
-        }
+        ScopedEvents<2> ev(kernel_ms != nullptr);
+        if (kernel_ms) HIP_CHECK(hipEventRecord(ev[0], o->stream));
         ntt_batch_enqueue(o->domain, o->ntt_batch, o->stream);
-        if (kernel_ms) HIP_CHECK(hipEventRecord(e1, o->stream));
+        if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], o->stream));
         HIP_CHECK(hipStreamSynchronize(o->stream));
-        if (kernel_ms) {
-            HIP_CHECK(hipEventElapsedTime(kernel_ms, e0, e1));
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-        }
+        if (kernel_ms) *kernel_ms = ev.ms(0, 1);
         return 0;
     });
 }
@@ -483,11 +468,11 @@ int dvgpu_ext_batch_create(void* owner, const DvgpuBatchSpec* spec, int use_grap
             b->report_dev.as<DvgpuG2ValidationReport>(1);
             b->report_host.as<DvgpuG2ValidationReport>(1);
         }
-        HIP_CHECK(hipEventCreate(&b->e0));
-        HIP_CHECK(hipEventCreate(&b->e1));
-        HIP_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
-        HIP_CHECK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+        create_event(b->e0);
+        create_event(b->e1);
+        create_event(b->fork, hipEventDisableTiming);
+        create_event(b->join, hipEventDisableTiming);
+        create_stream(b->side);
         // Warm-up run: sizes every remaining workspace before any capture.
         b->enqueue(st);
         HIP_CHECK(hipStreamSynchronize(st));

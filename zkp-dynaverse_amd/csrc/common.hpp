@@ -112,6 +112,57 @@ struct HostBuffer {
     T* as(size_t count) { return static_cast<T*>(ensure(count * sizeof(T))); }
 };
 
+// Create an event / a non-blocking stream; `out` is written only on success,
+// so an owner's destructor never releases a handle that was not made.
+inline void create_event(hipEvent_t& out, unsigned flags = hipEventDefault) {
+    hipEvent_t e = nullptr;
+    hip_check(hipEventCreateWithFlags(&e, flags), "hipEventCreateWithFlags");
+    out = e;
+}
+inline void create_stream(hipStream_t& out) {
+    hipStream_t s = nullptr;
+    hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    out = s;
+}
+
+// Events owned by one call: created together (none when `on` is false) and
+// destroyed on every exit path, a failed creation included.
+template <int N>
+struct ScopedEvents {
+    hipEvent_t e[N] = {};
+    explicit ScopedEvents(bool on = true) {
+        if (!on) return;
+        for (auto& x : e) {
+            hipEvent_t made = nullptr;
+            const hipError_t st = hipEventCreate(&made);
+            if (st != hipSuccess) {
+                destroy();
+                hip_check(st, "hipEventCreate");
+            }
+            x = made;
+        }
+    }
+    ~ScopedEvents() { destroy(); }
+    ScopedEvents(const ScopedEvents&) = delete;
+    ScopedEvents& operator=(const ScopedEvents&) = delete;
+    hipEvent_t operator[](int i) const { return e[i]; }
+    float ms(int a, int b) const {
+        float v = 0;
+        hip_check(hipEventElapsedTime(&v, e[a], e[b]), "hipEventElapsedTime");
+        return v;
+    }
+    uint64_t ns(int a, int b) const { return uint64_t(double(ms(a, b)) * 1e6); }
+
+  private:
+    void destroy() noexcept {
+        for (auto& x : e)
+            if (x) {
+                (void)hipEventDestroy(x);
+                x = nullptr;
+            }
+    }
+};
+
 inline unsigned grid_for(size_t n, unsigned block) { return unsigned((n + block - 1) / block); }
 
 }  // namespace dvgpu

@@ -243,21 +243,7 @@ SetupState& setup_of(Owner* o) {
     return *o->setup;
 }
 
-struct EventSet {
-    hipEvent_t e[8] = {};
-    EventSet() {
-        for (auto& x : e) HIP_CHECK(hipEventCreate(&x));
-    }
-    ~EventSet() {
-        for (auto& x : e)
-            if (x) (void)hipEventDestroy(x);
-    }
-    uint64_t ns(int a, int b) {
-        float ms = 0;
-        HIP_CHECK(hipEventElapsedTime(&ms, e[a], e[b]));
-        return uint64_t(double(ms) * 1e6);
-    }
-};
+using EventSet = ScopedEvents<8>;
 
 // [s_i] G into host `output` (canonical affine); events 0..5 are the
 // table/H2D/generate/normalize/D2H boundaries of DvgpuSetupStats.device_ns.

@@ -298,11 +298,8 @@ void precompute(KzgSrs& z, const uint8_t* coefficients, uint64_t coeff_count, ui
     const int LM = log2_pow2(M), Ln = log2_pow2(domain_size);
     Owner* o = z.owner;
     hipStream_t st = o->stream;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    if (timings) {
-        for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-        HIP_CHECK(hipEventRecord(ev[0], st));
-    }
+    ScopedEvents<4> ev(timings != nullptr);
+    if (timings) HIP_CHECK(hipEventRecord(ev[0], st));
     auto* cw = z.coeffs.as<uint32_t>(rows * coeff_count * 8);
     HIP_CHECK(hipMemcpyAsync(cw, coefficients, rows * coeff_count * 32, hipMemcpyHostToDevice, st));
     auto* sw = z.scal.as<uint32_t>((rows << LM) * 8);
@@ -343,14 +340,9 @@ void precompute(KzgSrs& z, const uint8_t* coefficients, uint64_t coeff_count, ui
     HIP_CHECK(hipMemcpyAsync(openings, ow, (rows << Ln) * 64, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     if (timings) {
-        float a = 0, b = 0, c = 0;
-        HIP_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
-        HIP_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));
-        HIP_CHECK(hipEventElapsedTime(&c, ev[2], ev[3]));
-        timings->scalar_and_basis_ms = a;
-        timings->convolution_ms = b;
-        timings->domain_transform_ms = c;
-        for (auto& e : ev) (void)hipEventDestroy(e);
+        timings->scalar_and_basis_ms = ev.ms(0, 1);
+        timings->convolution_ms = ev.ms(1, 2);
+        timings->domain_transform_ms = ev.ms(2, 3);
     }
 }
 

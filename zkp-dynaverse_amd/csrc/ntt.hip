@@ -338,7 +338,7 @@ void ntt_batch_prepare(NttDomain& d, const std::vector<NttBatchEntry>& entries, 
     std::memcpy(host, descs.data(), desc_bytes);
     auto* dev = plan.descs.as<NttBatchDesc>(descs.size());
     HIP_CHECK(hipMemcpyAsync(dev, host, desc_bytes, hipMemcpyHostToDevice, stream));
-    if (!plan.uploaded) HIP_CHECK(hipEventCreateWithFlags(&plan.uploaded, hipEventDisableTiming));
+    if (!plan.uploaded) create_event(plan.uploaded, hipEventDisableTiming);
     HIP_CHECK(hipEventRecord(plan.uploaded, stream));
     plan.upload_pending = true;
 }

@@ -356,11 +356,8 @@ extern "C" int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const
         auto* base = static_cast<uint8_t*>(o->stage_a.ensure(b1 + b2 + br + bg));
         HIP_CHECK(hipMemcpyAsync(base, g1, b1, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(base + b1, g2, b2, hipMemcpyHostToDevice, st));
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        if (kernel_ms) {
-            for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-            HIP_CHECK(hipEventRecord(ev[0], st));
-        }
+        ScopedEvents<2> ev(kernel_ms != nullptr);
+        if (kernel_ms) HIP_CHECK(hipEventRecord(ev[0], st));
         k_pairing_check<<<grid_for(count, 64), 64, 0, st>>>(
             reinterpret_cast<const uint32_t*>(base), reinterpret_cast<const uint32_t*>(base + b1), uint32_t(terms),
             count, base + b1 + b2, gt ? reinterpret_cast<uint32_t*>(base + b1 + b2 + br) : nullptr);
@@ -369,10 +366,7 @@ extern "C" int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const
         HIP_CHECK(hipMemcpyAsync(results, base + b1 + b2, count, hipMemcpyDeviceToHost, st));
         if (gt) HIP_CHECK(hipMemcpyAsync(gt, base + b1 + b2 + br, bg, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
-        if (kernel_ms) {
-            HIP_CHECK(hipEventElapsedTime(kernel_ms, ev[0], ev[1]));
-            for (auto& e : ev) (void)hipEventDestroy(e);
-        }
+        if (kernel_ms) *kernel_ms = ev.ms(0, 1);
         const void* bad = std::memchr(results, 2, count);
         if (bad) {
             const size_t i = static_cast<const uint8_t*>(bad) - results;
@@ -419,11 +413,8 @@ extern "C" int dvgpu_bn254_pairing_equations(void* backend, const uint8_t* g1, c
         HIP_CHECK(hipMemcpyAsync(base + b1, g2, b2, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(base + b1 + b2, offsets, (count + 1) * 4, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(base + b1 + b2 + bo, order.data(), count * 4, hipMemcpyHostToDevice, st));
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        if (kernel_ms) {
-            for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-            HIP_CHECK(hipEventRecord(ev[0], st));
-        }
+        ScopedEvents<2> ev(kernel_ms != nullptr);
+        if (kernel_ms) HIP_CHECK(hipEventRecord(ev[0], st));
         k_pairing_equations<<<grid_for(count, 64), 64, 0, st>>>(
             reinterpret_cast<const uint32_t*>(base), reinterpret_cast<const uint32_t*>(base + b1),
             reinterpret_cast<const uint32_t*>(base + b1 + b2), reinterpret_cast<const uint32_t*>(base + b1 + b2 + bo),
@@ -432,10 +423,7 @@ extern "C" int dvgpu_bn254_pairing_equations(void* backend, const uint8_t* g1, c
         if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
         HIP_CHECK(hipMemcpyAsync(results, base + b1 + b2 + bo + bq, count, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
-        if (kernel_ms) {
-            HIP_CHECK(hipEventElapsedTime(kernel_ms, ev[0], ev[1]));
-            for (auto& e : ev) (void)hipEventDestroy(e);
-        }
+        if (kernel_ms) *kernel_ms = ev.ms(0, 1);
         const void* bad = std::memchr(results, 2, count);
         if (bad) {
             const size_t i = static_cast<const uint8_t*>(bad) - results;
@@ -484,11 +472,8 @@ extern "C" int dvgpu_bn254_opening_cache_check(void* backend, const uint8_t* com
         HIP_CHECK(hipMemcpyAsync(dv, values, bv, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(dc, caches, bc, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(dm, cm.data(), bm, hipMemcpyHostToDevice, st));
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        if (kernel_ms) {
-            for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-            HIP_CHECK(hipEventRecord(ev[0], st));
-        }
+        ScopedEvents<2> ev(kernel_ms != nullptr);
+        if (kernel_ms) HIP_CHECK(hipEventRecord(ev[0], st));
         k_cache_divisors<<<grid_for(n, 64), 64, 0, st>>>(reinterpret_cast<const uint32_t*>(dz), n, h0.x, h0.y, h1.x,
                                                           h1.y, reinterpret_cast<uint32_t*>(dd), df);
         HIP_CHECK_LAUNCH();
@@ -502,10 +487,7 @@ extern "C" int dvgpu_bn254_opening_cache_check(void* backend, const uint8_t* com
         if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
         HIP_CHECK(hipMemcpyAsync(results, dr, total, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
-        if (kernel_ms) {
-            HIP_CHECK(hipEventElapsedTime(kernel_ms, ev[0], ev[1]));
-            for (auto& e : ev) (void)hipEventDestroy(e);
-        }
+        if (kernel_ms) *kernel_ms = ev.ms(0, 1);
         const void* bad = std::memchr(results, 2, total);
         if (bad) {
             const size_t t = static_cast<const uint8_t*>(bad) - results;
@@ -540,11 +522,8 @@ extern "C" int dvgpu_bn254_kzg_verify_batch(void* backend, const uint8_t* g1_0, 
         HIP_CHECK(hipMemcpyAsync(dv, values, bv, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(dw, openings, bw, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(dd, divisors, bd, hipMemcpyHostToDevice, st));
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        if (kernel_ms) {
-            for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-            HIP_CHECK(hipEventRecord(ev[0], st));
-        }
+        ScopedEvents<2> ev(kernel_ms != nullptr);
+        if (kernel_ms) HIP_CHECK(hipEventRecord(ev[0], st));
         k_prepare_lines<<<1, 64, 0, st>>>(h0.x, h0.y, reinterpret_cast<uint32_t*>(dl));
         HIP_CHECK_LAUNCH();
         k_kzg_verify<<<grid_for(count, 64), 64, 0, st>>>(
@@ -555,10 +534,7 @@ extern "C" int dvgpu_bn254_kzg_verify_batch(void* backend, const uint8_t* g1_0, 
         if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
         HIP_CHECK(hipMemcpyAsync(results, dr, count, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
-        if (kernel_ms) {
-            HIP_CHECK(hipEventElapsedTime(kernel_ms, ev[0], ev[1]));
-            for (auto& e : ev) (void)hipEventDestroy(e);
-        }
+        if (kernel_ms) *kernel_ms = ev.ms(0, 1);
         const void* bad = std::memchr(results, 2, count);
         if (bad) {
             const size_t t = static_cast<const uint8_t*>(bad) - results;

@@ -272,10 +272,6 @@ struct ResidentState {
         domain.set_quota(&o->quota);
         ntt.attach(&o->quota);
         for (DeviceBuffer* b : {&affine, &canon_scalars, &g2_points, &boundary}) b->quota = &o->quota;
-        for (Span* s : {&whole, &saves_span, &work_span}) {
-            HIP_CHECK(hipEventCreate(&s->begin));
-            HIP_CHECK(hipEventCreate(&s->end));
-        }
         slots.reserve(MAX_BUFFERS);
     }
 
@@ -350,8 +346,13 @@ struct ResidentState {
         return p;
     }
 
-    // The NTT tables and the journal arena, after the configuration is admitted.
+    // Events, the NTT tables and the journal arena, once the configuration is
+    // admitted (the destructor releases whatever was made before a failure).
     void build(const uint32_t (&root)[8]) {
+        for (Span* s : {&whole, &saves_span, &work_span}) {
+            create_event(s->begin);
+            create_event(s->end);
+        }
         std::memcpy(domain.pass_stages, owner->domain.pass_stages, sizeof(domain.pass_stages));
         ntt_domain_prepare(domain, root, stream());
         diag.resident_allocations += 4;  // power and stage tables, both directions

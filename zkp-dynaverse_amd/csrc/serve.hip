@@ -349,12 +349,8 @@ int dvgpu_ext_update_batch(void* owner, void* points, size_t point_count, int wi
         const uint32_t* d_sc = dev + off_sc;
         auto* pts = static_cast<uint32_t*>(points);
 
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (device_ms) {
-            HIP_CHECK(hipEventCreate(&e0));
-            HIP_CHECK(hipEventCreate(&e1));
-            HIP_CHECK(hipEventRecord(e0, st));
-        }
+        ScopedEvents<2> ev(device_ms != nullptr);
+        if (device_ms) HIP_CHECK(hipEventRecord(ev[0], st));
         if (entries) {
             k_scatter_add<<<grid_for(entries, 256), 256, 0, st>>>(d_desc, uint32_t(m), uint32_t(entries), d_loc, d_sc,
                                                                   frw);
@@ -369,18 +365,14 @@ int dvgpu_ext_update_batch(void* owner, void* points, size_t point_count, int wi
                                                            xyzz_out);
         HIP_CHECK_LAUNCH();
         ntt_batch_enqueue(o->domain, o->ntt_batch, st);
-        if (device_ms) HIP_CHECK(hipEventRecord(e1, st));
+        if (device_ms) HIP_CHECK(hipEventRecord(ev[1], st));
         uint8_t* host = o->msm.host_windows.as<uint8_t>(2 * m * 128 + m * 32 + 4);
         HIP_CHECK(hipMemcpyAsync(host, xyzz_out, 2 * m * 128, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipMemcpyAsync(host + 2 * m * 128, yout, m * 32, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipMemcpyAsync(host + 2 * m * 128 + m * 32, static_cast<uint32_t*>(o->msm.total.ptr) + 3, 4,
                                  hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
-        if (device_ms) {
-            HIP_CHECK(hipEventElapsedTime(device_ms, e0, e1));
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-        }
+        if (device_ms) *device_ms = ev.ms(0, 1);
         uint32_t flag = 0;
         std::memcpy(&flag, host + 2 * m * 128 + m * 32, 4);
         require(flag == 0, "MSM point index out of range");  // not reachable after the range checks
