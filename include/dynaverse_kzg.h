@@ -56,8 +56,11 @@ int dvgpu_kzg_srs_basis(void* backend, void* srs, uint64_t* degree, uint64_t* bu
  * coefficient) gives identities without touching the basis.  The
  * convolution length next_pow2(2 deg - 1) is limited to 2^20 points, and
  * poly_count times the convolution (and the domain) length to 2^24.
- * Calls on one SRS may come from several threads; creating or destroying SRS
- * objects must not overlap other calls on the same backend.
+ * Calls on one SRS may come from several threads (they run one at a time).
+ * If one of them fails on the device and retires the backend, retirement
+ * waits for the calls already in progress, and calls still waiting for the
+ * SRS return -4.  Destroying an SRS waits for calls in progress on it;
+ * creating SRS objects must not overlap other calls on the same backend.
  * timings may be NULL. */
 int dvgpu_kzg_precompute_single_openings(void* backend, void* srs, const uint8_t* coefficients,
                                          size_t coeff_count, size_t poly_count, size_t domain_size,

@@ -5,7 +5,8 @@ A child process runs with tests/native/libhip_fault_inject.so preloaded: the
 library makes the k-th hipMalloc, hipEventCreate* or hipStreamCreate* after
 arming fail without calling the runtime (no device work is involved) and
 counts live device allocations, pinned allocations, events and streams.  In
-each case the failing call returns -2, the next call on the same owner -4
+each case (one caller, or four threads on one KZG SRS) the failing call
+returns -2, the next call on the same owner -4
 ("retired"), a healthy owner created earlier and a fresh owner created
 afterwards both compute correct MSMs / NTTs, and once every owner is closed
 no allocation, event or stream is left alive.
@@ -90,6 +91,32 @@ def _child(case):
         report["first"] = status(lambda: DeviceBatch(a, ntt=[NttRows(d_in, d_out, 256, 1)], use_graph=False))
         report["second"] = status(lambda: a.msm_encoded(points[:64], ones[:32], 1))
         a.close()
+    elif case == "kzg_threads":
+        import threading
+        from dvgpu.kzg import KzgSrs
+        a = HipBackend()
+        z = KzgSrs(a, o.kzg_srs(7, 256))
+        small = [3, 1, 4, 1, 5, 9, 2, 6]
+        report["warm_ok"] = z.precompute_single_openings(small, 8) == o.kzg_openings(o.kzg_srs(7, 256), small, 8)
+        coeffs = [(i * i + 7) % o.R for i in range(200)]
+        results = []
+        gate = threading.Barrier(4)
+
+        def worker():
+            gate.wait()
+            results.append(status(lambda: z.precompute_single_openings(coeffs, 256)))
+
+        inj.dvgpu_inject_arm(MEMORY, 1)             # the first new buffer of the larger shape
+        step("four threads on one SRS, the first device allocation failing")
+        threads = [threading.Thread(target=worker) for _ in range(4)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        report["threads"] = sorted(r[0] for r in results)
+        report["second"] = status(lambda: z.precompute_single_openings(small, 8))
+        z.close()
+        a.close()
     report["fired"] = inj.dvgpu_inject_fired()
     step("fresh owner")
     fresh = HipBackend()
@@ -144,6 +171,20 @@ def test_event_failure_retires_owner():
     _retired(rep, "first", "second")
     _retired(rep, "resident_first", "resident_second")
     _recovered(rep, 2)
+
+
+@pytest.mark.gpu
+def test_kzg_failure_on_one_thread_retires_safely():
+    """Several threads call one SRS; the first call's device allocation
+    fails.  It returns -2, retirement waits for the calls in
+    progress, the threads still waiting for the SRS return -4, and nothing
+    runs on released state (the process completes and balances)."""
+    rep = _run("kzg_threads")
+    assert rep["warm_ok"], rep
+    statuses = rep["threads"]
+    assert statuses.count(-2) == 1 and all(s in (0, -2, -4) for s in statuses), rep
+    assert rep["second"][0] == -4 and "retired" in rep["second"][1], rep
+    _recovered(rep, 1)
 
 
 @pytest.mark.gpu

@@ -61,8 +61,9 @@ void Owner::release_device() noexcept {
 }
 
 void Owner::retire() noexcept {
-    if (retired) return;
-    retired = true;
+    if (retired.exchange(true)) return;
+    // Calls registered on other threads (KZG) finish before anything goes.
+    gate.close_and_drain();
     if (stream) (void)hipStreamSynchronize(stream);
     release_device();
     if (stream) (void)hipStreamDestroy(stream);
@@ -72,6 +73,7 @@ void Owner::retire() noexcept {
 
 Owner::~Owner() {
     if (!retired) {
+        gate.close_and_drain();
         if (stream) (void)hipStreamSynchronize(stream);
         release_device();
         if (stream) (void)hipStreamDestroy(stream);

@@ -16,6 +16,12 @@ struct DeviceError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 
+// The owner was retired (by another thread's failure) while this call waited:
+// maps to status -4 without retiring again.
+struct OwnerRetired : std::exception {
+    const char* what() const noexcept override { return "backend retired after a device error"; }
+};
+
 inline void require(bool ok, const char* message) {
     if (!ok) throw InvalidArgument(message);
 }

@@ -36,7 +36,8 @@ namespace dvgpu {
 
 struct KzgSrs {
     Owner* owner = nullptr;
-    std::mutex mu;
+    std::mutex mu;               // one call on this SRS at a time
+    int users = 0;               // calls registered on it (under the owner's gate lock)
     uint64_t count = 0;          // SRS powers
     DeviceBuffer powers;         // canonical affine, 16 words each
     NttDomain domain;            // Fr twiddles omega_(2^20)^x, x < 2^19
@@ -259,11 +260,49 @@ bool fr_canonical(const uint8_t* b) {
     return is_canonical<FrParams>(w);
 }
 
-KzgSrs* find_srs(Owner* o, void* srs) {
+KzgSrs* registered(Owner* o, void* srs) {
     auto it = std::find(o->kzg.begin(), o->kzg.end(), static_cast<KzgSrs*>(srs));
-    require(it != o->kzg.end(), "unknown KZG SRS handle for this backend");
-    return *it;
+    return it == o->kzg.end() ? nullptr : *it;
 }
+
+// One KZG call on one SRS.  It registers with the owner's gate (a retirement
+// on another thread then waits for it to leave) as a user of its SRS
+// (destroying the SRS waits for its users), and holds the SRS's lock.  A
+// call that gets the lock only after another thread retired the owner
+// leaves with status -4 without touching the device.
+struct KzgRegistration {
+    Owner* o;
+    KzgSrs* z = nullptr;
+    bool entered = false;
+    KzgRegistration(Owner* owner, void* handle) : o(owner) {
+        entered = o->gate.enter([&] {
+            z = registered(o, handle);
+            if (z) ++z->users;
+        });
+    }
+    ~KzgRegistration() {
+        if (entered) o->gate.leave([&] {
+            if (z) --z->users;
+        });
+    }
+    KzgRegistration(const KzgRegistration&) = delete;
+    KzgRegistration& operator=(const KzgRegistration&) = delete;
+};
+
+class KzgCall {
+  public:
+    KzgCall(Owner* o, void* handle) : reg_(o, handle) {
+        if (!reg_.entered) throw OwnerRetired();
+        require(reg_.z != nullptr, "unknown KZG SRS handle for this backend");
+        lock_ = std::unique_lock<std::mutex>(reg_.z->mu);
+        if (o->retired) throw OwnerRetired();
+    }
+    KzgSrs& srs() const { return *reg_.z; }
+
+  private:
+    KzgRegistration reg_;  // declared first: released last
+    std::unique_lock<std::mutex> lock_;
+};
 
 void precompute(KzgSrs& z, const uint8_t* coefficients, uint64_t coeff_count, uint64_t rows, uint64_t domain_size,
                 const uint8_t* offset, uint8_t* openings, DvgpuKzgTimings* timings) {
@@ -383,7 +422,7 @@ int dvgpu_kzg_srs_create(void* backend, const uint8_t* g1_powers, size_t count, 
         }
         ntt_domain_prepare(z->domain, root, o->stream);
         HIP_CHECK(hipStreamSynchronize(o->stream));
-        o->kzg.push_back(z.get());
+        o->gate.locked([&] { o->kzg.push_back(z.get()); });
         *srs = z.release();
         return 0;
     });
@@ -392,13 +431,15 @@ int dvgpu_kzg_srs_create(void* backend, const uint8_t* g1_powers, size_t count, 
 int dvgpu_kzg_srs_destroy(void* backend, void* srs) {
     Owner* o = as_owner(backend);
     if (!o) return -1;
-    auto it = std::find(o->kzg.begin(), o->kzg.end(), static_cast<KzgSrs*>(srs));
-    if (it == o->kzg.end()) return -1;
-    KzgSrs* z = *it;
-    {
-        std::lock_guard<std::mutex> lock(z->mu);
+    KzgSrs* z = nullptr;
+    o->gate.locked([&] {
+        auto it = std::find(o->kzg.begin(), o->kzg.end(), static_cast<KzgSrs*>(srs));
+        if (it == o->kzg.end()) return;
+        z = *it;
         o->kzg.erase(it);
-    }
+    });
+    if (!z) return -1;
+    o->gate.wait([&] { return z->users == 0; });  // calls already on it finish first
     if (o->stream) (void)hipStreamSynchronize(o->stream);
     kzg_destroy(z);
     return 0;
@@ -407,12 +448,14 @@ int dvgpu_kzg_srs_destroy(void* backend, void* srs) {
 int dvgpu_kzg_srs_basis(void* backend, void* srs, uint64_t* degree, uint64_t* builds) {
     Owner* o = as_owner(backend);
     if (!o || o->retired) return -1;
-    auto it = std::find(o->kzg.begin(), o->kzg.end(), static_cast<KzgSrs*>(srs));
-    if (it == o->kzg.end()) return -1;
-    std::lock_guard<std::mutex> lock((*it)->mu);
-    if (degree) *degree = (*it)->basis_degree;
-    if (builds) *builds = (*it)->basis_builds;
-    return 0;
+    try {
+        KzgCall call(o, srs);
+        if (degree) *degree = call.srs().basis_degree;
+        if (builds) *builds = call.srs().basis_builds;
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
 }
 
 int dvgpu_kzg_precompute_single_openings(void* backend, void* srs, const uint8_t* coefficients, size_t coeff_count,
@@ -420,9 +463,8 @@ int dvgpu_kzg_precompute_single_openings(void* backend, void* srs, const uint8_t
                                          uint8_t* openings, DvgpuKzgTimings* timings, char* error, size_t capacity) {
     Owner* o = as_owner(backend);
     return guarded(o, error, capacity, [&]() -> int {
-        KzgSrs* z = find_srs(o, srs);
-        std::lock_guard<std::mutex> lock(z->mu);
-        precompute(*z, coefficients, coeff_count, poly_count, domain_size, coset_offset, openings, timings);
+        KzgCall call(o, srs);
+        precompute(call.srs(), coefficients, coeff_count, poly_count, domain_size, coset_offset, openings, timings);
         return 0;
     });
 }

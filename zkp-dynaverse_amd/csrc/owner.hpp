@@ -7,6 +7,7 @@
 // owner (all its device memory is released and later calls return -4);
 // invalid arguments never do.
 #pragma once
+#include <atomic>
 #include <chrono>
 #include <memory>
 #include <mutex>
@@ -15,6 +16,7 @@
 
 #include "../../include/dynaverse_gpu.h"
 #include "../../include/dynaverse_hip_ext.h"
+#include "call_gate.hpp"
 #include "common.hpp"
 #include "msm.hpp"
 #include "ntt.hpp"
@@ -29,7 +31,10 @@ struct PolyTree;       // poly.hip
 
 struct Owner {
     hipStream_t stream = nullptr;
-    bool retired = false;
+    std::atomic<bool> retired{false};
+    // KZG calls (several threads per SRS allowed) register here; retire()
+    // closes the gate and waits for them before releasing anything.
+    CallGate gate;
     uint32_t large_bucket_factor = 0;
     std::string identity;
     Quota quota;                // limit 0 until the resident engine is configured
@@ -82,6 +87,8 @@ int guarded(Owner* owner, char* error, size_t capacity, Body&& body) {
         return fail(-4, "backend retired after a device error; create a new backend", error, capacity);
     try {
         return body();
+    } catch (const OwnerRetired&) {
+        return fail(-4, "backend retired after a device error; create a new backend", error, capacity);
     } catch (const InvalidArgument& e) {
         return fail(-1, e.what(), error, capacity);
     } catch (const std::exception& e) {
