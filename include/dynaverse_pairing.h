@@ -18,9 +18,12 @@
  * y.c0 || y.c1, little-endian canonical, all-zero = identity (the pair then
  * contributes one).  Points must be canonical and lie on E (G1) and on the
  * twist E' (G2), checked on the device; otherwise the call fails with -1 and
- * zeroes results and gt.  G2 points are taken to be in the r-torsion
- * (callers validate them, as for dvgpu_bn254_g2_msm).  Status codes as in
- * dynaverse_gpu.h.
+ * zeroes results and gt.  G2 subgroup membership is NOT checked here: G2
+ * points are taken to be in the r-torsion, as for dvgpu_bn254_g2_msm.  The
+ * arkworks typed points these checks replace are subgroup-checked when they
+ * are deserialised, so a caller holding untrusted G2 wire bytes runs
+ * dvgpu_bn254_g2_validation_inspect (dynaverse_gpu.h) on them first and
+ * passes only the points it accepts.  Status codes as in dynaverse_gpu.h.
  */
 #pragma once
 #include <stddef.h>
