@@ -153,8 +153,9 @@ typedef struct DvgpuNttRows {
 } DvgpuNttRows;
 
 /* All row batches in one launch sequence (one launch per pass for every
- * entry, then one bit-reversal launch), whatever their lengths.  Canonical in
- * and out; entries may not overlap each other (each may run in place). */
+ * entry, at most two; the last pass stores in natural order), whatever their
+ * lengths.  Canonical in and out; entries may not overlap each other (each
+ * may run in place). */
 int dvgpu_ext_ntt_batch_device(void* owner, const DvgpuNttRows* rows, size_t count, const uint8_t* root,
                                float* kernel_ms, char* error, size_t error_capacity);
 
