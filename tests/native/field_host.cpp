@@ -89,6 +89,27 @@ int main() {
                 if (op == "g1add") wr_aff(xyzz_to_affine(xyzz_add(xyzz_from_affine(A), xyzz_from_affine(B))));
                 else wr_aff(xyzz_to_affine(xyzz_madd(xyzz_from_affine(A), B)));
             }
+        } else if (op == "g1chain") {
+            // k_accumulate's loop: from infinity, add +-P_i (tokens "x y s",
+            // s = 1 negates) with the signed madd; then the generic path.
+            Xyzz<FqOps> fast = xyzz_inf<FqOps>(), plain = xyzz_inf<FqOps>();
+            for (size_t i = 1; i + 2 < t.size(); i += 3) {
+                const Fq x = rd<FqParams>(t[i]), y = rd<FqParams>(t[i + 1]);
+                const bool s = t[i + 2][0] == '1';
+                fast = xyzz_madd_signed<FqOps>(fast, x, y, s);
+                plain = xyzz_madd<FqOps>(plain, x, s ? neg(y) : y);
+            }
+            wr_aff(xyzz_to_affine(fast));
+            printf(" ");
+            wr_aff(xyzz_to_affine(plain));
+        } else if (op == "mul2raw" || op == "sub3raw") {
+            // 36 limbs (hex, little-endian limb order) of four raw operands;
+            // prints the nine output limbs.
+            Fq v[4];
+            for (int j = 0; j < 4; ++j)
+                for (int k = 0; k < LIMBS; ++k) v[j].l[k] = uint32_t(strtoul(a[9 * j + k], nullptr, 16));
+            const Fq r = op == "mul2raw" ? mul2(v[0], v[1], v[2], v[3]) : sub3_n(v[0], v[1], v[2], v[3]);
+            for (int k = 0; k < LIMBS; ++k) printf("%s%x", k ? " " : "", r.l[k]);
         } else if (op == "g1mul" || op == "g2mul") {
             // point then scalar (hex); result via double-and-add on XYZZ.
             uint32_t k[8];

@@ -108,3 +108,80 @@ def test_g2_mul_psi_oncurve(harness):
     out = [int(x, 16) for x in harness("g2psi", *args)]
     e = o.psi(g)
     assert out == [e[0][0], e[0][1], e[1][0], e[1][1]]
+
+
+LB, NL = 29, 9
+RM = 1 << (LB * NL)
+
+
+def _limbs(v, wide=False, rng=None):
+    """29-bit limbs of v; with `wide`, a limb-wise representation whose limbs
+    run up to 2^31 (as sub_w leaves them), same value."""
+    ls = [(v >> (LB * i)) & ((1 << LB) - 1) for i in range(NL)]
+    if wide:
+        for i in range(NL - 1):            # move up to 3 units of 2^29 down from limb i+1
+            k = min(ls[i + 1], rng.randrange(4))
+            ls[i + 1] -= k
+            ls[i] += k << LB
+    assert sum(x << (LB * i) for i, x in enumerate(ls)) == v
+    return ls
+
+
+def _value(ls):
+    return sum(x << (LB * i) for i, x in enumerate(ls))
+
+
+def test_mul2_bounds_and_value(harness):
+    """(a b + c d) / 2^261 mod q with a wide (limbs < 2^31, value < 6q), b < 6q,
+    c < 2q, d < 4q normalized: the output is normalized, < 2q and exact mod q,
+    including operands at the top of their ranges."""
+    rng = random.Random(21)
+    q = o.Q
+    rinv = pow(RM, -1, q)
+    tops = [(6 * q - 1, 6 * q - 1, 2 * q - 1, 4 * q - 1), (0, 6 * q - 1, 2 * q - 1, 0), (1, 1, 1, 1)]
+    cases = tops + [(rng.randrange(6 * q), rng.randrange(6 * q), rng.randrange(2 * q), rng.randrange(4 * q))
+                    for _ in range(300)]
+    for a, b, c, d in cases:
+        la = _limbs(a, wide=True, rng=rng)
+        assert max(la) < 1 << 31
+        args = ["%x" % x for x in la + _limbs(b) + _limbs(c) + _limbs(d)]
+        out = [int(x, 16) for x in harness("mul2raw", *args)]
+        assert all(x < 1 << LB for x in out)
+        v = _value(out)
+        assert v < 2 * q and v % q == (a * b + c * d) * rinv % q
+
+
+def test_sub3_bounds_and_value(harness):
+    """a + 8q - b - c - d with b + c + d < 6q (each < 2q): normalized, exact."""
+    rng = random.Random(22)
+    q = o.Q
+    cases = [(0, 2 * q - 1, 2 * q - 1, 2 * q - 1), (2 * q - 1, 0, 0, 0)] + \
+        [tuple(rng.randrange(2 * q) for _ in range(4)) for _ in range(300)]
+    for a, b, c, d in cases:
+        args = ["%x" % x for x in _limbs(a) + _limbs(b) + _limbs(c) + _limbs(d)]
+        out = [int(x, 16) for x in harness("sub3raw", *args)]
+        assert all(x < 1 << LB for x in out)
+        assert _value(out) == a + 8 * q - b - c - d
+
+
+def test_signed_madd_chain_equals_oracle(harness):
+    """The accumulation loop's signed lazy mixed add against the oracle and the
+    generic madd: random chains, repeated points (doubling), P then -P
+    (identity, then restart), and the same point with both signs."""
+    rng = random.Random(23)
+    pts = [o.g1_mul(rng.randrange(1, o.R)) for _ in range(12)]
+    P = pts[0]
+    chains = [[(P, 0), (P, 0)], [(P, 0), (P, 1)], [(P, 1), (P, 1), (P, 0)], [(P, 0), (P, 1), (pts[1], 1)],
+              [(pts[1], 0), (P, 0), (P, 0), (P, 1), (pts[2], 1)]]
+    for _ in range(40):
+        chains.append([(rng.choice(pts), rng.randrange(2)) for _ in range(rng.randrange(1, 20))])
+    for ch in chains:
+        want = None
+        args = []
+        for pt, s in ch:
+            want = o.add(want, o.neg(pt) if s else pt)
+            args += [h(pt[0]), h(pt[1]), str(s)]
+        out = harness("g1chain", *args)
+        fast, plain = (out[:1], out[1:]) if out[0] == "inf" else (out[:2], out[2:])
+        assert _parse_g1(fast) == want, ch
+        assert _parse_g1(plain) == want

@@ -125,6 +125,40 @@ HD Xyzz<F> xyzz_madd(const Xyzz<F>& p, const Affine<F>& a) {
     return xyzz_madd<F>(p, a.x, a.y);
 }
 
+// P + (x2, y2) or P + (x2, -y2) (the signed-digit MSM's bucket accumulation);
+// (x2, y2) finite and fully reduced.
+template <class F>
+HD Xyzz<F> xyzz_madd_signed(const Xyzz<F>& p, const typename F::E& x2, const typename F::E& y2, bool negate) {
+    return xyzz_madd<F>(p, x2, negate ? F::neg(y2) : y2);
+}
+
+// G1: madd-2008-s with lazy reductions.  The negation is folded into a wide
+// operand (4m - y2), R stays unreduced (< 6m) into its square and its product,
+// X3 = R^2 - PPP - 2Q takes one normalization and one reduction, and
+// Y3 = R (Q - X3) + Y (4m - PPP) is one double product with one Montgomery
+// reduction.  Bounds: S2, U2, PP, PPP, Q, R^2 < 2m (products); P < 2m; R < 6m
+// normalized; Q - X3 wide (< 6m); the double product < 36 m^2 + 8 m^2.  Outputs
+// are normalized and < 2m, as from xyzz_madd.
+template <>
+HD Xyzz<FqOps> xyzz_madd_signed<FqOps>(const Xyzz<FqOps>& p, const Fq& x2, const Fq& y2, bool negate) {
+    const Fq ys = negate ? sub_w(fp_zero<FqParams>(), y2) : y2;  // wide when negated, < 4m
+    if (is_zero(p.ZZ)) return {x2, negate ? reduce(norm(ys)) : y2, fp_one<FqParams>(), fp_one<FqParams>()};
+    const Fq U2 = mul(x2, p.ZZ);
+    const Fq S2 = mul(ys, p.ZZZ);
+    const Fq P = reduce(norm(sub_w(U2, p.X)));
+    const Fq R = norm(sub_w(S2, p.Y));  // < 6m
+    if (is_zero(P)) {
+        if (is_zero(reduce(R))) return xyzz_mdbl<FqOps>(x2, negate ? reduce(norm(ys)) : y2);
+        return xyzz_inf<FqOps>();
+    }
+    const Fq PP = sqr(P);
+    const Fq PPP = mul(P, PP);
+    const Fq Q = mul(p.X, PP);
+    const Fq X3 = reduce(sub3_n(sqr(R), PPP, Q, Q));
+    const Fq Y3 = mul2(sub_w(Q, X3), R, p.Y, norm(sub_w(fp_zero<FqParams>(), PPP)));
+    return {X3, Y3, mul(p.ZZ, PP), mul(p.ZZZ, PPP)};
+}
+
 // P + Q (add-2008-s).
 template <class F>
 HD Xyzz<F> xyzz_add(const Xyzz<F>& p, const Xyzz<F>& q) {

@@ -132,6 +132,44 @@ HD Fp<C> sqr(const Fp<C>& a) {
     return out;
 }
 
+// (a*b + c*d)/R mod m with one Montgomery reduction: a may be wide (limbs
+// < 2^31), b, c, d normalized, and a*b + c*d < 44 m^2 (then the output is
+// normalized and < 2m).  A column then holds at most 9 products below 2^60
+// and 18 below 2^58 (< 2^63.8): no overflow.
+template <class C>
+HD Fp<C> mul2(const Fp<C>& a, const Fp<C>& b, const Fp<C>& c, const Fp<C>& d) {
+    uint32_t m[LIMBS];
+    Fp<C> out;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < LIMBS; ++k) {
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+            acc += uint64_t(a.l[i]) * b.l[k - i];
+            acc += uint64_t(c.l[i]) * d.l[k - i];
+            acc += uint64_t(m[i]) * C::MOD[k - i];
+        }
+        acc += uint64_t(a.l[k]) * b.l[0];
+        acc += uint64_t(c.l[k]) * d.l[0];
+        m[k] = mul_lo32(uint32_t(acc), C::INV) & LIMB_MASK;
+        acc += uint64_t(m[k]) * C::MOD[0];
+        acc >>= LIMB_BITS;
+    }
+#pragma unroll
+    for (int k = LIMBS; k < 2 * LIMBS - 1; ++k) {
+#pragma unroll
+        for (int i = k - LIMBS + 1; i < LIMBS; ++i) {
+            acc += uint64_t(a.l[i]) * b.l[k - i];
+            acc += uint64_t(c.l[i]) * d.l[k - i];
+            acc += uint64_t(m[i]) * C::MOD[k - i];
+        }
+        out.l[k - LIMBS] = uint32_t(acc) & LIMB_MASK;
+        acc >>= LIMB_BITS;
+    }
+    out.l[LIMBS - 1] = uint32_t(acc);
+    return out;
+}
+
 // Carry-propagate non-negative limbs (each < 2^32) into normalized form.
 template <class C>
 HD Fp<C> norm(Fp<C> x) {
@@ -184,6 +222,16 @@ HD Fp<C> sub2_n(const Fp<C>& a, const Fp<C>& b, const Fp<C>& c) {
     Fp<C> r;
 #pragma unroll
     for (int i = 0; i < LIMBS; ++i) r.l[i] = a.l[i] + C::SUB8[i] - b.l[i] - c.l[i];
+    return norm(r);
+}
+
+// a + 8m - b - c - d (all normalized, b + c + d < 8m), normalized, < a + 8m:
+// the big borrow form of 8m keeps limbs 0..7 at >= 2^31 - 4 > 3 (2^29 - 1).
+template <class C>
+HD Fp<C> sub3_n(const Fp<C>& a, const Fp<C>& b, const Fp<C>& c, const Fp<C>& d) {
+    Fp<C> r;
+#pragma unroll
+    for (int i = 0; i < LIMBS; ++i) r.l[i] = a.l[i] + C::SUB8[i] - b.l[i] - c.l[i] - d.l[i];
     return norm(r);
 }
 

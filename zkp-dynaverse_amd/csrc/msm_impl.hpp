@@ -478,10 +478,9 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t* __restrict__
 #pragma unroll
             for (int k = 0; k < AW / 4; ++k) raw[k] = q[k];
         }
-        Affine<F> p = Mem<F>::load_affine(cur);
+        const Affine<F> p = Mem<F>::load_affine(cur);
         if (p.inf) continue;
-        if (sign) p.y = F::neg(p.y);
-        acc = xyzz_madd<F>(acc, p.x, p.y);
+        acc = xyzz_madd_signed<F>(acc, p.x, p.y, sign != 0);
     }
     Mem<F>::store_xyzz(partials + size_t(t) * PointLayout<F>::XYZZ_WORDS, acc);
 }
