@@ -102,6 +102,24 @@ int main() {
             wr_aff(xyzz_to_affine(fast));
             printf(" ");
             wr_aff(xyzz_to_affine(plain));
+        } else if (op == "g1combo") {
+            // u = A + B (signed madds from infinity, so ZZ != 1), then 2u, 3u,
+            // 3u + 3u, 3u - 3u, 2u + 4u (equal points in different
+            // coordinates) and u + B, each printed as an affine point.
+            const Fq ax = rd<FqParams>(a[0]), ay = rd<FqParams>(a[1]);
+            const Fq bx = rd<FqParams>(a[2]), by = rd<FqParams>(a[3]);
+            const Xyzz<FqOps> u = xyzz_madd_signed<FqOps>(xyzz_madd_signed<FqOps>(xyzz_inf<FqOps>(), ax, ay, false),
+                                                          bx, by, false);
+            const Xyzz<FqOps> v = xyzz_dbl<FqOps>(u);
+            const Xyzz<FqOps> w = xyzz_add<FqOps>(u, v);
+            const Xyzz<FqOps> outs[6] = {
+                v, w, xyzz_add<FqOps>(w, w), xyzz_add<FqOps>(w, xyzz_neg<FqOps>(w)),
+                xyzz_add<FqOps>(v, xyzz_dbl<FqOps>(xyzz_dbl<FqOps>(u))),
+                xyzz_add<FqOps>(u, xyzz_from_affine<FqOps>(Affine<FqOps>{bx, by, false}))};
+            for (int k = 0; k < 6; ++k) {
+                if (k) printf(" | ");
+                wr_aff(xyzz_to_affine(outs[k]));
+            }
         } else if (op == "mul2raw" || op == "sub3raw") {
             // 36 limbs (hex, little-endian limb order) of four raw operands;
             // prints the nine output limbs.

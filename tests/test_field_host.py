@@ -185,3 +185,20 @@ def test_signed_madd_chain_equals_oracle(harness):
         fast, plain = (out[:1], out[1:]) if out[0] == "inf" else (out[:2], out[2:])
         assert _parse_g1(fast) == want, ch
         assert _parse_g1(plain) == want
+
+
+def test_g1_add_and_dbl_in_xyzz(harness):
+    """The G1 general add and doubling (lazy reductions) on points whose ZZ is
+    not one: 2u, 3u, 3u + 3u (the add's doubling case), 3u - 3u, 2u + 4u (equal
+    points in different coordinates), u + B, for u = A + B, A == B and A == -B
+    included."""
+    rng = random.Random(24)
+    pts = [o.g1_mul(rng.randrange(1, o.R)) for _ in range(8)]
+    pairs = [(pts[0], pts[1]), (pts[2], pts[2]), (pts[3], o.neg(pts[3]))] + \
+        [(rng.choice(pts), rng.choice(pts)) for _ in range(20)]
+    for A, B in pairs:
+        out = " ".join(harness("g1combo", h(A[0]), h(A[1]), h(B[0]), h(B[1]))).split(" | ")
+        got = [_parse_g1(x.split()) for x in out]
+        u = o.add(A, B)
+        want = [o.mul(u, 2), o.mul(u, 3), o.mul(u, 6), None, o.mul(u, 6), o.add(u, B)]
+        assert got == want, (A, B)

@@ -183,6 +183,47 @@ HD Xyzz<F> xyzz_add(const Xyzz<F>& p, const Xyzz<F>& q) {
     return {X3, Y3, F::mul(F::mul(p.ZZ, q.ZZ), PP), F::mul(F::mul(p.ZZZ, q.ZZZ), PPP)};
 }
 
+// G1: dbl-2008-s-1 with lazy reductions.  U = 2Y (< 4m) and M = 3X^2 (< 6m)
+// stay unreduced into their products, X3 = M^2 - 2S takes one
+// normalization and one reduction, Y3 = M (S - X3) + Y (4m - W) is one double
+// product (< 36 m^2 + 8 m^2).  Inputs and outputs normalized and < 2m.
+template <>
+HD Xyzz<FqOps> xyzz_dbl<FqOps>(const Xyzz<FqOps>& p) {
+    if (is_zero(p.ZZ) || is_zero(p.Y)) return xyzz_inf<FqOps>();
+    const Fq U = norm(add_w(p.Y, p.Y));
+    const Fq V = sqr(U);
+    const Fq W = mul(U, V);
+    const Fq S = mul(p.X, V);
+    const Fq xx = sqr(p.X);
+    const Fq M = norm(add_w(add_w(xx, xx), xx));
+    const Fq X3 = reduce(sub2_n(sqr(M), S, S));
+    const Fq Y3 = mul2(sub_w(S, X3), M, p.Y, norm(sub_w(fp_zero<FqParams>(), W)));
+    return {X3, Y3, mul(V, p.ZZ), mul(W, p.ZZZ)};
+}
+
+// G1: add-2008-s with the same lazy steps as the mixed add below.
+template <>
+HD Xyzz<FqOps> xyzz_add<FqOps>(const Xyzz<FqOps>& p, const Xyzz<FqOps>& q) {
+    if (is_zero(p.ZZ)) return q;
+    if (is_zero(q.ZZ)) return p;
+    const Fq U1 = mul(p.X, q.ZZ);
+    const Fq U2 = mul(q.X, p.ZZ);
+    const Fq S1 = mul(p.Y, q.ZZZ);
+    const Fq S2 = mul(q.Y, p.ZZZ);
+    const Fq P = reduce(norm(sub_w(U2, U1)));
+    const Fq R = norm(sub_w(S2, S1));  // < 6m
+    if (is_zero(P)) {
+        if (is_zero(reduce(R))) return xyzz_dbl<FqOps>(p);
+        return xyzz_inf<FqOps>();
+    }
+    const Fq PP = sqr(P);
+    const Fq PPP = mul(P, PP);
+    const Fq Q = mul(U1, PP);
+    const Fq X3 = reduce(sub3_n(sqr(R), PPP, Q, Q));
+    const Fq Y3 = mul2(sub_w(Q, X3), R, S1, norm(sub_w(fp_zero<FqParams>(), PPP)));
+    return {X3, Y3, mul(mul(p.ZZ, q.ZZ), PP), mul(mul(p.ZZZ, q.ZZZ), PPP)};
+}
+
 template <class F>
 HD bool xyzz_eq(const Xyzz<F>& p, const Xyzz<F>& q) {
     const bool pi = F::is_zero(p.ZZ), qi = F::is_zero(q.ZZ);
