@@ -35,78 +35,72 @@ constexpr uint64_t neg_inv64(uint64_t p0) {
 }
 constexpr uint64_t PINV = neg_inv64(P[0]);
 
-inline bool geq_p(const uint64_t (&a)[4]) {
-    for (int i = 3; i >= 0; --i) {
-        if (a[i] != P[i]) return a[i] > P[i];
-    }
-    return true;
-}
-
-inline void sub_p(uint64_t (&a)[4]) {
-    u128 borrow = 0;
+// r - p when r >= p, else r (r < 2p), without branches.
+inline void reduce_once(uint64_t (&r)[4]) {
+    uint64_t s[4], borrow = 0;
     for (int i = 0; i < 4; ++i) {
-        const u128 d = u128(a[i]) - P[i] - borrow;
-        a[i] = uint64_t(d);
-        borrow = (d >> 64) & 1;
+        const u128 d = u128(r[i]) - P[i] - borrow;
+        s[i] = uint64_t(d);
+        borrow = uint64_t(d >> 64) & 1;
     }
+    const uint64_t keep = 0 - borrow;  // all ones when r < p
+    for (int i = 0; i < 4; ++i) r[i] = (r[i] & keep) | (s[i] & ~keep);
 }
 
+// Montgomery product by CIOS.  The top limb of p is below 2^62, so the
+// running value stays below 2p < 2^255 and the textbook form's extra carry
+// words are never needed (the high word of each step's two carries is
+// folded into the top limb directly).
 inline Fe mul(const Fe& a, const Fe& b) {
-    uint64_t t[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t t[4] = {0, 0, 0, 0};
     for (int i = 0; i < 4; ++i) {
-        u128 c = 0;
-        for (int j = 0; j < 4; ++j) {
-            c += u128(a.l[j]) * b.l[i] + t[j];
-            t[j] = uint64_t(c);
-            c >>= 64;
-        }
-        c += t[4];
-        t[4] = uint64_t(c);
-        t[5] = uint64_t(c >> 64);
-        const uint64_t m = t[0] * PINV;
-        c = u128(m) * P[0] + t[0];
-        c >>= 64;
+        const uint64_t bi = b.l[i];
+        u128 A = u128(a.l[0]) * bi + t[0];
+        const uint64_t lo = uint64_t(A);
+        const uint64_t m = lo * PINV;
+        u128 C = (u128(m) * P[0] + lo) >> 64;  // the low word cancels
+        A >>= 64;
         for (int j = 1; j < 4; ++j) {
-            c += u128(m) * P[j] + t[j];
-            t[j - 1] = uint64_t(c);
-            c >>= 64;
+            A += u128(a.l[j]) * bi + t[j];
+            const uint64_t tj = uint64_t(A);
+            A >>= 64;
+            C += u128(m) * P[j] + tj;
+            t[j - 1] = uint64_t(C);
+            C >>= 64;
         }
-        c += t[4];
-        t[3] = uint64_t(c);
-        t[4] = t[5] + uint64_t(c >> 64);
+        t[3] = uint64_t(C) + uint64_t(A);
     }
     uint64_t r[4] = {t[0], t[1], t[2], t[3]};
-    if (t[4] || geq_p(r)) sub_p(r);  // the sum is below 2p
+    reduce_once(r);
     return {{r[0], r[1], r[2], r[3]}};
 }
 
+// a + b for a, b < p: the sum is below 2^255, so one subtraction suffices.
 inline Fe add(const Fe& a, const Fe& b) {
     uint64_t r[4];
-    u128 c = 0;
+    uint64_t c = 0;
     for (int i = 0; i < 4; ++i) {
-        c += u128(a.l[i]) + b.l[i];
-        r[i] = uint64_t(c);
-        c >>= 64;
+        const u128 s = u128(a.l[i]) + b.l[i] + c;
+        r[i] = uint64_t(s);
+        c = uint64_t(s >> 64);
     }
-    if (c || geq_p(r)) sub_p(r);
+    reduce_once(r);
     return {{r[0], r[1], r[2], r[3]}};
 }
 
 inline Fe sub(const Fe& a, const Fe& b) {
-    uint64_t r[4];
-    u128 borrow = 0;
+    uint64_t r[4], borrow = 0;
     for (int i = 0; i < 4; ++i) {
         const u128 d = u128(a.l[i]) - b.l[i] - borrow;
         r[i] = uint64_t(d);
-        borrow = (d >> 64) & 1;
+        borrow = uint64_t(d >> 64) & 1;
     }
-    if (borrow) {
-        u128 c = 0;
-        for (int i = 0; i < 4; ++i) {
-            c += u128(r[i]) + P[i];
-            r[i] = uint64_t(c);
-            c >>= 64;
-        }
+    const uint64_t mask = 0 - borrow;  // add p back when the difference went negative
+    uint64_t c = 0;
+    for (int i = 0; i < 4; ++i) {
+        const u128 s = u128(r[i]) + (P[i] & mask) + c;
+        r[i] = uint64_t(s);
+        c = uint64_t(s >> 64);
     }
     return {{r[0], r[1], r[2], r[3]}};
 }

@@ -108,6 +108,33 @@ struct Mem<Fq2Ops> {
 
 constexpr uint32_t NO_DIGIT = 0xffffffffu;
 
+// Accumulation tasks: every bucket of c points is cut into floor(c / T)
+// full tasks of T points and, when T does not divide c, one task of c mod T
+// points.  Tasks run longest first (class 0: the full tasks, class k: tasks
+// of T - k points), so that the 64 lanes of a wave add about equally many
+// points; in bucket order a wave's lanes mix full tasks with short
+// remainders and all wait for the longest.
+constexpr uint32_t TASK_CLASS_MAX = 1024;  // task_points <= 1024
+// Words of the workspace's counter area: [0] entries, [1] tasks, [2] heavy
+// buckets, [3] error word, then the task-class counts and cursors.
+constexpr uint32_t TOTAL_CLASS_COUNTS = 4;
+constexpr uint32_t TOTAL_CLASS_CURSORS = TOTAL_CLASS_COUNTS + TASK_CLASS_MAX;
+constexpr uint32_t TOTAL_WORDS = TOTAL_CLASS_CURSORS + TASK_CLASS_MAX;
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Adds the task classes of `count` buckets (one per thread, 0 for none) to
+// the LDS histogram h (tp zeroed counters): full tasks summed per wave.
+__device__ __forceinline__ void task_class_add(uint32_t count, uint32_t tp, uint32_t* h) {
+    const uint32_t r = count % tp;
+    if (r) atomicAdd(&h[tp - r], 1u);
+    const uint32_t full = wave_sum(count / tp);
+    if ((threadIdx.x & 63) == 0 && full) atomicAdd(&h[0], full);
+}
+
 // bits [bit, bit + c) of a 256-bit little-endian scalar (c <= 24)
 // Signed c-bit digits in [-(2^(c-1) - 1), 2^(c-1)] by one addition: with
 // K = sum_k (2^(c-1) - 1) 2^(kc), digit k of s is window k of t = s + K minus
@@ -225,13 +252,15 @@ __device__ __forceinline__ uint32_t xcd_grouped_block(uint32_t bid, uint32_t gri
 // the signed digit of every point of the chunk in that window (the carry in
 // is recomputed from the lower windows) is counted in an LDS histogram, which
 // goes to global memory for the per-bucket prefix over blocks.  Placing pass:
-// the block loads its absolute start per bucket, recomputes the digits and
-// places every point with an LDS atomic on that start.  (The order within a
-// bucket is arbitrary; a bucket's sum does not depend on it.)
+// the block's start per bucket is the bucket's start (the scan's offsets)
+// plus the block's prefix among the bucket's blocks; the block recomputes
+// the digits and places every point with an LDS atomic on that start.  (The
+// order within a bucket is arbitrary; a bucket's sum does not depend on it.)
 template <bool PLACE>
 static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
     k_digits_lds(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t grid, const uint32_t* __restrict__ scalars,
-                 uint32_t* __restrict__ hist, const uint32_t* __restrict__ point_index, uint32_t point_limit,
+                 uint32_t* __restrict__ hist, const uint2* __restrict__ offsets,
+                 const uint32_t* __restrict__ point_index, uint32_t point_limit,
                  uint32_t* __restrict__ flags, uint32_t* __restrict__ sorted) {
     __shared__ uint32_t counts[MSM_LDS_BUCKETS];
     const uint32_t g = xcd_grouped_block(blockIdx.x, grid);
@@ -242,7 +271,8 @@ static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
     const uint32_t j = d.table ? local : local / d.digit_windows;
     const uint32_t w0 = d.table ? 0u : local % d.digit_windows, w1 = d.table ? d.digit_windows : w0 + 1;
     uint32_t* mine = hist + d.hist_base + size_t(local) * d.buckets;
-    for (uint32_t b = threadIdx.x; b < d.buckets; b += blockDim.x) counts[b] = PLACE ? mine[b] : 0u;
+    const uint2* starts = offsets + d.bucket_base + (d.table ? 0u : w0) * d.buckets;
+    for (uint32_t b = threadIdx.x; b < d.buckets; b += blockDim.x) counts[b] = PLACE ? mine[b] + starts[b].x : 0u;
     __syncthreads();
     const int c = int(d.c);
     const uint32_t i0 = j << d.chunk_shift, i1 = min(d.n, i0 + (1u << d.chunk_shift));
@@ -281,37 +311,36 @@ static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
 }
 
 // One thread per bucket: exclusive prefix of the bucket's per-block counts
-// over the blocks feeding it (in place), and the bucket's total.
-static __global__ void k_hist_prefix(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_buckets,
-                                     uint32_t* __restrict__ hist, uint32_t* __restrict__ counts) {
+// over the blocks feeding it (in place), the bucket's total, and the task
+// classes of the block's buckets added to the global class counts.
+static __global__ void __launch_bounds__(256) k_hist_prefix(const MsmDesc* __restrict__ descs, uint32_t m,
+                                                           uint32_t total_buckets, uint32_t* __restrict__ hist,
+                                                           uint32_t* __restrict__ counts, uint32_t tp,
+                                                           uint32_t* __restrict__ total) {
+    __shared__ uint32_t h[TASK_CLASS_MAX];
+    for (uint32_t i = threadIdx.x; i < tp; i += blockDim.x) h[i] = 0;
+    __syncthreads();
     const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gb >= total_buckets) return;
-    const MsmDesc d = descs[find_desc<&MsmDesc::bucket_base>(descs, m, gb)];
-    const uint32_t local = gb - d.bucket_base, row = local / d.buckets, b = local % d.buckets;
     uint32_t run = 0;
-    // Normal mode: bucket row w is fed by blocks (chunk j, window w); table
-    // mode: the single row is fed by every block of the MSM.
-    const uint32_t nblk = d.chunks;
-    for (uint32_t k = 0; k < nblk; ++k) {
-        const uint32_t blk = d.table ? k : k * d.digit_windows + row;
-        uint32_t* p = hist + d.hist_base + size_t(blk) * d.buckets + b;
-        const uint32_t v = *p;
-        *p = run;
-        run += v;
+    if (gb < total_buckets) {
+        const MsmDesc d = descs[find_desc<&MsmDesc::bucket_base>(descs, m, gb)];
+        const uint32_t local = gb - d.bucket_base, row = local / d.buckets, b = local % d.buckets;
+        // Normal mode: bucket row w is fed by blocks (chunk j, window w); table
+        // mode: the single row is fed by every block of the MSM.
+        const uint32_t nblk = d.chunks;
+        for (uint32_t k = 0; k < nblk; ++k) {
+            const uint32_t blk = d.table ? k : k * d.digit_windows + row;
+            uint32_t* p = hist + d.hist_base + size_t(blk) * d.buckets + b;
+            const uint32_t v = *p;
+            *p = run;
+            run += v;
+        }
+        counts[gb] = run;
     }
-    counts[gb] = run;
-}
-
-// After the bucket scan: every block's prefix becomes an absolute position
-// (bucket start + earlier blocks' entries), so the scatter reads one word.
-static __global__ void k_hist_absolute(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_hist,
-                                       const uint2* __restrict__ offsets, uint32_t* __restrict__ hist) {
-    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= total_hist) return;
-    const MsmDesc d = descs[find_desc<&MsmDesc::hist_base>(descs, m, h)];
-    const uint32_t local = h - d.hist_base, blk = local / d.buckets, b = local % d.buckets;
-    const uint32_t row = d.table ? 0u : blk % d.digit_windows;
-    hist[h] += offsets[d.bucket_base + row * d.buckets + b].x;
+    task_class_add(run, tp, h);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tp; i += blockDim.x)
+        if (h[i]) atomicAdd(&total[TOTAL_CLASS_COUNTS + i], h[i]);
 }
 
 static __global__ void k_scatter(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_entries,
@@ -382,8 +411,9 @@ static __global__ void k_scan_tiles(const uint32_t* __restrict__ counts, uint32_
 }
 
 // Single block: exclusive scan of the tile sums in place; writes the grand
-// total (points, tasks) to total[0..1].
-static __global__ void k_scan_sums(uint2* __restrict__ tile_sums, uint32_t tiles, uint32_t* __restrict__ total) {
+// total (points, tasks) to total[0..1] and the task-class cursors.
+static __global__ void k_scan_sums(uint2* __restrict__ tile_sums, uint32_t tiles, uint32_t tp,
+                                   uint32_t* __restrict__ total) {
     __shared__ uint2 smem[SCAN_THREADS];
     uint2 carry = make_uint2(0, 0);
     for (uint32_t start = 0; start < tiles; start += SCAN_THREADS) {
@@ -398,6 +428,22 @@ static __global__ void k_scan_sums(uint2* __restrict__ tile_sums, uint32_t tiles
     if (threadIdx.x == 0) {
         total[0] = carry.x;
         total[1] = carry.y;
+    }
+    // Task-class cursors: exclusive prefix of the class counts (classes
+    // [4t, 4t + 4) per thread).
+    constexpr uint32_t PER = TASK_CLASS_MAX / SCAN_THREADS;
+    uint32_t v[PER], sum = 0;
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t c = threadIdx.x * PER + k;
+        v[k] = c < tp ? total[TOTAL_CLASS_COUNTS + c] : 0u;
+        sum += v[k];
+    }
+    uint2 all;
+    uint32_t run = block_exclusive_scan(make_uint2(sum, 0u), smem, all).x;
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t c = threadIdx.x * PER + k;
+        if (c < tp) total[TOTAL_CLASS_CURSORS + c] = run;
+        run += v[k];
     }
 }
 
@@ -424,33 +470,101 @@ static __global__ void k_scan_apply(const uint32_t* __restrict__ counts, uint32_
     }
 }
 
-// One thread per bucket: emit (start, length) of each of its tasks.
-// Buckets cut into more than HEAVY_TASKS tasks have their partial sums
-// combined by a whole block (k_heavy) instead of serially in k_reduce_chunks.
+// A task's partial sum keeps its bucket-order slot (offsets[b].y + k), so
+// the bucket sums find a bucket's partials together.  Buckets cut into more
+// than HEAVY_TASKS tasks have their partial sums combined by a whole block
+// (k_heavy) instead of serially.
 constexpr uint32_t HEAVY_TASKS = 8;
+// The task kernels give each thread `per` buckets (strided by the block
+// size, so loads stay coalesced), about 256 blocks in all.
+constexpr uint32_t TASK_THREADS = 256;
 
-static __global__ void k_tasks(const uint32_t* __restrict__ counts, const uint2* __restrict__ offsets,
-                               uint32_t m, uint32_t task_points, uint2* __restrict__ tasks,
-                               uint32_t* __restrict__ heavy, uint32_t* __restrict__ heavy_count) {
-    const uint32_t wb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (wb >= m) return;
-    const uint32_t cnt = counts[wb];
-    const uint2 off = offsets[wb];
-    uint32_t k = 0;
-    for (uint32_t s = 0; s < cnt; ++k, s += task_points)
-        tasks[off.y + k] = make_uint2(off.x + s, min(task_points, cnt - s));
-    if (k > HEAVY_TASKS) heavy[atomicAdd(heavy_count, 1u)] = wb;
+inline uint32_t task_buckets_per_thread(uint32_t buckets) {
+    const uint32_t per = uint32_t((uint64_t(buckets) + 256 * TASK_THREADS - 1) / (256 * TASK_THREADS));
+    return std::max<uint32_t>(1, std::min<uint32_t>(16, per));
+}
+
+// Class counts of `per` buckets per thread (the path without the LDS
+// histogram pass, whose k_hist_prefix adds them itself).
+static __global__ void __launch_bounds__(TASK_THREADS) k_task_classes(const uint32_t* __restrict__ counts, uint32_t m,
+                                                                     uint32_t tp, uint32_t per,
+                                                                     uint32_t* __restrict__ total) {
+    __shared__ uint32_t h[TASK_CLASS_MAX];
+    for (uint32_t i = threadIdx.x; i < tp; i += TASK_THREADS) h[i] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * TASK_THREADS * per + threadIdx.x;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t wb = base + k * TASK_THREADS;
+        task_class_add(wb < m ? counts[wb] : 0u, tp, h);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tp; i += TASK_THREADS)
+        if (h[i]) atomicAdd(&total[TOTAL_CLASS_COUNTS + i], h[i]);
+}
+
+// Every bucket's tasks {first entry, points, partial slot} at positions of
+// their classes: the block takes one range per class, each thread one run
+// of full tasks for all its buckets.
+static __global__ void __launch_bounds__(TASK_THREADS) k_tasks(const uint32_t* __restrict__ counts,
+                                                              const uint2* __restrict__ offsets, uint32_t m,
+                                                              uint32_t tp, uint32_t per, uint4* __restrict__ tasks,
+                                                              uint32_t* __restrict__ heavy,
+                                                              uint32_t* __restrict__ total) {
+    __shared__ uint32_t h[TASK_CLASS_MAX];
+    __shared__ uint32_t wave_full[TASK_THREADS / 64];
+    for (uint32_t i = threadIdx.x; i < tp; i += TASK_THREADS) h[i] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * TASK_THREADS * per + threadIdx.x;
+    uint32_t full = 0;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t wb = base + k * TASK_THREADS;
+        const uint32_t cnt = wb < m ? counts[wb] : 0u, r = cnt % tp;
+        full += cnt / tp;
+        if (r) atomicAdd(&h[tp - r], 1u);
+    }
+    // this thread's first full task: block range + waves before + lanes before
+    uint32_t incl = full;
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63) wave_full[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t all = 0;
+        for (uint32_t w = 0; w < TASK_THREADS / 64; ++w) {
+            const uint32_t c = wave_full[w];
+            wave_full[w] = all;
+            all += c;
+        }
+        h[0] = all;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tp; i += TASK_THREADS)
+        if (h[i]) h[i] = atomicAdd(&total[TOTAL_CLASS_CURSORS + i], h[i]);
+    __syncthreads();
+    uint32_t pos = h[0] + wave_full[threadIdx.x >> 6] + incl - full;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t wb = base + k * TASK_THREADS;
+        if (wb >= m) break;
+        const uint32_t cnt = counts[wb], q = cnt / tp, r = cnt % tp;
+        const uint2 off = offsets[wb];
+        for (uint32_t j = 0; j < q; ++j) tasks[pos++] = make_uint4(off.x + j * tp, tp, off.y + j, 0u);
+        if (r) tasks[atomicAdd(&h[tp - r], 1u)] = make_uint4(off.x + q * tp, r, off.y + q, 0u);
+        if (q + (r ? 1u : 0u) > HEAVY_TASKS) heavy[atomicAdd(&total[2], 1u)] = wb;
+    }
 }
 
 template <class F>
 __global__ void __launch_bounds__(256) k_accumulate(const uint32_t* __restrict__ points,
                                                     const uint32_t* __restrict__ sorted,
-                                                    const uint2* __restrict__ tasks,
+                                                    const uint4* __restrict__ tasks,
                                                     const uint32_t* __restrict__ total,
                                                     uint32_t* __restrict__ partials) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total[1]) return;
-    const uint2 task = tasks[t];
+    const uint4 task = tasks[t];  // first entry, points, partial slot
     constexpr int AW = PointLayout<F>::AFFINE_WORDS;
     Xyzz<F> acc = xyzz_inf<F>();
     // Software pipeline: the next point's gather is in flight while the
@@ -482,7 +596,7 @@ __global__ void __launch_bounds__(256) k_accumulate(const uint32_t* __restrict__
         if (p.inf) continue;
         acc = xyzz_madd_signed<F>(acc, p.x, p.y, sign != 0);
     }
-    Mem<F>::store_xyzz(partials + size_t(t) * PointLayout<F>::XYZZ_WORDS, acc);
+    Mem<F>::store_xyzz(partials + size_t(task.z) * PointLayout<F>::XYZZ_WORDS, acc);
 }
 
 // One block per heavy bucket (grid-strided): strided serial sums, then an
@@ -873,7 +987,8 @@ inline int msm_window_for_impl(size_t n, const WindowTable* table) {
 
 inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const MsmConfig& cfg,
                                         const WindowTable* table) {
-    require(cfg.chunk >= 1 && cfg.chunk <= 8 && (cfg.chunk & (cfg.chunk - 1)) == 0 && cfg.task_points >= 1,
+    require(cfg.chunk >= 1 && cfg.chunk <= 8 && (cfg.chunk & (cfg.chunk - 1)) == 0 && cfg.task_points >= 1 &&
+                cfg.task_points <= int(TASK_CLASS_MAX),
             "invalid MSM config");
     MsmBatchPlan p;
     p.task_points = uint32_t(cfg.task_points);
@@ -965,8 +1080,8 @@ inline MsmBufferBytes msm_buffer_bytes(const MsmBatchPlan& plan, int xw) {
     b.ranks = b.digits;
     b.sorted = size_t(plan.total_entries) * 4;
     b.block_sums = tiles * sizeof(uint2);
-    b.total = 4 * 4;
-    b.tasks = plan.max_tasks * sizeof(uint2);
+    b.total = TOTAL_WORDS * 4;
+    b.tasks = plan.max_tasks * sizeof(uint4);
     b.partials = plan.max_tasks * size_t(xw) * 4;
     b.heavy = plan.max_heavy * 4;
     b.chunks = std::max<size_t>(size_t(plan.total_chunks) * (plan.chunk_planes ? 2 : 1),
@@ -1029,7 +1144,8 @@ inline void msm_batch_prepare_impl(const MsmBatchPlan& plan, int xw, MsmWorkspac
 struct MsmBuffers {
     const MsmDesc* descs;
     uint32_t *counts, *digits, *ranks, *sorted, *total, *partials, *heavy, *chunk_vals, *hist;
-    uint2 *offsets, *tile_sums, *tasks;
+    uint2 *offsets, *tile_sums;
+    uint4* tasks;
     uint32_t m, tiles;
 };
 
@@ -1039,6 +1155,7 @@ inline MsmBuffers msm_buffers(const MsmBatchPlan& plan, MsmWorkspace& ws, int xw
     b.tiles = uint32_t((plan.total_buckets + SCAN_TILE - 1) / SCAN_TILE);
     // The buffers were sized by msm_batch_prepare; check instead of growing.
     require(ws.counts.bytes >= size_t(plan.total_buckets) * 4 && ws.sorted.bytes >= size_t(plan.total_entries) * 4 &&
+                ws.total.bytes >= size_t(TOTAL_WORDS) * 4 && ws.tasks.bytes >= plan.max_tasks * sizeof(uint4) &&
                 ws.partials.bytes >= plan.max_tasks * xw * 4 && ws.chunks.bytes >= size_t(plan.total_chunks) * xw * 4 &&
                 ws.chunks.bytes >= size_t(plan.total_planes) * plan.plane_slices * xw * 4 &&
                 ws.chunks.bytes >= size_t(plan.total_chunks) * (plan.chunk_planes ? 2 : 1) * xw * 4 &&
@@ -1053,8 +1170,8 @@ inline MsmBuffers msm_buffers(const MsmBatchPlan& plan, MsmWorkspace& ws, int xw
     b.ranks = static_cast<uint32_t*>(ws.ranks.ptr);
     b.sorted = static_cast<uint32_t*>(ws.sorted.ptr);
     b.tile_sums = static_cast<uint2*>(ws.block_sums.ptr);
-    b.total = static_cast<uint32_t*>(ws.total.ptr);  // [0] entries, [1] tasks, [2] heavy buckets, [3] error
-    b.tasks = static_cast<uint2*>(ws.tasks.ptr);
+    b.total = static_cast<uint32_t*>(ws.total.ptr);  // TOTAL_WORDS counters (k_tasks)
+    b.tasks = static_cast<uint4*>(ws.tasks.ptr);
     b.partials = static_cast<uint32_t*>(ws.partials.ptr);
     b.heavy = static_cast<uint32_t*>(ws.heavy.ptr);
     b.chunk_vals = static_cast<uint32_t*>(ws.chunks.ptr);
@@ -1071,42 +1188,42 @@ inline void msm_mark(hipEvent_t* ev, int k, hipStream_t stream) {
 inline void msm_sort_stage(const uint32_t* point_index, const uint32_t* scalars, const MsmBatchPlan& plan,
                            const MsmBuffers& b, hipStream_t stream, hipEvent_t* ev, int k0) {
     const uint32_t m = b.m, tp = plan.task_points;
+    const uint32_t per = task_buckets_per_thread(plan.total_buckets);
+    const unsigned task_blocks = grid_for(plan.total_buckets, TASK_THREADS * per);
     msm_mark(ev, k0, stream);
-    k_fill<<<1, 64, 0, stream>>>(b.total, 4, 0u);
+    k_fill<<<1, 256, 0, stream>>>(b.total, TOTAL_CLASS_COUNTS + tp, 0u);
     if (plan.lds_digits) {
         k_digits_lds<false><<<plan.total_hist_blocks, plan.digit_threads, 0, stream>>>(
-            b.descs, m, plan.total_hist_blocks, scalars, b.hist, point_index, plan.point_limit, b.total + 3, b.sorted);
+            b.descs, m, plan.total_hist_blocks, scalars, b.hist, b.offsets, point_index, plan.point_limit, b.total + 3,
+            b.sorted);
         k_hist_prefix<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(b.descs, m, plan.total_buckets, b.hist,
-                                                                           b.counts);
+                                                                           b.counts, tp, b.total);
     } else {
         k_fill<<<unsigned(std::min<size_t>(grid_for(plan.total_buckets, 256), 4096)), 256, 0, stream>>>(
             b.counts, plan.total_buckets, 0u);
         k_digits<<<grid_for(plan.total_threads, 256), 256, 0, stream>>>(b.descs, m, plan.total_threads, scalars,
                                                                        b.counts, b.digits, b.ranks);
+        k_task_classes<<<task_blocks, TASK_THREADS, 0, stream>>>(b.counts, plan.total_buckets, tp, per, b.total);
     }
     HIP_CHECK_LAUNCH();
     msm_mark(ev, k0 + 1, stream);
     k_scan_tiles<<<b.tiles, SCAN_THREADS, 0, stream>>>(b.counts, plan.total_buckets, tp, b.tile_sums);
-    k_scan_sums<<<1, SCAN_THREADS, 0, stream>>>(b.tile_sums, b.tiles, b.total);
+    k_scan_sums<<<1, SCAN_THREADS, 0, stream>>>(b.tile_sums, b.tiles, tp, b.total);
     k_scan_apply<<<b.tiles, SCAN_THREADS, 0, stream>>>(b.counts, plan.total_buckets, tp, b.tile_sums, b.offsets);
     HIP_CHECK_LAUNCH();
-    if (plan.lds_digits) {
-        k_hist_absolute<<<grid_for(plan.total_hist, 256), 256, 0, stream>>>(b.descs, m, plan.total_hist, b.offsets,
-                                                                          b.hist);
-        HIP_CHECK_LAUNCH();
-    }
     msm_mark(ev, k0 + 2, stream);
     if (plan.lds_digits)
         k_digits_lds<true><<<plan.total_hist_blocks, plan.digit_threads, 0, stream>>>(
-            b.descs, m, plan.total_hist_blocks, scalars, b.hist, point_index, plan.point_limit, b.total + 3, b.sorted);
+            b.descs, m, plan.total_hist_blocks, scalars, b.hist, b.offsets, point_index, plan.point_limit, b.total + 3,
+            b.sorted);
     else
         k_scatter<<<grid_for(plan.total_entries, 256), 256, 0, stream>>>(b.descs, m, plan.total_entries, b.digits,
                                                                         b.ranks, b.offsets, point_index,
                                                                         plan.point_limit, b.total + 3, b.sorted);
     HIP_CHECK_LAUNCH();
     msm_mark(ev, k0 + 3, stream);
-    k_tasks<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(b.counts, b.offsets, plan.total_buckets, tp,
-                                                                  b.tasks, b.heavy, b.total + 2);
+    k_tasks<<<task_blocks, TASK_THREADS, 0, stream>>>(b.counts, b.offsets, plan.total_buckets, tp, per, b.tasks,
+                                                      b.heavy, b.total);
     HIP_CHECK_LAUNCH();
     msm_mark(ev, k0 + 4, stream);
 }
