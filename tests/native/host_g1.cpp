@@ -3,6 +3,7 @@
 // Montgomery form (x 2^261 mod p) on input, canonical on output.
 //   "mul a b" / "add a b" / "sub a b" / "inv a"     -> canonical result
 //   "dbl X Y ZZ ZZZ" / "addp P(4) Q(4)"             -> canonical affine x y ("inf")
+//   "dbln n X Y ZZ ZZZ"                             -> the same for 2^n P
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -73,6 +74,7 @@ int main() {
         else if (op == "inv") wr(inv(rd(t[1])));
         else if (op == "dbl") wra(dbl(rdp(&t[1])));
         else if (op == "addp") wra(add(rdp(&t[1]), rdp(&t[5])));
+        else if (op == "dbln") wra(dbl_n(rdp(&t[2]), atoi(t[1])));
         printf("\n");
         fflush(stdout);
     }

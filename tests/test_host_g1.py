@@ -81,3 +81,13 @@ def test_point_ops(call):
         assert aff(call("addp", *xyzz(None), *xyzz(Qp, z2))) == Qp                   # O + Q
         assert aff(call("addp", *xyzz(P, z1), *xyzz(None))) == P
     assert call("dbl", *xyzz(None)) == ["inf"]
+
+
+def test_repeated_doubling(call):
+    """dbl_n (Jacobian doublings between XYZZ conversions) against 2^n P."""
+    rng = random.Random(5)
+    for n in (1, 2, 3, 13, 15, 16):
+        P = o.g1_mul(rng.randrange(1, o.R))
+        z = rng.randrange(1, Q)
+        assert aff(call("dbln", str(n), *xyzz(P, z, n % 2 == 1))) == o.mul(P, 1 << n)
+    assert call("dbln", "5", *xyzz(None)) == ["inf"]

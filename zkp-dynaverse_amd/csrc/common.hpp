@@ -43,6 +43,9 @@ struct DeviceBuffer {
     size_t bytes = 0;
     Quota* quota = nullptr;
     bool borrowed = false;
+    // Changes whenever the buffer starts to view other memory (allocation,
+    // release, borrow), so a cached record of its contents can be checked.
+    uint64_t generation = 0;
     DeviceBuffer() = default;
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
@@ -55,6 +58,7 @@ struct DeviceBuffer {
         ptr = nullptr;
         bytes = 0;
         borrowed = false;
+        ++generation;
     }
     void borrow(void* p, size_t n) {
         release();

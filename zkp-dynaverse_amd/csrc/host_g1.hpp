@@ -179,6 +179,37 @@ inline Xyzz add(const Xyzz& p, const Xyzz& q) {
     return {X3, Y3, mul(mul(p.ZZ, q.ZZ), PP), mul(mul(p.ZZZ, q.ZZZ), PPP)};
 }
 
+// n >= 1 doublings.  They run in Jacobian coordinates (dbl-2009-l for
+// a = 0: 2M + 5S and a shorter dependency chain than the XYZZ doubling),
+// entered and left in XYZZ: (X, Y, ZZ, ZZZ) is the Jacobian point
+// (X ZZ ZZZ^2, Y ZZ^3 ZZZ^2, ZZ ZZZ), and Jacobian (X, Y, Z) is the XYZZ
+// point (X, Y, Z^2, Z^3).  The identity (Z = 0) stays the identity.
+inline Xyzz dbl_n(const Xyzz& p, int n) {
+    if (is_inf(p)) return p;
+    const Fe t = mul(p.ZZZ, p.ZZZ), zz2 = mul(p.ZZ, p.ZZ);
+    Fe X = mul(mul(p.X, p.ZZ), t);
+    Fe Y = mul(mul(p.Y, mul(zz2, p.ZZ)), t);
+    Fe Z = mul(p.ZZ, p.ZZZ);
+    for (int k = 0; k < n; ++k) {
+        const Fe A = mul(X, X), B = mul(Y, Y), C = mul(B, B);
+        const Fe xb = add(X, B);
+        const Fe D2 = sub(sub(mul(xb, xb), A), C);
+        const Fe D = add(D2, D2);
+        const Fe E = add(add(A, A), A);
+        const Fe F = mul(E, E);
+        const Fe X3 = sub(F, add(D, D));
+        const Fe C2 = add(C, C), C4 = add(C2, C2), C8 = add(C4, C4);
+        const Fe Y3 = sub(mul(E, sub(D, X3)), C8);
+        const Fe yz = mul(Y, Z);
+        Z = add(yz, yz);
+        X = X3;
+        Y = Y3;
+    }
+    if (is_zero(Z)) return infinity();
+    const Fe ZZ = mul(Z, Z);
+    return {X, Y, ZZ, mul(ZZ, Z)};
+}
+
 // Canonical affine words (x || y, zero words for the identity).
 inline void to_affine_words(const Xyzz& p, uint32_t (&x)[8], uint32_t (&y)[8]) {
     if (is_inf(p)) {

@@ -43,6 +43,43 @@ class HostPool {
         job->done_cv.wait(lk, [&] { return job->done.load() == job->count; });
     }
 
+    // Asynchronous form: start(count, fn) queues fn(i) for every i < count
+    // (claimed in increasing i) and returns; the caller may run claims
+    // itself with help(), and must call finish() (which waits for every
+    // claimed index) before fn or anything it references goes away.
+    struct Job;
+    template <class Fn>
+    std::shared_ptr<Job> start(size_t count, Fn& fn) {
+        auto job = std::make_shared<Job>();
+        job->count = count;
+        job->fn = [&fn](size_t i) { fn(i); };
+        if (count && !workers_.empty()) {
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                jobs_.push_back(job);
+            }
+            cv_.notify_all();
+        }
+        return job;
+    }
+    // Runs one unclaimed index of the job; false when none is left.
+    static bool help(Job& job) {
+        const size_t i = job.next.fetch_add(1);
+        if (i >= job.count) return false;
+        job.fn(i);
+        if (job.done.fetch_add(1) + 1 == job.count) {
+            std::lock_guard<std::mutex> lk(job.mu);
+            job.done_cv.notify_all();
+        }
+        return true;
+    }
+    static void finish(Job& job) {
+        while (help(job)) {
+        }
+        std::unique_lock<std::mutex> lk(job.mu);
+        job.done_cv.wait(lk, [&] { return job.done.load() >= job.count; });
+    }
+
     ~HostPool() {
         {
             std::lock_guard<std::mutex> lk(mu_);
@@ -52,7 +89,6 @@ class HostPool {
         for (auto& t : workers_) t.join();
     }
 
-  private:
     struct Job {
         size_t count = 0;
         std::function<void(size_t)> fn;
@@ -60,6 +96,8 @@ class HostPool {
         std::mutex mu;
         std::condition_variable done_cv;
     };
+
+  private:
 
     HostPool() {
         const unsigned hw = std::max(1u, std::thread::hardware_concurrency());

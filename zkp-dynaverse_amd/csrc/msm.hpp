@@ -117,6 +117,10 @@ struct MsmWorkspace {
     DeviceBuffer digits, ranks, counts, offsets, sorted, tasks, partials, chunks, block_sums, total,
         points, scalars, windows, heavy, descs, finals, hist;
     HostBuffer host_windows, host_descs;
+    // The descriptors last copied to `descs` (and that buffer's generation
+    // then): a plan whose descriptors are already there skips the upload.
+    std::vector<MsmDesc> uploaded;
+    uint64_t uploaded_generation = ~uint64_t(0);
     // Optional phase events (MSM_PHASE_EVENTS of them) recorded by the
     // enqueue: start, digits, scan, scatter, tasks, accumulate, reduce, tree.
     hipEvent_t* phase_events = nullptr;
@@ -129,6 +133,7 @@ struct MsmWorkspace {
         for (DeviceBuffer* b : all) b->release();
         host_windows.release();
         host_descs.release();
+        uploaded.clear();
     }
 };
 
