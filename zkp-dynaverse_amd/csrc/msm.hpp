@@ -115,7 +115,7 @@ constexpr int MSM_PHASE_EVENTS = 8;
 // Scratch owned by one backend; grows to the largest batch seen.
 struct MsmWorkspace {
     DeviceBuffer digits, ranks, counts, offsets, sorted, tasks, partials, chunks, block_sums, total,
-        points, scalars, windows, heavy, descs, finals, hist;
+        points, scalars, windows, heavy, descs, finals, hist, order;
     HostBuffer host_windows, host_descs;
     // The descriptors last copied to `descs` (and that buffer's generation
     // then): a plan whose descriptors are already there skips the upload.
@@ -124,8 +124,9 @@ struct MsmWorkspace {
     // Optional phase events (MSM_PHASE_EVENTS of them) recorded by the
     // enqueue: start, digits, scan, scatter, tasks, accumulate, reduce, tree.
     hipEvent_t* phase_events = nullptr;
-    DeviceBuffer* all[17] = {&digits, &ranks, &counts, &offsets, &sorted, &tasks, &partials, &chunks,
-                             &block_sums, &total, &points, &scalars, &windows, &heavy, &descs, &finals, &hist};
+    DeviceBuffer* all[18] = {&digits, &ranks, &counts, &offsets, &sorted, &tasks,  &partials,
+                             &chunks, &block_sums, &total, &points, &scalars, &windows, &heavy,
+                             &descs,  &finals, &hist, &order};
     void attach(Quota* q) {
         for (DeviceBuffer* b : all) b->quota = q;
     }

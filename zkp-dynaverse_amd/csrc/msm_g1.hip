@@ -20,7 +20,7 @@ uint64_t msm_run_workspace_bytes(size_t n, const MsmConfig& cfg, bool g2) {
     const MsmBufferBytes b = msm_buffer_bytes(plan, xw);
     uint64_t total = 0;
     for (size_t part : {b.counts, b.offsets, b.digits, b.ranks, b.sorted, b.block_sums, b.total, b.tasks, b.partials,
-                        b.heavy, b.chunks, b.hist, b.descs, b.windows})
+                        b.heavy, b.chunks, b.hist, b.descs, b.windows, b.order})
         if (part) total += DeviceBuffer::grown(part);
     return total;
 }

@@ -118,11 +118,29 @@ constexpr uint32_t NO_DIGIT = 0xffffffffu;
 // points; in bucket order a wave's lanes mix full tasks with short
 // remainders and all wait for the longest.
 constexpr uint32_t TASK_CLASS_MAX = 1024;  // task_points <= 1024
+// Buckets cut into more than HEAVY_TASKS tasks have their partial sums
+// combined by a whole block (k_heavy); the others with two or more tasks by
+// one thread (k_bucket_sums), which takes them in order of decreasing task
+// count (bucket-sum class HEAVY_TASKS - tasks) so that a wave's lanes chain
+// about equally many additions.
+constexpr uint32_t HEAVY_TASKS = 8;
 // Words of the workspace's counter area: [0] entries, [1] tasks, [2] heavy
-// buckets, [3] error word, then the task-class counts and cursors.
-constexpr uint32_t TOTAL_CLASS_COUNTS = 4;
+// buckets, [3] error word, then the bucket-sum class counts and cursors, the
+// number of buckets listed for k_bucket_sums, and the task-class counts and
+// cursors.
+constexpr uint32_t TOTAL_SUM_COUNTS = 4;
+constexpr uint32_t TOTAL_SUM_CURSORS = TOTAL_SUM_COUNTS + HEAVY_TASKS;
+constexpr uint32_t TOTAL_SUM_LISTED = TOTAL_SUM_CURSORS + HEAVY_TASKS;
+constexpr uint32_t TOTAL_CLASS_COUNTS = TOTAL_SUM_LISTED + 4;
 constexpr uint32_t TOTAL_CLASS_CURSORS = TOTAL_CLASS_COUNTS + TASK_CLASS_MAX;
 constexpr uint32_t TOTAL_WORDS = TOTAL_CLASS_CURSORS + TASK_CLASS_MAX;
+
+// Bucket-sum class of a bucket of `count` points, or HEAVY_TASKS when no
+// thread sums it (fewer than two tasks, or a heavy bucket).
+__host__ __device__ __forceinline__ uint32_t sum_class(uint32_t count, uint32_t tp) {
+    const uint32_t nt = (count + tp - 1) / tp;
+    return nt >= 2 && nt <= HEAVY_TASKS ? HEAVY_TASKS - nt : HEAVY_TASKS;
+}
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -130,12 +148,23 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 }
 
 // Adds the task classes of `count` buckets (one per thread, 0 for none) to
-// the LDS histogram h (tp zeroed counters): full tasks summed per wave.
-__device__ __forceinline__ void task_class_add(uint32_t count, uint32_t tp, uint32_t* h) {
+// the LDS histogram h (tp zeroed counters; full tasks summed per wave) and
+// their bucket-sum classes to hs (HEAVY_TASKS zeroed counters).
+__device__ __forceinline__ void task_class_add(uint32_t count, uint32_t tp, uint32_t* h, uint32_t* hs) {
     const uint32_t r = count % tp;
     if (r) atomicAdd(&h[tp - r], 1u);
     const uint32_t full = wave_sum(count / tp);
     if ((threadIdx.x & 63) == 0 && full) atomicAdd(&h[0], full);
+    const uint32_t sc = sum_class(count, tp);
+    if (sc < HEAVY_TASKS) atomicAdd(&hs[sc], 1u);
+}
+
+// The block's class histograms added to the global counts.
+__device__ __forceinline__ void task_class_flush(uint32_t tp, const uint32_t* h, const uint32_t* hs,
+                                                 uint32_t* __restrict__ total) {
+    for (uint32_t i = threadIdx.x; i < tp; i += blockDim.x)
+        if (h[i]) atomicAdd(&total[TOTAL_CLASS_COUNTS + i], h[i]);
+    if (threadIdx.x < HEAVY_TASKS && hs[threadIdx.x]) atomicAdd(&total[TOTAL_SUM_COUNTS + threadIdx.x], hs[threadIdx.x]);
 }
 
 // bits [bit, bit + c) of a 256-bit little-endian scalar (c <= 24)
@@ -320,8 +349,9 @@ static __global__ void __launch_bounds__(256) k_hist_prefix(const MsmDesc* __res
                                                            uint32_t total_buckets, uint32_t* __restrict__ hist,
                                                            uint32_t* __restrict__ counts, uint32_t tp,
                                                            uint32_t* __restrict__ total) {
-    __shared__ uint32_t h[TASK_CLASS_MAX];
+    __shared__ uint32_t h[TASK_CLASS_MAX], hs[HEAVY_TASKS];
     for (uint32_t i = threadIdx.x; i < tp; i += blockDim.x) h[i] = 0;
+    if (threadIdx.x < HEAVY_TASKS) hs[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t run = 0;
@@ -340,10 +370,9 @@ static __global__ void __launch_bounds__(256) k_hist_prefix(const MsmDesc* __res
         }
         counts[gb] = run;
     }
-    task_class_add(run, tp, h);
+    task_class_add(run, tp, h, hs);
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < tp; i += blockDim.x)
-        if (h[i]) atomicAdd(&total[TOTAL_CLASS_COUNTS + i], h[i]);
+    task_class_flush(tp, h, hs, total);
 }
 
 static __global__ void k_scatter(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_entries,
@@ -431,6 +460,12 @@ static __global__ void k_scan_sums(uint2* __restrict__ tile_sums, uint32_t tiles
     if (threadIdx.x == 0) {
         total[0] = carry.x;
         total[1] = carry.y;
+        uint32_t listed = 0;  // bucket-sum class cursors
+        for (uint32_t c = 0; c < HEAVY_TASKS; ++c) {
+            total[TOTAL_SUM_CURSORS + c] = listed;
+            listed += total[TOTAL_SUM_COUNTS + c];
+        }
+        total[TOTAL_SUM_LISTED] = listed;
     }
     // Task-class cursors: exclusive prefix of the class counts (classes
     // [4t, 4t + 4) per thread).
@@ -474,10 +509,7 @@ static __global__ void k_scan_apply(const uint32_t* __restrict__ counts, uint32_
 }
 
 // A task's partial sum keeps its bucket-order slot (offsets[b].y + k), so
-// the bucket sums find a bucket's partials together.  Buckets cut into more
-// than HEAVY_TASKS tasks have their partial sums combined by a whole block
-// (k_heavy) instead of serially.
-constexpr uint32_t HEAVY_TASKS = 8;
+// the bucket sums find a bucket's partials together.
 // The task kernels give each thread `per` buckets (strided by the block
 // size, so loads stay coalesced), about 256 blocks in all.
 constexpr uint32_t TASK_THREADS = 256;
@@ -492,17 +524,17 @@ inline uint32_t task_buckets_per_thread(uint32_t buckets) {
 static __global__ void __launch_bounds__(TASK_THREADS) k_task_classes(const uint32_t* __restrict__ counts, uint32_t m,
                                                                      uint32_t tp, uint32_t per,
                                                                      uint32_t* __restrict__ total) {
-    __shared__ uint32_t h[TASK_CLASS_MAX];
+    __shared__ uint32_t h[TASK_CLASS_MAX], hs[HEAVY_TASKS];
     for (uint32_t i = threadIdx.x; i < tp; i += TASK_THREADS) h[i] = 0;
+    if (threadIdx.x < HEAVY_TASKS) hs[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * TASK_THREADS * per + threadIdx.x;
     for (uint32_t k = 0; k < per; ++k) {
         const uint32_t wb = base + k * TASK_THREADS;
-        task_class_add(wb < m ? counts[wb] : 0u, tp, h);
+        task_class_add(wb < m ? counts[wb] : 0u, tp, h, hs);
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < tp; i += TASK_THREADS)
-        if (h[i]) atomicAdd(&total[TOTAL_CLASS_COUNTS + i], h[i]);
+    task_class_flush(tp, h, hs, total);
 }
 
 // Every bucket's tasks {first entry, points, partial slot} at positions of
@@ -512,18 +544,21 @@ static __global__ void __launch_bounds__(TASK_THREADS) k_tasks(const uint32_t* _
                                                               const uint2* __restrict__ offsets, uint32_t m,
                                                               uint32_t tp, uint32_t per, uint4* __restrict__ tasks,
                                                               uint32_t* __restrict__ heavy,
+                                                              uint32_t* __restrict__ order,
                                                               uint32_t* __restrict__ total) {
-    __shared__ uint32_t h[TASK_CLASS_MAX];
+    __shared__ uint32_t h[TASK_CLASS_MAX], hs[HEAVY_TASKS];
     __shared__ uint32_t wave_full[TASK_THREADS / 64];
     for (uint32_t i = threadIdx.x; i < tp; i += TASK_THREADS) h[i] = 0;
+    if (threadIdx.x < HEAVY_TASKS) hs[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * TASK_THREADS * per + threadIdx.x;
     uint32_t full = 0;
     for (uint32_t k = 0; k < per; ++k) {
         const uint32_t wb = base + k * TASK_THREADS;
-        const uint32_t cnt = wb < m ? counts[wb] : 0u, r = cnt % tp;
+        const uint32_t cnt = wb < m ? counts[wb] : 0u, r = cnt % tp, sc = sum_class(cnt, tp);
         full += cnt / tp;
         if (r) atomicAdd(&h[tp - r], 1u);
+        if (sc < HEAVY_TASKS) atomicAdd(&hs[sc], 1u);
     }
     // this thread's first full task: block range + waves before + lanes before
     uint32_t incl = full;
@@ -546,6 +581,8 @@ static __global__ void __launch_bounds__(TASK_THREADS) k_tasks(const uint32_t* _
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < tp; i += TASK_THREADS)
         if (h[i]) h[i] = atomicAdd(&total[TOTAL_CLASS_CURSORS + i], h[i]);
+    if (threadIdx.x < HEAVY_TASKS && hs[threadIdx.x])
+        hs[threadIdx.x] = atomicAdd(&total[TOTAL_SUM_CURSORS + threadIdx.x], hs[threadIdx.x]);
     __syncthreads();
     uint32_t pos = h[0] + wave_full[threadIdx.x >> 6] + incl - full;
     for (uint32_t k = 0; k < per; ++k) {
@@ -556,6 +593,8 @@ static __global__ void __launch_bounds__(TASK_THREADS) k_tasks(const uint32_t* _
         for (uint32_t j = 0; j < q; ++j) tasks[pos++] = make_uint4(off.x + j * tp, tp, off.y + j, 0u);
         if (r) tasks[atomicAdd(&h[tp - r], 1u)] = make_uint4(off.x + q * tp, r, off.y + q, 0u);
         if (q + (r ? 1u : 0u) > HEAVY_TASKS) heavy[atomicAdd(&total[2], 1u)] = wb;
+        const uint32_t sc = sum_class(cnt, tp);
+        if (sc < HEAVY_TASKS) order[atomicAdd(&hs[sc], 1u)] = wb;
     }
 }
 
@@ -636,18 +675,21 @@ __global__ void __launch_bounds__(256) k_heavy(const uint32_t* __restrict__ heav
 }
 
 
-// One thread per bucket of 2..HEAVY_TASKS tasks: the bucket's task partials
-// summed into its first task's partial (heavier buckets are summed by
-// k_heavy), so the chunk reduction reads one value per bucket.
+// One thread per bucket of 2..HEAVY_TASKS tasks (listed by k_tasks in order
+// of decreasing task count): the bucket's task partials summed into its
+// first task's partial (heavier buckets are summed by k_heavy), so the
+// chunk reduction reads one value per bucket.
 template <class F>
 __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* __restrict__ counts,
-                                                     const uint2* __restrict__ offsets, uint32_t total_buckets,
-                                                     uint32_t task_points, uint32_t* __restrict__ partials) {
+                                                     const uint2* __restrict__ offsets,
+                                                     const uint32_t* __restrict__ order,
+                                                     const uint32_t* __restrict__ total, uint32_t task_points,
+                                                     uint32_t* __restrict__ partials) {
     constexpr int XW = PointLayout<F>::XYZZ_WORDS;
-    const uint32_t wb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (wb >= total_buckets) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total[TOTAL_SUM_LISTED]) return;
+    const uint32_t wb = order[i];
     const uint32_t nt = (counts[wb] + task_points - 1) / task_points;
-    if (nt < 2 || nt > HEAVY_TASKS) return;
     const uint32_t t0 = offsets[wb].y;
     Xyzz<F> s = Mem<F>::load_xyzz(partials + size_t(t0) * XW);
     for (uint32_t k = 1; k < nt; ++k) s = xyzz_add<F>(s, Mem<F>::load_xyzz(partials + size_t(t0 + k) * XW));
@@ -1071,7 +1113,7 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
 // scratch area alike.
 struct MsmBufferBytes {
     size_t counts, offsets, digits, ranks, sorted, block_sums, total, tasks, partials, heavy, chunks, hist, descs,
-        windows;
+        windows, order;
 };
 
 inline MsmBufferBytes msm_buffer_bytes(const MsmBatchPlan& plan, int xw) {
@@ -1092,6 +1134,7 @@ inline MsmBufferBytes msm_buffer_bytes(const MsmBatchPlan& plan, int xw) {
     b.hist = plan.lds_digits ? size_t(plan.total_hist) * 4 : 0;
     b.descs = plan.descs.size() * sizeof(MsmDesc);
     b.windows = size_t(plan.total_windows) * size_t(xw) * 4;
+    b.order = size_t(plan.total_buckets) * 4;
     return b;
 }
 
@@ -1104,7 +1147,7 @@ inline size_t msm_workspace_borrow_impl(const MsmBatchPlan& plan, int xw, uint8_
         {&ws.counts, b.counts},   {&ws.offsets, b.offsets}, {&ws.digits, b.digits},   {&ws.ranks, b.ranks},
         {&ws.sorted, b.sorted},   {&ws.block_sums, b.block_sums}, {&ws.total, b.total}, {&ws.tasks, b.tasks},
         {&ws.partials, b.partials}, {&ws.heavy, b.heavy},   {&ws.chunks, b.chunks},   {&ws.hist, b.hist},
-        {&ws.descs, b.descs},     {&ws.windows, b.windows}};
+        {&ws.descs, b.descs},     {&ws.windows, b.windows}, {&ws.order, b.order}};
     size_t used = 0;
     for (const auto& p : parts) used += (std::max<size_t>(p.second, 1) + 255) & ~size_t(255);
     require(used <= capacity, "MSM batch needs more scratch than the allocation-free entry point holds");
@@ -1133,6 +1176,7 @@ inline void msm_batch_prepare_impl(const MsmBatchPlan& plan, int xw, MsmWorkspac
     ws.tasks.ensure(b.tasks);
     ws.partials.ensure(b.partials);
     ws.heavy.ensure(b.heavy);
+    ws.order.ensure(b.order);
     ws.chunks.ensure(b.chunks);
     if (plan.lds_digits) ws.hist.ensure(b.hist);
     MsmDesc* dev = ws.descs.as<MsmDesc>(m);
@@ -1156,7 +1200,7 @@ inline void msm_batch_prepare_impl(const MsmBatchPlan& plan, int xw, MsmWorkspac
 // Buffers of a prepared plan, checked once.
 struct MsmBuffers {
     const MsmDesc* descs;
-    uint32_t *counts, *digits, *ranks, *sorted, *total, *partials, *heavy, *chunk_vals, *hist;
+    uint32_t *counts, *digits, *ranks, *sorted, *total, *partials, *heavy, *chunk_vals, *hist, *order;
     uint2 *offsets, *tile_sums;
     uint4* tasks;
     uint32_t m, tiles;
@@ -1169,6 +1213,7 @@ inline MsmBuffers msm_buffers(const MsmBatchPlan& plan, MsmWorkspace& ws, int xw
     // The buffers were sized by msm_batch_prepare; check instead of growing.
     require(ws.counts.bytes >= size_t(plan.total_buckets) * 4 && ws.sorted.bytes >= size_t(plan.total_entries) * 4 &&
                 ws.total.bytes >= size_t(TOTAL_WORDS) * 4 && ws.tasks.bytes >= plan.max_tasks * sizeof(uint4) &&
+                ws.order.bytes >= size_t(plan.total_buckets) * 4 &&
                 ws.partials.bytes >= plan.max_tasks * xw * 4 && ws.chunks.bytes >= size_t(plan.total_chunks) * xw * 4 &&
                 ws.chunks.bytes >= size_t(plan.total_planes) * plan.plane_slices * xw * 4 &&
                 ws.chunks.bytes >= size_t(plan.total_chunks) * (plan.chunk_planes ? 2 : 1) * xw * 4 &&
@@ -1187,6 +1232,7 @@ inline MsmBuffers msm_buffers(const MsmBatchPlan& plan, MsmWorkspace& ws, int xw
     b.tasks = static_cast<uint4*>(ws.tasks.ptr);
     b.partials = static_cast<uint32_t*>(ws.partials.ptr);
     b.heavy = static_cast<uint32_t*>(ws.heavy.ptr);
+    b.order = static_cast<uint32_t*>(ws.order.ptr);
     b.chunk_vals = static_cast<uint32_t*>(ws.chunks.ptr);
     b.hist = plan.lds_digits ? static_cast<uint32_t*>(ws.hist.ptr) : nullptr;
     return b;
@@ -1236,7 +1282,7 @@ inline void msm_sort_stage(const uint32_t* point_index, const uint32_t* scalars,
     HIP_CHECK_LAUNCH();
     msm_mark(ev, k0 + 3, stream);
     k_tasks<<<task_blocks, TASK_THREADS, 0, stream>>>(b.counts, b.offsets, plan.total_buckets, tp, per, b.tasks,
-                                                      b.heavy, b.total);
+                                                      b.heavy, b.order, b.total);
     HIP_CHECK_LAUNCH();
     msm_mark(ev, k0 + 4, stream);
 }
@@ -1257,7 +1303,7 @@ void msm_reduce_stage(const MsmBatchPlan& plan, const MsmBuffers& b, hipStream_t
     const uint32_t m = b.m, tp = plan.task_points;
     k_heavy<F><<<unsigned(std::min<size_t>(plan.max_heavy, 1024)), 256, 0, stream>>>(b.heavy, b.total + 2, b.counts,
                                                                                      b.offsets, tp, b.partials);
-    k_bucket_sums<F><<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(b.counts, b.offsets, plan.total_buckets,
+    k_bucket_sums<F><<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(b.counts, b.offsets, b.order, b.total,
                                                                            tp, b.partials);
     if (plan.planes) {
         k_plane_sums<F><<<plan.total_planes * plan.plane_slices, PLANE_THREADS, 0, stream>>>(
