@@ -135,10 +135,31 @@ HD uint32_t xcd_order(uint32_t b, uint32_t blocks) {
     return (blocks & 7) ? b : (b & 7) * (blocks >> 3) + (b >> 3);
 }
 
-// One radix-2 round at local stage s.
+HD bool unit2(const NttPass& p, int s) { return p.mode != PASS_COLS && s == 0; }
+
+HD Fr tw2_of(const NttPass& p, uint32_t t, int s, uint32_t first) {
+    const uint32_t u = t & ((1u << p.logU) - 1), q = t >> p.logU, j = q & ((1u << s) - 1);
+    return unit2(p, s) ? fp_zero<FrParams>() : twiddle(p, s, j, first + u);
+}
+
+// Radix-2 butterfly t at local stage s with its twiddle w given (unused at
+// unit stages).
+HD void bfly2(uint32_t* __restrict__ lds, const NttPass& p, uint32_t t, int s, const Fr& w) {
+    const uint32_t logU = p.logU, h = 1u << s;
+    const uint32_t u = t & ((1u << logU) - 1), q = t >> logU;
+    const uint32_t j = q & (h - 1);
+    const uint32_t m1 = ((q >> s) << (s + 1)) | j;
+    const uint32_t e1 = (m1 << logU) + u, e2 = e1 + (h << logU);
+    const Fr x = lds_ld(lds, e1), y = lds_ld(lds, e2);
+    lds_st(lds, e1, reduce(norm(add_w(x, y))));
+    lds_st(lds, e2, unit2(p, s) ? reduce(norm(sub_w(x, y))) : mul(sub_w(x, y), w));
+}
+
+// One radix-2 round at local stage s (each butterfly loads its twiddle after
+// its LDS reads).
 HD void round2(uint32_t* __restrict__ lds, const NttPass& p, uint32_t tile, uint32_t first, int s) {
     const uint32_t logU = p.logU, h = 1u << s;
-    const bool unit = p.mode != PASS_COLS && s == 0;
+    const bool unit = unit2(p, s);
     for (uint32_t t = NTT_TID; t < (tile >> 1); t += NTT_BDIM) {
         const uint32_t u = t & ((1u << logU) - 1), q = t >> logU;
         const uint32_t j = q & (h - 1);
@@ -153,7 +174,9 @@ HD void round2(uint32_t* __restrict__ lds, const NttPass& p, uint32_t tile, uint
 
 // One radix-4 round over local stages s and s - 1 (s >= 1): the group at
 // positions m0 + {0, h2, 2 h2, 3 h2}, h2 = 2^(s-1).  Inputs < 2m; the two
-// sums of stage s stay unreduced (< 4m) into stage s - 1.
+// sums of stage s stay unreduced (< 4m) into stage s - 1.  (The twiddles are
+// loaded after the group's LDS reads: loading them first keeps them live
+// longer, and the extra registers cost the pass occupancy.)
 HD void round4(uint32_t* __restrict__ lds, const NttPass& p, uint32_t tile, uint32_t first, int s) {
     const uint32_t logU = p.logU, h2 = 1u << (s - 1);
     const bool unit = p.mode != PASS_COLS && s == 1;  // omega_4^0 and omega_2^0
@@ -192,7 +215,10 @@ HD void round4(uint32_t* __restrict__ lds, const NttPass& p, uint32_t tile, uint
 }
 
 // One tile (logical block lb) of pass p: load, DIF rounds, un-bit-reversing
-// store (with the optional scale), fully reduced.
+// store (with the optional scale), fully reduced.  On the device, when every
+// thread owns one radix-2 butterfly per round, each round's twiddle is loaded
+// during the round before (the first round's during the tile load), so its
+// latency overlaps work.
 template <bool R4>
 HD void ntt_tile(uint32_t* __restrict__ lds, const NttPass& p, uint32_t lb) {
     const uint32_t S = p.S, logU = p.logU, U = 1u << logU;
@@ -206,6 +232,18 @@ HD void ntt_tile(uint32_t* __restrict__ lds, const NttPass& p, uint32_t lb) {
         base = size_t(tr) << p.L;
     }
     const uint32_t first = g << logU;  // first row, column or k1 of the tile
+    int s = int(S) - 1;
+    // Device, radix-2, one butterfly per thread and round: prefetched
+    // twiddles.  (Radix-4 rounds keep loading their own: the three extra
+    // twiddles in flight cost the radix-4 pass more occupancy than the
+    // prefetch saves, measured on MI355X.)
+#if defined(__HIP_DEVICE_COMPILE__)
+    const bool one = !R4 && NTT_BDIM == (tile >> 1) && s >= 0;
+#else
+    const bool one = false;
+#endif
+    Fr w2;
+    if (one) w2 = tw2_of(p, NTT_TID, s, first);
 
     for (uint32_t i = NTT_TID; i < tile; i += NTT_BDIM) {
         uint32_t u, pos;
@@ -227,8 +265,14 @@ HD void ntt_tile(uint32_t* __restrict__ lds, const NttPass& p, uint32_t lb) {
     }
     NTT_SYNC();
 
-    int s = int(S) - 1;
-    if (R4) {
+    if (one) {
+        for (; s >= 0; --s) {
+            const Fr cur = w2;
+            if (s >= 1) w2 = tw2_of(p, NTT_TID, s - 1, first);
+            bfly2(lds, p, NTT_TID, s, cur);
+            NTT_SYNC();
+        }
+    } else if (R4) {
         if (S & 1) round2(lds, p, tile, first, s--);
         for (; s >= 1; s -= 2) round4(lds, p, tile, first, s);
     } else {
