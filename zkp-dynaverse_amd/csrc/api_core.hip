@@ -26,6 +26,21 @@ Owner::Owner() {
     stage_a.quota = stage_b.quota = stage_c.quota = &quota;
 }
 
+hipEvent_t* Owner::msm_phase_events() {
+    if (!msm_phase[0]) {
+        hipEvent_t made[MSM_PHASE_EVENTS] = {};
+        for (int k = 0; k < MSM_PHASE_EVENTS; ++k) {
+            const hipError_t st = hipEventCreate(&made[k]);
+            if (st != hipSuccess) {
+                for (int j = 0; j < k; ++j) (void)hipEventDestroy(made[j]);
+                hip_check(st, "hipEventCreate");
+            }
+        }
+        for (int k = 0; k < MSM_PHASE_EVENTS; ++k) msm_phase[k] = made[k];
+    }
+    return msm_phase;
+}
+
 void Owner::release_device() noexcept {
     if (resident) {
         retired_resident_stats = resident_stats(resident);
@@ -38,6 +53,10 @@ void Owner::release_device() noexcept {
         setup = nullptr;
     }
     msm.release();
+    for (hipEvent_t& e : msm_phase) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
     ntt.release();
     ntt_batch.release();
     ntt_batch_inv.release();

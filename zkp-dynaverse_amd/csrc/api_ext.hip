@@ -23,10 +23,10 @@ int msm_device(Owner* o, const void* points, const void* scalars, size_t n, int 
     constexpr int XW = PointLayout<F>::XYZZ_WORDS;
     MsmConfig cfg = o->msm_config;
     cfg.window_bits = window_bits ? window_bits : msm_window_for(n, &o->window_table);
-    Events ev(phase_ms != nullptr);
+    hipEvent_t* ev = phase_ms ? o->msm_phase_events() : nullptr;
     const MsmPlan p0 = msm_plan(n, cfg);
     uint32_t* window_words = o->msm.windows.as<uint32_t>(p0.outputs * XW);
-    o->msm.phase_events = phase_ms ? ev.e : nullptr;
+    o->msm.phase_events = ev;
     MsmPlan plan;
     try {
         plan = msm_enqueue<F>(static_cast<const uint32_t*>(points), static_cast<const uint32_t*>(scalars),
@@ -41,7 +41,7 @@ int msm_device(Owner* o, const void* points, const void* scalars, size_t n, int 
     HIP_CHECK(hipStreamSynchronize(o->stream));
     if (phase_ms)
         for (int k = 0; k + 1 < MSM_PHASE_EVENTS; ++k)
-            HIP_CHECK(hipEventElapsedTime(&phase_ms[k], ev.e[k], ev.e[k + 1]));
+            HIP_CHECK(hipEventElapsedTime(&phase_ms[k], ev[k], ev[k + 1]));
     affine_to_bytes<F>(msm_combine_windows<F>(host, plan), result);
     return 0;
 }

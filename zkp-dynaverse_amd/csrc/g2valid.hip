@@ -56,6 +56,44 @@ __device__ uint32_t parse_wire(const uint8_t* wire, size_t index, Affine<Fq2Ops>
     return 0;
 }
 
+// Non-adjacent form of the BN seed x, least significant digit first: 24
+// nonzero digits where the binary form has 28, so the [x]P ladder below
+// does 24 mixed additions (P or -P, both affine) after its doublings.
+struct SeedNaf {
+    int8_t d[66];
+    int len;
+};
+
+constexpr SeedNaf seed_naf() {
+    SeedNaf n{};
+    uint64_t k = BN_SEED;
+    unsigned __int128 kk = k;
+    while (kk) {
+        int8_t v = 0;
+        if (kk & 1) {
+            v = (kk & 3) == 1 ? 1 : -1;
+            kk = v > 0 ? kk - 1 : kk + 1;
+        }
+        n.d[n.len++] = v;
+        kk >>= 1;
+    }
+    return n;
+}
+
+constexpr SeedNaf SEED_NAF = seed_naf();
+
+// [x] P for an affine P by a left-to-right ladder over the NAF digits.
+__device__ G2 mul_seed(const Affine<Fq2Ops>& a) {
+    const Fq2 ny = f2_neg(a.y);
+    G2 r = xyzz_inf<Fq2Ops>();
+    for (int i = SEED_NAF.len - 1; i >= 0; --i) {
+        r = xyzz_dbl<Fq2Ops>(r);
+        const int8_t v = SEED_NAF.d[i];
+        if (v) r = xyzz_madd<Fq2Ops>(r, a.x, v > 0 ? a.y : ny);
+    }
+    return r;
+}
+
 __device__ bool in_subgroup(const Affine<Fq2Ops>& a, uint32_t algorithm) {
     const G2 P = xyzz_from_affine<Fq2Ops>(a);
     if (algorithm == 1) {
@@ -70,7 +108,7 @@ __device__ bool in_subgroup(const Affine<Fq2Ops>& a, uint32_t algorithm) {
         for (int k = 0; k < 4; ++k) lam[k] = G2_LAMBDA[k];
         return xyzz_eq<Fq2Ops>(psi(P), xyzz_mul_words<Fq2Ops, 4>(P, lam, 127));
     }
-    const G2 A = xyzz_mul_u64<Fq2Ops>(P, BN_SEED);
+    const G2 A = mul_seed(a);
     const G2 Bp = psi(A);
     const G2 C = psi(Bp);
     const G2 lhs = xyzz_add<Fq2Ops>(xyzz_add<Fq2Ops>(xyzz_add<Fq2Ops>(A, P), Bp), C);

@@ -71,6 +71,10 @@ struct Owner {
     // NTT stages per pass live in `domain`, the task shape in msm_config.
     WindowTable window_table;
     int msm_finalize_min = 0;  // 0 = built-in threshold for device-side window sums
+    // Phase events of the phase-timed device MSM, made on its first call
+    // (created together or not at all) and released with the device state.
+    hipEvent_t msm_phase[MSM_PHASE_EVENTS] = {};
+    hipEvent_t* msm_phase_events();
 
     Owner();
     ~Owner();
