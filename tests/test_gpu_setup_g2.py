@@ -142,3 +142,37 @@ def test_g2_config_resources_and_psi_probe(backend):
         assert (o.f2_mul(X, zi), o.f2_mul(Y, zi)) == p
     with pytest.raises(BackendError):
         backend.g2_psi_probe(bytes(191) + b"\x01", 1)
+
+
+@pytest.mark.parametrize("schedule,layout,fused", [(s, l, f) for s in (0, 1) for l in (0, 1) for f in (0, 1)])
+def test_g2_kernel_config_variants(backend, schedule, layout, fused):
+    """Every kernel configuration (the reference's schedule / layout / fused
+    knobs, icicle_g2_validation.cu:117-142) gives the same masks and reports:
+    encoding and curve corpus, subgroup / coset corpus, and a rejected
+    checked fixed-base chunk; the repack is counted when layout is 1."""
+    threads = (64, 128, 256)[(schedule + 2 * layout + fused) % 3]
+    print(f"case: g2 config threads={threads} schedule={schedule} layout={layout} fused={fused}", flush=True)
+    backend.set_g2_kernel_config(threads=threads, schedule=schedule, layout=layout, fused=fused)
+    try:
+        cases = encoding_cases()
+        wire = b"".join(w for w, _ in cases)
+        before = backend.g2_experiment_stats()["logical_repack_bytes"]
+        masks, rep = backend.inspect_g2_validation(wire, 3)
+        assert masks == [m for _, m in cases]
+        ref_masks, ref_rep = o.g2_masks(wire, 3)
+        assert masks == ref_masks and rep == ref_rep
+        after = backend.g2_experiment_stats()["logical_repack_bytes"]
+        assert after - before == (len(cases) * 256 if layout else 0)
+        sc = ChaCha20Stream(40 + schedule).uniform_fr(12)
+        wire, exp = corpus(sc)
+        for alg in (1, 2, 3):
+            masks, rep = backend.inspect_g2_validation(wire, alg)
+            assert masks == exp, alg
+        assert backend.g2_test_corrupt_next(3) == 0
+        raw, rep = backend.fixed_base_g2_checked_encoded(b"".join(o.fr_to_bytes(s) for s in sc), 12, 3)
+        assert raw is None and rep["invalid_points"] == 1 and rep["first_invalid"] == 3
+        raw, rep = backend.fixed_base_g2_checked_encoded(b"".join(o.fr_to_bytes(s) for s in sc[:5]), 5, 3)
+        assert rep["invalid_points"] == 0
+        assert [raw[128 * i:128 * i + 128] for i in range(5)] == [o.g2_to_bytes(o.g2_mul(s)) for s in sc[:5]]
+    finally:
+        backend.set_g2_kernel_config(threads=128)

@@ -69,8 +69,9 @@ struct BatchObj {
             for (const auto& g : ntt_groups) ntt_batch_enqueue(domain, *g, side);
             if (wire_n) {
                 auto* rep = static_cast<DvgpuG2ValidationReport*>(report_dev.ptr);
-                g2_validate_enqueue(wire, wire_n, algorithm, masks, rep, owner->g2_kernel.threads, side, nullptr,
-                                    nullptr);
+                G2CheckLaunch launch;  // the batch holds no repack scratch: records stay in place
+                launch.config = owner->g2_kernel;
+                g2_validate_enqueue(wire, wire_n, algorithm, masks, rep, launch, side);
                 HIP_CHECK(hipMemcpyAsync(report_host.ptr, rep, sizeof(DvgpuG2ValidationReport),
                                          hipMemcpyDeviceToHost, side));
             }

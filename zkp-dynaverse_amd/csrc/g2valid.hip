@@ -23,17 +23,9 @@ namespace {
 
 using G2 = Xyzz<Fq2Ops>;
 
-__device__ uint32_t parse_wire(const uint8_t* wire, size_t index, Affine<Fq2Ops>& p) {
-    const uint4* q = reinterpret_cast<const uint4*>(wire + index * 128);
-    uint32_t w[32];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint4 v = q[k];
-        w[4 * k] = v.x;
-        w[4 * k + 1] = v.y;
-        w[4 * k + 2] = v.z;
-        w[4 * k + 3] = v.w;
-    }
+// Status of one 128-byte record given as 32 LE words (0 valid or infinity,
+// 1 encoding error); p gets the point in Montgomery form.
+__device__ uint32_t parse_words(uint32_t (&w)[32], Affine<Fq2Ops>& p) {
     uint32_t any = 0;
 #pragma unroll
     for (int k = 0; k < 32; ++k) any |= w[k];
@@ -56,6 +48,20 @@ __device__ uint32_t parse_wire(const uint8_t* wire, size_t index, Affine<Fq2Ops>
     return 0;
 }
 
+__device__ uint32_t parse_wire(const uint8_t* wire, size_t index, Affine<Fq2Ops>& p) {
+    const uint4* q = reinterpret_cast<const uint4*>(wire + index * 128);
+    uint32_t w[32];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint4 v = q[k];
+        w[4 * k] = v.x;
+        w[4 * k + 1] = v.y;
+        w[4 * k + 2] = v.z;
+        w[4 * k + 3] = v.w;
+    }
+    return parse_words(w, p);
+}
+
 // Non-adjacent form of the BN seed x, least significant digit first: 24
 // nonzero digits where the binary form has 28, so the [x]P ladder below
 // does 24 mixed additions (P or -P, both affine) after its doublings.
@@ -66,8 +72,7 @@ struct SeedNaf {
 
 constexpr SeedNaf seed_naf() {
     SeedNaf n{};
-    uint64_t k = BN_SEED;
-    unsigned __int128 kk = k;
+    unsigned __int128 kk = BN_SEED;
     while (kk) {
         int8_t v = 0;
         if (kk & 1) {
@@ -82,37 +87,116 @@ constexpr SeedNaf seed_naf() {
 
 constexpr SeedNaf SEED_NAF = seed_naf();
 
+// ---- lane teams --------------------------------------------------------------
+//
+// One subgroup check is a serial chain of about 90 G2 operations, so with
+// one lane per point a call of 4096 points keeps 64 waves busy and the chain
+// sets the time.  TeamFq2Ops runs a point's chain on a team of four
+// consecutive lanes: every lane holds the same operands and follows the same
+// control flow, and the three Fq products of a Karatsuba Fq2 product (two
+// for a square) run on different lanes at once and are exchanged by lane
+// shuffles.  Additions are computed redundantly on every lane of the team.
+constexpr uint32_t TEAM = 4;
+
+__device__ __forceinline__ Fq team_take(const Fq& v, int lane) {
+    Fq r;
+#pragma unroll
+    for (int k = 0; k < LIMBS; ++k) r.l[k] = __shfl(v.l[k], lane, 64);
+    return r;
+}
+
+struct TeamFq2Ops : Fq2Ops {
+    static __device__ E mul(const E& a, const E& b) {
+        const uint32_t role = threadIdx.x & (TEAM - 1);
+        const int base = int(threadIdx.x & 63u & ~(TEAM - 1));
+        // role 0: a0 b0, 1: a1 b1, 2 (and 3, unused): (a0 + a1)(b0 + b1)
+        Fq x = add_w(a.c0, a.c1), y = norm(add_w(b.c0, b.c1));
+        if (role == 0) {
+            x = a.c0;
+            y = b.c0;
+        } else if (role == 1) {
+            x = a.c1;
+            y = b.c1;
+        }
+        const Fq p = bn254::mul(x, y);
+        const Fq t0 = team_take(p, base), t1 = team_take(p, base + 1), t2 = team_take(p, base + 2);
+        return {bn254::sub(t0, t1), reduce(sub2_n(t2, t0, t1))};
+    }
+    static __device__ E sqr(const E& a) {
+        const uint32_t role = threadIdx.x & (TEAM - 1);
+        const int base = int(threadIdx.x & 63u & ~(TEAM - 1));
+        // role 0: (a0 + a1)(a0 - a1), others: a0 a1
+        Fq x = a.c0, y = a.c1;
+        if (role == 0) {
+            x = add_w(a.c0, a.c1);
+            y = norm(sub_w(a.c0, a.c1));
+        }
+        const Fq p = bn254::mul(x, y);
+        const Fq s = team_take(p, base), t = team_take(p, base + 1);
+        return {s, bn254::add(t, t)};
+    }
+};
+
+template <class F>
+__device__ Xyzz<F> psi_of(const Xyzz<F>& p) {
+    return {F::mul(f2_conj(p.X), psi_cx()), F::mul(f2_conj(p.Y), psi_cy()), f2_conj(p.ZZ), f2_conj(p.ZZZ)};
+}
+
 // [x] P for an affine P by a left-to-right ladder over the NAF digits.
-__device__ G2 mul_seed(const Affine<Fq2Ops>& a) {
-    const Fq2 ny = f2_neg(a.y);
-    G2 r = xyzz_inf<Fq2Ops>();
+template <class F>
+__device__ Xyzz<F> mul_seed(const Fq2& x, const Fq2& y) {
+    const Fq2 ny = f2_neg(y);
+    Xyzz<F> r = xyzz_inf<F>();
     for (int i = SEED_NAF.len - 1; i >= 0; --i) {
-        r = xyzz_dbl<Fq2Ops>(r);
+        r = xyzz_dbl<F>(r);
         const int8_t v = SEED_NAF.d[i];
-        if (v) r = xyzz_madd<Fq2Ops>(r, a.x, v > 0 ? a.y : ny);
+        if (v) r = xyzz_madd<F>(r, x, v > 0 ? y : ny);
     }
     return r;
 }
 
-__device__ bool in_subgroup(const Affine<Fq2Ops>& a, uint32_t algorithm) {
-    const G2 P = xyzz_from_affine<Fq2Ops>(a);
+// compact: the schedule that keeps one running sum and one image live
+// (the reference's second schedule) instead of [x]P and both images.
+template <class F>
+__device__ bool in_subgroup(const Affine<Fq2Ops>& a, uint32_t algorithm, bool compact) {
+    const Xyzz<F> P = {a.x, a.y, f2_one(), f2_one()};
     if (algorithm == 1) {
         uint32_t r[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) r[k] = FR_MOD32[k];
-        return xyzz_is_inf<Fq2Ops>(xyzz_mul_words<Fq2Ops, 8>(P, r, 254));
+        return xyzz_is_inf<F>(xyzz_mul_words<F, 8>(P, r, 254));
     }
     if (algorithm == 2) {
         uint32_t lam[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) lam[k] = G2_LAMBDA[k];
-        return xyzz_eq<Fq2Ops>(psi(P), xyzz_mul_words<Fq2Ops, 4>(P, lam, 127));
+        return xyzz_eq<F>(psi_of<F>(P), xyzz_mul_words<F, 4>(P, lam, 127));
     }
-    const G2 A = mul_seed(a);
-    const G2 Bp = psi(A);
-    const G2 C = psi(Bp);
-    const G2 lhs = xyzz_add<Fq2Ops>(xyzz_add<Fq2Ops>(xyzz_add<Fq2Ops>(A, P), Bp), C);
-    return xyzz_eq<Fq2Ops>(lhs, xyzz_dbl<Fq2Ops>(psi(C)));
+    if (compact) {
+        Xyzz<F> t = mul_seed<F>(a.x, a.y);
+        Xyzz<F> sum = xyzz_madd<F>(t, a.x, a.y);
+        t = psi_of<F>(t);
+        sum = xyzz_add<F>(sum, t);
+        t = psi_of<F>(t);
+        sum = xyzz_add<F>(sum, t);
+        return xyzz_eq<F>(sum, xyzz_dbl<F>(psi_of<F>(t)));
+    }
+    const Xyzz<F> A = mul_seed<F>(a.x, a.y);
+    const Xyzz<F> Bp = psi_of<F>(A);
+    const Xyzz<F> C = psi_of<F>(Bp);
+    const Xyzz<F> lhs = xyzz_add<F>(xyzz_add<F>(xyzz_madd<F>(A, a.x, a.y), Bp), C);
+    return xyzz_eq<F>(lhs, xyzz_dbl<F>(psi_of<F>(C)));
+}
+
+// Point i from the wire (128-byte records) or from word planes (word w of
+// point i at planes[w n + i], the repacked layout).
+__device__ uint32_t parse_point(const uint8_t* input, size_t n, size_t i, bool soa, Affine<Fq2Ops>& p) {
+    if (!soa) return parse_wire(input, i, p);
+    const uint32_t* planes = reinterpret_cast<const uint32_t*>(input);
+    uint32_t w[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) w[k] = planes[size_t(k) * n + i];
+    return parse_words(w, p);
 }
 
 __global__ void k_parse_curve(const uint8_t* __restrict__ wire, size_t n, uint32_t* __restrict__ masks) {
@@ -123,14 +207,33 @@ __global__ void k_parse_curve(const uint8_t* __restrict__ wire, size_t n, uint32
     masks[i] = status ? status : (affine_on_curve<Fq2Ops>(p) ? 0u : 2u);
 }
 
-__global__ void k_subgroup(const uint8_t* __restrict__ wire, size_t n, uint32_t algorithm,
+// One team of TEAM lanes per point (i = thread / TEAM); fused: the encoding
+// and curve checks run here too (no k_parse_curve pass).
+// (At most 256 threads per block, the largest configuration: the bound
+// lets a point's chain keep its operands in registers.)
+template <bool FUSED>
+__global__ void __launch_bounds__(256) k_subgroup(const uint8_t* __restrict__ input, size_t n, uint32_t algorithm, bool soa, bool compact,
                            uint32_t* __restrict__ masks) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n || masks[i]) return;
+    const size_t i = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) / TEAM;
+    if (i >= n) return;
+    const bool writer = (threadIdx.x & (TEAM - 1)) == 0;
+    if (!FUSED && masks[i]) return;
     Affine<Fq2Ops> p;
-    parse_wire(wire, i, p);
+    const uint32_t status = parse_point(input, n, i, soa, p);
+    if (FUSED) {
+        const uint32_t m = status ? status : (affine_on_curve<Fq2Ops>(p) ? 0u : 2u);
+        if (writer) masks[i] = m;
+        if (m) return;
+    }
     if (p.inf) return;
-    masks[i] = in_subgroup(p, algorithm) ? 0u : 4u;
+    const bool ok = in_subgroup<TeamFq2Ops>(p, algorithm, compact);
+    if (writer) masks[i] = ok ? 0u : 4u;
+}
+
+// Word planes of n 128-byte records: planes[w n + i] = word w of record i.
+__global__ void k_repack(const uint32_t* __restrict__ wire, uint32_t* __restrict__ planes, size_t n) {
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n * 32) planes[(t % 32) * n + t / 32] = wire[t];
 }
 
 __global__ void k_reduce_status(const uint32_t* __restrict__ masks, size_t n, uint32_t algorithm,
@@ -174,7 +277,7 @@ __global__ void k_reduce_status(const uint32_t* __restrict__ masks, size_t n, ui
 __global__ void k_corrupt(uint8_t* wire, size_t index) { wire[index * 128] ^= 1u; }
 
 // psi images of homogeneous (X : Y : Z) points, canonical 192-byte records.
-__global__ void k_psi_probe(const uint8_t* __restrict__ in, size_t n, uint8_t* __restrict__ out,
+__global__ void __launch_bounds__(64) k_psi_probe(const uint8_t* __restrict__ in, size_t n, uint8_t* __restrict__ out,
                             uint32_t* __restrict__ masks) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -215,14 +318,29 @@ __global__ void k_psi_probe(const uint8_t* __restrict__ in, size_t n, uint8_t* _
 }  // namespace
 
 void g2_validate_enqueue(const uint8_t* wire, size_t n, uint32_t algorithm, uint32_t* masks,
-                         DvgpuG2ValidationReport* report, unsigned threads, hipStream_t stream,
-                         hipEvent_t parsed, hipEvent_t subgroup_done) {
-    k_parse_curve<<<grid_for(n, threads), threads, 0, stream>>>(wire, n, masks);
+                         DvgpuG2ValidationReport* report, const G2CheckLaunch& l, hipStream_t stream) {
+    const unsigned threads = l.config.threads;
+    const bool fused = l.config.fused != 0, soa = l.config.layout != 0 && l.scratch;
+    if (!fused) {
+        k_parse_curve<<<grid_for(n, threads), threads, 0, stream>>>(wire, n, masks);
+        HIP_CHECK_LAUNCH();
+    }
+    if (l.parsed) HIP_CHECK(hipEventRecord(l.parsed, stream));
+    const uint8_t* input = wire;
+    if (soa) {
+        k_repack<<<grid_for(n * 32, 256), 256, 0, stream>>>(reinterpret_cast<const uint32_t*>(wire),
+                                                          reinterpret_cast<uint32_t*>(l.scratch), n);
+        HIP_CHECK_LAUNCH();
+        input = l.scratch;
+    }
+    if (l.repacked) HIP_CHECK(hipEventRecord(l.repacked, stream));
+    const bool compact = l.config.schedule != 0;
+    if (fused)
+        k_subgroup<true><<<grid_for(n * TEAM, threads), threads, 0, stream>>>(input, n, algorithm, soa, compact, masks);
+    else
+        k_subgroup<false><<<grid_for(n * TEAM, threads), threads, 0, stream>>>(input, n, algorithm, soa, compact, masks);
     HIP_CHECK_LAUNCH();
-    if (parsed) HIP_CHECK(hipEventRecord(parsed, stream));
-    k_subgroup<<<grid_for(n, threads), threads, 0, stream>>>(wire, n, algorithm, masks);
-    HIP_CHECK_LAUNCH();
-    if (subgroup_done) HIP_CHECK(hipEventRecord(subgroup_done, stream));
+    if (l.subgroup_done) HIP_CHECK(hipEventRecord(l.subgroup_done, stream));
     k_reduce_status<<<1, 256, 0, stream>>>(masks, n, algorithm, report);
     HIP_CHECK_LAUNCH();
 }
@@ -239,9 +357,9 @@ void g2_psi_probe_enqueue(const uint8_t* points, size_t n, uint8_t* out, uint32_
 
 void g2_kernel_attributes(uint32_t, unsigned threads, DvgpuG2KernelResources& out) {
     hipFuncAttributes a{};
-    HIP_CHECK(hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_subgroup)));
+    HIP_CHECK(hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_subgroup<false>)));
     int blocks = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(k_subgroup),
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(k_subgroup<false>),
                                                            int(threads), 0));
     int device = 0, maximum = 0;
     HIP_CHECK(hipGetDevice(&device));
@@ -281,7 +399,7 @@ SetupState& setup_of(Owner* o) {
     return *o->setup;
 }
 
-using EventSet = ScopedEvents<8>;
+using EventSet = ScopedEvents<9>;  // 8: after the repack of a layout-1 validation
 
 // [s_i] G into host `output` (canonical affine); events 0..5 are the
 // table/H2D/generate/normalize/D2H boundaries of DvgpuSetupStats.device_ns.
@@ -310,7 +428,13 @@ void run_fixed_base(Owner* o, const uint8_t* scalars, size_t n, uint8_t* output,
         if (corrupt >= 0) g2_corrupt_point(dpts, size_t(corrupt), o->stream);
         uint32_t* masks = st.masks.as<uint32_t>(n);
         auto* rep = st.report.as<DvgpuG2ValidationReport>(1);
-        g2_validate_enqueue(dpts, n, algorithm, masks, rep, o->g2_kernel.threads, o->stream, ev.e[6], ev.e[7]);
+        G2CheckLaunch launch;
+        launch.config = o->g2_kernel;
+        if (launch.config.layout) launch.scratch = st.planes.as<uint8_t>(n * 128);
+        launch.parsed = ev.e[6];
+        launch.repacked = ev.e[8];
+        launch.subgroup_done = ev.e[7];
+        g2_validate_enqueue(dpts, n, algorithm, masks, rep, launch, o->stream);
         HIP_CHECK(hipMemcpyAsync(report_out, rep, sizeof(*rep), hipMemcpyDeviceToHost, o->stream));
         (void)dev_masks_out;
     }
@@ -393,6 +517,9 @@ int dvgpu_bn254_fixed_base_g2_checked(void* backend, const uint8_t* scalars, siz
         auto& s = o->g2_stats;
         s.parse_curve_device_ns += ev.ns(4, 6);
         s.subgroup_device_ns += ev.ns(6, 7);
+        o->g2_experiment.repack_device_ns += ev.ns(6, 8);
+        o->g2_experiment.subgroup_kernel_ns += ev.ns(8, 7);
+        o->g2_experiment.logical_repack_bytes += o->g2_kernel.layout ? uint64_t(n) * 256 : 0;
         s.d2h_device_ns += ev.ns(7, 5);
         s.native_wall_ns += wall;
         s.checked_points += n;
@@ -429,7 +556,13 @@ int dvgpu_bn254_g2_validation_inspect(void* backend, const uint8_t* wire, size_t
         HIP_CHECK(hipEventRecord(ev.e[0], o->stream));
         HIP_CHECK(hipMemcpyAsync(dwire, wire, n * 128, hipMemcpyHostToDevice, o->stream));
         HIP_CHECK(hipEventRecord(ev.e[1], o->stream));
-        g2_validate_enqueue(dwire, n, algorithm, dmasks, drep, o->g2_kernel.threads, o->stream, ev.e[2], ev.e[3]);
+        G2CheckLaunch launch;
+        launch.config = o->g2_kernel;
+        if (launch.config.layout) launch.scratch = st.planes.as<uint8_t>(n * 128);
+        launch.parsed = ev.e[2];
+        launch.repacked = ev.e[6];
+        launch.subgroup_done = ev.e[3];
+        g2_validate_enqueue(dwire, n, algorithm, dmasks, drep, launch, o->stream);
         HIP_CHECK(hipEventRecord(ev.e[4], o->stream));
         HIP_CHECK(hipMemcpyAsync(staged, dmasks, n * 4, hipMemcpyDeviceToHost, o->stream));
         HIP_CHECK(hipMemcpyAsync(staged + n * 4, drep, sizeof(DvgpuG2ValidationReport), hipMemcpyDeviceToHost,
@@ -444,7 +577,9 @@ int dvgpu_bn254_g2_validation_inspect(void* backend, const uint8_t* wire, size_t
         s.d2h_device_ns += ev.ns(4, 5);
         s.native_wall_ns += now_ns() - t0;
         s.checked_points += n;
-        o->g2_experiment.subgroup_kernel_ns += ev.ns(2, 3);
+        o->g2_experiment.repack_device_ns += ev.ns(2, 6);
+        o->g2_experiment.subgroup_kernel_ns += ev.ns(6, 3);
+        o->g2_experiment.logical_repack_bytes += o->g2_kernel.layout ? uint64_t(n) * 256 : 0;
         o->g2_traffic.point_h2d_bytes += n * 128;
         o->g2_traffic.status_d2h_bytes += n * 4 + sizeof(DvgpuG2ValidationReport);
         std::memcpy(masks, staged, n * 4);

@@ -147,7 +147,7 @@ __global__ void k_transpose_fr(const uint32_t* __restrict__ in, uint32_t* __rest
     stfr(out + (col * rows + row) * 8, v);
 }
 
-__global__ void k_transpose_g1(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, size_t rows,
+__global__ void __launch_bounds__(64) k_transpose_g1(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, size_t rows,
                                size_t cols, size_t start, size_t n, int twiddle, Fr root) {
     const size_t off = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (off >= n) return;
@@ -166,7 +166,7 @@ __global__ void k_zip(const uint32_t* __restrict__ a, const uint32_t* __restrict
     stfr(out + i * 8, canon(r));
 }
 
-__global__ void k_scale(const uint32_t* __restrict__ pts, const uint32_t* __restrict__ sc, uint32_t* __restrict__ out,
+__global__ void __launch_bounds__(64) k_scale(const uint32_t* __restrict__ pts, const uint32_t* __restrict__ sc, uint32_t* __restrict__ out,
                         size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;

@@ -13,9 +13,9 @@ struct SetupState {
     DvgpuSetupStats stats{};
     DeviceBuffer table_g1, table_g2;  // affine Montgomery words, 64 x 16 entries
     bool have_g1 = false, have_g2 = false;
-    DeviceBuffer scalars, points, masks, report;
+    DeviceBuffer scalars, points, masks, report, planes;  // planes: the repacked layout
     void attach(Quota* q) {
-        for (DeviceBuffer* b : {&table_g1, &table_g2, &scalars, &points, &masks, &report}) b->quota = q;
+        for (DeviceBuffer* b : {&table_g1, &table_g2, &scalars, &points, &masks, &report, &planes}) b->quota = q;
     }
 };
 
@@ -31,11 +31,22 @@ template <class F>
 void fixed_base_mont_enqueue(SetupState& st, const uint32_t* scalars, size_t n, uint32_t* out_mont,
                              hipStream_t stream);
 
+// How one validation runs: the owner's kernel configuration (threads per
+// block; schedule 1: the compact order of the seed predicate; layout 1: the
+// records repacked into word planes in `scratch` (n * 128 bytes) first, when
+// given; fused 1: the encoding and curve checks inside the subgroup kernel)
+// and optional events after the parse, the repack and the subgroup kernel.
+struct G2CheckLaunch {
+    DvgpuG2KernelConfig config{128, 0, 0, 0};
+    uint8_t* scratch = nullptr;
+    hipEvent_t parsed = nullptr, repacked = nullptr, subgroup_done = nullptr;
+};
+
 // G2 wire validation of n points (128-byte canonical wire) on the device:
 // per-point masks (1 encoding, 2 curve, 4 subgroup) and the 32-byte report.
+// Kernels only (capturable).
 void g2_validate_enqueue(const uint8_t* wire, size_t n, uint32_t algorithm, uint32_t* masks,
-                         DvgpuG2ValidationReport* report, unsigned threads, hipStream_t stream,
-                         hipEvent_t parsed, hipEvent_t subgroup_done);
+                         DvgpuG2ValidationReport* report, const G2CheckLaunch& launch, hipStream_t stream);
 
 // Flip bit 0 of byte 0 of wire point `index` (test-only corruption).
 void g2_corrupt_point(uint8_t* wire, size_t index, hipStream_t stream);
