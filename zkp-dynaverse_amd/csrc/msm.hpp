@@ -62,7 +62,7 @@ struct MsmItem {
 constexpr uint32_t MSM_NO_SLOT = 0xffffffffu;
 
 struct MsmDesc {
-    uint32_t n, c, windows, buckets;
+    uint32_t n, c, windows, buckets;  // c: the widest window's bits; buckets 2^(c-1) per window
     uint32_t point_base, scalar_base;
     uint32_t thread_base, entry_base, bucket_base, chunk_base, window_base;
     uint32_t out_index;    // position of this MSM's result in the caller's list
@@ -74,7 +74,10 @@ struct MsmDesc {
     uint32_t hist_base;        // first counter of its histograms (one bucket set per block)
     uint32_t chunk_shift;      // log2 of the points per histogram block
     uint32_t chunk;            // buckets per first-level reduction thread
-    uint32_t plane_base;       // first bit-plane sum of this MSM (plane mode)
+    uint32_t plane_base;       // first bit-plane sum of this MSM (plane mode)    // Windows below `wide` are c bits wide, the others c - 1 (the widths sum
+    // to at least 255, and the top window is not left with a few bits and
+    // as few buckets for all points).
+    uint32_t wide;
 };
 
 // Digits are ranked per block of 2^chunk_shift points and one window in LDS
@@ -182,6 +185,7 @@ void device_fill_words(uint32_t* p, size_t words, uint32_t value, hipStream_t st
 
 struct MsmPlan {
     int c, windows;
+    int wide = 0;  // windows below this index are c bits wide, the others c - 1
     uint32_t buckets;
     size_t n;
     int planes = 0;      // plane sums per window (the single-MSM path's window output)

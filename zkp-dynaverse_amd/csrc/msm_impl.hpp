@@ -111,6 +111,20 @@ struct Mem<Fq2Ops> {
 
 constexpr uint32_t NO_DIGIT = 0xffffffffu;
 
+// Width and lowest bit of window w (MsmDesc::wide).
+__host__ __device__ __forceinline__ uint32_t window_width(const MsmDesc& d, uint32_t w) {
+    return w < d.wide ? d.c : d.c - 1;
+}
+__host__ __device__ __forceinline__ uint32_t window_offset(const MsmDesc& d, uint32_t w) {
+    return w <= d.wide ? w * d.c : d.wide * d.c + (w - d.wide) * (d.c - 1);
+}
+
+// Full-width windows of W windows of c or c - 1 bits covering 255 bits.
+__host__ __device__ __forceinline__ uint32_t wide_windows(uint32_t c, uint32_t W) {
+    const int64_t wide = 255 - int64_t(W) * int64_t(c - 1);
+    return uint32_t(wide < 0 ? 0 : (wide > int64_t(W) ? int64_t(W) : wide));
+}
+
 // Accumulation tasks: every bucket of c points is cut into floor(c / T)
 // full tasks of T points and, when T does not divide c, one task of c mod T
 // points.  Tasks run longest first (class 0: the full tasks, class k: tasks
@@ -168,16 +182,17 @@ __device__ __forceinline__ void task_class_flush(uint32_t tp, const uint32_t* h,
 }
 
 // bits [bit, bit + c) of a 256-bit little-endian scalar (c <= 24)
-// Signed c-bit digits in [-(2^(c-1) - 1), 2^(c-1)] by one addition: with
-// K = sum_k (2^(c-1) - 1) 2^(kc), digit k of s is window k of t = s + K minus
-// (2^(c-1) - 1) (the representation is unique, so this equals the
-// carry-propagating recoding).  t needs nine words once W c exceeds 256.
-__device__ __forceinline__ void recode_constant(int c, uint32_t windows, uint32_t (&k)[9]) {
+// Signed digits in [-(2^(b-1) - 1), 2^(b-1)] (b the window's width) by one
+// addition: with K = sum_k (2^(b_k-1) - 1) 2^(o_k) (o_k the window's lowest
+// bit), digit k of s is window k of t = s + K minus (2^(b_k-1) - 1) (the
+// representation is unique, so this equals the carry-propagating
+// recoding).  t needs nine words once the windows cover more than 256 bits.
+__device__ __forceinline__ void recode_constant(const MsmDesc& d, uint32_t (&k)[9]) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) k[i] = 0;
-    const uint64_t low = (uint64_t(1) << (c - 1)) - 1;
-    for (uint32_t w = 0; w < windows; ++w) {
-        const uint32_t bit = w * uint32_t(c), q = bit >> 5, sh = bit & 31;
+    for (uint32_t w = 0; w < d.digit_windows; ++w) {
+        const uint64_t low = (uint64_t(1) << (window_width(d, w) - 1)) - 1;
+        const uint32_t bit = window_offset(d, w), q = bit >> 5, sh = bit & 31;
         const uint64_t v = low << sh;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -246,14 +261,13 @@ static __global__ void k_digits(const MsmDesc* __restrict__ descs, uint32_t m, u
     const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(d.scalar_base + i) * 8);
     const uint4 a = sp[0], b = sp[1];
     const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    const int c = int(d.c);
-    const uint32_t full = 1u << c;
     uint32_t carry = 0;
     for (uint32_t w = 0; w < d.digit_windows; ++w) {
-        uint32_t v = scalar_bits(s, int(w) * c, c) + carry;
+        const int c = int(window_width(d, w));
+        uint32_t v = scalar_bits(s, int(window_offset(d, w)), c) + carry;
         uint32_t sign = 0;
-        if (v > d.buckets) {
-            v = full - v;
+        if (v > (1u << (c - 1))) {
+            v = (1u << c) - v;
             sign = 1;
             carry = 1;
         } else {
@@ -306,10 +320,9 @@ static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
     const uint2* starts = offsets + d.bucket_base + (d.table ? 0u : w0) * d.buckets;
     for (uint32_t b = threadIdx.x; b < d.buckets; b += blockDim.x) counts[b] = PLACE ? mine[b] + starts[b].x : 0u;
     __syncthreads();
-    const int c = int(d.c);
     const uint32_t i0 = j << d.chunk_shift, i1 = min(d.n, i0 + (1u << d.chunk_shift));
     uint32_t kconst[9];
-    recode_constant(c, d.digit_windows, kconst);
+    recode_constant(d, kconst);
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(d.scalar_base + i) * 8);
         const uint4 a = sp[0], b = sp[1];
@@ -317,7 +330,8 @@ static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
         uint32_t t[9];
         recode_add(s, kconst, t);
         for (uint32_t w = w0; w < w1; ++w) {
-            const int32_t digit = int32_t(window_of(t, w * uint32_t(c), c)) - int32_t(d.buckets - 1);
+            const int c = int(window_width(d, w));
+            const int32_t digit = int32_t(window_of(t, window_offset(d, w), c)) - int32_t((1u << (c - 1)) - 1);
             if (digit == 0) continue;
             const uint32_t sign = digit < 0 ? 1u : 0u;
             const uint32_t v = uint32_t(digit < 0 ? -digit : digit);
@@ -959,7 +973,7 @@ __global__ void __launch_bounds__(64) k_finalize(const MsmDesc* __restrict__ des
     const uint32_t* base = windows + size_t(d.window_base) * XW;
     Xyzz<F> acc = Mem<F>::load_xyzz(base + size_t(d.windows - 1) * XW);
     for (int w = int(d.windows) - 2; w >= 0; --w) {
-        for (uint32_t k = 0; k < d.c; ++k) acc = xyzz_dbl<F>(acc);
+        for (uint32_t k = 0; k < window_width(d, uint32_t(w)); ++k) acc = xyzz_dbl<F>(acc);
         acc = xyzz_add<F>(acc, Mem<F>::load_xyzz(base + size_t(w) * XW));
     }
     const Affine<F> a = xyzz_to_affine<F>(acc);
@@ -1049,10 +1063,12 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
         d.c = uint32_t(c);
         d.windows = uint32_t((255 + c - 1) / c);
         d.digit_windows = d.windows;
+        d.wide = wide_windows(d.c, d.windows);
         if (it.table_windows) {
             require(it.table_windows == d.windows, "window-table rows must hold ceil(255 / c) points");
             d.table = 1;
             d.windows = 1;
+            d.wide = d.digit_windows;  // the tables hold multiples [2^(c w)] B: every window c bits
         }
         d.buckets = 1u << (c - 1);
         {
@@ -1420,7 +1436,7 @@ Affine<F> combine_one(const uint32_t* words, const MsmDesc& d) {
     const uint32_t* base = words + size_t(d.window_base) * XW;
     typename H::P acc = H::load(base + size_t(d.windows - 1) * XW);
     for (int w = int(d.windows) - 2; w >= 0; --w) {
-        acc = H::add(H::dbl_n(acc, int(d.c)), H::load(base + size_t(w) * XW));
+        acc = H::add(H::dbl_n(acc, int(window_width(d, uint32_t(w)))), H::load(base + size_t(w) * XW));
     }
     return H::affine(acc);
 }
@@ -1439,6 +1455,7 @@ inline MsmPlan msm_plan_impl(size_t n, const MsmConfig& cfg) {
     MsmPlan p;
     p.c = cfg.window_bits ? cfg.window_bits : msm_default_window_impl(n);
     p.windows = (255 + p.c - 1) / p.c;
+    p.wide = int(wide_windows(uint32_t(p.c), uint32_t(p.windows)));
     p.buckets = 1u << (p.c - 1);
     p.n = n;
     p.planes = int(plane_bits(p.buckets));
@@ -1463,6 +1480,7 @@ inline MsmPlan msm_outputs_of(const MsmBatchPlan& plan) {
     MsmPlan p;
     p.c = int(plan.descs[0].c);
     p.windows = int(plan.descs[0].windows);
+    p.wide = int(plan.descs[0].wide);
     p.buckets = plan.descs[0].buckets;
     p.n = plan.descs[0].n;
     p.planes = plan.planes ? int(plane_bits(p.buckets))
@@ -1520,6 +1538,7 @@ Affine<F> msm_combine_windows(const uint32_t* words, const MsmPlan& plan) {
         MsmDesc d{};
         d.c = uint32_t(plan.c);
         d.windows = uint32_t(plan.windows);
+        d.wide = uint32_t(plan.wide);
         d.window_base = 0;
         return combine_one<F>(words, d);
     }
@@ -1542,7 +1561,7 @@ Affine<F> msm_combine_windows(const uint32_t* words, const MsmPlan& plan) {
     wait_for(W - 1);
     typename H::P acc = win[W - 1];
     for (size_t w = W - 1; w-- > 0;) {
-        acc = H::dbl_n(acc, plan.c);
+        acc = H::dbl_n(acc, int(w) < plan.wide ? plan.c : plan.c - 1);
         wait_for(w);
         acc = H::add(acc, win[w]);
     }
