@@ -137,7 +137,7 @@ constexpr uint32_t TASK_CLASS_MAX = 1024;  // task_points <= 1024
 // one thread (k_bucket_sums), which takes them in order of decreasing task
 // count (bucket-sum class HEAVY_TASKS - tasks) so that a wave's lanes chain
 // about equally many additions.
-constexpr uint32_t HEAVY_TASKS = 8;
+constexpr uint32_t HEAVY_TASKS = 32;
 // Words of the workspace's counter area: [0] entries, [1] tasks, [2] heavy
 // buckets, [3] error word, then the bucket-sum class counts and cursors, the
 // number of buckets listed for k_bucket_sums, and the task-class counts and
