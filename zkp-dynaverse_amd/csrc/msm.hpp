@@ -92,6 +92,11 @@ struct MsmBatchPlan {
     uint32_t total_threads = 0, total_entries = 0, total_buckets = 0, total_chunks = 0, total_windows = 0;
     uint32_t total_hist_blocks = 0, total_hist = 0, digit_threads = MSM_DIGIT_THREADS, max_window_chunks = 0;
     bool lds_digits = false;  // every bucket set fits the LDS histogram
+    // Group placement (lds_digits batches without window tables whose bucket
+    // sets are multiples of 64 and whose tasks hold at most 32 points): the
+    // entries are staged per group of 64 buckets in the partial-sum buffer,
+    // then placed group by group.
+    bool group_place = false;
     size_t max_tasks = 0, max_heavy = 0;
     uint32_t task_points = 16;
     // Indexed points at or beyond this limit are counted in the workspace's

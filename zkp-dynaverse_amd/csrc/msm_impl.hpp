@@ -356,6 +356,95 @@ static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
     }
 }
 
+// Placing pass in two steps whose stores stay within short runs (group
+// mode).  Placing every entry straight at its bucket slot, as above, leaves
+// a block's entries spread over its window's whole entry range (about one
+// entry per 128-byte line per block), so each line is written in pieces by
+// many blocks at different times.  Here buckets are taken in groups of
+// GROUP_BUCKETS consecutive buckets, whose entries are contiguous in the
+// sorted order:
+//   k_digits_group: the blocks of the counting pass again; the block's run
+//     in a group starts at the group's first entry plus the block's prefixes
+//     summed over the group's buckets, and every point goes to its group's
+//     run (LDS atomic) as (MSM-local point index | bucket mod GROUP_BUCKETS
+//     << 24 | sign << 31) in a staging array;
+//   k_group_place: one block per group moves the group's staged entries
+//     (one contiguous range) to their buckets' slots within the same range.
+// Requires !table, buckets a multiple of GROUP_BUCKETS and n <= 2^24.
+constexpr uint32_t GROUP_SHIFT = 6;
+constexpr uint32_t GROUP_BUCKETS = 1u << GROUP_SHIFT;
+
+static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
+    k_digits_group(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t grid, const uint32_t* __restrict__ scalars,
+                   const uint32_t* __restrict__ hist, const uint2* __restrict__ offsets,
+                   uint32_t* __restrict__ staged) {
+    __shared__ uint32_t cursor[MSM_LDS_BUCKETS / GROUP_BUCKETS];
+    const uint32_t g = xcd_grouped_block(blockIdx.x, grid);
+    const MsmDesc d = descs[find_desc<&MsmDesc::hist_block_base>(descs, m, g)];
+    const uint32_t local = g - d.hist_block_base;
+    const uint32_t j = local / d.digit_windows, w = local % d.digit_windows;
+    const uint32_t* mine = hist + d.hist_base + size_t(local) * d.buckets;
+    const uint2* starts = offsets + d.bucket_base + w * d.buckets;
+    const uint32_t groups = d.buckets >> GROUP_SHIFT, lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+    for (uint32_t s = threadIdx.x >> 6; s < groups; s += waves) {  // one wave per group
+        const uint32_t sum = wave_sum(mine[s * GROUP_BUCKETS + lane]);
+        if (lane == 0) cursor[s] = starts[s * GROUP_BUCKETS].x + sum;
+    }
+    __syncthreads();
+    const uint32_t i0 = j << d.chunk_shift, i1 = min(d.n, i0 + (1u << d.chunk_shift));
+    uint32_t kconst[9];
+    recode_constant(d, kconst);
+    const int c = int(window_width(d, w));
+    const uint32_t bit = window_offset(d, w);
+    const int32_t bias = int32_t((1u << (c - 1)) - 1);
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(d.scalar_base + i) * 8);
+        const uint4 a = sp[0], b = sp[1];
+        const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        uint32_t t[9];
+        recode_add(s, kconst, t);
+        const int32_t digit = int32_t(window_of(t, bit, c)) - bias;
+        if (digit == 0) continue;
+        const uint32_t sign = digit < 0 ? 1u : 0u;
+        const uint32_t bucket = uint32_t(digit < 0 ? -digit : digit) - 1u;
+        const uint32_t pos = atomicAdd(&cursor[bucket >> GROUP_SHIFT], 1u);
+        staged[pos] = i | ((bucket & (GROUP_BUCKETS - 1u)) << 24) | (sign << 31);
+    }
+}
+
+static __global__ void __launch_bounds__(256)
+    k_group_place(const MsmDesc* __restrict__ descs, uint32_t m, const uint32_t* __restrict__ counts,
+                  const uint2* __restrict__ offsets, const uint32_t* __restrict__ staged,
+                  const uint32_t* __restrict__ point_index, uint32_t point_limit, uint32_t* __restrict__ flags,
+                  uint32_t* __restrict__ sorted) {
+    __shared__ uint32_t cursor[GROUP_BUCKETS];
+    __shared__ uint32_t range[2];
+    const uint32_t gb0 = blockIdx.x * GROUP_BUCKETS;
+    const MsmDesc d = descs[find_desc<&MsmDesc::bucket_base>(descs, m, gb0)];
+    if (threadIdx.x < GROUP_BUCKETS) {  // the first wave: one bucket per lane
+        const uint32_t start = offsets[gb0 + threadIdx.x].x;
+        cursor[threadIdx.x] = start;
+        const uint32_t len = wave_sum(counts[gb0 + threadIdx.x]);
+        if (threadIdx.x == 0) {
+            range[0] = start;
+            range[1] = len;
+        }
+    }
+    __syncthreads();
+    const uint32_t start = range[0], len = range[1];
+    for (uint32_t k = threadIdx.x; k < len; k += blockDim.x) {
+        const uint32_t e = staged[start + k];
+        const uint32_t i = e & 0xffffffu;
+        uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
+        if (pidx >= point_limit) {  // flagged for the host; point 0 keeps the access in bounds
+            flags[0] = 1u;
+            pidx = 0;
+        }
+        const uint32_t pos = atomicAdd(&cursor[(e >> 24) & (GROUP_BUCKETS - 1u)], 1u);
+        sorted[pos] = pidx | (e & 0x80000000u);
+    }
+}
+
 // One thread per bucket: exclusive prefix of the bucket's per-block counts
 // over the blocks feeding it (in place), the bucket's total, and the task
 // classes of the block's buckets added to the global class counts.
@@ -1053,7 +1142,7 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
     p.task_points = uint32_t(cfg.task_points);
     uint64_t threads = 0, entries = 0, buckets = 0, chunks = 0, windows = 0, hist_blocks = 0, hist_words = 0;
     uint64_t digit_threads = 0, max_window_chunks = 0;
-    bool lds = true;
+    bool lds = true, group = cfg.task_points <= 32;
     for (const MsmItem& it : items) {
         require(it.n > 0 && it.n <= (size_t(1) << 24), "MSM length must be within 1..2^24");
         const int c = it.c ? it.c : (cfg.window_bits ? cfg.window_bits : msm_window_for_impl(it.n, table));
@@ -1097,6 +1186,7 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
         hist_words += blocks * d.buckets;
         digit_threads = std::max<uint64_t>(digit_threads, std::min<uint64_t>(it.n, uint64_t(1) << d.chunk_shift));
         lds &= d.buckets <= MSM_LDS_BUCKETS;
+        group &= !d.table && d.buckets >= GROUP_BUCKETS;
         threads += it.n;
         entries += uint64_t(d.digit_windows) * it.n;
         buckets += uint64_t(d.windows) * d.buckets;
@@ -1113,6 +1203,9 @@ inline MsmBatchPlan msm_batch_plan_impl(const std::vector<MsmItem>& items, const
     p.digit_threads = uint32_t(std::min<uint64_t>(MSM_DIGIT_THREADS, std::max<uint64_t>(64, (digit_threads + 63) / 64 * 64)));
     p.total_hist = uint32_t(hist_words);
     p.lds_digits = lds && hist_words <= uint64_t(entries) * 2 + (uint64_t(1) << 20);
+    // The staging array lives in the partial-sum buffer (at least
+    // entries / task_points + 1 XYZZ values of 32 or more words each).
+    p.group_place = p.lds_digits && group;
     p.total_threads = uint32_t(threads);
     p.total_entries = uint32_t(entries);
     p.total_buckets = uint32_t(buckets);
@@ -1235,7 +1328,8 @@ inline MsmBuffers msm_buffers(const MsmBatchPlan& plan, MsmWorkspace& ws, int xw
                 ws.chunks.bytes >= size_t(plan.total_chunks) * (plan.chunk_planes ? 2 : 1) * xw * 4 &&
                 ws.descs.bytes >= b.m * sizeof(MsmDesc) && ws.heavy.bytes >= plan.max_heavy * 4 &&
                 ws.block_sums.bytes >= b.tiles * sizeof(uint2) &&
-                (!plan.lds_digits || ws.hist.bytes >= size_t(plan.total_hist) * 4),
+                (!plan.lds_digits || ws.hist.bytes >= size_t(plan.total_hist) * 4) &&
+                (!plan.group_place || ws.partials.bytes >= size_t(plan.total_entries) * 4),
             "MSM workspace not prepared for this batch");
     b.descs = static_cast<const MsmDesc*>(ws.descs.ptr);
     b.counts = static_cast<uint32_t*>(ws.counts.ptr);
@@ -1287,7 +1381,12 @@ inline void msm_sort_stage(const uint32_t* point_index, const uint32_t* scalars,
     k_scan_apply<<<b.tiles, SCAN_THREADS, 0, stream>>>(b.counts, plan.total_buckets, tp, b.tile_sums, b.offsets);
     HIP_CHECK_LAUNCH();
     msm_mark(ev, k0 + 2, stream);
-    if (plan.lds_digits)
+    if (plan.group_place) {
+        k_digits_group<<<plan.total_hist_blocks, plan.digit_threads, 0, stream>>>(
+            b.descs, m, plan.total_hist_blocks, scalars, b.hist, b.offsets, b.partials);
+        k_group_place<<<plan.total_buckets / GROUP_BUCKETS, 256, 0, stream>>>(
+            b.descs, m, b.counts, b.offsets, b.partials, point_index, plan.point_limit, b.total + 3, b.sorted);
+    } else if (plan.lds_digits)
         k_digits_lds<true><<<plan.total_hist_blocks, plan.digit_threads, 0, stream>>>(
             b.descs, m, plan.total_hist_blocks, scalars, b.hist, b.offsets, point_index, plan.point_limit, b.total + 3,
             b.sorted);
