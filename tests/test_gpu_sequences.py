@@ -108,3 +108,34 @@ def test_kzg_verify_input_checks(backend):
     trivial = o.g1_to_bytes(o.g1_mul(f_z)) + bytes(64) + f_z.to_bytes(32, "little")
     assert verify_update_proofs(backend, [trivial, good], [d, d]) == [True, True]
     assert verify_update_proofs(backend, [], []) == []
+
+
+def test_kzg_verify_many_divisors_both_paths(backend):
+    """4200 proofs over 4200 distinct divisors [tau - z_i] g2: the one-wave
+    path runs them in two chunks of the 4096-point line cache (the second
+    after the cache starts over); 16 385 items (above the one-wave limit)
+    take the one-lane path.  Every fifth item carries a changed value and is
+    rejected, the others hold (the points come from the fixed-base kernel,
+    checked against the oracle elsewhere; the decisions are fixed by the
+    algebra)."""
+    from dvgpu.pairing import kzg_verify_encoded
+    tau, n = 0x1234567890ABCDEF, 4200
+    z = [(7 * i + 3) % R for i in range(n)]
+    a = [(11 * i + 5) % R for i in range(n)]
+    y = [(13 * i + 1) % R for i in range(n)]
+    w = [(a[i] - y[i]) * pow(tau - z[i], R - 2, R) % R for i in range(n)]
+    enc = lambda vals: b"".join(encode_fr(v) for v in vals)  # noqa: E731
+    d = b"".join(backend.fixed_base_encoded(enc([(tau - zz) % R for zz in z[c:c + 4096]]),
+                                            len(z[c:c + 4096]), True) for c in range(0, n, 4096))
+    c = b"".join(backend.fixed_base_encoded(enc(a[k:k + 4096]), len(a[k:k + 4096]), False) for k in range(0, n, 4096))
+    wp = b"".join(backend.fixed_base_encoded(enc(w[k:k + 4096]), len(w[k:k + 4096]), False) for k in range(0, n, 4096))
+    vals = enc([(y[i] + (1 if i % 5 == 0 else 0)) % R for i in range(n)])
+    want = bytes(0 if i % 5 == 0 else 1 for i in range(n))
+    print("case: 4200 distinct divisors, one-wave path", flush=True)
+    res, _ = kzg_verify_encoded(backend, c, vals, wp, d, n)
+    assert res == want
+    print("case: 16385 items, one-lane path", flush=True)
+    m = 16385
+    rep = lambda buf, size: (buf * (m // n + 1))[:m * size]  # noqa: E731
+    res, _ = kzg_verify_encoded(backend, rep(c, 64), rep(vals, 32), rep(wp, 64), rep(d, 128), m)
+    assert res == (want * (m // n + 1))[:m]

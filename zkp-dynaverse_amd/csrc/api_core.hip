@@ -72,6 +72,10 @@ void Owner::release_device() noexcept {
     kzg.clear();
     for (PolyTree* t : poly_trees) poly_tree_destroy(t);
     poly_trees.clear();
+    if (pairing) {
+        pairing_cache_destroy(pairing);
+        pairing = nullptr;
+    }
     for (auto& kv : ext_allocations) {
         (void)hipFree(kv.first);
         quota.release(kv.second);

@@ -28,6 +28,7 @@ struct BatchObj;       // batch.hip
 struct SetupState;     // setup.hip
 struct KzgSrs;         // kzg.hip
 struct PolyTree;       // poly.hip
+struct PairingCache;   // pairing.hip
 
 struct Owner {
     hipStream_t stream = nullptr;
@@ -67,6 +68,8 @@ struct Owner {
     std::vector<KzgSrs*> kzg;
     // Subproduct trees created through dynaverse_poly.h (freed with the owner).
     std::vector<PolyTree*> poly_trees;
+    // Prepared G2 lines and the G1 doubling table of the pairing checks.
+    PairingCache* pairing = nullptr;
     // Measured window width per log2 MSM size (dvgpu_ext_set_tuning); the
     // NTT stages per pass live in `domain`, the task shape in msm_config.
     WindowTable window_table;
@@ -125,5 +128,7 @@ void setup_destroy(SetupState* state) noexcept;
 void kzg_destroy(KzgSrs* srs) noexcept;
 // Polynomial hooks (poly.hip).
 void poly_tree_destroy(PolyTree* tree) noexcept;
+// Pairing hooks (pairing.hip).
+void pairing_cache_destroy(PairingCache* cache) noexcept;
 
 }  // namespace dvgpu

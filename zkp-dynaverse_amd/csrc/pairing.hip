@@ -3,6 +3,7 @@
 // exponentiation, the product compared with one.
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/dynaverse_pairing.h"
@@ -10,8 +11,27 @@
 #include "glv.hpp"
 #include "owner.hpp"
 #include "pairing.hpp"
+#include "pairing_team.hpp"
 
 namespace dvgpu {
+
+// Prepared G2 lines kept per owner: the lines of a G2 point depend on the
+// point alone (arkworks' G2Prepared), and the verifiers' G2 points repeat
+// call after call (the generator; a session's frozen divisor [tau - z] g2).
+// Slots are handed out in order and all of them are dropped together when a
+// call needs more than remain.
+constexpr uint32_t LINE_SLOTS = 4096;
+
+struct PairingCache {
+    std::unordered_map<std::string, uint32_t> slot_of;  // 128-byte wire point -> slot
+    DeviceBuffer lines;                                 // LINE_SLOTS x (lines of one point)
+    std::string g1_key;                                 // 64-byte wire point of g1_table
+    DeviceBuffer g1_table;                              // [2^i] g1 for i < 256, XYZZ words
+    DeviceBuffer staged;                                // points to prepare, their slots
+    DeviceBuffer program;                               // team_program's operations
+    int program_len = 0;
+};
+
 namespace {
 
 using namespace bn254;
@@ -305,6 +325,156 @@ __global__ void __launch_bounds__(64) k_kzg_verify(const uint32_t* __restrict__ 
     ok[t] = f12_is_one(final_exponentiation(f)) ? 1 : 0;
 }
 
+// ---- wave-per-check KZG verification ---------------------------------------
+
+constexpr uint32_t LINE_WORDS = PAIRING_LINES * 48;
+
+// Canonical wire G2 point (32 words) -> affine Montgomery; false when it is
+// the identity, not canonical or off the twist.
+__device__ __forceinline__ bool load_g2(const uint32_t* q, Fq2& x, Fq2& y) {
+    if (all_zero(q, 32)) return false;
+    for (int k = 0; k < 4; ++k)
+        if (!canonical_fq(q + 8 * k)) return false;
+    x = {ld_canon_fq(q), ld_canon_fq(q + 8)};
+    y = {ld_canon_fq(q + 16), ld_canon_fq(q + 24)};
+    return affine_on_curve<Fq2Ops>({x, y, false});
+}
+
+__device__ __forceinline__ void store_line(uint32_t* out, const Line& l) {
+    const Fq* c[6] = {&l.l0.c0, &l.l0.c1, &l.l1.c0, &l.l1.c1, &l.l3.c0, &l.l3.c1};
+    for (int q = 0; q < 6; ++q) stfq(out + q * 8, *c[q]);
+}
+
+// One lane per G2 point (canonical wire words): its prepared lines, in the
+// order of prepare_lines, at lines + slots[i] * LINE_WORDS.  Points that do
+// not parse are skipped (their checks fail on their own validation).
+__global__ void __launch_bounds__(64) k_lines_many(const uint32_t* __restrict__ wire, uint32_t count,
+                                                   const uint32_t* __restrict__ slots, uint32_t* __restrict__ lines) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    Fq2 x2, y2;
+    if (!load_g2(wire + size_t(i) * 32, x2, y2)) return;
+    uint32_t* out = lines + size_t(slots[i]) * LINE_WORDS;
+    G2Proj t = {x2, y2, f2_one()};
+    int n = 0;
+    for (int b = PAIRING_ATE_BITS - 2; b >= 0; --b) {
+        store_line(out + 48 * n++, miller_dbl_coeffs(t));
+        if (ate_bit(b)) store_line(out + 48 * n++, miller_add_coeffs(t, x2, y2));
+    }
+    Fq2 q1x, q1y, q2x, q2y;
+    twist_frobenius(x2, y2, q1x, q1y);
+    twist_frobenius(q1x, q1y, q2x, q2y);
+    store_line(out + 48 * n++, miller_add_coeffs(t, q1x, q1y));
+    store_line(out + 48 * n++, miller_add_coeffs(t, q2x, f2_neg(q2y)));
+}
+
+// [2^i] P for i < 256 as XYZZ words (one lane).
+__global__ void k_g1_doublings(Fq x, Fq y, uint32_t* __restrict__ out) {
+    if (blockIdx.x || threadIdx.x) return;
+    G1 p = {x, y, fp_one<FqParams>(), fp_one<FqParams>()};
+    for (int i = 0; i < 256; ++i) {
+        stg1(out + 32 * i, p);
+        p = xyzz_dbl<FqOps>(p);
+    }
+}
+
+__device__ __forceinline__ G1 shfl_down_g1(const G1& p, int off) {
+    G1 r;
+    const Fq* src[4] = {&p.X, &p.Y, &p.ZZ, &p.ZZZ};
+    Fq* dst[4] = {&r.X, &r.Y, &r.ZZ, &r.ZZZ};
+    for (int k = 0; k < 4; ++k)
+        for (int i = 0; i < LIMBS; ++i) dst[k]->l[i] = __shfl_down(src[k]->l[i], off, 64);
+    return r;
+}
+
+// One team (one wave) per item t (the check of k_kzg_verify): lane 0
+// validates the inputs; the wave forms [y_t] g1_0 from the doubling table
+// (four bits a lane, then a shuffle tree); the pair (C_t - [y_t] g1_0, g2_0) is taken in
+// XYZZ form, its lines scaled by Y ZZ, X ZZZ and ZZ ZZZ (the affine line times
+// ZZ ZZZ, a factor in Fq that the final exponentiation removes), the pair
+// (-W_t, D_t) by -y, x and 1; both lines come prepared (g2_slot and
+// div_slots[t] in `lines`).  Then the team Miller product and final
+// exponentiation.
+__global__ void __launch_bounds__(TEAM_THREADS) k_kzg_team(const uint32_t* __restrict__ commitments,
+                                                 const uint32_t* __restrict__ values,
+                                                 const uint32_t* __restrict__ openings,
+                                                 const uint32_t* __restrict__ divisors,
+                                                 const uint32_t* __restrict__ g1_table,
+                                                 const uint32_t* __restrict__ lines, uint32_t g2_slot,
+                                                 const uint32_t* __restrict__ div_slots,
+                                                 const uint32_t* __restrict__ program, int program_len,
+                                                 uint8_t* __restrict__ ok) {
+    TeamShared& s = team_lds;
+    const uint64_t t = blockIdx.x;
+    const int lane = threadIdx.x;
+    const uint32_t* cw = commitments + t * 16;
+    const uint32_t* v = values + t * 8;
+    const uint32_t* pi = openings + t * 16;
+    const uint32_t* dv = divisors + t * 32;
+    __shared__ G1 c_point;
+    if (lane == 0) {
+        bool valid = canonical_fr(v);
+        const bool c0 = all_zero(cw, 16), pi0 = all_zero(pi, 16), dinf = all_zero(dv, 32);
+        G1 c = xyzz_inf<FqOps>();
+        if (!c0) {
+            valid = valid && canonical_fq(cw) && canonical_fq(cw + 8);
+            if (valid) {
+                const Fq x = ld_canon_fq(cw), y = ld_canon_fq(cw + 8);
+                valid = affine_on_curve<FqOps>({x, y, false});
+                c = G1{x, y, fp_one<FqParams>(), fp_one<FqParams>()};
+            }
+        }
+        Fq px = fp_zero<FqParams>(), py = fp_zero<FqParams>();
+        if (!pi0) {
+            valid = valid && canonical_fq(pi) && canonical_fq(pi + 8);
+            if (valid) {
+                px = ld_canon_fq(pi);
+                py = ld_canon_fq(pi + 8);
+                valid = affine_on_curve<FqOps>({px, py, false});
+            }
+        }
+        if (!dinf) {
+            Fq2 dx, dy;
+            valid = valid && load_g2(dv, dx, dy);
+        }
+        c_point = c;
+        s.scale[3] = {neg(py), fp_zero<FqParams>()};
+        s.scale[4] = {px, fp_zero<FqParams>()};
+        s.scale[5] = f2_one();
+        s.flags[0] = valid;
+        s.flags[2] = !pi0 && !dinf;
+    }
+    team_sync();
+    if (!s.flags[0]) {
+        if (lane == 0) ok[t] = 2;
+        return;
+    }
+    // [y] g1_0: lane L adds the table points of bits 4L .. 4L + 3, then a
+    // shuffle tree
+    G1 acc = xyzz_inf<FqOps>();
+    if (lane < 64) {
+        const uint32_t nib = (v[lane >> 3] >> ((lane & 7) * 4)) & 15u;
+        for (int b = 0; b < 4; ++b)
+            if ((nib >> b) & 1u) acc = xyzz_add<FqOps>(acc, ldg1(g1_table + 32 * (4 * lane + b)));
+        for (int off = 32; off > 0; off >>= 1) acc = xyzz_add<FqOps>(acc, shfl_down_g1(acc, off));
+    }
+    if (lane == 0) {
+        const G1 d = xyzz_add<FqOps>(c_point, xyzz_neg<FqOps>(acc));
+        const bool live = !xyzz_is_inf<FqOps>(d);
+        if (live) {
+            s.scale[0] = {mul(d.Y, d.ZZ), fp_zero<FqParams>()};
+            s.scale[1] = {mul(d.X, d.ZZZ), fp_zero<FqParams>()};
+            s.scale[2] = {mul(d.ZZ, d.ZZZ), fp_zero<FqParams>()};
+        }
+        s.flags[1] = live;
+    }
+    team_sync();
+    const bool live_a = s.flags[1], live_b = s.flags[2];
+    const uint32_t* lines_b = live_b ? lines + size_t(div_slots[t]) * LINE_WORDS : lines;
+    team_run(s, program, program_len, lines + size_t(g2_slot) * LINE_WORDS, lines_b, live_a, live_b);
+    if (lane == 0) ok[t] = f12_is_one(w12_to_tower(s.w[TW_OUT])) ? 1 : 0;
+}
+
 Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
 
 Affine<FqOps> host_g1(const uint8_t* b, const char* what) {
@@ -335,10 +505,163 @@ Affine<Fq2Ops> host_g2(const uint8_t* b, const char* what) {
     return a;
 }
 
+PairingCache& pairing_cache(Owner* o) {
+    if (!o->pairing) o->pairing = new PairingCache();
+    return *o->pairing;
+}
+
+// Slots of the wire G2 points pts[i] (128 bytes each; at most LINE_SLOTS
+// distinct ones); points without lines are prepared on `st`.
+std::vector<uint32_t> line_slots(Owner* o, const std::vector<const uint8_t*>& pts, hipStream_t st) {
+    PairingCache& c = pairing_cache(o);
+    c.lines.ensure(size_t(LINE_SLOTS) * LINE_WORDS * 4);
+    std::unordered_map<std::string, uint32_t> fresh;  // points of this call without lines
+    for (const uint8_t* p : pts) {
+        std::string key(reinterpret_cast<const char*>(p), 128);
+        if (!c.slot_of.count(key)) fresh.emplace(std::move(key), 0u);
+    }
+    if (c.slot_of.size() + fresh.size() > LINE_SLOTS) {  // start over: every point of the call is fresh
+        c.slot_of.clear();
+        fresh.clear();
+        for (const uint8_t* p : pts) fresh.emplace(std::string(reinterpret_cast<const char*>(p), 128), 0u);
+    }
+    require(fresh.size() <= LINE_SLOTS, "too many distinct G2 points for the prepared-line cache");
+    if (!fresh.empty()) {
+        std::vector<uint32_t> host(fresh.size() * 33);
+        uint32_t* words = host.data();
+        uint32_t* slots = host.data() + fresh.size() * 32;
+        size_t k = 0;
+        for (auto& kv : fresh) {
+            const uint32_t slot = uint32_t(c.slot_of.size());
+            c.slot_of.emplace(kv.first, slot);
+            std::memcpy(words + 32 * k, kv.first.data(), 128);
+            slots[k++] = slot;
+        }
+        auto* dev = static_cast<uint32_t*>(c.staged.ensure(host.size() * 4));
+        HIP_CHECK(hipMemcpyAsync(dev, host.data(), host.size() * 4, hipMemcpyHostToDevice, st));
+        k_lines_many<<<grid_for(k, 64), 64, 0, st>>>(dev, uint32_t(k), dev + 32 * k, static_cast<uint32_t*>(c.lines.ptr));
+        HIP_CHECK_LAUNCH();
+        // the pageable source must outlive the copy
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
+    std::vector<uint32_t> out(pts.size());
+    for (size_t i = 0; i < pts.size(); ++i)
+        out[i] = c.slot_of.at(std::string(reinterpret_cast<const char*>(pts[i]), 128));
+    return out;
+}
+
+// The team program of a two-pair check, uploaded once per owner.
+const uint32_t* team_program_device(Owner* o, int& len, hipStream_t st) {
+    PairingCache& c = pairing_cache(o);
+    if (!c.program_len) {
+        std::vector<uint32_t> prog(size_t(team_program(nullptr)));
+        team_program(prog.data());
+        auto* dev = static_cast<uint32_t*>(c.program.ensure(prog.size() * 4));
+        HIP_CHECK(hipMemcpyAsync(dev, prog.data(), prog.size() * 4, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        c.program_len = int(prog.size());
+    }
+    len = c.program_len;
+    return static_cast<const uint32_t*>(c.program.ptr);
+}
+
+// The doubling table of the wire G1 point g (canonical, finite).
+const uint32_t* g1_doublings(Owner* o, const uint8_t* g, const Affine<FqOps>& a, hipStream_t st) {
+    PairingCache& c = pairing_cache(o);
+    const std::string key(reinterpret_cast<const char*>(g), 64);
+    if (c.g1_key != key || !c.g1_table.ptr) {
+        c.g1_key.clear();
+        c.g1_table.ensure(256 * 32 * 4);
+        k_g1_doublings<<<1, 64, 0, st>>>(a.x, a.y, static_cast<uint32_t*>(c.g1_table.ptr));
+        HIP_CHECK_LAUNCH();
+        c.g1_key = key;
+    }
+    return static_cast<const uint32_t*>(c.g1_table.ptr);
+}
+
 }  // namespace
 }  // namespace dvgpu
 
 using namespace dvgpu;
+
+namespace {
+
+// Up to this many items a call runs one wave per item (k_kzg_team); above,
+// one lane per item (k_kzg_verify), whose 64 items per wave make up for its
+// serial latency once the device is full.
+constexpr size_t KZG_TEAM_MAX = 16384;
+
+int kzg_verify_team(Owner* o, const uint8_t* g1_0, const uint8_t* g2_0, const Affine<FqOps>& g1,
+                    const uint8_t* commitments, const uint8_t* values, const uint8_t* openings,
+                    const uint8_t* divisors, size_t count, uint8_t* results, float* kernel_ms) {
+    hipStream_t st = o->stream;
+    const uint32_t* table = g1_doublings(o, g1_0, g1, st);
+    int program_len = 0;
+    const uint32_t* program = team_program_device(o, program_len, st);
+    const size_t bc = count * 64, bv = count * 32, bw = count * 64, bd = count * 128,
+                 bs = (count * 4 + 255) & ~size_t(255), br = (count + 255) & ~size_t(255);
+    auto* base = static_cast<uint8_t*>(o->stage_a.ensure(bc + bv + bw + bd + bs + br));
+    uint8_t *dc = base, *dv = dc + bc, *dw = dv + bv, *dd = dw + bw, *ds = dd + bd, *dr = ds + bs;
+    HIP_CHECK(hipMemcpyAsync(dc, commitments, bc, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(dv, values, bv, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(dw, openings, bw, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(dd, divisors, bd, hipMemcpyHostToDevice, st));
+    ScopedEvents<2> ev(kernel_ms != nullptr);
+    if (kernel_ms) HIP_CHECK(hipEventRecord(ev[0], st));
+    std::vector<uint8_t> zero(128, 0);
+    // chunks whose G2 points (g2_0 and the finite divisors) fit the line cache
+    for (size_t first = 0; first < count;) {
+        std::vector<const uint8_t*> pts = {g2_0};
+        std::vector<size_t> idx;  // pts index of each item's divisor (0: none)
+        std::unordered_map<std::string, size_t> seen;
+        size_t last = first;
+        for (; last < count; ++last) {
+            const uint8_t* d = divisors + 128 * last;
+            if (std::memcmp(d, zero.data(), 128) == 0) {
+                idx.push_back(0);
+                continue;
+            }
+            std::string key(reinterpret_cast<const char*>(d), 128);
+            auto it = seen.find(key);
+            if (it == seen.end()) {
+                if (pts.size() == LINE_SLOTS) break;
+                it = seen.emplace(std::move(key), pts.size()).first;
+                pts.push_back(d);
+            }
+            idx.push_back(it->second);
+        }
+        const std::vector<uint32_t> slots = line_slots(o, pts, st);
+        std::vector<uint32_t> item_slots(idx.size());
+        for (size_t k = 0; k < idx.size(); ++k) item_slots[k] = slots[idx[k]];
+        uint32_t* dslots = reinterpret_cast<uint32_t*>(ds) + first;
+        HIP_CHECK(hipMemcpyAsync(dslots, item_slots.data(), item_slots.size() * 4, hipMemcpyHostToDevice, st));
+        const uint32_t* lines = static_cast<const uint32_t*>(pairing_cache(o).lines.ptr);
+        k_kzg_team<<<unsigned(last - first), TEAM_THREADS, 0, st>>>(
+            reinterpret_cast<const uint32_t*>(dc) + first * 16, reinterpret_cast<const uint32_t*>(dv) + first * 8,
+            reinterpret_cast<const uint32_t*>(dw) + first * 16, reinterpret_cast<const uint32_t*>(dd) + first * 32,
+            table, lines, slots[0], dslots, program, program_len, dr + first);
+        HIP_CHECK_LAUNCH();
+        // the pageable slot array must outlive its copy
+        HIP_CHECK(hipStreamSynchronize(st));
+        first = last;
+    }
+    if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
+    HIP_CHECK(hipMemcpyAsync(results, dr, count, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    if (kernel_ms) *kernel_ms = ev.ms(0, 1);
+    const void* bad = std::memchr(results, 2, count);
+    if (bad) {
+        const size_t t = static_cast<const uint8_t*>(bad) - results;
+        std::memset(results, 0, count);
+        throw InvalidArgument(("KZG opening " + std::to_string(t) +
+                               ": a point or value is not canonical or not on its curve").c_str());
+    }
+    return 0;
+}
+
+}  // namespace
+
+void dvgpu::pairing_cache_destroy(PairingCache* c) noexcept { delete c; }
 
 extern "C" int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const uint8_t* g2, size_t terms,
                                          size_t count, uint8_t* results, uint8_t* gt, float* kernel_ms,
@@ -514,6 +837,8 @@ extern "C" int dvgpu_bn254_kzg_verify_batch(void* backend, const uint8_t* g1_0, 
         const Affine<Fq2Ops> h0 = host_g2(g2_0, "g2 generator is not a canonical twist point");
         require(!h0.inf, "g2 generator is the identity");
         hipStream_t st = o->stream;
+        if (count <= KZG_TEAM_MAX) return kzg_verify_team(o, g1_0, g2_0, g1, commitments, values, openings, divisors,
+                                                          count, results, kernel_ms);
         const size_t bc = count * 64, bv = count * 32, bw = count * 64, bd = count * 128,
                      br = (count + 255) & ~size_t(255), bl = PAIRING_LINES * 48 * 4;
         auto* base = static_cast<uint8_t*>(o->stage_a.ensure(bc + bv + bw + bd + br + bl));
