@@ -28,8 +28,8 @@ struct PairingCache {
     std::string g1_key;                                 // 64-byte wire point of g1_table
     DeviceBuffer g1_table;                              // [2^i] g1 for i < 256, XYZZ words
     DeviceBuffer staged;                                // points to prepare, their slots
-    DeviceBuffer program;                               // team_program's operations
-    int program_len = 0;
+    DeviceBuffer program;  // team_program for 1..PAIRING_MAX_TERMS pairs, then (offset, length) per count
+    uint32_t program_words = 0;
 };
 
 namespace {
@@ -402,8 +402,8 @@ __global__ void __launch_bounds__(TEAM_THREADS) k_kzg_team(const uint32_t* __res
                                                  const uint32_t* __restrict__ g1_table,
                                                  const uint32_t* __restrict__ lines, uint32_t g2_slot,
                                                  const uint32_t* __restrict__ div_slots,
-                                                 const uint32_t* __restrict__ program, int program_len,
-                                                 uint8_t* __restrict__ ok) {
+                                                 const uint32_t* __restrict__ programs,
+                                                 const uint32_t* __restrict__ prog_off, uint8_t* __restrict__ ok) {
     TeamShared& s = team_lds;
     const uint64_t t = blockIdx.x;
     const int lane = threadIdx.x;
@@ -438,11 +438,11 @@ __global__ void __launch_bounds__(TEAM_THREADS) k_kzg_team(const uint32_t* __res
             valid = valid && load_g2(dv, dx, dy);
         }
         c_point = c;
-        s.scale[3] = {neg(py), fp_zero<FqParams>()};
-        s.scale[4] = {px, fp_zero<FqParams>()};
-        s.scale[5] = f2_one();
+        s.scale[1][0] = {neg(py), fp_zero<FqParams>()};
+        s.scale[1][1] = {px, fp_zero<FqParams>()};
+        s.scale[1][2] = f2_one();
         s.flags[0] = valid;
-        s.flags[2] = !pi0 && !dinf;
+        s.live[1] = !pi0 && !dinf;
     }
     team_sync();
     if (!s.flags[0]) {
@@ -462,17 +462,78 @@ __global__ void __launch_bounds__(TEAM_THREADS) k_kzg_team(const uint32_t* __res
         const G1 d = xyzz_add<FqOps>(c_point, xyzz_neg<FqOps>(acc));
         const bool live = !xyzz_is_inf<FqOps>(d);
         if (live) {
-            s.scale[0] = {mul(d.Y, d.ZZ), fp_zero<FqParams>()};
-            s.scale[1] = {mul(d.X, d.ZZZ), fp_zero<FqParams>()};
-            s.scale[2] = {mul(d.ZZ, d.ZZZ), fp_zero<FqParams>()};
+            s.scale[0][0] = {mul(d.Y, d.ZZ), fp_zero<FqParams>()};
+            s.scale[0][1] = {mul(d.X, d.ZZZ), fp_zero<FqParams>()};
+            s.scale[0][2] = {mul(d.ZZ, d.ZZZ), fp_zero<FqParams>()};
         }
-        s.flags[1] = live;
+        s.live[0] = live;
+        s.lines[0] = lines + size_t(g2_slot) * LINE_WORDS;
+        s.lines[1] = s.live[1] ? lines + size_t(div_slots[t]) * LINE_WORDS : lines;
     }
-    team_sync();
-    const bool live_a = s.flags[1], live_b = s.flags[2];
-    const uint32_t* lines_b = live_b ? lines + size_t(div_slots[t]) * LINE_WORDS : lines;
-    team_run(s, program, program_len, lines + size_t(g2_slot) * LINE_WORDS, lines_b, live_a, live_b);
+    team_clear_lines(s);
+    team_run(s, programs + prog_off[4], int(prog_off[5]));  // the two-pair program
     if (lane == 0) ok[t] = f12_is_one(w12_to_tower(s.w[TW_OUT])) ? 1 : 0;
+}
+
+// One team per check of `count` groups of pairs: group t holds the pairs
+// first[t] .. first[t + 1] - 1 (g1 64 B, g2 128 B canonical wire words, G2
+// lines prepared at slots[pair]); in product form, pairs after the first get
+// their G1 point negated when `negate_rest` (an equation e(L) == prod e(R_j)
+// as one product).  Lane 0 validates every pair (ok[t] = 2 on a bad point);
+// a pair with an identity contributes one.  The program of the group's pair
+// count comes from programs + prog_off[k], prog_len[k].  gt (optional):
+// the reduced product, tower order, canonical words.
+__global__ void __launch_bounds__(TEAM_THREADS) k_pairs_team(const uint32_t* __restrict__ g1,
+                                                             const uint32_t* __restrict__ g2,
+                                                             const uint32_t* __restrict__ first,
+                                                             const uint32_t* __restrict__ order, uint32_t terms,
+                                                             int negate_rest, const uint32_t* __restrict__ slots,
+                                                             const uint32_t* __restrict__ lines,
+                                                             const uint32_t* __restrict__ programs,
+                                                             const uint32_t* __restrict__ prog_off,
+                                                             uint8_t* __restrict__ ok, uint32_t* __restrict__ gt) {
+    TeamShared& s = team_lds;
+    const uint64_t t = order ? order[blockIdx.x] : blockIdx.x;
+    const int lane = threadIdx.x;
+    const uint64_t p0 = first ? first[t] : t * terms;
+    const int k = int(first ? first[t + 1] - first[t] : terms);
+    if (lane == 0) {
+        bool valid = true;
+        for (int j = 0; j < k; ++j) {
+            PairingInput e;
+            bool finite;
+            valid = load_pair(g1 + (p0 + j) * 16, g2 + (p0 + j) * 32, e, finite) && valid;
+            s.live[j] = finite;
+            if (finite) {
+                s.scale[j][0] = {negate_rest && j ? neg(e.yp) : e.yp, fp_zero<FqParams>()};
+                s.scale[j][1] = {e.xp, fp_zero<FqParams>()};
+                s.scale[j][2] = f2_one();
+                s.lines[j] = lines + size_t(slots[p0 + j]) * LINE_WORDS;
+            }
+        }
+        s.flags[0] = valid;
+    }
+    team_clear_lines(s);
+    if (!s.flags[0]) {
+        if (lane == 0) ok[t] = 2;
+        return;
+    }
+    team_run(s, programs + prog_off[2 * k], int(prog_off[2 * k + 1]));
+    if (lane == 0) {
+        const Fq12 r = w12_to_tower(s.w[TW_OUT]);
+        ok[t] = f12_is_one(r) ? 1 : 0;
+        if (gt) {
+            uint32_t* o = gt + t * 96;
+            const Fq6* c[2] = {&r.c0, &r.c1};
+            for (int h = 0; h < 2; ++h) {
+                const Fq2* a[3] = {&c[h]->a0, &c[h]->a1, &c[h]->a2};
+                for (int i = 0; i < 3; ++i) {
+                    st_canon_fq(o + (h * 3 + i) * 16, a[i]->c0);
+                    st_canon_fq(o + (h * 3 + i) * 16 + 8, a[i]->c1);
+                }
+            }
+        }
+    }
 }
 
 Owner* as_owner(void* p) { return static_cast<Owner*>(p); }
@@ -550,19 +611,31 @@ std::vector<uint32_t> line_slots(Owner* o, const std::vector<const uint8_t*>& pt
     return out;
 }
 
-// The team program of a two-pair check, uploaded once per owner.
-const uint32_t* team_program_device(Owner* o, int& len, hipStream_t st) {
+// The team programs of checks of 1..PAIRING_MAX_TERMS pairs, uploaded once
+// per owner; returns the programs and sets off to the table of (offset,
+// length) pairs indexed by 2 * pairs.
+const uint32_t* team_programs(Owner* o, const uint32_t*& off, hipStream_t st) {
     PairingCache& c = pairing_cache(o);
-    if (!c.program_len) {
-        std::vector<uint32_t> prog(size_t(team_program(nullptr)));
-        team_program(prog.data());
-        auto* dev = static_cast<uint32_t*>(c.program.ensure(prog.size() * 4));
-        HIP_CHECK(hipMemcpyAsync(dev, prog.data(), prog.size() * 4, hipMemcpyHostToDevice, st));
+    if (!c.program_words) {
+        std::vector<uint32_t> table(2 * (PAIRING_MAX_TERMS + 1), 0u), prog;
+        for (int k = 1; k <= PAIRING_MAX_TERMS; ++k) {
+            const size_t at = prog.size(), len = size_t(team_program(k, nullptr));
+            prog.resize(at + len);
+            team_program(k, prog.data() + at);
+            table[2 * k] = uint32_t(at);
+            table[2 * k + 1] = uint32_t(len);
+        }
+        const size_t words = table.size() + prog.size();
+        std::vector<uint32_t> all(prog);
+        all.insert(all.end(), table.begin(), table.end());
+        auto* dev = static_cast<uint32_t*>(c.program.ensure(words * 4));
+        HIP_CHECK(hipMemcpyAsync(dev, all.data(), words * 4, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipStreamSynchronize(st));
-        c.program_len = int(prog.size());
+        c.program_words = uint32_t(prog.size());
     }
-    len = c.program_len;
-    return static_cast<const uint32_t*>(c.program.ptr);
+    const uint32_t* base = static_cast<const uint32_t*>(c.program.ptr);
+    off = base + c.program_words;
+    return base;
 }
 
 // The doubling table of the wire G1 point g (canonical, finite).
@@ -589,15 +662,15 @@ namespace {
 // Up to this many items a call runs one wave per item (k_kzg_team); above,
 // one lane per item (k_kzg_verify), whose 64 items per wave make up for its
 // serial latency once the device is full.
-constexpr size_t KZG_TEAM_MAX = 16384;
+constexpr size_t KZG_TEAM_MAX = 8192;
 
 int kzg_verify_team(Owner* o, const uint8_t* g1_0, const uint8_t* g2_0, const Affine<FqOps>& g1,
                     const uint8_t* commitments, const uint8_t* values, const uint8_t* openings,
                     const uint8_t* divisors, size_t count, uint8_t* results, float* kernel_ms) {
     hipStream_t st = o->stream;
     const uint32_t* table = g1_doublings(o, g1_0, g1, st);
-    int program_len = 0;
-    const uint32_t* program = team_program_device(o, program_len, st);
+    const uint32_t* prog_off = nullptr;
+    const uint32_t* programs = team_programs(o, prog_off, st);
     const size_t bc = count * 64, bv = count * 32, bw = count * 64, bd = count * 128,
                  bs = (count * 4 + 255) & ~size_t(255), br = (count + 255) & ~size_t(255);
     auto* base = static_cast<uint8_t*>(o->stage_a.ensure(bc + bv + bw + bd + bs + br));
@@ -639,7 +712,7 @@ int kzg_verify_team(Owner* o, const uint8_t* g1_0, const uint8_t* g2_0, const Af
         k_kzg_team<<<unsigned(last - first), TEAM_THREADS, 0, st>>>(
             reinterpret_cast<const uint32_t*>(dc) + first * 16, reinterpret_cast<const uint32_t*>(dv) + first * 8,
             reinterpret_cast<const uint32_t*>(dw) + first * 16, reinterpret_cast<const uint32_t*>(dd) + first * 32,
-            table, lines, slots[0], dslots, program, program_len, dr + first);
+            table, lines, slots[0], dslots, programs, prog_off, dr + first);
         HIP_CHECK_LAUNCH();
         // the pageable slot array must outlive its copy
         HIP_CHECK(hipStreamSynchronize(st));
@@ -657,6 +730,51 @@ int kzg_verify_team(Owner* o, const uint8_t* g1_0, const uint8_t* g2_0, const Af
                                ": a point or value is not canonical or not on its curve").c_str());
     }
     return 0;
+}
+
+// Up to this many checks (pairing_check, pairing_equations in product mode)
+// run one team each, when their distinct G2 points fit the line cache.
+constexpr size_t PAIRS_TEAM_MAX = 4096;
+
+// Line-cache slots of the n wire G2 points at g2 (128 bytes each; identity
+// points get slot 0, never read), or an empty vector when the distinct
+// points do not fit the cache.
+std::vector<uint32_t> pair_slots(Owner* o, const uint8_t* g2, size_t n, hipStream_t st) {
+    static const uint8_t zero[128] = {0};
+    std::unordered_map<std::string, size_t> seen;
+    std::vector<const uint8_t*> pts;
+    std::vector<size_t> idx(n, SIZE_MAX);
+    for (size_t i = 0; i < n; ++i) {
+        const uint8_t* q = g2 + 128 * i;
+        if (std::memcmp(q, zero, 128) == 0) continue;
+        auto it = seen.emplace(std::string(reinterpret_cast<const char*>(q), 128), pts.size()).first;
+        if (it->second == pts.size()) pts.push_back(q);
+        idx[i] = it->second;
+        if (pts.size() > LINE_SLOTS) return {};
+    }
+    const std::vector<uint32_t> slots = pts.empty() ? std::vector<uint32_t>() : line_slots(o, pts, st);
+    std::vector<uint32_t> out(n, 0u);
+    for (size_t i = 0; i < n; ++i)
+        if (idx[i] != SIZE_MAX) out[i] = slots[idx[i]];
+    return out;
+}
+
+// The team kernel over count groups already on the device (g1, g2 at dg1,
+// dg2; group boundaries at dfirst or `terms` pairs each); results at dres.
+void pairs_team_launch(Owner* o, const uint32_t* dg1, const uint32_t* dg2, const uint32_t* dfirst, uint32_t terms,
+                       int negate_rest, const std::vector<uint32_t>& slots, size_t count, uint8_t* dres,
+                       uint32_t* dgt, hipStream_t st) {
+    PairingCache& c = pairing_cache(o);
+    const uint32_t* prog_off = nullptr;
+    const uint32_t* programs = team_programs(o, prog_off, st);
+    auto* dslots = static_cast<uint32_t*>(c.staged.ensure(std::max<size_t>(slots.size(), 1) * 4));
+    HIP_CHECK(hipMemcpyAsync(dslots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, st));
+    k_pairs_team<<<unsigned(count), TEAM_THREADS, 0, st>>>(dg1, dg2, dfirst, nullptr, terms, negate_rest, dslots,
+                                                          static_cast<const uint32_t*>(c.lines.ptr), programs,
+                                                          prog_off, dres, dgt);
+    HIP_CHECK_LAUNCH();
+    // the pageable slot array must outlive its copy
+    HIP_CHECK(hipStreamSynchronize(st));
 }
 
 }  // namespace
@@ -681,9 +799,15 @@ extern "C" int dvgpu_bn254_pairing_check(void* backend, const uint8_t* g1, const
         HIP_CHECK(hipMemcpyAsync(base + b1, g2, b2, hipMemcpyHostToDevice, st));
         ScopedEvents<2> ev(kernel_ms != nullptr);
         if (kernel_ms) HIP_CHECK(hipEventRecord(ev[0], st));
-        k_pairing_check<<<grid_for(count, 64), 64, 0, st>>>(
-            reinterpret_cast<const uint32_t*>(base), reinterpret_cast<const uint32_t*>(base + b1), uint32_t(terms),
-            count, base + b1 + b2, gt ? reinterpret_cast<uint32_t*>(base + b1 + b2 + br) : nullptr);
+        const std::vector<uint32_t> slots = count <= PAIRS_TEAM_MAX ? pair_slots(o, g2, n, st) : std::vector<uint32_t>();
+        if (!slots.empty())
+            pairs_team_launch(o, reinterpret_cast<const uint32_t*>(base), reinterpret_cast<const uint32_t*>(base + b1),
+                              nullptr, uint32_t(terms), 0, slots, count, base + b1 + b2,
+                              gt ? reinterpret_cast<uint32_t*>(base + b1 + b2 + br) : nullptr, st);
+        else
+            k_pairing_check<<<grid_for(count, 64), 64, 0, st>>>(
+                reinterpret_cast<const uint32_t*>(base), reinterpret_cast<const uint32_t*>(base + b1), uint32_t(terms),
+                count, base + b1 + b2, gt ? reinterpret_cast<uint32_t*>(base + b1 + b2 + br) : nullptr);
         HIP_CHECK_LAUNCH();
         if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
         HIP_CHECK(hipMemcpyAsync(results, base + b1 + b2, count, hipMemcpyDeviceToHost, st));
@@ -738,10 +862,17 @@ extern "C" int dvgpu_bn254_pairing_equations(void* backend, const uint8_t* g1, c
         HIP_CHECK(hipMemcpyAsync(base + b1 + b2 + bo, order.data(), count * 4, hipMemcpyHostToDevice, st));
         ScopedEvents<2> ev(kernel_ms != nullptr);
         if (kernel_ms) HIP_CHECK(hipEventRecord(ev[0], st));
-        k_pairing_equations<<<grid_for(count, 64), 64, 0, st>>>(
-            reinterpret_cast<const uint32_t*>(base), reinterpret_cast<const uint32_t*>(base + b1),
-            reinterpret_cast<const uint32_t*>(base + b1 + b2), reinterpret_cast<const uint32_t*>(base + b1 + b2 + bo),
-            count, mode == DVGPU_PAIRING_REFERENCE, base + b1 + b2 + bo + bq);
+        const std::vector<uint32_t> slots = mode == DVGPU_PAIRING_PRODUCT && count <= PAIRS_TEAM_MAX
+                                                ? pair_slots(o, g2, n, st) : std::vector<uint32_t>();
+        if (!slots.empty())  // product form: the right-hand G1 points negated
+            pairs_team_launch(o, reinterpret_cast<const uint32_t*>(base), reinterpret_cast<const uint32_t*>(base + b1),
+                              reinterpret_cast<const uint32_t*>(base + b1 + b2), 0u, 1, slots, count,
+                              base + b1 + b2 + bo + bq, nullptr, st);
+        else
+            k_pairing_equations<<<grid_for(count, 64), 64, 0, st>>>(
+                reinterpret_cast<const uint32_t*>(base), reinterpret_cast<const uint32_t*>(base + b1),
+                reinterpret_cast<const uint32_t*>(base + b1 + b2), reinterpret_cast<const uint32_t*>(base + b1 + b2 + bo),
+                count, mode == DVGPU_PAIRING_REFERENCE, base + b1 + b2 + bo + bq);
         HIP_CHECK_LAUNCH();
         if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
         HIP_CHECK(hipMemcpyAsync(results, base + b1 + b2 + bo + bq, count, hipMemcpyDeviceToHost, st));

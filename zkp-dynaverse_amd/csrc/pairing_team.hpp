@@ -37,8 +37,11 @@ constexpr int TEAM_THREADS = 64;
 struct TeamShared {
     Fq2 prod[36];
     W12 w[TW_SLOTS];
-    Fq2 scale[6];       // line scale factors (s, 0): pair A l0, l1, l3, pair B l0, l1, l3
-    uint32_t flags[4];  // [0] inputs valid, [1] pair A live, [2] pair B live
+    W12 line[PAIRING_MAX_TERMS];  // pair p's current line, evaluated (w^2, w^4, w^5 stay zero)
+    Fq2 scale[PAIRING_MAX_TERMS][3];  // pair p's line scale factors (s, 0) for l0, l1, l3
+    const uint32_t* lines[PAIRING_MAX_TERMS];  // pair p's prepared lines
+    uint32_t live[PAIRING_MAX_TERMS];          // pair p contributes (both points finite)
+    uint32_t flags[4];  // kernel-specific
 };
 
 // One team per block: the block's LDS.
@@ -79,11 +82,11 @@ __device__ __forceinline__ W12 w12_inv(const W12& a) {
 // never makes a function call (a call would save and restore the live
 // registers through scratch memory around every product).
 
-// w[out] = w[a] * w[b]: lane 6i + j (< 36) forms a_i b_j, then lane 2k + e
+// out = a * b: lane 6i + j (< 36) forms a_i b_j, then lane 2k + e
 // (< 12) forms component e of the coefficient of w^k.
-__device__ __forceinline__ void team_mul(TeamShared& s, int a, int b, int out) {
+__device__ __forceinline__ void team_mul(TeamShared& s, const W12& a, const W12& b, W12& out) {
     const int lane = threadIdx.x;
-    if (lane < 36) s.prod[lane] = f2_mul(s.w[a].c[lane / 6], s.w[b].c[lane % 6]);
+    if (lane < 36) s.prod[lane] = f2_mul(a.c[lane / 6], b.c[lane % 6]);
     team_sync();
     if (lane < 12) {
         const int k = lane >> 1, e = lane & 1;
@@ -122,8 +125,8 @@ __device__ __forceinline__ void team_mul(TeamShared& s, int a, int b, int out) {
             }
         }
         r = reduce(r);
-        if (e) s.w[out].c[k].c1 = r;
-        else s.w[out].c[k].c0 = r;
+        if (e) out.c[k].c1 = r;
+        else out.c[k].c0 = r;
     }
     team_sync();
 }
@@ -135,39 +138,40 @@ enum TeamOp : uint32_t {
     TOP_CONJ = 3,    // w[out] = w[a]^(q^6)
     TOP_FROB = 4,    // w[out] = w[a]^q
     TOP_INV = 5,     // w[out] = w[a]^-1
-    TOP_LINES = 6,   // w[TW_G], w[TW_H] = line n of pairs A and B, evaluated
-    TOP_MUL_A = 7,   // w[TW_F] *= w[TW_G] when pair A is live
-    TOP_MUL_B = 8,   // w[TW_F] *= w[TW_H] when pair B is live
+    TOP_LINES = 6,   // line[p] = line n of every live pair p < a, evaluated
+    TOP_MUL_LINE = 7,  // w[TW_F] *= line[a] when pair a is live
 };
 
 __host__ __device__ constexpr uint32_t team_op(uint32_t op, uint32_t a, uint32_t b, uint32_t out, uint32_t n = 0) {
     return op | (a << 4) | (b << 9) | (out << 14) | (n << 19);
 }
 
-// Lines of pairs A and B in the layout of prepare_lines (PAIRING_LINES x
-// {l0, l1, l3}, 48 Montgomery words per line), pair p's line evaluated by
-// scaling l0, l1, l3 with scale[3p .. 3p + 2].
-__device__ __forceinline__ void team_eval_lines(TeamShared& s, const uint32_t* lines_a, const uint32_t* lines_b,
-                                                int n, bool live_a, bool live_b) {
+// Before a program: every pair's line has zero coefficients at w^2, w^4
+// and w^5 (prepared lines are l0 + l1 w + l3 w^3).
+__device__ __forceinline__ void team_clear_lines(TeamShared& s) {
     const int lane = threadIdx.x;
-    if (lane < 6) {
-        const int p = lane / 3, q = lane % 3;  // pair, coefficient (l0, l1, l3)
-        if (p ? live_b : live_a) {
-            const uint32_t* src = (p ? lines_b : lines_a) + n * 48 + q * 16;
+    if (lane < 3 * PAIRING_MAX_TERMS) s.line[lane / 3].c[lane % 3 == 0 ? 2 : lane % 3 + 3] = f2_zero();
+    team_sync();
+}
+
+// Line n of pairs 0 .. k-1 (prepared lines in the layout of prepare_lines,
+// PAIRING_LINES x {l0, l1, l3}, 48 Montgomery words per line), pair p's
+// coefficients scaled by scale[p][0..2]; lane 3p + q forms coefficient q.
+__device__ __forceinline__ void team_eval_lines(TeamShared& s, int k, int n) {
+    const int lane = threadIdx.x;
+    if (lane < 3 * k) {
+        const int p = lane / 3, q = lane % 3;
+        if (s.live[p]) {
+            const uint32_t* src = s.lines[p] + n * 48 + q * 16;
             const Fq2 c = {dvgpu::ldfq(src), dvgpu::ldfq(src + 8)};
-            s.w[p ? TW_H : TW_G].c[q == 2 ? 3 : q] = f2_mul(c, s.scale[lane]);  // w^0, w^1, w^3
+            s.line[p].c[q == 2 ? 3 : q] = f2_mul(c, s.scale[p][q]);  // w^0, w^1, w^3
         }
-    } else if (lane < 12) {  // the coefficients of w^2, w^4, w^5 are zero
-        const int p = (lane - 6) / 3, q = (lane - 6) % 3;
-        s.w[p ? TW_H : TW_G].c[q == 0 ? 2 : q + 3] = f2_zero();
     }
     team_sync();
 }
 
-// Runs `len` operations of `prog`.
-__device__ __forceinline__ void team_run(TeamShared& s, const uint32_t* __restrict__ prog, int len,
-                                         const uint32_t* lines_a, const uint32_t* lines_b, bool live_a,
-                                         bool live_b) {
+// Runs `len` operations of `prog` (pairs, lines and scales set in s).
+__device__ __forceinline__ void team_run(TeamShared& s, const uint32_t* __restrict__ prog, int len) {
     const int lane = threadIdx.x;
     for (int pc = 0; pc < len; ++pc) {
         const uint32_t op = prog[pc];
@@ -175,16 +179,13 @@ __device__ __forceinline__ void team_run(TeamShared& s, const uint32_t* __restri
                   out = int((op >> 14) & 31u), n = int(op >> 19);
         switch (kind) {
             case TOP_MUL:
-                team_mul(s, a, b, out);
+                team_mul(s, s.w[a], s.w[b], s.w[out]);
                 break;
-            case TOP_MUL_A:
-                if (live_a) team_mul(s, TW_F, TW_G, TW_F);
-                break;
-            case TOP_MUL_B:
-                if (live_b) team_mul(s, TW_F, TW_H, TW_F);
+            case TOP_MUL_LINE:
+                if (s.live[a]) team_mul(s, s.w[TW_F], s.line[a], s.w[TW_F]);
                 break;
             case TOP_LINES:
-                team_eval_lines(s, lines_a, lines_b, n, live_a, live_b);
+                team_eval_lines(s, a, n);
                 break;
             case TOP_INV: {
                 if (lane == 0) s.w[out] = w12_inv(s.w[a]);
@@ -209,21 +210,21 @@ __device__ __forceinline__ void team_run(TeamShared& s, const uint32_t* __restri
     }
 }
 
-// The program of one pairing-product check over two prepared pairs: the
+// The program of one pairing-product check over k prepared pairs: the
 // Miller product into w[TW_F] (the loop order of miller_product_impl: the
 // accumulator squared once per bit below the top, each live pair's tangent,
 // its chord where the bit is set, the two Frobenius chords), then the final
 // exponentiation into w[TW_OUT] (the chain of final_exponentiation).
-inline int team_program(uint32_t* out) {
+// Returns the number of operations (written to out when out is not null).
+inline int team_program(int pairs, uint32_t* out) {
     int k = 0;
     auto op = [&](uint32_t kind, int a = 0, int b = 0, int o = 0, int n = 0) {
         if (out) out[k] = team_op(kind, uint32_t(a), uint32_t(b), uint32_t(o), uint32_t(n));
         ++k;
     };
     auto lines = [&](int n) {
-        op(TOP_LINES, 0, 0, 0, n);
-        op(TOP_MUL_A);
-        op(TOP_MUL_B);
+        op(TOP_LINES, pairs, 0, 0, n);
+        for (int p = 0; p < pairs; ++p) op(TOP_MUL_LINE, p);
     };
     auto mul = [&](int a, int b, int o) { op(TOP_MUL, a, b, o); };
     auto pow_x = [&](int src, int o) {
