@@ -189,6 +189,8 @@ def main():
         secondary["pairing_cache_check_2p16"] = benchmarks.pairing_cache_check(backend, log=log)
         log("# verifier pairing equations: 2^16 equations of 2..5 pairs, both modes")
         secondary["pairing_equations_2p16"] = benchmarks.pairing_equations_mix(backend, log=log)
+        log("# KZG update-proof verification by call size")
+        secondary["kzg_verify"] = benchmarks.kzg_verify_latency(backend, log=log)
         log("# polynomial tools: subproduct tree over 2^12 points")
         secondary["poly_tree_2p12"] = benchmarks.poly_tree(backend, log=log)
         log("# per-row figures: G2 MSM, G2 validation, resident transform / MSM / readback")

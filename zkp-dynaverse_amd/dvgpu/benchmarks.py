@@ -5,6 +5,7 @@ one-call-at-a-time path.  Everything runs on seeded synthetic data; every
 served proof is compared with the plain path's bytes.
 """
 import gc
+import statistics
 import time
 from typing import Dict, List
 
@@ -542,6 +543,50 @@ def pairing_cache_check(backend: HipBackend, log_n: int = 16, caches: int = 3, r
     return out
 
 
+def kzg_verify_latency(backend: HipBackend, counts=(1, 16, 256, 4096, 16384), reps: int = 5, log=print) -> Dict:
+    """dvgpu_bn254_kzg_verify_batch (the pairing check of an update proof,
+    dynaverse.rs:447-451) by call size: proofs e(C - [y] g1, g2) == e(W, D)
+    built from scalars (C = [a] g1, W = [(a - y)/(tau - z)] g1,
+    D = [tau - z] g2, points by the fixed-base kernels) over 64 divisors,
+    every third proof with a changed value (rejected).  Calls of up to 8192
+    proofs run one wave per proof, larger ones one lane per proof; the
+    divisors' prepared lines are cached after the first call (the first call
+    of a size is not timed)."""
+    from .codec import FR_MODULUS, encode_fr
+    from .pairing import kzg_verify_encoded
+    r, tau, distinct = FR_MODULUS, 0x5DEECE66D1234567, 64
+    z = [(977 * i + 13) % r for i in range(distinct)]
+    a = [(7919 * i + 5) % r for i in range(distinct)]
+    y = [(104729 * i + 1) % r for i in range(distinct)]
+    w = [(a[i] - y[i]) * pow(tau - z[i], r - 2, r) % r for i in range(distinct)]
+    enc = lambda vals: b"".join(encode_fr(v) for v in vals)  # noqa: E731
+    d = backend.fixed_base_encoded(enc([(tau - zz) % r for zz in z]), distinct, True)
+    c = backend.fixed_base_encoded(enc(a), distinct, False)
+    wp = backend.fixed_base_encoded(enc(w), distinct, False)
+    out = {}
+    for count in counts:
+        idx = [i % distinct for i in range(count)]
+        vals = enc([(y[j] + (1 if i % 3 == 2 else 0)) % r for i, j in enumerate(idx)])
+        want = bytes(0 if i % 3 == 2 else 1 for i in range(count))
+        cs = b"".join(c[64 * j:64 * j + 64] for j in idx)
+        ws = b"".join(wp[64 * j:64 * j + 64] for j in idx)
+        ds = b"".join(d[128 * j:128 * j + 128] for j in idx)
+        kzg_verify_encoded(backend, cs, vals, ws, ds, count)  # lines of the divisors prepared
+        kms, walls, ok = [], [], True
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res, ms = kzg_verify_encoded(backend, cs, vals, ws, ds, count)
+            walls.append((time.perf_counter() - t0) * 1e3)
+            kms.append(ms)
+            ok = ok and res == want
+        km = statistics.median(kms)
+        out[str(count)] = {"kernel_ms": round(km, 3), "wall_ms": round(statistics.median(walls), 3),
+                           "proofs_per_s": round(count / (km * 1e-3), 1), "decisions_exact": ok,
+                           "path": "one wave per proof" if count <= 8192 else "one lane per proof"}
+        log(f"KZG verification, {count} proofs: {km:.3f} ms ({out[str(count)]['path']}), decisions exact={ok}")
+    return out
+
+
 def pairing_equations_mix(backend: HipBackend, distinct: int = 8, equations: int = 65536, reps: int = 3,
                           log=print) -> Dict:
     """The verifiers' pairing equations (verification.rs PairingChecks::equal)
@@ -599,9 +644,10 @@ def pairing_equations_mix(backend: HipBackend, distinct: int = 8, equations: int
         log(f"pairing equations ({name} mode): {count} equations / {pairs} pairs in {kms:.1f} ms "
             f"({out[name]['equations_per_s']} eq/s), decisions exact={ok}")
     # One verification's worth (the first 16 equations: 5-pair shapes and
-    # their twins) in one call: the latency of a lane's Miller products and
-    # final exponentiation, whatever the batch size.
+    # their twins) in one call: one wave per equation, the G2 lines prepared
+    # by an untimed first call.
     small = offsets[16]
+    pairing_equations_encoded(backend, g1[:64 * small], g2[:128 * small], offsets[:17], VerificationMode.PRODUCT)
     times = []
     for _ in range(reps):
         res, ms = pairing_equations_encoded(backend, g1[:64 * small], g2[:128 * small], offsets[:17],
@@ -786,7 +832,8 @@ def main(argv=None):
     import json
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("workload", choices=["opening_cache", "pairing_cache_check", "pairing_equations", "poly_tree",
-                                         "row_figures", "batched_launch", "incremental", "served_stream"])
+                                         "row_figures", "batched_launch", "incremental", "served_stream",
+                                         "kzg_verify_latency"])
     ap.add_argument("--log-n", type=int, default=16)
     ap.add_argument("--rows", type=int, default=3)
     ap.add_argument("--reps", type=int, default=3)
