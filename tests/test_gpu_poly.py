@@ -109,3 +109,36 @@ def test_large_tree_batches_levels(backend):
         inter = tree.stats()
         assert inter["rounds"] <= 12                       # at most one device round per level
     print("tree build", build, "interpolate", inter, flush=True)
+
+
+def test_mid_size_tree_device_rounds_checked_everywhere(backend):
+    """400 points, so the upper levels' products (up to 200 x 200
+    coefficient pairs) run as device rounds: the vanishing polynomial, the
+    values at every point, the interpolation, the inverse derivatives and the
+    Cauchy sums all against the oracle's direct forms; Newton division with a
+    degree-300 divisor through the same device products."""
+    rng = random.Random(46)
+    n = 400
+    points = rng.sample(range(1, 1 << 40), n)
+    with SubproductTree(backend, points) as tree:
+        assert tree.stats()["device_products"] >= 1        # device rounds, not the host schoolbook
+        van = [1]
+        for x in points:
+            van = o.poly_mul(van, [(-x) % o.R, 1])
+        assert tree.vanishing() == van
+        poly = [rng.randrange(o.R) for _ in range(n)]
+        vals = tree.multipoint_eval(poly)
+        assert vals == [o.poly_eval(poly, x) for x in points]
+        assert tree.interpolate(vals) == o.poly_trim(poly)
+        der = o.poly_trim([i * c for i, c in enumerate(van)][1:])
+        assert tree.inverse_derivatives() == [pow(o.poly_eval(der, x), o.R - 2, o.R) for x in points]
+        weights = [rng.randrange(o.R) for _ in range(n)]
+        assert tree.cauchy_sums(weights) == o.cauchy_sums_direct(points, weights)
+    den = [rng.randrange(o.R) for _ in range(300)] + [7]
+    q = [rng.randrange(o.R) for _ in range(257)]
+    num = o.poly_mul(den, q)
+    assert newton_divide_exact(backend, num, den) == o.poly_trim(q)
+    bad = list(num)
+    bad[5] = (bad[5] + 1) % o.R
+    with pytest.raises(BackendError):
+        newton_divide_exact(backend, bad, den)

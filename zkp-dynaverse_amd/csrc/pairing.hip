@@ -200,19 +200,6 @@ __global__ void __launch_bounds__(64) k_cache_divisors(const uint32_t* __restric
     }
 }
 
-// The P-independent lines of g2_0's Miller loop (PAIRING_LINES x 3 Fq2,
-// Montgomery words), shared by every check of the call.
-__global__ void k_prepare_lines(Fq2 x, Fq2 y, uint32_t* __restrict__ out) {
-    if (blockIdx.x || threadIdx.x) return;
-    Line lines[PAIRING_LINES];
-    prepare_lines(x, y, lines);
-    for (int i = 0; i < PAIRING_LINES; ++i) {
-        const Fq* c[6] = {&lines[i].l0.c0, &lines[i].l0.c1, &lines[i].l1.c0, &lines[i].l1.c1,
-                          &lines[i].l3.c0, &lines[i].l3.c1};
-        for (int q = 0; q < 6; ++q) stfq(out + i * 48 + q * 8, *c[q]);
-    }
-}
-
 struct GlobalLines {
     const uint32_t* p;
     __device__ Line operator()(int i) const {
@@ -918,10 +905,9 @@ extern "C" int dvgpu_bn254_opening_cache_check(void* backend, const uint8_t* com
         hipStream_t st = o->stream;
         const size_t total = wires * n;
         const size_t bz = n * 32, bv = total * 32, bc = total * 64, bd = n * 128, bf = (n + 255) & ~size_t(255),
-                     bm = wires * 64, br = (total + 255) & ~size_t(255), bl = PAIRING_LINES * 48 * 4;
-        auto* base = static_cast<uint8_t*>(o->stage_a.ensure(bz + bv + bc + bd + bf + bm + br + bl));
-        uint8_t *dz = base, *dv = dz + bz, *dc = dv + bv, *dd = dc + bc, *df = dd + bd, *dm = df + bf, *dr = dm + bm,
-                *dl = dr + br;
+                     bm = wires * 64, br = (total + 255) & ~size_t(255);
+        auto* base = static_cast<uint8_t*>(o->stage_a.ensure(bz + bv + bc + bd + bf + bm + br));
+        uint8_t *dz = base, *dv = dz + bz, *dc = dv + bv, *dd = dc + bc, *df = dd + bd, *dm = df + bf, *dr = dm + bm;
         HIP_CHECK(hipMemcpyAsync(dz, points, bz, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(dv, values, bv, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(dc, caches, bc, hipMemcpyHostToDevice, st));
@@ -931,12 +917,13 @@ extern "C" int dvgpu_bn254_opening_cache_check(void* backend, const uint8_t* com
         k_cache_divisors<<<grid_for(n, 64), 64, 0, st>>>(reinterpret_cast<const uint32_t*>(dz), n, h0.x, h0.y, h1.x,
                                                           h1.y, reinterpret_cast<uint32_t*>(dd), df);
         HIP_CHECK_LAUNCH();
-        k_prepare_lines<<<1, 64, 0, st>>>(h0.x, h0.y, reinterpret_cast<uint32_t*>(dl));
-        HIP_CHECK_LAUNCH();
+        // g2_0's lines from the owner's line cache (prepared on first use)
+        const uint32_t g2_slot = line_slots(o, std::vector<const uint8_t*>{g2_0}, st)[0];
+        const uint32_t* lines = static_cast<const uint32_t*>(pairing_cache(o).lines.ptr) + size_t(g2_slot) * LINE_WORDS;
         k_cache_check<<<grid_for(total, 64), 64, 0, st>>>(
             reinterpret_cast<const uint32_t*>(dv), reinterpret_cast<const uint32_t*>(dc),
             reinterpret_cast<const uint32_t*>(dd), df, reinterpret_cast<const uint32_t*>(dm), uint32_t(wires), n,
-            g1.x, g1.y, reinterpret_cast<const uint32_t*>(dl), dr);
+            g1.x, g1.y, lines, dr);
         HIP_CHECK_LAUNCH();
         if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
         HIP_CHECK(hipMemcpyAsync(results, dr, total, hipMemcpyDeviceToHost, st));
@@ -971,21 +958,22 @@ extern "C" int dvgpu_bn254_kzg_verify_batch(void* backend, const uint8_t* g1_0, 
         if (count <= KZG_TEAM_MAX) return kzg_verify_team(o, g1_0, g2_0, g1, commitments, values, openings, divisors,
                                                           count, results, kernel_ms);
         const size_t bc = count * 64, bv = count * 32, bw = count * 64, bd = count * 128,
-                     br = (count + 255) & ~size_t(255), bl = PAIRING_LINES * 48 * 4;
-        auto* base = static_cast<uint8_t*>(o->stage_a.ensure(bc + bv + bw + bd + br + bl));
-        uint8_t *dc = base, *dv = dc + bc, *dw = dv + bv, *dd = dw + bw, *dr = dd + bd, *dl = dr + br;
+                     br = (count + 255) & ~size_t(255);
+        auto* base = static_cast<uint8_t*>(o->stage_a.ensure(bc + bv + bw + bd + br));
+        uint8_t *dc = base, *dv = dc + bc, *dw = dv + bv, *dd = dw + bw, *dr = dd + bd;
         HIP_CHECK(hipMemcpyAsync(dc, commitments, bc, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(dv, values, bv, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(dw, openings, bw, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(dd, divisors, bd, hipMemcpyHostToDevice, st));
         ScopedEvents<2> ev(kernel_ms != nullptr);
         if (kernel_ms) HIP_CHECK(hipEventRecord(ev[0], st));
-        k_prepare_lines<<<1, 64, 0, st>>>(h0.x, h0.y, reinterpret_cast<uint32_t*>(dl));
-        HIP_CHECK_LAUNCH();
+        // g2_0's lines from the owner's line cache (prepared on first use)
+        const uint32_t g2_slot = line_slots(o, std::vector<const uint8_t*>{g2_0}, st)[0];
+        const uint32_t* lines = static_cast<const uint32_t*>(pairing_cache(o).lines.ptr) + size_t(g2_slot) * LINE_WORDS;
         k_kzg_verify<<<grid_for(count, 64), 64, 0, st>>>(
             reinterpret_cast<const uint32_t*>(dc), reinterpret_cast<const uint32_t*>(dv),
             reinterpret_cast<const uint32_t*>(dw), reinterpret_cast<const uint32_t*>(dd), count, g1.x, g1.y,
-            reinterpret_cast<const uint32_t*>(dl), dr);
+            lines, dr);
         HIP_CHECK_LAUNCH();
         if (kernel_ms) HIP_CHECK(hipEventRecord(ev[1], st));
         HIP_CHECK(hipMemcpyAsync(results, dr, count, hipMemcpyDeviceToHost, st));
