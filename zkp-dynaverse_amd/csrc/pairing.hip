@@ -382,7 +382,8 @@ __device__ __forceinline__ G1 shfl_down_g1(const G1& p, int off) {
 // (-W_t, D_t) by -y, x and 1; both lines come prepared (g2_slot and
 // div_slots[t] in `lines`).  Then the team Miller product and final
 // exponentiation.
-__global__ void __launch_bounds__(TEAM_THREADS) k_kzg_team(const uint32_t* __restrict__ commitments,
+template <int WAVES>
+__global__ void __launch_bounds__(TEAM_THREADS, WAVES) k_kzg_team(const uint32_t* __restrict__ commitments,
                                                  const uint32_t* __restrict__ values,
                                                  const uint32_t* __restrict__ openings,
                                                  const uint32_t* __restrict__ divisors,
@@ -470,7 +471,8 @@ __global__ void __launch_bounds__(TEAM_THREADS) k_kzg_team(const uint32_t* __res
 // a pair with an identity contributes one.  The program of the group's pair
 // count comes from programs + prog_off[k], prog_len[k].  gt (optional):
 // the reduced product, tower order, canonical words.
-__global__ void __launch_bounds__(TEAM_THREADS) k_pairs_team(const uint32_t* __restrict__ g1,
+template <int WAVES>
+__global__ void __launch_bounds__(TEAM_THREADS, WAVES) k_pairs_team(const uint32_t* __restrict__ g1,
                                                              const uint32_t* __restrict__ g2,
                                                              const uint32_t* __restrict__ first,
                                                              const uint32_t* __restrict__ order, uint32_t terms,
@@ -646,6 +648,11 @@ using namespace dvgpu;
 
 namespace {
 
+// Team kernels are built for one and for two waves per SIMD: two hold the
+// registers to 128 per lane at the cost of ~8% more time per check, which
+// pays once a call has more checks than the device has SIMDs (MI355X: 1024).
+constexpr size_t TEAM_ONE_WAVE_MAX = 1024;
+
 // Up to this many items a call runs one wave per item (k_kzg_team); above,
 // one lane per item (k_kzg_verify), whose 64 items per wave make up for its
 // serial latency once the device is full.
@@ -696,7 +703,8 @@ int kzg_verify_team(Owner* o, const uint8_t* g1_0, const uint8_t* g2_0, const Af
         uint32_t* dslots = reinterpret_cast<uint32_t*>(ds) + first;
         HIP_CHECK(hipMemcpyAsync(dslots, item_slots.data(), item_slots.size() * 4, hipMemcpyHostToDevice, st));
         const uint32_t* lines = static_cast<const uint32_t*>(pairing_cache(o).lines.ptr);
-        k_kzg_team<<<unsigned(last - first), TEAM_THREADS, 0, st>>>(
+        auto* kernel = last - first > TEAM_ONE_WAVE_MAX ? k_kzg_team<2> : k_kzg_team<1>;
+        kernel<<<unsigned(last - first), TEAM_THREADS, 0, st>>>(
             reinterpret_cast<const uint32_t*>(dc) + first * 16, reinterpret_cast<const uint32_t*>(dv) + first * 8,
             reinterpret_cast<const uint32_t*>(dw) + first * 16, reinterpret_cast<const uint32_t*>(dd) + first * 32,
             table, lines, slots[0], dslots, programs, prog_off, dr + first);
@@ -756,9 +764,10 @@ void pairs_team_launch(Owner* o, const uint32_t* dg1, const uint32_t* dg2, const
     const uint32_t* programs = team_programs(o, prog_off, st);
     auto* dslots = static_cast<uint32_t*>(c.staged.ensure(std::max<size_t>(slots.size(), 1) * 4));
     HIP_CHECK(hipMemcpyAsync(dslots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, st));
-    k_pairs_team<<<unsigned(count), TEAM_THREADS, 0, st>>>(dg1, dg2, dfirst, nullptr, terms, negate_rest, dslots,
-                                                          static_cast<const uint32_t*>(c.lines.ptr), programs,
-                                                          prog_off, dres, dgt);
+    auto* kernel = count > TEAM_ONE_WAVE_MAX ? k_pairs_team<2> : k_pairs_team<1>;
+    kernel<<<unsigned(count), TEAM_THREADS, 0, st>>>(dg1, dg2, dfirst, nullptr, terms, negate_rest, dslots,
+                                                    static_cast<const uint32_t*>(c.lines.ptr), programs, prog_off,
+                                                    dres, dgt);
     HIP_CHECK_LAUNCH();
     // the pageable slot array must outlive its copy
     HIP_CHECK(hipStreamSynchronize(st));
