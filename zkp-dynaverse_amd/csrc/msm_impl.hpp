@@ -28,6 +28,7 @@
 #include <utility>
 
 #include "host_g1.hpp"
+#include "host_g2.hpp"
 #include "host_pool.hpp"
 
 namespace dvgpu {
@@ -1477,26 +1478,8 @@ void msm_batch_finalize_enqueue(const uint32_t* window_words, const MsmBatchPlan
     HIP_CHECK_LAUNCH();
 }
 
-template <class F>
-struct HostMem;
-template <>
-struct HostMem<FqOps> {
-    static Fq ld(const uint32_t* p) {
-        uint32_t w[8];
-        std::memcpy(w, p, 32);
-        return unpack<FqParams>(w);
-    }
-    static Xyzz<FqOps> load_xyzz(const uint32_t* p) { return {ld(p), ld(p + 8), ld(p + 16), ld(p + 24)}; }
-};
-template <>
-struct HostMem<Fq2Ops> {
-    static Fq2 ld(const uint32_t* p) { return {HostMem<FqOps>::ld(p), HostMem<FqOps>::ld(p + 8)}; }
-    static Xyzz<Fq2Ops> load_xyzz(const uint32_t* p) { return {ld(p), ld(p + 16), ld(p + 32), ld(p + 48)}; }
-};
-
-
-// Point arithmetic of the host epilogues: G1 in the host's own 64-bit-limb
-// field (host_g1.hpp), G2 in the device code compiled for the host.
+// Point arithmetic of the host epilogues, in the host's own 64-bit-limb
+// field: G1 (host_g1.hpp) and G2 (host_g2.hpp).
 template <class F>
 struct HostArith;
 template <>
@@ -1516,16 +1499,19 @@ struct HostArith<FqOps> {
 };
 template <>
 struct HostArith<Fq2Ops> {
-    using P = Xyzz<Fq2Ops>;
-    static P load(const uint32_t* w) { return HostMem<Fq2Ops>::load_xyzz(w); }
-    static P inf() { return xyzz_inf<Fq2Ops>(); }
-    static P add(const P& a, const P& b) { return xyzz_add<Fq2Ops>(a, b); }
-    static P dbl(const P& a) { return xyzz_dbl<Fq2Ops>(a); }
-    static P dbl_n(P a, int n) {
-        for (int k = 0; k < n; ++k) a = xyzz_dbl<Fq2Ops>(a);
-        return a;
+    using P = hostg2::Xyzz;
+    static P load(const uint32_t* w) { return hostg2::from_device_xyzz(w); }
+    static P inf() { return hostg2::infinity(); }
+    static P add(const P& a, const P& b) { return hostg2::add(a, b); }
+    static P dbl(const P& a) { return hostg2::dbl(a); }
+    static P dbl_n(const P& a, int n) { return hostg2::dbl_n(a, n); }
+    static Affine<Fq2Ops> affine(const P& p) {
+        if (hostg2::is_inf(p)) return {f2_zero(), f2_zero(), true};
+        uint32_t w[4][8];
+        hostg2::to_affine_words(p, w);
+        return {{to_mont<FqParams>(w[0]), to_mont<FqParams>(w[1])}, {to_mont<FqParams>(w[2]), to_mont<FqParams>(w[3])},
+                false};
     }
-    static Affine<Fq2Ops> affine(const P& p) { return xyzz_to_affine<Fq2Ops>(p); }
 };
 
 template <class F>
