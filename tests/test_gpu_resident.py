@@ -2,10 +2,12 @@
 semantics of crates/protocol/src/incremental_rebuild/operators.rs:387-590
 (restated here with the oracle's field and group arithmetic) and the
 reference's native rules (native/src/icicle_resident.cu)."""
+import copy
+
 import pytest
 
 from dvgpu import BackendError, HipBackend
-from dvgpu.resident import ADD, FR, G1, MUL, SUB, ResidentBackend
+from dvgpu.resident import ADD, FR, G1, MUL, SUB, FourStepPlan, ResidentBackend
 from dvgpu.rng import ChaCha20Stream
 from oracle import bn254 as o
 
@@ -170,6 +172,84 @@ def test_four_step_packet_g1(rb):
     rb.write_points(a, 0, fwd)
     rb.transform_four_step(G1, a, s, c, 16, True)
     assert rb.read_points(c, 0, 16) == pts
+
+
+def drain(plan, rb):
+    steps = 0
+    while not plan.next_step(rb):
+        steps += 1
+        assert steps < 100_000
+    return steps + 1
+
+
+def test_four_step_cursor_session_65536_rebuild_length(rb):
+    """The rebuild's 2^17 transform of an n = 65536 session as the reference
+    schedules it (program.rs:84-104: tile 512, row batch 16; 816 resident
+    calls) against the oracle's NTT, and back."""
+    n = 1 << 17
+    vals = rand_fr(40, n)
+    a, s, c, d = rb.alloc(FR, n), rb.alloc(FR, n), rb.alloc(FR, n), rb.alloc(FR, n)
+    rb.write_fields(a, 0, vals)
+    fwd = FourStepPlan(FR, n, a, s, c, 512, 16, False)
+    assert drain(fwd, rb) == fwd.total_steps == 816
+    got = rb.read_fields(c, 0, n)
+    assert got == o.ntt(vals)
+    inv = FourStepPlan(FR, n, c, s, d, 512, 16, True)
+    drain(inv, rb)
+    assert rb.read_fields(d, 0, n) == vals
+
+
+@pytest.mark.parametrize("tile,row_batch", [(64, 1), (100, 4), (4096, 16), (4096, 128)])
+def test_four_step_cursor_equals_packet(rb, tile, row_batch):
+    n = 4096
+    vals = rand_fr(41, n)
+    a, s, c, p = rb.alloc(FR, n), rb.alloc(FR, n), rb.alloc(FR, n), rb.alloc(FR, n)
+    rb.write_fields(a, 0, vals)
+    for inverse in (False, True):
+        FourStepPlan(FR, n, a, s, p, tile, row_batch, inverse).execute_packet(rb)
+        drain(FourStepPlan(FR, n, a, s, c, tile, row_batch, inverse), rb)
+        assert rb.read_fields(c, 0, n) == rb.read_fields(p, 0, n), (tile, row_batch, inverse)
+
+
+def test_four_step_cursor_g1(rb):
+    """Group transform by the cursor (every 17th point the identity, as in
+    the reference's packet probe, transform_packet.rs:218) against the
+    oracle's group FFT, forward and inverse."""
+    n = 64
+    ks = rand_fr(42, n)
+    pts = [None if i % 17 == 0 else o.g1_mul(k) for i, k in enumerate(ks)]
+    a, s, c = rb.alloc(G1, n), rb.alloc(G1, n), rb.alloc(G1, n)
+    rb.write_points(a, 0, pts)
+    for inverse in (False, True):
+        plan = FourStepPlan(G1, n, a, s, c, 8, 16, inverse)
+        assert drain(plan, rb) == plan.total_steps
+        assert rb.read_points(c, 0, n) == o.ecntt_fft(pts, inverse), inverse
+
+
+def test_four_step_committed_cursor_retried_after_rollback():
+    """operators_tests.rs:280-303 on the device: 15 steps of a tentative
+    copy inside a transaction, rollback restores the zeroed output, and the
+    committed cursor then runs to the natural-order transform."""
+    b = HipBackend()
+    r = ResidentBackend(b, limit=1 << 30, journal=1 << 20)
+    a, s, c = r.alloc(FR, 32), r.alloc(FR, 32, True), r.alloc(FR, 32, True)
+    vals = [(i * i + 3 * i + 11) % o.R for i in range(32)]
+    r.write_fields(a, 0, vals)
+    r.seal()
+    committed = FourStepPlan(FR, 32, a, s, c, 3, 2, False)
+    r.begin_transaction()
+    tentative = copy.copy(committed)
+    for _ in range(15):
+        assert not tentative.next_step(r)
+    with pytest.raises(BackendError):
+        r.write_fields(c, 32, [1])                 # out of bounds: the call fails
+    r.rollback()
+    assert r.read_fields(c, 0, 32) == [0] * 32
+    r.begin_transaction()
+    drain(committed, r)
+    r.commit_transaction()
+    assert r.read_fields(c, 0, 32) == o.ntt(vals)
+    b.close()
 
 
 def test_resident_msms(rb):

@@ -747,7 +747,8 @@ def row_figures(backend: HipBackend, reps: int = 5, log=print) -> Dict:
     fixed-base kernel), G2 wire validation of 4096 subgroup points (all
     accepted), and through the resident engine: an ECNTT over 2^12 G1 points
     (forward then inverse restores the input), a resident G1 MSM over 2^16
-    bases, and a 64-point G1 read normalised on the device."""
+    bases, a 64-point G1 read normalised on the device, and the rebuild's
+    four-step cursor over resident calls."""
     from .codec import FR_MODULUS, encode_fr
     from .resident import FR, G1, ResidentBackend
     from .workload import poly_eval, srs_scalars
@@ -822,6 +823,31 @@ def row_figures(backend: HipBackend, reps: int = 5, log=print) -> Dict:
     t, _ = med(ecntt16)
     ok16 = rb.read_points(d16, 0, 8) == bases[:8] and rb.read_points(d16, nb - 8, 8) == bases[nb - 8:]
     out["resident_ecntt_2p16_fwd_inv"] = {"ms": round(t, 3), "ok": bool(ok16)}
+    # The rebuild's four-step cursor (operators.rs:505-597) as the session
+    # schedules it (program.rs:84-104): Fr at 2n = 2^17 with tile 512 and
+    # row batch 16 (816 resident calls), its initialization plan (12 calls),
+    # and G1 at 2^16 with tile 256 and row batch 16; each forward then
+    # inverse, the round trip checked.
+    from .resident import FourStepPlan
+    n17 = 1 << 17
+    fv = ChaCha20Stream(2034).uniform_fr(n17)
+    fa, fs, fc, fd = (rb.alloc(FR, n17) for _ in range(4))
+    rb.write_fields(fa, 0, fv)
+
+    def cursor(kind, a, s, c, d, n, tile, batch, init=False):
+        calls = 0
+        for src, dst, inv in ((a, c, False), (c, d, True)):
+            plan = FourStepPlan(kind, n, src, s, dst, tile, batch, inv)
+            calls += (plan.initialization_plan() if init else plan).run(rb)
+        return calls
+    for name, init in (("resident_four_step_fr_2p17_row16", False), ("resident_four_step_fr_2p17_init", True)):
+        t, calls = med(lambda: cursor(FR, fa, fs, fc, fd, n17, 512, 16, init))
+        ok = rb.read_fields(fd, 0, 8) == fv[:8] and rb.read_fields(fd, n17 - 8, 8) == fv[n17 - 8:]
+        out[name] = {"ms": round(t, 3), "calls": calls, "ok": bool(ok)}
+    e16 = rb.alloc(G1, nb)
+    t, calls = med(lambda: cursor(G1, s16, m16, d16, e16, nb, 256, 16))
+    ok = rb.read_points(e16, 0, 8) == bases[:8] and rb.read_points(e16, nb - 8, 8) == bases[nb - 8:]
+    out["resident_four_step_g1_2p16_row16"] = {"ms": round(t, 3), "calls": calls, "ok": bool(ok)}
     log("row figures: " + ", ".join(f"{key} {v['ms']} ms ok={v['ok']}" for key, v in out.items()))
     return out
 

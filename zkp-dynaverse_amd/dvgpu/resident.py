@@ -11,7 +11,7 @@ from typing import List, Optional, Sequence
 
 from . import _lib
 from .backend import BackendError, HipBackend, _check, _err
-from .codec import decode_g1, decode_g2, encode_fr, encode_g1, encode_g2, root_of_unity_2_20
+from .codec import FR_MODULUS, decode_g1, decode_g2, encode_fr, encode_g1, encode_g2, root_of_unity_2_20
 
 ALLOC, LENGTH, KIND, BEGIN, ROLLBACK, WRITE, READ, COPY, REVERSE, TRANSPOSE, TRANSFORM, ZIP, SCALE, \
     MSM_G1, MSM_G2, ALLOC_BASES = range(1, 17)
@@ -144,3 +144,101 @@ class ResidentBackend:
         if self.b.lib.dvgpu_resident_diagnostics(self.b.handle, ctypes.byref(st)) != 0:
             raise BackendError(-1, "invalid resident diagnostics owner")
         return {k: getattr(st, k) for k, _ in st._fields_}
+
+
+class FourStepPlan:
+    """Natural-order four-step transform of length n as a resumable cursor
+    of resident calls (crates/protocol/src/incremental_rebuild/operators.rs:
+    505-597).  With a = 2^floor(log2 n / 2) and b = n / a:
+
+      phase 0  transpose input (a x b) -> output        tile_len elements a call
+      phase 1  b transforms of length a, output -> scratch   row_batch rows a call
+      phase 2  transpose scratch (b x a) -> output, element (row, col) times
+               w^(row * col)                              tile_len elements a call
+      phase 3  a transforms of length b, output -> scratch   row_batch rows a call
+      phase 4  transpose scratch (a x b) -> output        tile_len elements a call
+
+    w is the generator of order n (its inverse for the inverse transform;
+    the row transforms scale by 1/a and 1/b, so the whole scales by 1/n).
+    A failed step leaves the cursor where it was; copy.copy() of a plan is
+    the reference's clone, kept by a caller until its transaction commits.
+    """
+
+    PHASES = 5
+
+    def __init__(self, kind, n, input, scratch, output, tile_len, row_batch, inverse):
+        if n < 1 or n & (n - 1):
+            raise ValueError("four-step length must be a power of two")
+        if tile_len <= 0 or row_batch <= 0:
+            raise ValueError("four-step block bounds must be positive")
+        if input == scratch or input == output or scratch == output:
+            raise ValueError("four-step buffers must be distinct")
+        if n > 1 << 20:
+            raise ValueError("unsupported four-step root order")
+        log = n.bit_length() - 1
+        self.kind, self.n, self.inverse = kind, n, inverse
+        self.a = 1 << (log // 2)
+        self.b = n // self.a
+        self.input, self.scratch, self.output = input, scratch, output
+        self.tile_len, self.row_batch = tile_len, row_batch
+        root = pow(root_of_unity_2_20(), (1 << 20) // n, FR_MODULUS)
+        self.root = pow(root, FR_MODULUS - 2, FR_MODULUS) if inverse else root
+        tiles = -(-n // tile_len)
+        self.total_steps = 3 * tiles + -(-self.a // row_batch) + -(-self.b // row_batch)
+        self.phase, self.cursor = 0, 0
+
+    def dimensions(self):
+        return self.a, self.b
+
+    def is_complete(self) -> bool:
+        return self.phase == self.PHASES
+
+    def initialization_plan(self) -> "FourStepPlan":
+        """Initialization pays no per-update latency bound: contiguous rows
+        and tiles of up to 65536 elements a call."""
+        cap = 65536
+        return FourStepPlan(self.kind, self.n, self.input, self.scratch, self.output, min(self.n, cap),
+                            max(1, cap // max(self.a, self.b)), self.inverse)
+
+    def execute_packet(self, backend: ResidentBackend) -> None:
+        """The whole transform as one resident packet call (n <= 4096)."""
+        if self.phase or self.cursor:
+            raise ValueError("transform packet must start at an untouched plan")
+        backend.transform_four_step(self.kind, self.input, self.scratch, self.output, self.n, self.inverse)
+
+    def next_step(self, backend: ResidentBackend) -> bool:
+        """Run one tile or row batch; True once the last one has run."""
+        if self.is_complete():
+            return True
+        if self.phase == 0 and self.cursor == 0:
+            for h in (self.input, self.scratch, self.output):
+                if backend.buffer_kind(h) != self.kind:
+                    raise ValueError("four-step buffer kind mismatch")
+                if backend.buffer_len(h) < self.n:
+                    raise ValueError("four-step buffer too short")
+        phase_len = {1: self.b, 3: self.a}.get(self.phase, self.n)
+        block = self.row_batch if self.phase in (1, 3) else self.tile_len
+        count = min(block, phase_len - self.cursor)
+        a, b, k = self.a, self.b, self.kind
+        if self.phase == 0:
+            backend.transpose(k, self.input, self.output, a, b, self.cursor, count)
+        elif self.phase == 1:
+            backend.transform_rows(k, self.output, self.scratch, a, self.cursor, count, self.inverse)
+        elif self.phase == 2:
+            backend.transpose(k, self.scratch, self.output, b, a, self.cursor, count, twiddle=self.root)
+        elif self.phase == 3:
+            backend.transform_rows(k, self.output, self.scratch, b, self.cursor, count, self.inverse)
+        else:
+            backend.transpose(k, self.scratch, self.output, a, b, self.cursor, count)
+        self.cursor += count
+        if self.cursor == phase_len:
+            self.phase, self.cursor = self.phase + 1, 0
+        return self.is_complete()
+
+    def run(self, backend: ResidentBackend) -> int:
+        """Every remaining step; returns the number run."""
+        steps = 0
+        while not self.is_complete():
+            self.next_step(backend)
+            steps += 1
+        return steps
