@@ -112,11 +112,11 @@ def test_large_tree_batches_levels(backend):
 
 
 def test_mid_size_tree_device_rounds_checked_everywhere(backend):
-    """400 points, so the upper levels' products (up to 200 x 200
-    coefficient pairs) run as device rounds: the vanishing polynomial, the
+    """400 points, a tree kept in HBM (nodes up to 256 leaves, the last
+    nodes of every level partly filled): the vanishing polynomial, the
     values at every point, the interpolation, the inverse derivatives and the
     Cauchy sums all against the oracle's direct forms; Newton division with a
-    degree-300 divisor through the same device products."""
+    degree-300 divisor through the batched device products."""
     rng = random.Random(46)
     n = 400
     points = rng.sample(range(1, 1 << 40), n)
@@ -142,3 +142,57 @@ def test_mid_size_tree_device_rounds_checked_everywhere(backend):
     bad[5] = (bad[5] + 1) % o.R
     with pytest.raises(BackendError):
         newton_divide_exact(backend, bad, den)
+
+
+@pytest.mark.parametrize("n", [64, 513])
+def test_device_tree_boundary_shapes(backend, n):
+    """Trees in HBM where they start (64 points, a power of two) and with a
+    one-point node at the top split (513 = 512 + 1): every output against the
+    oracle's direct forms, and a polynomial longer than the point count
+    (reduced modulo the vanishing polynomial before the remainder tree)."""
+    rng = random.Random(47 + n)
+    points = rng.sample(range(1, 1 << 40), n)
+    with SubproductTree(backend, points) as tree:
+        assert tree.stats()["device_products"] >= 1
+        van = [1]
+        for x in points:
+            van = o.poly_mul(van, [(-x) % o.R, 1])
+        assert tree.vanishing() == van
+        poly = [rng.randrange(o.R) for _ in range(n)]
+        vals = tree.multipoint_eval(poly)
+        assert vals == [o.poly_eval(poly, x) for x in points]
+        assert tree.interpolate(vals) == o.poly_trim(poly)
+        longer = [rng.randrange(o.R) for _ in range(2 * n + 3)]
+        assert tree.multipoint_eval(longer) == [o.poly_eval(longer, x) for x in points]
+        der = o.poly_trim([i * c for i, c in enumerate(van)][1:])
+        assert tree.inverse_derivatives() == [pow(o.poly_eval(der, x), o.R - 2, o.R) for x in points]
+        weights = [rng.randrange(o.R) for _ in range(n)]
+        assert tree.cauchy_sums(weights) == o.cauchy_sums_direct(points, weights)
+        assert tree.interpolate([0] * n) == [0]
+
+
+def test_device_tree_keeps_its_root(backend):
+    """A call naming another root of order 2^20 gets the same values: the
+    tree's transformed rows were made with the root of its build and its
+    rounds keep to that root; the owner's later products agree too."""
+    import ctypes
+
+    from dvgpu import _lib
+    from dvgpu.poly import _buf, _dec, _enc
+    rng = random.Random(49)
+    n = 100
+    points = rng.sample(range(1, 1 << 40), n)
+    poly = [rng.randrange(o.R) for _ in range(n)]
+    expect = [o.poly_eval(poly, x) for x in points]
+    other = _buf(o.fr_to_bytes(pow(o.ROOT_2_20, 3, o.R)))
+    with SubproductTree(backend, points) as tree:
+        out = ctypes.create_string_buffer(32 * n)
+        err = ctypes.create_string_buffer(256)
+        st = _lib.DvgpuPolyStats()
+        assert backend.lib.dvgpu_poly_tree_eval(backend.handle, other, tree.handle, _buf(_enc(poly)), n, out,
+                                                ctypes.byref(st), err, 256) == 0, err.value
+        assert _dec(out.raw, n) == expect
+        assert tree.multipoint_eval(poly) == expect
+        assert tree.interpolate(expect) == o.poly_trim(poly)
+    a = [rng.randrange(o.R) for _ in range(70)]
+    assert fft_multiply(backend, a, poly) == o.poly_mul(a, poly)

@@ -10,6 +10,8 @@
 // fused inverse batch, one download.  Polynomials live on the host as
 // canonical little-endian words; the orchestration mirrors the reference's
 // recursion level by level, so every result is the same polynomial or value.
+// Subproduct trees over DEV_TREE_MIN_POINTS points or more are kept in HBM
+// instead and run their rounds there (poly_tree.hip).
 // Intermediate remainders are kept untrimmed (same polynomials, possibly
 // with zero high coefficients); the outputs are trimmed as the reference's.
 #include <algorithm>
@@ -24,6 +26,7 @@
 #include "../../include/dynaverse_poly.h"
 #include "g1mem.hpp"
 #include "owner.hpp"
+#include "poly_tree.hpp"
 
 namespace dvgpu {
 
@@ -390,7 +393,9 @@ void write_poly(const Poly& p, uint8_t* out, size_t cap, size_t* len) {
 
 // A balanced subproduct tree (fast.rs:9-64): level 0 holds X - x_i, pairs
 // multiply upward, an unpaired node is carried.  recip[l][i] is
-// rev(levels[l][i])^-1 to the length the remainder tree needs.
+// rev(levels[l][i])^-1 to the length the remainder tree needs.  From
+// DEV_TREE_MIN_POINTS points the tree lives in HBM instead (`dev`, built and
+// used by poly_tree.hip) and the host fields other than `points` stay empty.
 struct PolyTree {
     Owner* owner = nullptr;
     std::vector<W8> points;
@@ -398,6 +403,8 @@ struct PolyTree {
     std::vector<std::vector<Poly>> recip;
     std::vector<W8> derivatives;          // A'(x_i)
     std::vector<W8> inverse_derivatives;  // 1 / A'(x_i)
+    DevTree* dev = nullptr;
+    ~PolyTree() { dev_tree_destroy(dev); }
 };
 
 void poly_tree_destroy(PolyTree* t) noexcept { delete t; }
@@ -492,6 +499,12 @@ Poly tree_interpolate(PolyEngine& e, const PolyTree& t, const std::vector<W8>& v
     return out;
 }
 
+void add_stats(PolyEngine& e, const DevTreeStats& st) {
+    e.stats_.rounds += st.rounds;
+    e.stats_.products += st.products;
+    e.stats_.device_products += st.products;
+}
+
 PolyTree* tree_build(PolyEngine& e, Owner* o, const std::vector<W8>& pts) {
     auto t = std::make_unique<PolyTree>();
     t->owner = o;
@@ -500,6 +513,12 @@ PolyTree* tree_build(PolyEngine& e, Owner* o, const std::vector<W8>& pts) {
     for (const W8& p : pts)
         require(seen.insert(std::string(reinterpret_cast<const char*>(p.data()), 32)).second,
                 "duplicate interpolation point");
+    if (pts.size() >= DEV_TREE_MIN_POINTS) {
+        DevTreeStats st;
+        t->dev = dev_tree_build(o, pts[0].data(), pts.size(), st);
+        add_stats(e, st);
+        return t.release();
+    }
     std::vector<Poly> leaves;
     const W8 one = from_fr(fp_one<FrParams>());
     for (const W8& p : pts) leaves.push_back(Poly{neg8(p), one});
@@ -557,6 +576,28 @@ PolyTree* tree_build(PolyEngine& e, Owner* o, const std::vector<W8>& pts) {
         }
     }
     return t.release();
+}
+
+// Values of p at a device tree's points; p of degree >= k is first reduced
+// modulo the vanishing polynomial (the top division of fast.rs:73-84).
+std::vector<W8> dev_eval(PolyEngine& e, const PolyTree& t, Poly p) {
+    const size_t k = t.points.size();
+    if (p.size() > k) {
+        Poly A(k + 1);
+        dev_tree_vanishing(t.dev, A[0].data());
+        std::vector<const Poly*> num{&p}, den{&A};
+        std::vector<Poly> rem;
+        e.div_rem(num, den, nullptr, nullptr, rem);
+        p = std::move(rem[0]);
+    }
+    std::vector<W8> v(k);
+    const uint32_t* pp = p[0].data();
+    const size_t len = p.size();
+    uint32_t* vp = v[0].data();
+    DevTreeStats st;
+    dev_tree_eval(t.dev, &pp, &len, 1, &vp, st);
+    add_stats(e, st);
+    return v;
 }
 
 template <class Body>
@@ -646,6 +687,12 @@ int dvgpu_poly_tree_vanishing(void* backend, void* tree, uint8_t* out, size_t ou
     Owner* o = as_owner(backend);
     return guarded(o, error, capacity, [&]() -> int {
         PolyTree* t = find_tree(o, tree);
+        if (t->dev) {
+            Poly v(t->points.size() + 1);
+            dev_tree_vanishing(t->dev, v[0].data());
+            write_poly(v, out, out_capacity, out_len);
+            return 0;
+        }
         write_poly(t->levels.back()[0], out, out_capacity, out_len);
         return 0;
     });
@@ -656,6 +703,15 @@ int dvgpu_poly_tree_inverse_derivatives(void* backend, void* tree, uint8_t* out,
     Owner* o = as_owner(backend);
     return guarded(o, error, capacity, [&]() -> int {
         PolyTree* t = find_tree(o, tree);
+        if (t->dev) {
+            require(out_len && out, "null output");
+            require(t->points.size() <= out_capacity, "output capacity too small");
+            Poly v(t->points.size());
+            dev_tree_inverse_derivatives(t->dev, v[0].data());
+            std::memcpy(out, v.data(), v.size() * 32);
+            *out_len = v.size();
+            return 0;
+        }
         require(out_len && (out || t->inverse_derivatives.empty()), "null output");
         require(t->inverse_derivatives.size() <= out_capacity, "output capacity too small");
         for (size_t i = 0; i < t->inverse_derivatives.size(); ++i)
@@ -673,7 +729,7 @@ int dvgpu_poly_tree_eval(void* backend, const uint8_t* root, void* tree, const u
         require(values || t->points.empty(), "null values");
         Poly p = read_poly(poly, len);
         trim(p);
-        const std::vector<W8> v = tree_eval(e, *t, p);
+        const std::vector<W8> v = t->dev ? dev_eval(e, *t, p) : tree_eval(e, *t, p);
         for (size_t i = 0; i < v.size(); ++i) std::memcpy(values + 32 * i, v[i].data(), 32);
     });
 }
@@ -685,6 +741,16 @@ int dvgpu_poly_tree_interpolate(void* backend, const uint8_t* root, void* tree, 
         PolyTree* t = find_tree(o, tree);
         require(values || !count, "null values");
         const Poly v = read_poly(values, count);
+        if (t->dev) {
+            require(v.size() == t->points.size(), "interpolation value count differs from points");
+            Poly c(v.size());
+            DevTreeStats st;
+            dev_tree_interpolate(t->dev, v[0].data(), c[0].data(), st);
+            add_stats(e, st);
+            trim(c);
+            write_poly(c, out, out_capacity, out_len);
+            return;
+        }
         write_poly(tree_interpolate(e, *t, v), out, out_capacity, out_len);
     });
 }
@@ -700,6 +766,12 @@ int dvgpu_poly_tree_cauchy_sums(void* backend, const uint8_t* root, void* tree, 
         require(sums || !count, "null sums");
         if (!count) return;
         const Poly w = read_poly(weights, count);
+        if (t->dev) {
+            DevTreeStats st;
+            dev_tree_cauchy(t->dev, w[0].data(), reinterpret_cast<uint32_t*>(sums), st);
+            add_stats(e, st);
+            return;
+        }
         std::vector<W8> nv;
         for (size_t i = 0; i < count; ++i) nv.push_back(from_fr(mul(to_fr(w[i]), to_fr(t->derivatives[i]))));
         const Poly num = tree_interpolate(e, *t, nv);
