@@ -30,6 +30,7 @@
 #include "ntt_tile.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <vector>
 
@@ -109,11 +110,20 @@ bool fr_is_one(const Fr& x) {
     return true;
 }
 
-// The Montgomery limbs of 1/2^L.
+// The Montgomery limbs of 1/2^L, from a table made once per process (a
+// Fermat inversion on the host per inverse transform would hold the stream
+// for several microseconds before every launch).
 Fr inverse_scale(int L) {
-    uint32_t nw[8] = {0};
-    nw[L / 32] = 1u << (L % 32);
-    return canon(inv(to_mont<FrParams>(nw)));
+    static const std::array<Fr, NTT_MAX_LOG + 1> table = [] {
+        std::array<Fr, NTT_MAX_LOG + 1> t{};
+        for (int l = 0; l <= NTT_MAX_LOG; ++l) {
+            uint32_t nw[8] = {0};
+            nw[l / 32] = 1u << (l % 32);
+            t[size_t(l)] = canon(inv(to_mont<FrParams>(nw)));
+        }
+        return t;
+    }();
+    return table[size_t(L)];
 }
 
 // The pass-1 table of length 2^L for one direction, built on first use.
@@ -236,6 +246,11 @@ static void run_rows(NttDomain& d, const uint32_t* in, uint32_t* out, int L, siz
     const LengthPlan lp = plan_entry(d, in, out, work, L, rows, inverse, stream);
     for (int p = 0; p < lp.passes; ++p)
         launch_pass(lp.pass[p], radix4_for(d, L, lp.blocks[p], lp.tile[p]), lp.blocks[p], lp.tile[p], stream);
+}
+
+Fr ntt_inverse_scale(int log_n) {
+    require(log_n >= 0 && log_n <= NTT_MAX_LOG, "NTT length must be a power of two <= 2^20");
+    return inverse_scale(log_n);
 }
 
 void ntt_rows(NttDomain& d, const uint32_t* in, uint32_t* out, int log_n, size_t rows, bool inverse,

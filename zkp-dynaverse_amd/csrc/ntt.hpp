@@ -74,6 +74,9 @@ struct NttWorkspace {
     void release() { a.release(); b.release(); }
 };
 
+// 1 / 2^log_n as canonical Montgomery limbs (a table made once per process).
+bn254::Fr ntt_inverse_scale(int log_n);
+
 // rows independent transforms of length n = 2^log_n, each row contiguous.
 // The transform is linear and its twiddles are Montgomery values, so words
 // in give words out in the same form: canonical in, canonical out

@@ -608,9 +608,10 @@ void dev_tree_cauchy(DevTree* t, const uint32_t* weights, uint32_t* sums, DevTre
     uint32_t* vals = t->remainder_tree(2);
     // the weights again (ra and rb were overwritten), then the combination
     t->upload(t->P(t->z), weights, k, 0);
-    // the word of 1/2 (k_cauchy's operands are words)
-    uint32_t two[8] = {2, 0, 0, 0, 0, 0, 0, 0}, hw[8];
-    from_mont(inv(to_mont<FrParams>(two)), hw);
+    // the word of 1/2 = (r + 1) / 2 = (r >> 1) + 1 (r odd; k_cauchy's operands are words)
+    uint32_t hw[8];
+    for (int q = 0; q < 8; ++q) hw[q] = (FR_MOD32[q] >> 1) | (q < 7 ? FR_MOD32[q + 1] << 31 : 0u);
+    hw[0] += 1;  // r >> 1 ends in ...f8000000: no carry
     k_cauchy<<<grid_for(k, BLOCK), BLOCK, 0, s>>>(vals, vals + K * 8, t->P(t->z), t->P(t->ideriv), t->P(t->y), k,
                                                   unpack<FrParams>(hw));
     HIP_CHECK_LAUNCH();

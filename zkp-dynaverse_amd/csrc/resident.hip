@@ -489,9 +489,7 @@ struct ResidentState {
         k_ecntt_bitrev<<<grid_for(total, 64), 64, 0, stream()>>>(in, work, L, total);
         HIP_CHECK_LAUNCH();
         gfft_rows(domain, work, L, rows, true, stream());
-        uint32_t nw[8] = {0};
-        nw[L / 32] = 1u << (L % 32);
-        const Fr ninv = canon(inv(to_mont<FrParams>(nw)));
+        const Fr ninv = ntt_inverse_scale(L);
         k_g1_mul_const<<<grid_for(total, 64), 64, 0, stream()>>>(work, out, total, ninv);
     }
 
