@@ -196,3 +196,21 @@ def test_device_tree_keeps_its_root(backend):
         assert tree.interpolate(expect) == o.poly_trim(poly)
     a = [rng.randrange(o.R) for _ in range(70)]
     assert fft_multiply(backend, a, poly) == o.poly_mul(a, poly)
+
+
+def test_device_tree_at_the_point_limit(backend):
+    """2^19 points, the largest tree (the entry point refuses one more):
+    values of a degree-(2^19 - 1) polynomial spot-checked by Horner at both
+    ends and inside, and the interpolation of all 2^19 values giving the
+    polynomial back exactly (a size-independent round trip)."""
+    rng = random.Random(51)
+    n = 1 << 19
+    points = rng.sample(range(1, 1 << 60), n)
+    poly = [rng.randrange(o.R) for _ in range(n)]
+    with SubproductTree(backend, points) as tree:
+        vals = tree.multipoint_eval(poly)
+        for i in (0, 1, 4097, n // 2, n - 2, n - 1):
+            assert vals[i] == o.poly_eval(poly, points[i]), i
+        assert tree.interpolate(vals) == o.poly_trim(poly)
+    with pytest.raises(BackendError):
+        SubproductTree(backend, points + [(1 << 60) + 5])
