@@ -286,6 +286,33 @@ static __global__ void k_digits(const MsmDesc* __restrict__ descs, uint32_t m, u
     }
 }
 
+// Digit planes (group mode): every scalar is read and recoded once, its
+// signed digits written as 16-bit entries, plane w of MSM d at
+// d.entry_base + w d.n (magnitude | sign << 15; 0 = no digit; magnitudes are
+// at most 2^14 as bucket sets fit the LDS histogram).  The counting and
+// staging blocks, one per (chunk, window), then read two bytes per point
+// instead of the whole scalar.
+static __global__ void __launch_bounds__(256)
+    k_digit_planes(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t total_threads,
+                   const uint32_t* __restrict__ scalars, uint16_t* __restrict__ planes) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total_threads) return;
+    const MsmDesc d = descs[find_desc<&MsmDesc::thread_base>(descs, m, t)];
+    const uint32_t i = t - d.thread_base;
+    const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(d.scalar_base + i) * 8);
+    const uint4 a = sp[0], b = sp[1];
+    const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t kconst[9], r[9];
+    recode_constant(d, kconst);
+    recode_add(s, kconst, r);
+    for (uint32_t w = 0; w < d.digit_windows; ++w) {
+        const int c = int(window_width(d, w));
+        const int32_t digit = int32_t(window_of(r, window_offset(d, w), c)) - int32_t((1u << (c - 1)) - 1);
+        const uint32_t v = uint32_t(digit < 0 ? -digit : digit), sign = digit < 0 ? 1u : 0u;
+        planes[size_t(d.entry_base) + size_t(w) * d.n + i] = uint16_t(v | (sign << 15));
+    }
+}
+
 // Blocks of the LDS digit path map to XCDs round-robin; consecutive logical
 // blocks (the windows of one chunk of points) are placed on one XCD so that
 // its L2 serves the chunk's scalars to all of them.
@@ -303,12 +330,15 @@ __device__ __forceinline__ uint32_t xcd_grouped_block(uint32_t bid, uint32_t gri
 // plus the block's prefix among the bucket's blocks; the block recomputes
 // the digits and places every point with an LDS atomic on that start.  (The
 // order within a bucket is arbitrary; a bucket's sum does not depend on it.)
-template <bool PLACE>
+// PLANES (the counting pass of group mode): the digits come from the
+// planes of k_digit_planes instead of the scalars.
+template <bool PLACE, bool PLANES = false>
 static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
     k_digits_lds(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t grid, const uint32_t* __restrict__ scalars,
                  uint32_t* __restrict__ hist, const uint2* __restrict__ offsets,
                  const uint32_t* __restrict__ point_index, uint32_t point_limit,
-                 uint32_t* __restrict__ flags, uint32_t* __restrict__ sorted) {
+                 uint32_t* __restrict__ flags, uint32_t* __restrict__ sorted,
+                 const uint16_t* __restrict__ planes = nullptr) {
     __shared__ uint32_t counts[MSM_LDS_BUCKETS];
     const uint32_t g = xcd_grouped_block(blockIdx.x, grid);
     const MsmDesc d = descs[find_desc<&MsmDesc::hist_block_base>(descs, m, g)];
@@ -325,17 +355,26 @@ static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
     uint32_t kconst[9];
     recode_constant(d, kconst);
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(d.scalar_base + i) * 8);
-        const uint4 a = sp[0], b = sp[1];
-        const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        uint32_t t[9];
-        recode_add(s, kconst, t);
+        uint32_t t[9] = {0};
+        if (!PLANES) {
+            const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(d.scalar_base + i) * 8);
+            const uint4 a = sp[0], b = sp[1];
+            const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            recode_add(s, kconst, t);
+        }
         for (uint32_t w = w0; w < w1; ++w) {
-            const int c = int(window_width(d, w));
-            const int32_t digit = int32_t(window_of(t, window_offset(d, w), c)) - int32_t((1u << (c - 1)) - 1);
-            if (digit == 0) continue;
-            const uint32_t sign = digit < 0 ? 1u : 0u;
-            const uint32_t v = uint32_t(digit < 0 ? -digit : digit);
+            uint32_t v, sign;
+            if (PLANES) {
+                const uint32_t e = planes[size_t(d.entry_base) + size_t(w) * d.n + i];
+                v = e & 0x7fffu;
+                sign = e >> 15;
+            } else {
+                const int c = int(window_width(d, w));
+                const int32_t digit = int32_t(window_of(t, window_offset(d, w), c)) - int32_t((1u << (c - 1)) - 1);
+                v = uint32_t(digit < 0 ? -digit : digit);
+                sign = digit < 0 ? 1u : 0u;
+            }
+            if (v == 0) continue;
             const uint32_t pos = atomicAdd(&counts[v - 1], 1u);
             if (PLACE) {
                 uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
@@ -376,7 +415,7 @@ constexpr uint32_t GROUP_SHIFT = 6;
 constexpr uint32_t GROUP_BUCKETS = 1u << GROUP_SHIFT;
 
 static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
-    k_digits_group(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t grid, const uint32_t* __restrict__ scalars,
+    k_digits_group(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t grid, const uint16_t* __restrict__ planes,
                    const uint32_t* __restrict__ hist, const uint2* __restrict__ offsets,
                    uint32_t* __restrict__ staged) {
     __shared__ uint32_t cursor[MSM_LDS_BUCKETS / GROUP_BUCKETS];
@@ -393,21 +432,12 @@ static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
     }
     __syncthreads();
     const uint32_t i0 = j << d.chunk_shift, i1 = min(d.n, i0 + (1u << d.chunk_shift));
-    uint32_t kconst[9];
-    recode_constant(d, kconst);
-    const int c = int(window_width(d, w));
-    const uint32_t bit = window_offset(d, w);
-    const int32_t bias = int32_t((1u << (c - 1)) - 1);
+    const uint16_t* plane = planes + size_t(d.entry_base) + size_t(w) * d.n;
     for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const uint4* sp = reinterpret_cast<const uint4*>(scalars + size_t(d.scalar_base + i) * 8);
-        const uint4 a = sp[0], b = sp[1];
-        const uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        uint32_t t[9];
-        recode_add(s, kconst, t);
-        const int32_t digit = int32_t(window_of(t, bit, c)) - bias;
-        if (digit == 0) continue;
-        const uint32_t sign = digit < 0 ? 1u : 0u;
-        const uint32_t bucket = uint32_t(digit < 0 ? -digit : digit) - 1u;
+        const uint32_t e = plane[i];
+        if ((e & 0x7fffu) == 0) continue;
+        const uint32_t sign = e >> 15;
+        const uint32_t bucket = (e & 0x7fffu) - 1u;
         const uint32_t pos = atomicAdd(&cursor[bucket >> GROUP_SHIFT], 1u);
         staged[pos] = i | ((bucket & (GROUP_BUCKETS - 1u)) << 24) | (sign << 31);
     }
@@ -1231,8 +1261,10 @@ inline MsmBufferBytes msm_buffer_bytes(const MsmBatchPlan& plan, int xw) {
     MsmBufferBytes b;
     b.counts = size_t(plan.total_buckets) * 4;
     b.offsets = size_t(plan.total_buckets) * sizeof(uint2);
-    b.digits = plan.lds_digits ? 0 : size_t(plan.total_entries) * 4;
-    b.ranks = b.digits;
+    // LDS digits keep no digit array, except the 16-bit planes of group mode
+    b.digits = plan.lds_digits ? (plan.group_place ? size_t(plan.total_entries) * 2 : 0)
+                               : size_t(plan.total_entries) * 4;
+    b.ranks = plan.lds_digits ? 0 : b.digits;
     b.sorted = size_t(plan.total_entries) * 4;
     b.block_sums = tiles * sizeof(uint2);
     b.total = TOTAL_WORDS * 4;
@@ -1279,6 +1311,8 @@ inline void msm_batch_prepare_impl(const MsmBatchPlan& plan, int xw, MsmWorkspac
     if (!plan.lds_digits) {
         ws.digits.ensure(b.digits);
         ws.ranks.ensure(b.ranks);
+    } else if (plan.group_place) {
+        ws.digits.ensure(b.digits);  // the digit planes
     }
     ws.sorted.ensure(b.sorted);
     ws.block_sums.ensure(b.block_sums);
@@ -1330,7 +1364,8 @@ inline MsmBuffers msm_buffers(const MsmBatchPlan& plan, MsmWorkspace& ws, int xw
                 ws.descs.bytes >= b.m * sizeof(MsmDesc) && ws.heavy.bytes >= plan.max_heavy * 4 &&
                 ws.block_sums.bytes >= b.tiles * sizeof(uint2) &&
                 (!plan.lds_digits || ws.hist.bytes >= size_t(plan.total_hist) * 4) &&
-                (!plan.group_place || ws.partials.bytes >= size_t(plan.total_entries) * 4),
+                (!plan.group_place || (ws.partials.bytes >= size_t(plan.total_entries) * 4 &&
+                                       ws.digits.bytes >= size_t(plan.total_entries) * 2)),
             "MSM workspace not prepared for this batch");
     b.descs = static_cast<const MsmDesc*>(ws.descs.ptr);
     b.counts = static_cast<uint32_t*>(ws.counts.ptr);
@@ -1362,7 +1397,16 @@ inline void msm_sort_stage(const uint32_t* point_index, const uint32_t* scalars,
     const unsigned task_blocks = grid_for(plan.total_buckets, TASK_THREADS * per);
     msm_mark(ev, k0, stream);
     k_fill<<<1, 256, 0, stream>>>(b.total, TOTAL_CLASS_COUNTS + tp, 0u);
-    if (plan.lds_digits) {
+    uint16_t* planes = reinterpret_cast<uint16_t*>(b.digits);
+    if (plan.group_place) {
+        k_digit_planes<<<grid_for(plan.total_threads, 256), 256, 0, stream>>>(b.descs, m, plan.total_threads, scalars,
+                                                                             planes);
+        k_digits_lds<false, true><<<plan.total_hist_blocks, plan.digit_threads, 0, stream>>>(
+            b.descs, m, plan.total_hist_blocks, scalars, b.hist, b.offsets, point_index, plan.point_limit, b.total + 3,
+            b.sorted, planes);
+        k_hist_prefix<<<grid_for(plan.total_buckets, 256), 256, 0, stream>>>(b.descs, m, plan.total_buckets, b.hist,
+                                                                           b.counts, tp, b.total);
+    } else if (plan.lds_digits) {
         k_digits_lds<false><<<plan.total_hist_blocks, plan.digit_threads, 0, stream>>>(
             b.descs, m, plan.total_hist_blocks, scalars, b.hist, b.offsets, point_index, plan.point_limit, b.total + 3,
             b.sorted);
@@ -1384,7 +1428,7 @@ inline void msm_sort_stage(const uint32_t* point_index, const uint32_t* scalars,
     msm_mark(ev, k0 + 2, stream);
     if (plan.group_place) {
         k_digits_group<<<plan.total_hist_blocks, plan.digit_threads, 0, stream>>>(
-            b.descs, m, plan.total_hist_blocks, scalars, b.hist, b.offsets, b.partials);
+            b.descs, m, plan.total_hist_blocks, planes, b.hist, b.offsets, b.partials);
         k_group_place<<<plan.total_buckets / GROUP_BUCKETS, 256, 0, stream>>>(
             b.descs, m, b.counts, b.offsets, b.partials, point_index, plan.point_limit, b.total + 3, b.sorted);
     } else if (plan.lds_digits)
