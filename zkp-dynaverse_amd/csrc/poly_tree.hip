@@ -387,8 +387,10 @@ struct DevTree {
             else k_conv_small<false><<<grid, BLOCK, 0, stream()>>>(xs, h, xs, h, total, L2, m1, m2);
             HIP_CHECK_LAUNCH();
         } else {
-            ntt_rows(o->domain, xs, xs, L2, rows, false, o->ntt, stream());
-            if (two) ntt_rows(o->domain, x2, x2, L2, rows, false, o->ntt, stream());
+            // a second operand stored right after the first: one transform call for both
+            const bool adjacent = two && x2 == xs + (rows << L2) * 8;
+            ntt_rows(o->domain, xs, xs, L2, adjacent ? 2 * rows : rows, false, o->ntt, stream());
+            if (two && !adjacent) ntt_rows(o->domain, x2, x2, L2, rows, false, o->ntt, stream());
             if (two) k_pointwise_rows<true><<<grid, BLOCK, 0, stream()>>>(xs, h, x2, h2, total, L2, m1, m2);
             else k_pointwise_rows<false><<<grid, BLOCK, 0, stream()>>>(xs, h, xs, h, total, L2, m1, m2);
             HIP_CHECK_LAUNCH();
@@ -398,8 +400,20 @@ struct DevTree {
         st->products += two ? 2 * rows : rows;
     }
 
-    // X <- X * Y row by row (Y is left transformed).
+    // X <- X * Y row by row (Y is left transformed).  Y stored right after X
+    // is transformed in the same call as X.
     void mul(uint32_t* xs, uint32_t* ys, uint64_t rows, int L2) {
+        const uint64_t total = rows << L2;
+        if (L2 > SMALL_LOG && ys == xs + total * 8) {
+            ntt_rows(o->domain, xs, xs, L2, 2 * rows, false, o->ntt, stream());
+            const unsigned grid = grid_for(total, BLOCK);
+            k_pointwise_rows<false><<<grid, BLOCK, 0, stream()>>>(xs, ys, xs, ys, total, L2, SAME_ROW, SAME_ROW);
+            HIP_CHECK_LAUNCH();
+            ntt_rows(o->domain, xs, xs, L2, rows, true, o->ntt, stream());
+            ++st->rounds;
+            st->products += rows;
+            return;
+        }
         prep(ys, rows, L2);
         mul_pre(xs, ys, nullptr, nullptr, rows, L2, SAME_ROW, SAME_ROW);
     }
@@ -431,10 +445,10 @@ struct DevTree {
         uint32_t *vin = P(ra), *vout = P(rb);
         for (int l = 0; l < L; ++l) {
             const uint64_t pairs = K >> (l + 1);
-            k_pair_gather<<<grid_for(K, BLOCK), BLOCK, 0, stream()>>>(vin, P(x), P(y), K, l);
+            k_pair_gather<<<grid_for(K, BLOCK), BLOCK, 0, stream()>>>(vin, P(x), P(x, K), K, l);
             HIP_CHECK_LAUNCH();
             // P_A B + P_B A: right child's row with P_A, left child's with P_B
-            mul_pre(P(x), P(nh, uint64_t(l) * 2 * K), P(y), P(nh, uint64_t(l) * 2 * K), pairs, l + 1,
+            mul_pre(P(x), P(nh, uint64_t(l) * 2 * K), P(x, K), P(nh, uint64_t(l) * 2 * K), pairs, l + 1,
                     RowMap{~uint64_t(0), 2, 1}, RowMap{~uint64_t(0), 2, 0});
             k_pair_fix<<<grid_for(K, BLOCK), BLOCK, 0, stream()>>>(vin, P(x), vout, k, K, l);
             HIP_CHECK_LAUNCH();
@@ -496,9 +510,9 @@ DevTree* dev_tree_build(Owner* o, const uint32_t* points, size_t k, DevTreeStats
     for (int l = 0; l < L; ++l) {
         const uint64_t pairs = K >> (l + 1);
         uint32_t* src = t->P(t->lev, uint64_t(l) * K);
-        k_pair_gather<<<grid_for(K, BLOCK), BLOCK, 0, s>>>(src, t->P(t->x), t->P(t->y), K, l);
+        k_pair_gather<<<grid_for(K, BLOCK), BLOCK, 0, s>>>(src, t->P(t->x), t->P(t->x, K), K, l);
         HIP_CHECK_LAUNCH();
-        t->mul(t->P(t->x), t->P(t->y), pairs, l + 1);
+        t->mul(t->P(t->x), t->P(t->x, K), pairs, l + 1);
         k_pair_fix<<<grid_for(K, BLOCK), BLOCK, 0, s>>>(src, t->P(t->x), t->P(t->lev, uint64_t(l + 1) * K), k, K,
                                                          l);
         HIP_CHECK_LAUNCH();
@@ -518,13 +532,14 @@ DevTree* dev_tree_build(Owner* o, const uint32_t* points, size_t k, DevTreeStats
             count += K >> l;
         }
         const uint64_t total = count * 2 * m;
-        k_newton_gather<<<grid_for(total, BLOCK), BLOCK, 0, s>>>(t->P(t->lev), t->P(t->rh), t->P(t->x), t->P(t->y),
-                                                                 total, k, K, m, rows);
+        uint32_t *xs = t->P(t->x), *ys = t->P(t->x, total);  // Y right after X (total <= 2K)
+        k_newton_gather<<<grid_for(total, BLOCK), BLOCK, 0, s>>>(t->P(t->lev), t->P(t->rh), xs, ys, total, k, K, m,
+                                                                 rows);
         HIP_CHECK_LAUNCH();
-        t->mul(t->P(t->x), t->P(t->y), count, sbit + 1);
-        k_newton_mid<<<grid_for(total, BLOCK), BLOCK, 0, s>>>(t->P(t->x), t->P(t->z), total, m);
+        t->mul(xs, ys, count, sbit + 1);
+        k_newton_mid<<<grid_for(total, BLOCK), BLOCK, 0, s>>>(xs, t->P(t->z), total, m);
         HIP_CHECK_LAUNCH();
-        t->mul_pre(t->P(t->z), t->P(t->y), nullptr, nullptr, count, sbit + 1, SAME_ROW, SAME_ROW);
+        t->mul_pre(t->P(t->z), ys, nullptr, nullptr, count, sbit + 1, SAME_ROW, SAME_ROW);
         const uint64_t puts = count * m;
         k_newton_put<<<grid_for(puts, BLOCK), BLOCK, 0, s>>>(t->P(t->z), t->P(t->rh), puts, K, m, rows);
         HIP_CHECK_LAUNCH();
