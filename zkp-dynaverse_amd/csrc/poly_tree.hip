@@ -370,8 +370,8 @@ struct DevTree {
 
     void domain() { ntt_domain_prepare(o->domain, root, o->stream); }
 
-    // The transform of rows x 2^L operand rows, in place (nothing for the
-    // schoolbook lengths, which keep coefficients).
+    // Transform `rows` operand rows of 2^L2 elements in place (nothing for
+    // the schoolbook lengths, whose operands stay coefficients).
     void prep(uint32_t* h, uint64_t rows, int L2) {
         if (L2 > SMALL_LOG) ntt_rows(o->domain, h, h, L2, rows, false, o->ntt, stream());
     }
