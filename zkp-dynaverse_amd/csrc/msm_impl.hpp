@@ -414,32 +414,79 @@ static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
 constexpr uint32_t GROUP_SHIFT = 6;
 constexpr uint32_t GROUP_BUCKETS = 1u << GROUP_SHIFT;
 
+// Entries a staging block sorts by group in LDS before writing them out, so
+// that each group's run leaves as contiguous stores (16-byte aligned pieces
+// of whole lines) instead of one scattered 4-byte store per entry.
+constexpr uint32_t GROUP_STAGE = 15360;
+
 static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
     k_digits_group(const MsmDesc* __restrict__ descs, uint32_t m, uint32_t grid, const uint16_t* __restrict__ planes,
                    const uint32_t* __restrict__ hist, const uint2* __restrict__ offsets,
                    uint32_t* __restrict__ staged) {
-    __shared__ uint32_t cursor[MSM_LDS_BUCKETS / GROUP_BUCKETS];
+    constexpr uint32_t GROUPS = MSM_LDS_BUCKETS / GROUP_BUCKETS;
+    __shared__ uint32_t cursor[GROUPS];  // this block's next slot in each group's run
+    __shared__ uint32_t lcount[GROUPS], lfill[GROUPS];
+    __shared__ uint32_t ents[GROUP_STAGE];
     const uint32_t g = xcd_grouped_block(blockIdx.x, grid);
     const MsmDesc d = descs[find_desc<&MsmDesc::hist_block_base>(descs, m, g)];
     const uint32_t local = g - d.hist_block_base;
     const uint32_t j = local / d.digit_windows, w = local % d.digit_windows;
     const uint32_t* mine = hist + d.hist_base + size_t(local) * d.buckets;
     const uint2* starts = offsets + d.bucket_base + w * d.buckets;
-    const uint32_t groups = d.buckets >> GROUP_SHIFT, lane = threadIdx.x & 63, waves = blockDim.x >> 6;
-    for (uint32_t s = threadIdx.x >> 6; s < groups; s += waves) {  // one wave per group
+    const uint32_t groups = d.buckets >> GROUP_SHIFT, lane = threadIdx.x & 63, wave = threadIdx.x >> 6,
+                   waves = blockDim.x >> 6;
+    for (uint32_t s = wave; s < groups; s += waves) {  // one wave per group
         const uint32_t sum = wave_sum(mine[s * GROUP_BUCKETS + lane]);
         if (lane == 0) cursor[s] = starts[s * GROUP_BUCKETS].x + sum;
     }
-    __syncthreads();
     const uint32_t i0 = j << d.chunk_shift, i1 = min(d.n, i0 + (1u << d.chunk_shift));
     const uint16_t* plane = planes + size_t(d.entry_base) + size_t(w) * d.n;
-    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const uint32_t e = plane[i];
-        if ((e & 0x7fffu) == 0) continue;
-        const uint32_t sign = e >> 15;
-        const uint32_t bucket = (e & 0x7fffu) - 1u;
-        const uint32_t pos = atomicAdd(&cursor[bucket >> GROUP_SHIFT], 1u);
-        staged[pos] = i | ((bucket & (GROUP_BUCKETS - 1u)) << 24) | (sign << 31);
+    for (uint32_t r0 = i0; r0 < i1; r0 += GROUP_STAGE) {
+        const uint32_t r1 = min(i1, r0 + GROUP_STAGE);
+        for (uint32_t s = threadIdx.x; s < groups; s += blockDim.x) lcount[s] = 0;
+        __syncthreads();
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+            const uint32_t v = plane[i] & 0x7fffu;
+            if (v) atomicAdd(&lcount[(v - 1u) >> GROUP_SHIFT], 1u);
+        }
+        __syncthreads();
+        if (wave == 0) {  // exclusive prefix of the group counts (up to four groups a lane)
+            uint32_t c[4], sum = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t s = lane * 4 + q;
+                c[q] = s < groups ? lcount[s] : 0u;
+                sum += c[q];
+            }
+            uint32_t incl = sum;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t t = __shfl_up(incl, off, 64);
+                if (lane >= uint32_t(off)) incl += t;
+            }
+            uint32_t at = incl - sum;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t s = lane * 4 + q;
+                if (s < groups) lfill[s] = at;
+                at += c[q];
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+            const uint32_t e = plane[i];
+            if ((e & 0x7fffu) == 0) continue;
+            const uint32_t bucket = (e & 0x7fffu) - 1u;
+            const uint32_t pos = atomicAdd(&lfill[bucket >> GROUP_SHIFT], 1u);
+            ents[pos] = i | ((bucket & (GROUP_BUCKETS - 1u)) << 24) | ((e >> 15) << 31);
+        }
+        __syncthreads();
+        for (uint32_t s = wave; s < groups; s += waves) {  // one wave copies a group's run
+            const uint32_t n = lcount[s], src = lfill[s] - n, dst = cursor[s];
+            for (uint32_t k = lane; k < n; k += 64) staged[dst + k] = ents[src + k];
+            if (lane == 0) cursor[s] = dst + n;
+        }
+        __syncthreads();
     }
 }
 
