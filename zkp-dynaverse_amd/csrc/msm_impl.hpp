@@ -490,36 +490,68 @@ static __global__ void __launch_bounds__(MSM_DIGIT_THREADS)
     }
 }
 
+// Entries k_group_place sorts by bucket in LDS per round, so that each
+// bucket's share of the round leaves as one contiguous piece.
+constexpr uint32_t PLACE_STAGE = 4096;
+
 static __global__ void __launch_bounds__(256)
     k_group_place(const MsmDesc* __restrict__ descs, uint32_t m, const uint32_t* __restrict__ counts,
                   const uint2* __restrict__ offsets, const uint32_t* __restrict__ staged,
                   const uint32_t* __restrict__ point_index, uint32_t point_limit, uint32_t* __restrict__ flags,
                   uint32_t* __restrict__ sorted) {
-    __shared__ uint32_t cursor[GROUP_BUCKETS];
+    __shared__ uint32_t cursor[GROUP_BUCKETS], lcount[GROUP_BUCKETS], lfill[GROUP_BUCKETS];
+    __shared__ uint32_t ents[PLACE_STAGE];
     __shared__ uint32_t range[2];
     const uint32_t gb0 = blockIdx.x * GROUP_BUCKETS;
     const MsmDesc d = descs[find_desc<&MsmDesc::bucket_base>(descs, m, gb0)];
-    if (threadIdx.x < GROUP_BUCKETS) {  // the first wave: one bucket per lane
-        const uint32_t start = offsets[gb0 + threadIdx.x].x;
-        cursor[threadIdx.x] = start;
-        const uint32_t len = wave_sum(counts[gb0 + threadIdx.x]);
-        if (threadIdx.x == 0) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    if (wave == 0) {  // one bucket per lane
+        const uint32_t start = offsets[gb0 + lane].x;
+        cursor[lane] = start;
+        const uint32_t len = wave_sum(counts[gb0 + lane]);
+        if (lane == 0) {
             range[0] = start;
             range[1] = len;
         }
     }
     __syncthreads();
     const uint32_t start = range[0], len = range[1];
-    for (uint32_t k = threadIdx.x; k < len; k += blockDim.x) {
-        const uint32_t e = staged[start + k];
-        const uint32_t i = e & 0xffffffu;
-        uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
-        if (pidx >= point_limit) {  // flagged for the host; point 0 keeps the access in bounds
-            flags[0] = 1u;
-            pidx = 0;
+    for (uint32_t r0 = 0; r0 < len; r0 += PLACE_STAGE) {
+        const uint32_t r1 = min(len, r0 + PLACE_STAGE);
+        if (threadIdx.x < GROUP_BUCKETS) lcount[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t k = r0 + threadIdx.x; k < r1; k += blockDim.x)
+            atomicAdd(&lcount[(staged[start + k] >> 24) & (GROUP_BUCKETS - 1u)], 1u);
+        __syncthreads();
+        if (wave == 0) {  // exclusive prefix of the bucket counts, one a lane
+            const uint32_t c = lcount[lane];
+            uint32_t incl = c;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t t = __shfl_up(incl, off, 64);
+                if (lane >= uint32_t(off)) incl += t;
+            }
+            lfill[lane] = incl - c;
         }
-        const uint32_t pos = atomicAdd(&cursor[(e >> 24) & (GROUP_BUCKETS - 1u)], 1u);
-        sorted[pos] = pidx | (e & 0x80000000u);
+        __syncthreads();
+        for (uint32_t k = r0 + threadIdx.x; k < r1; k += blockDim.x) {
+            const uint32_t e = staged[start + k];
+            const uint32_t i = e & 0xffffffu;
+            uint32_t pidx = point_index ? point_index[d.point_base + i] : d.point_base + i;
+            if (pidx >= point_limit) {  // flagged for the host; point 0 keeps the access in bounds
+                flags[0] = 1u;
+                pidx = 0;
+            }
+            const uint32_t pos = atomicAdd(&lfill[(e >> 24) & (GROUP_BUCKETS - 1u)], 1u);
+            ents[pos] = pidx | (e & 0x80000000u);
+        }
+        __syncthreads();
+        for (uint32_t b = wave; b < GROUP_BUCKETS; b += waves) {  // one wave copies a bucket's piece
+            const uint32_t n = lcount[b], src = lfill[b] - n, dst = cursor[b];
+            for (uint32_t k = lane; k < n; k += 64) sorted[dst + k] = ents[src + k];
+            if (lane == 0) cursor[b] = dst + n;
+        }
+        __syncthreads();
     }
 }
 
