@@ -110,3 +110,20 @@ def test_msm_prefix_sizes(env, log_n):
     n = 1 << log_n
     print(f"G1 MSM 2^{log_n}, device entry", flush=True)
     assert b.msm_device(d_srs.ptr, d_sc.ptr, n) == oracle_commitment(scal[:n], tau)
+
+
+@pytest.mark.parametrize("window", [0, 14])
+def test_msm_2p20_one_scalar_everywhere(env, window):
+    """Every scalar equal: in every window all 2^20 entries fall into one
+    bucket, so one placement group holds them all and every staging block
+    sends its whole chunk (2^15 points at c = 15, 2^14 at c = 14) to that
+    group over several LDS rounds, and the group's placement runs 256 rounds.
+    Expected: [s * sum_i tau^i] g1 by the geometric sum, on the host."""
+    b, tau, _, _, d_srs, _ = env
+    s = ChaCha20Stream(9).nonzero_fr()
+    d = b.to_device(np.frombuffer(s.to_bytes(32, "little") * N, dtype=np.uint8))
+    geo = (pow(tau, N, o.R) - 1) * pow(tau - 1, o.R - 2, o.R) % o.R
+    try:
+        assert b.msm_device(d_srs.ptr, d.ptr, N, window_bits=window) == o.g1_to_bytes(o.g1_mul(s * geo % o.R))
+    finally:
+        d.free()
